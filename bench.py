@@ -1,0 +1,194 @@
+"""Benchmark: standalone Legendre transform pair on one MI355X.
+
+Workload (the headline configuration): Lmax=255, nth=384 Gauss points,
+Nr=96 radial points, 12 field components (4 vectors), seeded Gaussian
+spectra with amplitude (l+1)^-1.5.  One step = one backward + one forward
+transform, inputs resident in HBM.  Every precision mode runs; each mode's
+output is compared with the FP64 GPU path, and the FP64 path with the
+reference's own org-loop transforms (oracle/_ref, compiled from the reference
+sources) on a bounded sample of radial levels, timed on the host cores as the
+CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes ...]
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+L, NTH, NR, NV, NS = 255, 384, 96, 4, 0
+NCOMP = 3 * NV + NS
+ERR_BOUND = 1e-11
+PEAK = {"hip_fp64": 78.6, "hip_fp32": 157.3, "hip_bf16x3": 2500.0}   # TFLOP/s dense
+HBM_PEAK = 8000.0   # GB/s
+
+
+def legendre_flops(L, nth, nr, nvec, nscl):
+    """Symmetric-form work of one direction: 2 th_h Nr (L+1)^2 (5 nvec + nscl)."""
+    th_h = (nth + 1) // 2
+    return 2.0 * th_h * nr * (L + 1) ** 2 * (5 * nvec + nscl)
+
+
+def make_spectrum(torch, J, seed=1234):
+    l = torch.floor(torch.sqrt(torch.arange(J, dtype=torch.float64, device="cuda")))
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    x = torch.randn((NR, J, NCOMP), generator=gen, dtype=torch.float64, device="cuda")
+    return (x * ((l + 1.0) ** -1.5)[None, :, None]).reshape(-1).contiguous()
+
+
+def field_err(got, ref, ncomp):
+    g = got.reshape(-1, ncomp)
+    r = ref.reshape(-1, ncomp)
+    s = r.abs().amax(dim=0).clamp_min(1e-300)
+    return ((g - r).abs().amax(dim=0) / s).max().item()
+
+
+def time_mode(torch, calypso_amd, mode, radius, sp, steps, warmup):
+    lt = calypso_amd.LegendreTransform(L, NTH, radius, NCOMP, NV, NS, mode)
+    vr = torch.empty(lt.n_rtm(), dtype=torch.float64, device="cuda")
+    sp2 = torch.empty_like(sp)
+    for _ in range(warmup):
+        lt.backward(sp, vr)
+        lt.forward(vr, sp2)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+            torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1, e2 in evs:
+        e0.record()
+        lt.backward(sp, vr)
+        e1.record()
+        lt.forward(vr, sp2)
+        e2.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    bwd_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in evs]))
+    fwd_ms = float(np.mean([b.elapsed_time(c) for _, b, c in evs]))
+    info = lt.info()
+    lt.close()
+    return {"wall_s": wall, "bwd_ms": bwd_ms, "fwd_ms": fwd_ms, "vr": vr, "sp2": sp2,
+            "info": info}
+
+
+def cpu_baseline(calypso_amd, radius, sp_host, vr_gpu_host, nsample, threads):
+    """Reference org loops (oracle/_ref) on `nsample` radial levels."""
+    import oracle
+    rs = radius[:nsample]
+    grid = oracle.Grid(L, NTH, rs)
+    J = (L + 1) ** 2
+    sp_s = sp_host.reshape(NR, J, NCOMP)[:nsample].ravel()
+    kind = "reference" if oracle.ref_available() else "port"
+    if kind == "reference":
+        ref = oracle.RefLegendre(grid, nthreads=threads)
+        t0 = time.perf_counter()
+        vr_c = ref.bwd(NCOMP, NV, NS, sp_s)
+        t1 = time.perf_counter()
+        sp_c = ref.fwd(NCOMP, NV, NS, vr_c)
+        t2 = time.perf_counter()
+    else:
+        threads = 1
+        t0 = time.perf_counter()
+        vr_c = oracle.leg_bwd(grid, NCOMP, NV, NS, sp_s)
+        t1 = time.perf_counter()
+        sp_c = oracle.leg_fwd(grid, NCOMP, NV, NS, vr_c)
+        t2 = time.perf_counter()
+    sec = (t2 - t0) * (NR / nsample)
+    # GPU FP64 output on the same levels vs the reference
+    M = 2 * L + 1
+    g = vr_gpu_host.reshape(M, NR, NTH, NCOMP)[:, :nsample]
+    c = vr_c.reshape(M, nsample, NTH, NCOMP)
+    errs = [np.max(np.abs(g[..., nd] - c[..., nd])) / np.max(np.abs(c[..., nd]))
+            for nd in range(NCOMP)]
+    return {"value": 1.0 / sec, "unit": "steps/s", "cores": threads, "kind": kind,
+            "sample": f"{nsample} of {NR} radial levels (bwd {t1 - t0:.2f}s + fwd {t2 - t1:.2f}s, "
+                      f"scaled x{NR / nsample:.0f}); reference org-loop transforms "
+                      f"legendre_{{b,f}}_trans_*_org with np_smp={threads}",
+            "gpu_fp64_err_vs_cpu": float(max(errs))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--modes", default="hip_fp64,hip_fp32,hip_bf16x3")
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded "
+              f"design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import calypso_amd
+
+    radius = np.linspace(7.0 / 13.0, 20.0 / 13.0, NR)
+    J = (L + 1) ** 2
+    sp = make_spectrum(torch, J)
+    flops = legendre_flops(L, NTH, NR, NV, NS)
+    modes = [m for m in args.modes.split(",") if m]
+    res = {}
+    ref_vr = ref_sp = None
+    for mode in modes:
+        r = time_mode(torch, calypso_amd, mode, radius, sp, args.steps, args.warmup)
+        if mode == "hip_fp64":
+            ref_vr, ref_sp = r["vr"].clone(), r["sp2"].clone()
+        res[mode] = r
+        print(f"# {mode}: bwd {r['bwd_ms']:.3f} ms  fwd {r['fwd_ms']:.3f} ms", file=sys.stderr,
+              flush=True)
+    out_modes = {}
+    for mode, r in res.items():
+        step_ms = r["wall_s"] * 1e3 / args.steps
+        e = {"steps_per_s": 1e3 / step_ms, "ms_per_step": step_ms,
+             "bwd_ms": r["bwd_ms"], "fwd_ms": r["fwd_ms"],
+             "legendre_tflops": 2 * flops / ((r["bwd_ms"] + r["fwd_ms"]) * 1e-3) / 1e12,
+             "kt": [r["info"]["kt_bwd"], r["info"]["kt_fwd"]]}
+        if ref_vr is not None:
+            e["err_vs_fp64_bwd"] = field_err(r["vr"], ref_vr, NCOMP)
+            e["err_vs_fp64_fwd"] = field_err(r["sp2"], ref_sp, NCOMP)
+        out_modes[mode] = e
+    # headline: fastest mode meeting the error bound (FP64 itself is the judge)
+    ok = [m for m in out_modes if m == "hip_fp64" or
+          max(out_modes[m].get("err_vs_fp64_bwd", 1), out_modes[m].get("err_vs_fp64_fwd", 1)) <= ERR_BOUND]
+    head = max(ok, key=lambda m: out_modes[m]["steps_per_s"]) if ok else modes[0]
+    h = out_modes[head]
+    dom = "bwd" if h["bwd_ms"] >= h["fwd_ms"] else "fwd"
+    dom_ms = h[f"{dom}_ms"]
+    achieved = flops / (dom_ms * 1e-3) / 1e12
+    line = {
+        "metric": "steps/s + Legendre TFLOP/s, mixed-prec vs FP64 GPU, Lmax=255 Nr=96, err<=1e-11",
+        "value": h["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"hip_fp64": "f64", "hip_fp32": "f32", "hip_bf16x3": "bf16x3"}[head],
+        "data": "synthetic (seeded Gaussian spectra, amplitude (l+1)^-1.5)",
+        "config": {"workload": "legendre bwd+fwd pair", "Lmax": L, "nth": NTH, "Nr": NR,
+                   "components": NCOMP, "vectors": NV, "scalars": NS, "headline_mode": head,
+                   "err_bound": ERR_BOUND},
+        "legendre_tflops": h["legendre_tflops"],
+        "modes": out_modes,
+        "roofline": {"kernel": f"k_leg_{dom} ({head})", "bound": "mfma", "achieved": achieved,
+                     "peak": PEAK[head], "unit": "TFLOP/s", "frac": achieved / PEAK[head],
+                     "traffic": None},
+    }
+    if not args.no_cpu:
+        vr_host = ref_vr.cpu().numpy() if ref_vr is not None else res[modes[0]]["vr"].cpu().numpy()
+        line["cpu_baseline"] = cpu_baseline(calypso_amd, radius, sp.cpu().numpy(), vr_host,
+                                            args.cpu_sample, args.cpu_threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,65 @@
+"""ctypes binding of libcalypso_amd.so (include/calypso_amd.h).
+
+The library is loaded from ``calypso_amd/lib`` only.  There is no CPU
+fallback: if the HIP library is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libcalypso_amd.so")
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ("calypso_last_error", ctypes.c_char_p, []),
+    ("calypso_legendre_mode_id", _I, [ctypes.c_char_p]),
+    ("calypso_gauss_points", _I, [_I, _P, _P, _P]),
+    ("calypso_leg_create", _I, [_I, _I, _I, _P, _P, _P, _I, _I, _I, _I, ctypes.POINTER(_P)]),
+    ("calypso_leg_destroy", _I, [_P]),
+    ("calypso_leg_backward_device", _I, [_P, _I, _I, _I, _P, _P, _P]),
+    ("calypso_leg_forward_device", _I, [_P, _I, _I, _I, _P, _P, _P]),
+    ("calypso_leg_backward", _I, [_P, _I, _I, _I, _I, _I, _P, _P]),
+    ("calypso_leg_forward", _I, [_P, _I, _I, _I, _I, _I, _P, _P]),
+    ("calypso_leg_tables", _I, [_P, _P, _P]),
+    ("calypso_leg_info", _I, [_P, _P]),
+    ("calypso_leg_stream", _P, [_P]),
+    ("calypso_sel_init_legendre_trans", _I, [_I, _I, _I, _I, _P, _P, _P, _I, _I, _I]),
+    ("calypso_sel_backward_legendre_trans", _I, [_I, _I, _I, _I, _I, _P, _P]),
+    ("calypso_sel_forward_legendre_trans", _I, [_I, _I, _I, _I, _I, _P, _P]),
+    ("calypso_sel_finalize_legendre_trans", _I, []),
+]
+
+_lib = None
+
+
+class CalypsoError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CalypsoError(
+                f"{LIB_PATH} not found: build it with `python -m calypso_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = load().calypso_last_error().decode(errors="replace")
+        raise CalypsoError(f"{what}: {msg}" if what else msg)

@@ -1,0 +1,79 @@
+"""Build the in-tree libraries for gfx950.
+
+* ``calypso_amd/lib/libcalypso_amd.so``   HIP kernels + C ABI (hipcc)
+* ``calypso_amd/lib/libcalypso_amd_f.so`` Fortran module shim (amdflang),
+  ``legendre_trans_hip`` and ``calypso_amd_capi`` (.mod files beside it)
+* ``calypso_amd/lib/leg_driver``           Fortran program used by the tests
+  to drive the shim end to end
+
+Run ``python -m calypso_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "lib")
+CSRC = os.path.join(HERE, "csrc")
+FSRC = os.path.join(HERE, "fortran")
+
+HIP_SOURCES = ["legendre.hip", "capi.cpp"]
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        print(out)
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd)}")
+    return out
+
+
+def build_hip():
+    os.makedirs(LIB, exist_ok=True)
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    out = os.path.join(LIB, "libcalypso_amd.so")
+    tmp = out + ".tmp"
+    _run([hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"),
+          *[os.path.join(CSRC, s) for s in HIP_SOURCES], "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_fortran():
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        print("amdflang not found: Fortran shim skipped", flush=True)
+        return None
+    obj = os.path.join(LIB, "fobj")
+    os.makedirs(obj, exist_ok=True)
+    srcs = ["calypso_amd_capi.f90", "legendre_trans_hip.f90"]
+    for s in srcs:
+        _run([fc, "-O2", "-fPIC", "-c", os.path.join(FSRC, s), "-module-dir", LIB,
+              "-o", os.path.join(obj, s.replace(".f90", ".o"))])
+    so = os.path.join(LIB, "libcalypso_amd_f.so")
+    _run([fc, "-shared", "-o", so, *[os.path.join(obj, s.replace(".f90", ".o")) for s in srcs],
+          "-L", LIB, "-lcalypso_amd", "-Wl,-rpath,$ORIGIN"])
+    drv = os.path.join(LIB, "leg_driver")
+    _run([fc, "-O2", os.path.join(FSRC, "leg_driver.f90"), "-I", LIB, "-o", drv,
+          "-L", LIB, "-lcalypso_amd_f", "-lcalypso_amd", "-Wl,-rpath,$ORIGIN"])
+    return so
+
+
+def main():
+    build_hip()
+    build_fortran()
+
+
+if __name__ == "__main__":
+    sys.exit(main())

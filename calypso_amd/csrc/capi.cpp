@@ -1,0 +1,220 @@
+// capi.cpp -- extern "C" boundary of libcalypso_amd.so (include/calypso_amd.h).
+#include "../../include/calypso_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cctype>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "leg_internal.hpp"
+
+using calypso::LegContext;
+
+struct calypso_leg {
+    LegContext c;
+};
+
+namespace {
+
+thread_local std::string g_err;
+calypso_leg *g_sel = nullptr;   // process-wide selector context
+
+int fail(const std::string &m)
+{
+    g_err = m;
+    return -1;
+}
+
+#define CHECK_HIP(expr)                                                   \
+    do {                                                                  \
+        hipError_t e_ = (expr);                                           \
+        if (e_ != hipSuccess) return fail(std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+int ensure(double **buf, size_t *cap, size_t n)
+{
+    if (*cap >= n) return 0;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(buf, n * sizeof(double));
+    if (e != hipSuccess) return fail(std::string("hipMalloc staging: ") + hipGetErrorString(e));
+    *cap = n;
+    return 0;
+}
+
+int host_transform(calypso_leg *h, bool fwd, int ncomp, int nvec, int nscl, int n_WR,
+                   int n_WS, const double *WR, double *WS)
+{
+    LegContext &c = h->c;
+    const size_t nrlm = (size_t)ncomp * c.g.J * c.g.nr;
+    const size_t nrtm = (size_t)ncomp * c.g.nth * c.g.nr * c.g.M;
+    const size_t nin = fwd ? nrtm : nrlm, nout = fwd ? nrlm : nrtm;
+    if ((size_t)n_WR < nin) return fail("n_WR smaller than ncomp * nnod of the input space");
+    if ((size_t)n_WS < nout) return fail("n_WS smaller than ncomp * nnod of the output space");
+    if (ensure(&c.d_rlm, &c.cap_rlm, nrlm) || ensure(&c.d_rtm, &c.cap_rtm, nrtm)) return -1;
+    double *din = fwd ? c.d_rtm : c.d_rlm, *dout = fwd ? c.d_rlm : c.d_rtm;
+    CHECK_HIP(hipMemcpyAsync(din, WR, nin * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    std::string err;
+    int rc = fwd ? calypso::leg_forward_dev(&c, ncomp, nvec, nscl, din, dout, c.stream, &err)
+                 : calypso::leg_backward_dev(&c, ncomp, nvec, nscl, din, dout, c.stream, &err);
+    if (rc) return fail(err);
+    CHECK_HIP(hipMemcpyAsync(WS, dout, nout * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CHECK_HIP(hipStreamSynchronize(c.stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *calypso_last_error(void) { return g_err.c_str(); }
+
+int calypso_legendre_mode_id(const char *name)
+{
+    if (!name) return -1;
+    std::string s;
+    for (const char *p = name; *p; ++p) {
+        if (std::isspace((unsigned char)*p)) continue;
+        s.push_back((char)std::tolower((unsigned char)*p));
+    }
+    if (s == "hip_fp64") return CALYPSO_LEG_FP64;
+    if (s == "hip_fp32") return CALYPSO_LEG_FP32;
+    if (s == "hip_bf16x3") return CALYPSO_LEG_BF16X3;
+    return -1;
+}
+
+int calypso_gauss_points(int n, double *point, double *weight, double *colat)
+{
+    if (n < 1 || !point || !weight || !colat) return fail("calypso_gauss_points: bad arguments");
+    calypso::gauss_points_host(n, point, weight, colat);
+    return 0;
+}
+
+int calypso_leg_create(int l_truncation, int nth, int nr, const double *radius,
+                       const double *colat, const double *weight, int ncomp, int nvector,
+                       int nscalar, int mode, calypso_leg_t **out)
+{
+    if (!out || !radius) return fail("calypso_leg_create: null argument");
+    *out = nullptr;
+    calypso_leg *h = new (std::nothrow) calypso_leg();
+    if (!h) return fail("calypso_leg_create: out of host memory");
+    std::string err;
+    if (calypso::leg_context_build(&h->c, l_truncation, nth, nr, radius, colat, weight, ncomp,
+                                   nvector, nscalar, mode, &err)) {
+        calypso::leg_context_free(&h->c);
+        delete h;
+        return fail(err);
+    }
+    *out = h;
+    return 0;
+}
+
+int calypso_leg_destroy(calypso_leg_t *ctx)
+{
+    if (!ctx) return 0;
+    calypso::leg_context_free(&ctx->c);
+    delete ctx;
+    return 0;
+}
+
+int calypso_leg_backward_device(calypso_leg_t *ctx, int ncomp, int nvector, int nscalar,
+                                const double *d_sp_rlm, double *d_vr_rtm, void *stream)
+{
+    if (!ctx || !d_sp_rlm || !d_vr_rtm) return fail("calypso_leg_backward_device: null argument");
+    std::string err;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->c.stream;
+    if (calypso::leg_backward_dev(&ctx->c, ncomp, nvector, nscalar, d_sp_rlm, d_vr_rtm, s, &err))
+        return fail(err);
+    return 0;
+}
+
+int calypso_leg_forward_device(calypso_leg_t *ctx, int ncomp, int nvector, int nscalar,
+                               const double *d_vr_rtm, double *d_sp_rlm, void *stream)
+{
+    if (!ctx || !d_sp_rlm || !d_vr_rtm) return fail("calypso_leg_forward_device: null argument");
+    std::string err;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->c.stream;
+    if (calypso::leg_forward_dev(&ctx->c, ncomp, nvector, nscalar, d_vr_rtm, d_sp_rlm, s, &err))
+        return fail(err);
+    return 0;
+}
+
+int calypso_leg_backward(calypso_leg_t *ctx, int ncomp, int nvector, int nscalar, int n_WR,
+                         int n_WS, const double *WR, double *WS)
+{
+    if (!ctx || !WR || !WS) return fail("calypso_leg_backward: null argument");
+    return host_transform(ctx, false, ncomp, nvector, nscalar, n_WR, n_WS, WR, WS);
+}
+
+int calypso_leg_forward(calypso_leg_t *ctx, int ncomp, int nvector, int nscalar, int n_WR,
+                        int n_WS, const double *WR, double *WS)
+{
+    if (!ctx || !WR || !WS) return fail("calypso_leg_forward: null argument");
+    return host_transform(ctx, true, ncomp, nvector, nscalar, n_WR, n_WS, WR, WS);
+}
+
+int calypso_leg_tables(calypso_leg_t *ctx, double *P, double *dP)
+{
+    if (!ctx || !P || !dP) return fail("calypso_leg_tables: null argument");
+    std::string err;
+    if (calypso::leg_tables_to_host(&ctx->c, P, dP, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_leg_info(calypso_leg_t *ctx, long long *info)
+{
+    if (!ctx || !info) return fail("calypso_leg_info: null argument");
+    LegContext &c = ctx->c;
+    std::string err;
+    const calypso::LegPlan *pb = calypso::leg_plan(&c, c.ncomp, c.nvec, c.nscl, false, &err);
+    const calypso::LegPlan *pf = calypso::leg_plan(&c, c.ncomp, c.nvec, c.nscl, true, &err);
+    if (!pb || !pf) return fail(err);
+    info[0] = c.g.L;
+    info[1] = c.g.nth;
+    info[2] = c.g.nr;
+    info[3] = c.prec;
+    info[4] = (long long)c.tab_bytes;
+    info[5] = pb->KT;
+    info[6] = pf->KT;
+    info[7] = pb->n_items;
+    return 0;
+}
+
+void *calypso_leg_stream(calypso_leg_t *ctx) { return ctx ? (void *)ctx->c.stream : nullptr; }
+
+int calypso_sel_init_legendre_trans(int mode, int l_truncation, int nth, int nr,
+                                    const double *radius, const double *colat,
+                                    const double *weight, int ncomp, int nvector, int nscalar)
+{
+    if (g_sel) calypso_leg_destroy(g_sel);
+    g_sel = nullptr;
+    return calypso_leg_create(l_truncation, nth, nr, radius, colat, weight, ncomp, nvector,
+                              nscalar, mode, &g_sel);
+}
+
+int calypso_sel_backward_legendre_trans(int ncomp, int nvector, int nscalar, int n_WR, int n_WS,
+                                        const double *WR, double *WS)
+{
+    if (!g_sel) return fail("sel_backward_legendre_trans: selector not initialised");
+    return calypso_leg_backward(g_sel, ncomp, nvector, nscalar, n_WR, n_WS, WR, WS);
+}
+
+int calypso_sel_forward_legendre_trans(int ncomp, int nvector, int nscalar, int n_WR, int n_WS,
+                                       const double *WR, double *WS)
+{
+    if (!g_sel) return fail("sel_forward_legendre_trans: selector not initialised");
+    if (ncomp <= 0) return 0;  // legendre_transform_select.f90:391
+    return calypso_leg_forward(g_sel, ncomp, nvector, nscalar, n_WR, n_WS, WR, WS);
+}
+
+int calypso_sel_finalize_legendre_trans(void)
+{
+    int rc = calypso_leg_destroy(g_sel);
+    g_sel = nullptr;
+    return rc;
+}
+
+}  // extern "C"

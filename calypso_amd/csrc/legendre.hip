@@ -1,0 +1,1121 @@
+// legendre.hip -- backward / forward spherical-harmonic Legendre transforms
+// for gfx950 (CDNA4) as per-order MFMA contractions.
+//
+// Arithmetic follows the reference org loops
+//   backward: legendre_b_trans_vector_org / _scalar_org
+//             (SPH_SHELL_src/legendre_bwd_trans_org.f90:45-197)
+//   forward:  legendre_f_trans_vector_org / _scalar_org
+//             (SPH_SHELL_src/legendre_fwd_trans_org.f90:45-160)
+// in the equatorially folded form of the reference's symmetric variants
+// (legendre_bwd_sym_matmul.f90, legendre_fwd_sym_matmul.f90):
+// P_l^m(pi-theta) = (-1)^(l-m) P_l^m(theta) and dP/dtheta flips the other way,
+// so each order m needs only the hemisphere rows and the degree list split
+// by the parity of l-m.
+//
+// Data layouts are the reference's single-domain ones (component fastest):
+//   sp_rlm[nd + ncomp*(j + J*k)]              j in rlm order (m-major)
+//   vr_rtm[nd + ncomp*(t + nth*k + nth*nr*mp)] mp = m + L
+//
+// Kernels (one work-group = one |m|, a tile of KT radial points, a group of
+// 16-row tiles; the +m and -m orders share every P tile):
+//   backward: rows = colatitude, K = degree;  out(t) = S+A, out(pi-t) = S-A
+//   forward : rows = degree,     K = colatitude, input folded N+-S on load
+// Precisions: FP64 MFMA 16x16x4, FP32 MFMA 16x16x4, split-BF16x3 MFMA
+// 16x16x32 (three bf16 slices per operand, six slice products, FP32
+// accumulation).  P_l^m tables are computed on the GPU in FP64 and packed in
+// the lane order the MFMA A operand is read in (one 64-lane unit per
+// 16 x KSTEP tile), so every A fetch is a contiguous 512 B - 1 KiB load.
+#include "leg_internal.hpp"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace calypso {
+
+#define HIP_TRY(expr)                                                         \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            if (err) *err = std::string(#expr " failed: ") + hipGetErrorString(e_); \
+            return -1;                                                        \
+        }                                                                     \
+    } while (0)
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned short us8v __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------ traits --
+
+struct TF64 {
+    using elem = double;
+    using frag = double;
+    using acc = d4v;
+    static constexpr int KSTEP = 4, NS = 1, EPL = 1;
+    __device__ static inline void store(elem *dst, double x, int stride) { dst[0] = x; }
+    __device__ static inline frag load(const elem *p) { return *p; }
+    __device__ static inline frag neg(frag a) { return -a; }
+    __device__ static inline void mma(acc &c, const frag *a, const frag *b) {
+        c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], c, 0, 0, 0);
+    }
+    __device__ static inline double get(const acc &c, int v) { return c[v]; }
+};
+
+struct TF32 {
+    using elem = float;
+    using frag = float;
+    using acc = f4v;
+    static constexpr int KSTEP = 4, NS = 1, EPL = 1;
+    __device__ static inline void store(elem *dst, double x, int stride) { dst[0] = (float)x; }
+    __device__ static inline frag load(const elem *p) { return *p; }
+    __device__ static inline frag neg(frag a) { return -a; }
+    __device__ static inline void mma(acc &c, const frag *a, const frag *b) {
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
+    }
+    __device__ static inline double get(const acc &c, int v) { return (double)c[v]; }
+};
+
+__device__ inline unsigned short bf16_bits(double x) {
+    __bf16 h = (__bf16)(float)x;
+    return __builtin_bit_cast(unsigned short, h);
+}
+__device__ inline double bf16_val(unsigned short b) {
+    return (double)(float)__builtin_bit_cast(__bf16, b);
+}
+
+struct TBF3 {
+    using elem = unsigned short;  // bf16 bit pattern
+    using frag = bf8v;
+    using acc = f4v;
+    static constexpr int KSTEP = 32, NS = 3, EPL = 8;
+    // three-slice split x = hi + mid + lo; each residual is exact in FP64
+    __device__ static inline void store(elem *dst, double x, int stride) {
+        unsigned short h = bf16_bits(x);
+        double r1 = x - bf16_val(h);
+        unsigned short m = bf16_bits(r1);
+        double r2 = r1 - bf16_val(m);
+        dst[0] = h;
+        dst[stride] = m;
+        dst[2 * stride] = bf16_bits(r2);
+    }
+    __device__ static inline frag load(const elem *p) {
+        us8v u = *reinterpret_cast<const us8v *>(p);
+        return __builtin_bit_cast(bf8v, u);
+    }
+    __device__ static inline frag neg(frag a) {
+        us8v u = __builtin_bit_cast(us8v, a);
+        u ^= (unsigned short)0x8000;
+        return __builtin_bit_cast(bf8v, u);
+    }
+    // six slice products, smallest first, FP32 accumulation
+    __device__ static inline void mma(acc &c, const frag *a, const frag *b) {
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], c, 0, 0, 0);
+    }
+    __device__ static inline double get(const acc &c, int v) { return (double)c[v]; }
+};
+
+// unit = one 16 x KSTEP operand tile in lane order: NS slices x 64 lanes x EPL
+template <class T> constexpr int unit_elems() { return T::NS * 64 * T::EPL; }
+
+// ------------------------------------------------- P_l^m recursion (FP64) --
+
+// Running Schmidt semi-normalised P_l^q(cos theta) for l = 0,1,2,...
+// (schmidt_legendre_m, schmidt_fix_m.f90:89-141), evaluated one l at a time.
+struct SchmidtSeq {
+    int q;
+    double c, s, sq, df_q, d1, d2, p1, p2;
+    __device__ void init(int q_, double c_, double s_) {
+        q = q_; c = c_; s = s_;
+        if (q <= 0) return;
+        double prod = 1.0;
+        for (int k = 1; k <= q; ++k) prod = prod * (double)(2 * k - 1) / (double)(2 * k);
+        df_q = prod;
+        sq = 1.0;
+        for (int k = 1; k <= q; ++k) sq = sq * s;
+    }
+    // value at degree l; must be called for l = 0,1,2,... in order
+    __device__ double at(int l) {
+        if (q < 0) return 0.0;
+        if (q == 0) {
+            double v;
+            if (l == 0) v = 1.0;
+            else if (l == 1) v = c;
+            else v = p1 * (double)(2 * l - 1) / (double)l * c - p2 * (double)(l - 1) / (double)l;
+            p2 = p1; p1 = v;
+            return v;
+        }
+        if (l < q) return 0.0;
+        double df;
+        if (l == q) {
+            df = sqrt(2.0 * df_q);
+            d1 = df;
+        } else if (l == q + 1) {
+            df = sqrt(2.0 * df_q * (double)(2 * q + 1)) * c;
+            d2 = d1; d1 = df;
+        } else {
+            df = (c * (double)(2 * l - 1) * d1 - sqrt((double)((l - 1) * (l - 1) - q * q)) * d2)
+                 / sqrt((double)(l * l - q * q));
+            d2 = d1; d1 = df;
+        }
+        return df * sq;
+    }
+};
+
+// raw tables P, dP/dtheta and m P / sin(theta) at the hemisphere rows:
+// raw[(pos(m) + l - m) * th_h + t], pos(m) = sum_{m'<m} (L - m' + 1).
+// dP follows diff_schmidt_legendre_m (schmidt_fix_m.f90:145-175).
+__global__ void k_schmidt_raw(int L, int th_h, const double *colat,
+                              double *P, double *dP, double *PM)
+{
+    const int m = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= th_h) return;
+    const double th = colat[t];
+    const double c = cos(th), s = sin(th), as = 1.0 / s;
+    const long long pos = (long long)m * (L + 1) - (long long)m * (m - 1) / 2;
+    SchmidtSeq sn, sm, sp;
+    sn.init(m - 1, c, s);
+    sm.init(m, c, s);
+    sp.init(m + 1, c, s);
+    for (int l = 0; l <= L; ++l) {
+        double pn = sn.at(l), pm = sm.at(l), pp = sp.at(l);
+        if (l < m) continue;
+        double d;
+        if (m == 0) {
+            d = (l == 0) ? 0.0 : -sqrt((double)(l * (l + 1) / 2)) * pp;
+        } else if (m == 1) {
+            d = (l == 1) ? pn
+                         : 0.5 * (sqrt((double)(2 * l * (l + 1))) * pn
+                                  - sqrt((double)((l - 1) * (l + 2))) * pp);
+        } else {
+            d = (l == m) ? 0.5 * sqrt((double)(2 * m)) * pn
+                         : 0.5 * (sqrt((double)((l + m) * (l - m + 1))) * pn
+                                  - sqrt((double)((l - m) * (l + m + 1))) * pp);
+        }
+        const long long o = (pos + (l - m)) * th_h + t;
+        P[o] = pm;
+        dP[o] = d;
+        PM[o] = pm * (double)m * as;
+    }
+}
+
+// bwd units: for m: for rt < RT: for ks < KS(m): for w < 6
+// (P even, P odd, dP even, dP odd, PM even, PM odd); element (s, lane, e):
+// row t = rt*16 + lane%16, degree index i = ks*KSTEP + (KSTEP/4)*(lane/16) + e
+template <class T>
+__global__ void k_pack_bwd(int L, int th_h, int RT, const double *P,
+                           const double *dP, const double *PM,
+                           const long long *off, typename T::elem *tab)
+{
+    const int m = blockIdx.y;
+    const long long u0 = off[m], nu = off[m + 1] - off[m];
+    const int K = L - m + 1, K0 = (K + 1) / 2;
+    const int KS = (K0 + T::KSTEP - 1) / T::KSTEP;
+    const long long pos = (long long)m * (L + 1) - (long long)m * (m - 1) / 2;
+    constexpr int UE = unit_elems<T>() / T::NS;  // per slice
+    for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < nu * UE;
+         x += (long long)gridDim.x * blockDim.x) {
+        const long long u = x / UE;
+        const int r = (int)(x % UE);
+        const int lane = r / T::EPL, e = r % T::EPL;
+        const int w = (int)(u % 6);
+        const long long u2 = u / 6;
+        const int ks = (int)(u2 % KS), rt = (int)(u2 / KS);
+        const int par = w & 1;
+        const int t = rt * 16 + (lane & 15);
+        const int i = ks * T::KSTEP + (T::KSTEP / 4) * (lane >> 4) + e;
+        const int l = m + par + 2 * i;
+        double v = 0.0;
+        if (t < th_h && l <= L) {
+            const double *src = (w < 2) ? P : (w < 4) ? dP : PM;
+            v = src[(pos + (l - m)) * th_h + t];
+        }
+        typename T::elem *dst = tab + (u0 + u) * unit_elems<T>() + lane * T::EPL + e;
+        T::store(dst, v, 64 * T::EPL);
+    }
+}
+
+// fwd units: for m: for par: for rt < RT_par(m): for ks < KSt: for w < 3
+// (P, dP, PM); element (s, lane, e): degree index i = rt*16 + lane%16,
+// row t = ks*KSTEP + (KSTEP/4)*(lane/16) + e
+template <class T>
+__global__ void k_pack_fwd(int L, int th_h, int KSt, const double *P,
+                           const double *dP, const double *PM,
+                           const long long *off, typename T::elem *tab)
+{
+    const int mp = blockIdx.y;  // 2*m + par
+    const int m = mp >> 1, par = mp & 1;
+    const long long u0 = off[mp], nu = off[mp + 1] - off[mp];
+    const long long pos = (long long)m * (L + 1) - (long long)m * (m - 1) / 2;
+    constexpr int UE = unit_elems<T>() / T::NS;
+    for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < nu * UE;
+         x += (long long)gridDim.x * blockDim.x) {
+        const long long u = x / UE;
+        const int r = (int)(x % UE);
+        const int lane = r / T::EPL, e = r % T::EPL;
+        const int w = (int)(u % 3);
+        const long long u2 = u / 3;
+        const int ks = (int)(u2 % KSt), rt = (int)(u2 / KSt);
+        const int i = rt * 16 + (lane & 15);
+        const int t = ks * T::KSTEP + (T::KSTEP / 4) * (lane >> 4) + e;
+        const int l = m + par + 2 * i;
+        double v = 0.0;
+        if (t < th_h && l <= L) {
+            const double *src = (w == 0) ? P : (w == 1) ? dP : PM;
+            v = src[(pos + (l - m)) * th_h + t];
+        }
+        typename T::elem *dst = tab + (u0 + u) * unit_elems<T>() + lane * T::EPL + e;
+        T::store(dst, v, 64 * T::EPL);
+    }
+}
+
+// ------------------------------------------------------ kernel params --
+
+struct LegArgs {
+    const double *in;
+    double *out;
+    const void *tab;
+    const long long *off;
+    const int *lstack;
+    const double *radius;
+    const double *weight;
+    const WorkItem *items;
+    int L, nth, nr, th_h, J;
+    int ncomp, nvec, nscl;
+    int KT, RT, KSt;
+};
+
+constexpr int kThreads = 256;
+constexpr int kChunkB = 32;   // degree pairs staged per LDS chunk (bwd)
+constexpr int kChunkF = 64;   // colatitudes staged per LDS chunk (fwd)
+
+template <class T> __device__ inline int lds_pad() { return T::EPL == 8 ? 8 : 1; }
+
+// ------------------------------------------------------- backward ------
+
+// Column map of the backward kernel for one work item.
+//  GEMM1 (A = P):       c = (kk*2+sg)*(nv+ns) + q
+//      q <  nv: v_r of vector q = sum P * l(l+1)/r^2 S_pol
+//      q >= nv: scalar q-nv     = sum P * S
+//  GEMM2 (A = dP | PM): c = ((kk*2+sg)*nv + q)*2 + a, a = 0 theta, 1 phi
+//      v_theta(sm) = sum dP dS(sm)/r + (sm/sin) sum P T(-sm)/r
+//      v_phi(sm)   = -sum dP T(sm)/r + (sm/sin) sum P dS(-sm)/r
+// (legendre_bwd_trans_org.f90:71-151 with the +-m partner mn = M-mp+1).
+template <class T, int RPW, int MCT1, int MCT2>
+__global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
+{
+    using elem = typename T::elem;
+    using frag = typename T::frag;
+    using acc = typename T::acc;
+    constexpr int KS_ = T::KSTEP, NS = T::NS, EPL = T::EPL;
+    constexpr int KC = (kChunkB / KS_ > 0 ? kChunkB : KS_);
+    const int KCP = KC + lds_pad<T>();
+
+    const WorkItem it = a.items[blockIdx.x];
+    const int m = it.m, kt0 = it.ktile * a.KT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int K = a.L - m + 1, K0 = (K + 1) / 2;
+    const int KS = (K0 + KS_ - 1) / KS_;
+    const int nkk = min(a.KT, a.nr - kt0);
+    const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
+    const int ncol = a.KT * 2 * ncomp;
+    const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 2 * 2 * nv;
+    const int nct1 = (c1n + 15) / 16, nct2 = (c2n + 15) / 16;
+    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    elem *X = reinterpret_cast<elem *>(smem_raw);  // [NS][2 par][ncol][KCP]
+    const int slice = 2 * ncol * KCP;
+
+    // per-lane B column descriptors
+    int b1col[MCT1];
+    int b2x[MCT2], b2y[MCT2];
+    bool b2xn[MCT2], b2yn[MCT2];
+#pragma unroll
+    for (int ct = 0; ct < MCT1; ++ct) {
+        int c = ct * 16 + (lane & 15);
+        b1col[ct] = 0;
+        if (c < c1n) {
+            int per = nv + ns, kk = c / (2 * per), rem = c % (2 * per);
+            int sg = rem / per, q = rem % per;
+            int comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+            b1col[ct] = (kk * 2 + sg) * ncomp + comp;
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < MCT2; ++ct) {
+        int c = ct * 16 + (lane & 15);
+        b2x[ct] = b2y[ct] = 0;
+        b2xn[ct] = b2yn[ct] = false;
+        if (c < c2n) {
+            int kk = c / (4 * nv), rem = c % (4 * nv);
+            int sg = rem / (2 * nv), r2 = rem % (2 * nv);
+            int q = r2 >> 1, ax = r2 & 1;
+            int xs = (kk * 2 + sg) * ncomp, ys = (kk * 2 + (1 - sg)) * ncomp;
+            if (ax == 0) {  // theta: X = dS(sg), Y = +-T(-sg)
+                b2x[ct] = xs + 3 * q + 1; b2xn[ct] = false;
+                b2y[ct] = ys + 3 * q + 2; b2yn[ct] = (sg == 1);
+            } else {        // phi: X = -T(sg), Y = +-dS(-sg)
+                b2x[ct] = xs + 3 * q + 2; b2xn[ct] = true;
+                b2y[ct] = ys + 3 * q + 1; b2yn[ct] = (sg == 1);
+            }
+        }
+    }
+
+    acc a1[RPW][MCT1][2], a2[RPW][MCT2][2];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+        for (int ct = 0; ct < MCT1; ++ct) { a1[r][ct][0] = acc{}; a1[r][ct][1] = acc{}; }
+#pragma unroll
+        for (int ct = 0; ct < MCT2; ++ct) { a2[r][ct][0] = acc{}; a2[r][ct][1] = acc{}; }
+    }
+
+    const elem *tab = reinterpret_cast<const elem *>(a.tab);
+    const long long ubase = a.off[m];
+    constexpr int UE = unit_elems<T>();
+
+    for (int c0 = 0; c0 < KS * KS_; c0 += KC) {
+        __syncthreads();
+        // stage degree pairs [c0, c0+KC) of both parities, both +-m, KT radii
+        const int nstage = 2 * KC * ncol;
+        for (int e = threadIdx.x; e < nstage; e += kThreads) {
+            int comp = e % ncomp, rest = e / ncomp;
+            int li = rest % (2 * KC), rest2 = rest / (2 * KC);
+            int sg = rest2 & 1, kk = rest2 >> 1;
+            int l = m + 2 * c0 + li;
+            double v = 0.0;
+            if (l <= a.L && kk < nkk && (sg == 0 || m > 0)) {
+                int k = kt0 + kk;
+                int j = (sg ? j0n : j0p) + (l - m);
+                v = a.in[comp + (size_t)ncomp * (j + (size_t)a.J * k)];
+                double a1r = 1.0 / a.radius[k];
+                if (comp < 3 * nv) {
+                    if (comp % 3 == 0) {
+                        double g3 = (l == 0) ? 0.5 : (double)(l * (l + 1));
+                        v = v * (a1r * a1r) * g3;
+                    } else {
+                        v = v * a1r;
+                    }
+                }
+            }
+            int par = li & 1, il = li >> 1;
+            int col = (kk * 2 + sg) * ncomp + comp;
+            T::store(X + (par * ncol + col) * KCP + il, v, slice);
+        }
+        __syncthreads();
+
+        const int ks_end = min(KS, (c0 + KC) / KS_);
+        for (int ks = c0 / KS_; ks < ks_end; ++ks) {
+            const int kl = (ks * KS_ - c0) + (KS_ / 4) * (lane >> 4);
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int rt = (it.rgroup * 4 + wave) * RPW + r;
+                if (rt >= a.RT) continue;
+                frag A[6][NS];
+                const elem *ap = tab + (ubase + ((long long)rt * KS + ks) * 6) * UE + lane * EPL;
+#pragma unroll
+                for (int w = 0; w < 6; ++w)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) A[w][s] = T::load(ap + w * UE + s * 64 * EPL);
+#pragma unroll
+                for (int ct = 0; ct < MCT1; ++ct) {
+                    if (ct >= nct1) break;
+                    frag be[NS], bo[NS];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        be[s] = T::load(X + s * slice + (0 * ncol + b1col[ct]) * KCP + kl);
+                        bo[s] = T::load(X + s * slice + (1 * ncol + b1col[ct]) * KCP + kl);
+                    }
+                    T::mma(a1[r][ct][0], A[0], be);
+                    T::mma(a1[r][ct][1], A[1], bo);
+                }
+#pragma unroll
+                for (int ct = 0; ct < MCT2; ++ct) {
+                    if (ct >= nct2) break;
+                    frag xe[NS], xo[NS], ye[NS], yo[NS];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        xe[s] = T::load(X + s * slice + (0 * ncol + b2x[ct]) * KCP + kl);
+                        xo[s] = T::load(X + s * slice + (1 * ncol + b2x[ct]) * KCP + kl);
+                        ye[s] = T::load(X + s * slice + (0 * ncol + b2y[ct]) * KCP + kl);
+                        yo[s] = T::load(X + s * slice + (1 * ncol + b2y[ct]) * KCP + kl);
+                        if (b2xn[ct]) { xe[s] = T::neg(xe[s]); xo[s] = T::neg(xo[s]); }
+                        if (b2yn[ct]) { ye[s] = T::neg(ye[s]); yo[s] = T::neg(yo[s]); }
+                    }
+                    // symmetric set: dP odd, PM even; antisymmetric: dP even, PM odd
+                    T::mma(a2[r][ct][0], A[3], xo);
+                    T::mma(a2[r][ct][0], A[4], ye);
+                    T::mma(a2[r][ct][1], A[2], xe);
+                    T::mma(a2[r][ct][1], A[5], yo);
+                }
+            }
+        }
+    }
+
+    // epilogue: hemisphere row t -> out(t) = S + A, mirror nth-1-t -> S - A
+    const size_t sl = (size_t)a.nth * a.nr;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int rt = (it.rgroup * 4 + wave) * RPW + r;
+        if (rt >= a.RT) continue;
+#pragma unroll
+        for (int ct = 0; ct < MCT1; ++ct) {
+            if (ct >= nct1) break;
+            int c = ct * 16 + (lane & 15);
+            if (c >= c1n) continue;
+            int per = nv + ns, kk = c / (2 * per), rem = c % (2 * per);
+            int sg = rem / per, q = rem % per;
+            if (kk >= nkk || (sg == 1 && m == 0)) continue;
+            int comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+            int mp = a.L + (sg ? -m : m), k = kt0 + kk;
+            double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int t = rt * 16 + 4 * (lane >> 4) + v;
+                if (t >= a.th_h) continue;
+                double S = T::get(a1[r][ct][0], v), An = T::get(a1[r][ct][1], v);
+                int tm = a.nth - 1 - t;
+                o[(size_t)ncomp * t] = S + An;
+                if (tm != t) o[(size_t)ncomp * tm] = S - An;
+            }
+        }
+#pragma unroll
+        for (int ct = 0; ct < MCT2; ++ct) {
+            if (ct >= nct2) break;
+            int c = ct * 16 + (lane & 15);
+            if (c >= c2n) continue;
+            int kk = c / (4 * nv), rem = c % (4 * nv);
+            int sg = rem / (2 * nv), r2 = rem % (2 * nv);
+            int q = r2 >> 1, ax = r2 & 1;
+            if (kk >= nkk || (sg == 1 && m == 0)) continue;
+            int comp = 3 * q + 1 + ax;
+            int mp = a.L + (sg ? -m : m), k = kt0 + kk;
+            double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int t = rt * 16 + 4 * (lane >> 4) + v;
+                if (t >= a.th_h) continue;
+                double S = T::get(a2[r][ct][0], v), An = T::get(a2[r][ct][1], v);
+                int tm = a.nth - 1 - t;
+                o[(size_t)ncomp * t] = S + An;
+                if (tm != t) o[(size_t)ncomp * tm] = S - An;
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------- forward ------
+
+// Column map of the forward kernel (rows = degree index of one parity):
+//  GEMM1 (A = P):  c = (kk*2+sg)*(nv+ns) + q
+//      S_pol(sm) = r^2 g7 sum w P v_r(sm);  scalar = g6 sum w P f(sm)
+//  GEMM2 (A = dP | PM): c = ((kk*2+sg)*nv + q)*2 + a
+//      dS(sm) = r g7 [ sum w dP v_th(sm) - (sm/sin) sum w P v_ph(-sm) ]
+//      T(sm)  = -r g7 [ sum w dP v_ph(sm) + (sm/sin) sum w P v_th(-sm) ]
+// (legendre_fwd_trans_org.f90:70-115).  The sphere sum is folded onto the
+// hemisphere: P-terms of an even (l-m) row use V(t)+V(pi-t), dP-terms use
+// V(t)-V(pi-t); odd rows swap the two.
+template <class T, int RPW, int MCT1, int MCT2>
+__global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
+{
+    using elem = typename T::elem;
+    using frag = typename T::frag;
+    using acc = typename T::acc;
+    constexpr int KS_ = T::KSTEP, NS = T::NS, EPL = T::EPL;
+    constexpr int TC = kChunkF;
+    const int TCP = TC + lds_pad<T>();
+
+    const WorkItem it = a.items[blockIdx.x];
+    const int m = it.m, kt0 = it.ktile * a.KT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int K = a.L - m + 1, K0 = (K + 1) / 2, K1 = K / 2;
+    const int RTe = (K0 + 15) / 16, RTo = (K1 + 15) / 16;
+    const int nkk = min(a.KT, a.nr - kt0);
+    const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
+    const int ncol = a.KT * 2 * ncomp;
+    const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 2 * 2 * nv;
+    const int nct1 = (c1n + 15) / 16, nct2 = (c2n + 15) / 16;
+    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    elem *V = reinterpret_cast<elem *>(smem_raw);  // [NS][2 fold][ncol][TCP]
+    const int slice = 2 * ncol * TCP;
+
+    int b1col[MCT1];
+    int b2x[MCT2], b2y[MCT2];
+    bool b2xn[MCT2], b2yn[MCT2];
+#pragma unroll
+    for (int ct = 0; ct < MCT1; ++ct) {
+        int c = ct * 16 + (lane & 15);
+        b1col[ct] = 0;
+        if (c < c1n) {
+            int per = nv + ns, kk = c / (2 * per), rem = c % (2 * per);
+            int sg = rem / per, q = rem % per;
+            int comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+            b1col[ct] = (kk * 2 + sg) * ncomp + comp;
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < MCT2; ++ct) {
+        int c = ct * 16 + (lane & 15);
+        b2x[ct] = b2y[ct] = 0;
+        b2xn[ct] = b2yn[ct] = false;
+        if (c < c2n) {
+            int kk = c / (4 * nv), rem = c % (4 * nv);
+            int sg = rem / (2 * nv), r2 = rem % (2 * nv);
+            int q = r2 >> 1, ax = r2 & 1;
+            int xs = (kk * 2 + sg) * ncomp, ys = (kk * 2 + (1 - sg)) * ncomp;
+            if (ax == 0) {  // dS: X = v_th(sg), Y = -+ v_ph(-sg)
+                b2x[ct] = xs + 3 * q + 1; b2xn[ct] = false;
+                b2y[ct] = ys + 3 * q + 2; b2yn[ct] = (sg == 0);
+            } else {        // T: X = -v_ph(sg), Y = -+ v_th(-sg)
+                b2x[ct] = xs + 3 * q + 2; b2xn[ct] = true;
+                b2y[ct] = ys + 3 * q + 1; b2yn[ct] = (sg == 0);
+            }
+        }
+    }
+
+    acc a1[RPW][MCT1], a2[RPW][MCT2];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+        for (int ct = 0; ct < MCT1; ++ct) a1[r][ct] = acc{};
+#pragma unroll
+        for (int ct = 0; ct < MCT2; ++ct) a2[r][ct] = acc{};
+    }
+
+    const elem *tab = reinterpret_cast<const elem *>(a.tab);
+    constexpr int UE = unit_elems<T>();
+    const size_t sl = (size_t)a.nth * a.nr;
+
+    for (int t0 = 0; t0 < a.KSt * KS_; t0 += TC) {
+        __syncthreads();
+        const int nstage = TC * 2 * a.KT * ncomp;
+        for (int e = threadIdx.x; e < nstage; e += kThreads) {
+            int comp = e % ncomp, rest = e / ncomp;
+            int tl = rest % TC, rest2 = rest / TC;
+            int sg = rest2 & 1, kk = rest2 >> 1;
+            int t = t0 + tl;
+            double vs = 0.0, va = 0.0;
+            if (t < a.th_h && kk < nkk && (sg == 0 || m > 0)) {
+                int mp = a.L + (sg ? -m : m), k = kt0 + kk;
+                const double *src = a.in + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
+                int tm = a.nth - 1 - t;
+                double w = a.weight[t];
+                double v1 = src[(size_t)ncomp * t];
+                if (tm != t) {
+                    double v2 = src[(size_t)ncomp * tm];
+                    vs = w * (v1 + v2);
+                    va = w * (v1 - v2);
+                } else {
+                    vs = w * v1;
+                }
+            }
+            int col = (kk * 2 + sg) * ncomp + comp;
+            T::store(V + (0 * ncol + col) * TCP + tl, vs, slice);
+            T::store(V + (1 * ncol + col) * TCP + tl, va, slice);
+        }
+        __syncthreads();
+
+        const int ks_end = min(a.KSt, (t0 + TC) / KS_);
+        for (int ks = t0 / KS_; ks < ks_end; ++ks) {
+            const int kl = (ks * KS_ - t0) + (KS_ / 4) * (lane >> 4);
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int rr = (it.rgroup * 4 + wave) * RPW + r;
+                if (rr >= RTe + RTo) continue;
+                const int par = rr < RTe ? 0 : 1;
+                const int rt = par ? rr - RTe : rr;
+                frag A[3][NS];
+                const elem *ap = tab + (a.off[2 * m + par] + ((long long)rt * a.KSt + ks) * 3) * UE
+                               + lane * EPL;
+#pragma unroll
+                for (int w = 0; w < 3; ++w)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) A[w][s] = T::load(ap + w * UE + s * 64 * EPL);
+                const int fP = par, fD = 1 - par;  // fold index: 0 sym, 1 anti
+#pragma unroll
+                for (int ct = 0; ct < MCT1; ++ct) {
+                    if (ct >= nct1) break;
+                    frag b[NS];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        b[s] = T::load(V + s * slice + (fP * ncol + b1col[ct]) * TCP + kl);
+                    T::mma(a1[r][ct], A[0], b);
+                }
+#pragma unroll
+                for (int ct = 0; ct < MCT2; ++ct) {
+                    if (ct >= nct2) break;
+                    frag x[NS], y[NS];
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        x[s] = T::load(V + s * slice + (fD * ncol + b2x[ct]) * TCP + kl);
+                        y[s] = T::load(V + s * slice + (fP * ncol + b2y[ct]) * TCP + kl);
+                        if (b2xn[ct]) x[s] = T::neg(x[s]);
+                        if (b2yn[ct]) y[s] = T::neg(y[s]);
+                    }
+                    T::mma(a2[r][ct], A[1], x);
+                    T::mma(a2[r][ct], A[2], y);
+                }
+            }
+        }
+    }
+
+    // epilogue: scale by r^2 g7, r g7 or g6 (sph_normalizations,
+    // spherical_harmonics.f90:121-139) and store in rlm order
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int rr = (it.rgroup * 4 + wave) * RPW + r;
+        if (rr >= RTe + RTo) continue;
+        const int par = rr < RTe ? 0 : 1;
+        const int rt = par ? rr - RTe : rr;
+        const int Kp = par ? K1 : K0;
+#pragma unroll
+        for (int ct = 0; ct < MCT1 + MCT2; ++ct) {
+            const bool g1 = ct < MCT1;
+            const int cc = g1 ? ct : ct - MCT1;
+            if (g1 ? (cc >= nct1) : (cc >= nct2)) continue;
+            int c = cc * 16 + (lane & 15);
+            int kk, sg, comp, kind;  // kind 0: r^2 g7, 1: r g7, 2: g6
+            if (g1) {
+                if (c >= c1n) continue;
+                int per = nv + ns;
+                kk = c / (2 * per);
+                int rem = c % (2 * per);
+                sg = rem / per;
+                int q = rem % per;
+                comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+                kind = q < nv ? 0 : 2;
+            } else {
+                if (c >= c2n) continue;
+                kk = c / (4 * nv);
+                int rem = c % (4 * nv);
+                sg = rem / (2 * nv);
+                int r2 = rem % (2 * nv);
+                comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
+                kind = 1;
+            }
+            if (kk >= nkk || (sg == 1 && m == 0)) continue;
+            const int k = kt0 + kk;
+            const double rad = a.radius[k];
+            const int j0 = sg ? j0n : j0p;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + 4 * (lane >> 4) + v;
+                if (i >= Kp) continue;
+                const int l = m + par + 2 * i;
+                double fac;
+                if (kind == 2) {
+                    fac = (l == 0) ? 0.5 : (m == 0 ? (double)(2 * l + 1) / 2.0
+                                                   : (double)(2 * l + 1) / 4.0);
+                } else {
+                    double g7 = (l == 0) ? 1.0
+                              : (m == 0 ? (double)(2 * l + 1) / (double)(2 * l * (l + 1))
+                                        : (double)(2 * l + 1) / (double)(4 * l * (l + 1)));
+                    fac = (kind == 0) ? rad * rad * g7 : rad * g7;
+                }
+                double val = (g1 ? T::get(a1[r][cc], v) : T::get(a2[r][cc], v)) * fac;
+                a.out[comp + (size_t)ncomp * ((j0 + (l - m)) + (size_t)a.J * k)] = val;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------- host ----
+
+// Gauss-Legendre points on [-1,1] by Newton iteration on [0,1] with the
+// same start values, stopping rule (3e-12) and mapping as the reference
+// construct_gauss_coefs / set_gauss_colatitude (m_gauss_points.f90:117-195),
+// so colatitudes and weights agree with the reference grid.
+void gauss_points_host(int n, double *point, double *weight, double *colat)
+{
+    const double pi = 4.0 * std::atan(1.0);
+    if (n == 1) {
+        point[0] = 0.5;
+        weight[0] = 1.0;
+    } else {
+        const int half_n = (n + 1 - ((n + 1) % 2)) / 2;
+        for (int i = 1; i <= half_n; ++i) {
+            double z = std::cos(pi * ((double)i - 0.25) / ((double)n + 0.5));
+            double z1, pp, p1, p2, p3;
+            for (;;) {
+                p1 = 1.0;
+                p2 = 0.0;
+                for (int jj = 1; jj <= n; ++jj) {
+                    p3 = p2;
+                    p2 = p1;
+                    p1 = ((2.0 * (double)jj - 1.0) * z * p2 - ((double)jj - 1.0) * p3) / (double)jj;
+                }
+                pp = (double)n * (z * p1 - p2) / (z * z - 1.0);
+                z1 = z;
+                z = z1 - p1 / pp;
+                if (std::fabs(z - z1) < 3.0e-12) break;
+            }
+            point[i - 1] = 0.5 - 0.5 * z;
+            point[n - i] = 0.5 + 0.5 * z;
+            weight[i - 1] = 2.0 * 0.5 / ((1.0 - z * z) * pp * pp);
+            weight[n - i] = weight[i - 1];
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        point[i] = point[i] * 2.0 - 1.0;
+        weight[i] = weight[i] * 2.0;
+        colat[i] = std::acos(point[i]);
+    }
+}
+
+template <class T>
+static size_t elem_size() { return sizeof(typename T::elem); }
+
+template <class T>
+static int build_tables(LegContext *c, const double *dP_raw_P, const double *dP_raw_dP,
+                        const double *dP_raw_PM, std::string *err)
+{
+    const LegGeometry &g = c->g;
+    const int L = g.L, th_h = g.th_h;
+    c->kstep = T::KSTEP;
+    c->rt_bwd = (th_h + 15) / 16;
+    c->ks_fwd = (th_h + T::KSTEP - 1) / T::KSTEP;
+
+    std::vector<long long> ob(L + 2), of(2 * (L + 1) + 1);
+    ob[0] = 0;
+    for (int m = 0; m <= L; ++m) {
+        int K = L - m + 1, K0 = (K + 1) / 2;
+        int KS = (K0 + T::KSTEP - 1) / T::KSTEP;
+        ob[m + 1] = ob[m] + (long long)c->rt_bwd * KS * 6;
+    }
+    of[0] = 0;
+    for (int m = 0; m <= L; ++m) {
+        int K = L - m + 1, K0 = (K + 1) / 2, K1 = K / 2;
+        of[2 * m + 1] = of[2 * m] + (long long)((K0 + 15) / 16) * c->ks_fwd * 3;
+        of[2 * m + 2] = of[2 * m + 1] + (long long)((K1 + 15) / 16) * c->ks_fwd * 3;
+    }
+    const size_t ue = (size_t)unit_elems<T>();
+    const size_t bb = (size_t)ob[L + 1] * ue * sizeof(typename T::elem);
+    const size_t bf = (size_t)of[2 * (L + 1)] * ue * sizeof(typename T::elem);
+    HIP_TRY(hipMalloc(&c->d_off_bwd, sizeof(long long) * ob.size()));
+    HIP_TRY(hipMalloc(&c->d_off_fwd, sizeof(long long) * of.size()));
+    HIP_TRY(hipMemcpy(c->d_off_bwd, ob.data(), sizeof(long long) * ob.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_off_fwd, of.data(), sizeof(long long) * of.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_tab_bwd, bb));
+    HIP_TRY(hipMalloc(&c->d_tab_fwd, bf));
+    c->tab_bytes = bb + bf;
+
+    dim3 gb(256, L + 1), gf(256, 2 * (L + 1));
+    hipLaunchKernelGGL(k_pack_bwd<T>, gb, dim3(256), 0, c->stream, L, th_h, c->rt_bwd,
+                       dP_raw_P, dP_raw_dP, dP_raw_PM, c->d_off_bwd,
+                       reinterpret_cast<typename T::elem *>(c->d_tab_bwd));
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_pack_fwd<T>, gf, dim3(256), 0, c->stream, L, th_h, c->ks_fwd,
+                       dP_raw_P, dP_raw_dP, dP_raw_PM, c->d_off_fwd,
+                       reinterpret_cast<typename T::elem *>(c->d_tab_fwd));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Radial tile: the largest KT whose column tiles fit one kernel instance.
+// Instances (mct1, mct2, rows-per-wave): (1,1,3), (1,2,3), (2,4,2|3).
+static void choose_plan(LegPlan *p, int nv, int ns, bool fwd)
+{
+    const int per1 = 2 * (nv + ns), per2 = 4 * nv;
+    static const int inst[3][2] = {{1, 1}, {1, 2}, {2, 4}};
+    int best_kt = 1, best_i = 2;
+    double best_eff = -1.0;
+    for (int i = 0; i < 3; ++i) {
+        for (int kt = 1; kt <= 8; ++kt) {
+            int n1 = (per1 * kt + 15) / 16, n2 = (per2 * kt + 15) / 16;
+            if (n1 > inst[i][0] || n2 > inst[i][1]) continue;
+            double used = per1 * kt + per2 * kt;
+            double tiles = 16.0 * (n1 + n2);
+            // wider radial tiles re-read P fewer times; the widest instance
+            // needs fewer rows per wave in the backward kernel
+            double eff = (used / tiles) * std::min(1.0, (double)kt / 2.0);
+            if (i == 2 && !fwd) eff *= 0.9;
+            if (eff > best_eff + 1e-9) { best_eff = eff; best_kt = kt; best_i = i; }
+        }
+    }
+    p->KT = best_kt;
+    p->mct1 = inst[best_i][0];
+    p->mct2 = inst[best_i][1];
+    p->rpw = (best_i == 2 && !fwd) ? 2 : 3;
+}
+
+static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
+{
+    const LegGeometry &g = c->g;
+    const int nkt = (g.nr + p->KT - 1) / p->KT;
+    std::vector<WorkItem> items;
+    const int rows_per_group = 4 * p->rpw;
+    for (int m = 0; m <= g.L; ++m) {
+        int nrow;
+        if (fwd) {
+            int K = g.L - m + 1, K0 = (K + 1) / 2, K1 = K / 2;
+            nrow = (K0 + 15) / 16 + (K1 + 15) / 16;
+        } else {
+            nrow = c->rt_bwd;
+        }
+        int ngroup = (nrow + rows_per_group - 1) / rows_per_group;
+        for (int rg = 0; rg < ngroup; ++rg)
+            for (int kt = 0; kt < nkt; ++kt) items.push_back({m, kt, rg, 0});
+    }
+    p->n_items = (int)items.size();
+    HIP_TRY(hipMalloc(&p->d_items, sizeof(WorkItem) * items.size()));
+    HIP_TRY(hipMemcpy(p->d_items, items.data(), sizeof(WorkItem) * items.size(),
+                      hipMemcpyHostToDevice));
+    return 0;
+}
+
+const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
+                        std::string *err)
+{
+    if (ncomp != 3 * nvec + nscl || nvec < 0 || nscl < 0 || ncomp < 1) {
+        *err = "ncomp must equal 3*nvector + nscalar";
+        return nullptr;
+    }
+    for (const LegPlan &p : c->plans)
+        if (p.ncomp == ncomp && p.nvec == nvec && p.nscl == nscl && p.fwd == fwd) return &p;
+    LegPlan p;
+    p.ncomp = ncomp;
+    p.nvec = nvec;
+    p.nscl = nscl;
+    p.fwd = fwd;
+    choose_plan(&p, nvec, nscl, fwd);
+    if (build_items(c, &p, fwd, err)) return nullptr;
+    c->plans.push_back(p);
+    return &c->plans.back();
+}
+
+int leg_context_build(LegContext *c, int L, int nth, int nr, const double *radius,
+                      const double *colat, const double *weight, int ncomp, int nvec,
+                      int nscl, int prec, std::string *err)
+{
+    if (L < 1 || nth < 2 || nr < 1) { *err = "invalid grid size"; return -1; }
+    if (ncomp != 3 * nvec + nscl || nvec < 0 || nscl < 0 || ncomp < 1) {
+        *err = "ncomp must equal 3*nvector + nscalar";
+        return -1;
+    }
+    if (prec != kF64 && prec != kF32 && prec != kBF16x3) { *err = "unknown precision"; return -1; }
+    c->g.L = L;
+    c->g.nth = nth;
+    c->g.nr = nr;
+    c->g.th_h = (nth + 1) / 2;
+    c->g.J = (L + 1) * (L + 1);
+    c->g.M = 2 * L + 1;
+    c->ncomp = ncomp;
+    c->nvec = nvec;
+    c->nscl = nscl;
+    c->prec = prec;
+
+    std::vector<double> pt(nth), wt(nth), ct(nth);
+    gauss_points_host(nth, pt.data(), wt.data(), ct.data());
+    if (colat) std::memcpy(ct.data(), colat, sizeof(double) * nth);
+    if (weight) std::memcpy(wt.data(), weight, sizeof(double) * nth);
+    // the folded kernels need a mirror-symmetric grid
+    for (int t = 0; t < nth / 2; ++t) {
+        if (std::fabs(wt[t] - wt[nth - 1 - t]) > 1e-14 * std::fabs(wt[t]) ||
+            std::fabs(ct[t] + ct[nth - 1 - t] - 3.14159265358979323846) > 1e-12) {
+            *err = "colatitude grid is not equatorially symmetric";
+            return -1;
+        }
+    }
+    std::vector<int> lstack(2 * L + 2);
+    lstack[0] = 0;
+    for (int mp = 0; mp < 2 * L + 1; ++mp) lstack[mp + 1] = lstack[mp] + (L - std::abs(mp - L) + 1);
+
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&c->ev0));
+    HIP_TRY(hipEventCreate(&c->ev1));
+    HIP_TRY(hipMalloc(&c->d_radius, sizeof(double) * nr));
+    HIP_TRY(hipMalloc(&c->d_colat, sizeof(double) * nth));
+    HIP_TRY(hipMalloc(&c->d_weight, sizeof(double) * nth));
+    HIP_TRY(hipMalloc(&c->d_lstack, sizeof(int) * lstack.size()));
+    HIP_TRY(hipMemcpy(c->d_radius, radius, sizeof(double) * nr, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_colat, ct.data(), sizeof(double) * nth, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_weight, wt.data(), sizeof(double) * nth, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_lstack, lstack.data(), sizeof(int) * lstack.size(), hipMemcpyHostToDevice));
+
+    // FP64 recursion on the device, then pack into MFMA operand order
+    const size_t nraw = (size_t)(L + 1) * (L + 2) / 2 * c->g.th_h;
+    double *rP = nullptr, *rdP = nullptr, *rPM = nullptr;
+    HIP_TRY(hipMalloc(&rP, sizeof(double) * nraw));
+    HIP_TRY(hipMalloc(&rdP, sizeof(double) * nraw));
+    HIP_TRY(hipMalloc(&rPM, sizeof(double) * nraw));
+    dim3 grid((c->g.th_h + 127) / 128, L + 1);
+    hipLaunchKernelGGL(k_schmidt_raw, grid, dim3(128), 0, c->stream, L, c->g.th_h, c->d_colat,
+                       rP, rdP, rPM);
+    HIP_TRY(hipGetLastError());
+    int rc;
+    if (prec == kF64) rc = build_tables<TF64>(c, rP, rdP, rPM, err);
+    else if (prec == kF32) rc = build_tables<TF32>(c, rP, rdP, rPM, err);
+    else rc = build_tables<TBF3>(c, rP, rdP, rPM, err);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(rP);
+    (void)hipFree(rdP);
+    (void)hipFree(rPM);
+    if (rc) return rc;
+
+    if (!leg_plan(c, ncomp, nvec, nscl, false, err)) return -1;
+    if (!leg_plan(c, ncomp, nvec, nscl, true, err)) return -1;
+    return 0;
+}
+
+void leg_context_free(LegContext *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_radius);
+    (void)hipFree(c->d_colat);
+    (void)hipFree(c->d_weight);
+    (void)hipFree(c->d_lstack);
+    (void)hipFree(c->d_tab_bwd);
+    (void)hipFree(c->d_tab_fwd);
+    (void)hipFree(c->d_off_bwd);
+    (void)hipFree(c->d_off_fwd);
+    for (LegPlan &p : c->plans) (void)hipFree(p.d_items);
+    c->plans.clear();
+    (void)hipFree(c->d_rlm);
+    (void)hipFree(c->d_rtm);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+static LegArgs make_args(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out)
+{
+    LegArgs a;
+    a.in = in;
+    a.out = out;
+    a.tab = fwd ? c->d_tab_fwd : c->d_tab_bwd;
+    a.off = fwd ? c->d_off_fwd : c->d_off_bwd;
+    a.lstack = c->d_lstack;
+    a.radius = c->d_radius;
+    a.weight = c->d_weight;
+    a.items = p.d_items;
+    a.L = c->g.L;
+    a.nth = c->g.nth;
+    a.nr = c->g.nr;
+    a.th_h = c->g.th_h;
+    a.J = c->g.J;
+    a.ncomp = p.ncomp;
+    a.nvec = p.nvec;
+    a.nscl = p.nscl;
+    a.KT = p.KT;
+    a.RT = c->rt_bwd;
+    a.KSt = c->ks_fwd;
+    return a;
+}
+
+template <class T>
+static size_t lds_bytes(const LegPlan &p, bool fwd)
+{
+    const int ncol = p.KT * 2 * p.ncomp;
+    const int pad = (T::EPL == 8) ? 8 : 1;
+    const int kc = fwd ? kChunkF : (kChunkB / T::KSTEP > 0 ? kChunkB : T::KSTEP);
+    return (size_t)T::NS * 2 * ncol * (kc + pad) * sizeof(typename T::elem);
+}
+
+template <class T, int RPW, int MCT1, int MCT2>
+static int launch_t(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
+                    hipStream_t s, std::string *err)
+{
+    LegArgs a = make_args(c, p, fwd, in, out);
+    size_t lds = lds_bytes<T>(p, fwd);
+    if (lds > 160 * 1024) { *err = "LDS tile too large for this component count"; return -1; }
+    if (fwd)
+        hipLaunchKernelGGL((k_leg_fwd<T, RPW, MCT1, MCT2>), dim3(p.n_items), dim3(kThreads), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_leg_bwd<T, RPW, MCT1, MCT2>), dim3(p.n_items), dim3(kThreads), lds, s, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <class T>
+static int launch_p(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
+                    hipStream_t s, std::string *err)
+{
+    if (p.mct1 == 1 && p.mct2 == 1) return launch_t<T, 3, 1, 1>(c, p, fwd, in, out, s, err);
+    if (p.mct1 == 1 && p.mct2 == 2) return launch_t<T, 3, 1, 2>(c, p, fwd, in, out, s, err);
+    if (p.rpw == 2) return launch_t<T, 2, 2, 4>(c, p, fwd, in, out, s, err);
+    return launch_t<T, 3, 2, 4>(c, p, fwd, in, out, s, err);
+}
+
+static int launch(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, const double *in,
+                  double *out, hipStream_t s, std::string *err)
+{
+    const LegPlan *p = leg_plan(c, ncomp, nvec, nscl, fwd, err);
+    if (!p) return -1;
+    if (c->prec == kF64) return launch_p<TF64>(c, *p, fwd, in, out, s, err);
+    if (c->prec == kF32) return launch_p<TF32>(c, *p, fwd, in, out, s, err);
+    return launch_p<TBF3>(c, *p, fwd, in, out, s, err);
+}
+
+int leg_backward_dev(LegContext *c, int ncomp, int nvec, int nscl, const double *d_sp,
+                     double *d_vr, hipStream_t s, std::string *err)
+{
+    return launch(c, ncomp, nvec, nscl, false, d_sp, d_vr, s, err);
+}
+
+int leg_forward_dev(LegContext *c, int ncomp, int nvec, int nscl, const double *d_vr,
+                    double *d_sp, hipStream_t s, std::string *err)
+{
+    return launch(c, ncomp, nvec, nscl, true, d_vr, d_sp, s, err);
+}
+
+// FP64 P and dP at all nth colatitudes in the reference's P_rtm(l_rtm, j)
+// order (rlm modes, m-major), for checking the device recursion.
+int leg_tables_to_host(LegContext *c, double *P, double *dP, std::string *err)
+{
+    const int L = c->g.L, th_h = c->g.th_h, nth = c->g.nth;
+    const size_t nraw = (size_t)(L + 1) * (L + 2) / 2 * th_h;
+    double *rP = nullptr, *rdP = nullptr, *rPM = nullptr;
+    HIP_TRY(hipMalloc(&rP, sizeof(double) * nraw));
+    HIP_TRY(hipMalloc(&rdP, sizeof(double) * nraw));
+    HIP_TRY(hipMalloc(&rPM, sizeof(double) * nraw));
+    dim3 grid((th_h + 127) / 128, L + 1);
+    hipLaunchKernelGGL(k_schmidt_raw, grid, dim3(128), 0, c->stream, L, th_h, c->d_colat, rP, rdP,
+                       rPM);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> hP(nraw), hdP(nraw);
+    HIP_TRY(hipMemcpyAsync(hP.data(), rP, sizeof(double) * nraw, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hdP.data(), rdP, sizeof(double) * nraw, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(rP);
+    (void)hipFree(rdP);
+    (void)hipFree(rPM);
+    // expand hemisphere rows to the full grid by parity; rlm order of modes
+    int j = 0;
+    for (int mp = 0; mp < 2 * L + 1; ++mp) {
+        int m = std::abs(mp - L);
+        long long pos = (long long)m * (L + 1) - (long long)m * (m - 1) / 2;
+        for (int l = m; l <= L; ++l, ++j) {
+            double sgn = ((l - m) & 1) ? -1.0 : 1.0;
+            for (int t = 0; t < th_h; ++t) {
+                double p = hP[(pos + l - m) * th_h + t], d = hdP[(pos + l - m) * th_h + t];
+                P[t + (size_t)nth * j] = p;
+                dP[t + (size_t)nth * j] = d;
+                int tm = nth - 1 - t;
+                if (tm != t) {
+                    P[tm + (size_t)nth * j] = sgn * p;
+                    dP[tm + (size_t)nth * j] = -sgn * d;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+}  // namespace calypso

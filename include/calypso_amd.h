@@ -1,0 +1,109 @@
+/*
+ * calypso_amd.h -- C ABI of the MI355X (gfx950) spherical-harmonic transform
+ * library libcalypso_amd.so.  Plain pointers and sizes only; every entry
+ * point returns 0 on success and a negative status on failure, with the
+ * message available from calypso_last_error() (the reference aborts through
+ * calypso_MPI_abort, calypso_mpi.f90:114-126; the Fortran shim does the same
+ * on a nonzero status).
+ *
+ * Layouts are the reference's single-domain ones (1 MPI rank, identity
+ * send/receive tables, set_local_sphere_param.f90:86-126):
+ *   sp_rlm[nd + ncomp*(j + J*k)]                 J = (L+1)^2, rlm order
+ *   vr_rtm[nd + ncomp*(t + nth*k + nth*nr*mp)]   mp = m + L, m = -L..L
+ * with vectors first as (pol, dpol/dr, tor) <-> (r, theta, phi) triplets and
+ * scalars after them (legendre_transform_select.f90:309-439).
+ */
+#ifndef CALYPSO_AMD_H
+#define CALYPSO_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision / mode ids of the GPU Legendre transform. */
+#define CALYPSO_LEG_FP64    1  /* FP64 MFMA 16x16x4                          */
+#define CALYPSO_LEG_FP32    2  /* FP32 MFMA 16x16x4                          */
+#define CALYPSO_LEG_BF16X3  3  /* split-BF16x3 MFMA 16x16x32, FP32 accumulate */
+
+typedef struct calypso_leg calypso_leg_t;
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *calypso_last_error(void);
+
+/* Mode string -> id, the GPU counterpart of set_legendre_trans_mode_ctl
+ * (legendre_transform_select.f90:181-224).  Accepts "hip_fp64", "hip_fp32",
+ * "hip_bf16x3" (case-insensitive); returns -1 for anything else. */
+int calypso_legendre_mode_id(const char *name);
+
+/* Gauss-Legendre points/weights/colatitudes of construct_gauss_coefs and
+ * set_gauss_colatitude (m_gauss_points.f90:117-195); host-side, no GPU. */
+int calypso_gauss_points(int n, double *point, double *weight, double *colat);
+
+/* Create a transform context: the GPU counterpart of initialize_legendre_trans
+ * + sel_init_legendre_trans (init_sph_trans.f90:64-92,
+ * legendre_transform_select.f90:228-265).  P_l^m and dP_l^m/dtheta are
+ * computed on the device in FP64 from the colatitudes.
+ *   radius[nr]          radius_1d_rlm_r
+ *   colat, weight[nth]  g_colat_rtm, weight_rtm (NULL: computed here)
+ *   ncomp/nvector/nscalar  field set the plan is prepared for (others are
+ *                       planned lazily on first use)                        */
+int calypso_leg_create(int l_truncation, int nth, int nr, const double *radius,
+                       const double *colat, const double *weight, int ncomp,
+                       int nvector, int nscalar, int mode,
+                       calypso_leg_t **out);
+
+/* sel_finalize_legendre_trans (legendre_transform_select.f90:269-304). */
+int calypso_leg_destroy(calypso_leg_t *ctx);
+
+/* Device-resident transforms on HIP stream `stream` (NULL: the context's own
+ * stream).  Asynchronous.  d_sp_rlm / d_vr_rtm are device pointers.
+ * backward = sel_backward_legendre_trans (legendre_transform_select.f90:309-371)
+ * forward  = sel_forward_legendre_trans  (legendre_transform_select.f90:375-439)
+ * The output array is fully overwritten. */
+int calypso_leg_backward_device(calypso_leg_t *ctx, int ncomp, int nvector,
+                                int nscalar, const double *d_sp_rlm,
+                                double *d_vr_rtm, void *stream);
+int calypso_leg_forward_device(calypso_leg_t *ctx, int ncomp, int nvector,
+                               int nscalar, const double *d_vr_rtm,
+                               double *d_sp_rlm, void *stream);
+
+/* Host-buffer transforms with the reference signature
+ * (ncomp, nvector, nscalar, n_WR, n_WS, WR, WS): WR is read, WS written;
+ * synchronous (copies in, transform, copies out). */
+int calypso_leg_backward(calypso_leg_t *ctx, int ncomp, int nvector,
+                         int nscalar, int n_WR, int n_WS, const double *WR,
+                         double *WS);
+int calypso_leg_forward(calypso_leg_t *ctx, int ncomp, int nvector,
+                        int nscalar, int n_WR, int n_WS, const double *WR,
+                        double *WS);
+
+/* Device P_l^m / dP_l^m/dtheta at all nth points in the reference's
+ * P_rtm(l_rtm, j_rlm) order (m_schmidt_poly_on_rtm.f90), arrays nth*J. */
+int calypso_leg_tables(calypso_leg_t *ctx, double *P, double *dP);
+
+/* info[0..7]: L, nth, nr, mode, table bytes, radial tile (bwd), radial tile
+ * (fwd), work-groups per backward launch (for the plan of the creation set). */
+int calypso_leg_info(calypso_leg_t *ctx, long long *info);
+
+/* The context's HIP stream (hipStream_t). */
+void *calypso_leg_stream(calypso_leg_t *ctx);
+
+/* ---- process-wide selector, the shape the Fortran module binds ---------- */
+/* One active context, like the reference's module-level state
+ * (legendre_transform_select.f90:169-175). */
+int calypso_sel_init_legendre_trans(int mode, int l_truncation, int nth,
+                                    int nr, const double *radius,
+                                    const double *colat, const double *weight,
+                                    int ncomp, int nvector, int nscalar);
+int calypso_sel_backward_legendre_trans(int ncomp, int nvector, int nscalar,
+                                        int n_WR, int n_WS, const double *WR,
+                                        double *WS);
+int calypso_sel_forward_legendre_trans(int ncomp, int nvector, int nscalar,
+                                       int n_WR, int n_WS, const double *WR,
+                                       double *WS);
+int calypso_sel_finalize_legendre_trans(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

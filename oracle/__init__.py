@@ -1,0 +1,173 @@
+"""Test-only checkers for the spherical-harmonic transform path.
+
+``oracle`` wraps two CPU libraries built by ``oracle/Makefile``:
+
+* ``liboracle.so``            -- the plain-C FP64 restatement (calypso_oracle.c);
+* ``_ref/libcalypso_ref.so``  -- the reference's own Fortran routines compiled
+  from the reference sources, driven through ``ref_harness.f90``.
+
+Only ``tests/``, ``__graft_entry__.smoke`` and the CPU-baseline leg of
+``bench.py`` import this package, always as the checker, never as the thing
+measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+_lib = None
+_ref = None
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_P)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle oracle`")
+        _lib = ctypes.CDLL(path)
+        _lib.orc_num_modes.restype = _I
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(os.path.join(_HERE, "_ref", "libcalypso_ref.so"))
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        path = os.path.join(_HERE, "_ref", "libcalypso_ref.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run `make -C oracle ref`")
+        _ref = ctypes.CDLL(path)
+    return _ref
+
+
+# ---------------------------------------------------------------- tables --
+
+def rlm_index(L: int):
+    J = (L + 1) ** 2
+    jlm = np.zeros((J, 3), dtype=np.int32)
+    lstack = np.zeros(2 * L + 2, dtype=np.int32)
+    lib().orc_rlm_index(_I(L), _ptr(jlm), _ptr(lstack))
+    return jlm, lstack
+
+
+def gauss_points(n: int, use_ref: bool = False):
+    point = np.zeros(n)
+    weight = np.zeros(n)
+    colat = np.zeros(n)
+    f = ref().ref_gauss_points if use_ref else lib().orc_gauss_points
+    f(_I(n), _ptr(point), _ptr(weight), _ptr(colat))
+    return point, weight, colat
+
+
+def schmidt_legendres_m(ltr: int, m: int, theta: float, use_ref: bool = False):
+    p = np.zeros(ltr + 1)
+    dp = np.zeros(ltr + 1)
+    f = ref().ref_schmidt_legendres_m if use_ref else lib().orc_schmidt_legendres_m
+    f(_I(ltr), _I(m), _D(theta), _ptr(p), _ptr(dp))
+    return p, dp
+
+
+def sph_normalizations(l: int, m: int):
+    g = np.zeros(17)
+    lib().orc_sph_normalizations(_I(l), _I(m), _ptr(g))
+    return g
+
+
+def g_sph_rlm(L: int):
+    g = np.zeros(((L + 1) ** 2, 17))
+    lib().orc_g_sph_rlm(_I(L), _ptr(g))
+    return g
+
+
+def legendre_tables(L: int, colat: np.ndarray):
+    nth = colat.shape[0]
+    J = (L + 1) ** 2
+    P = np.zeros((J, nth))
+    dP = np.zeros((J, nth))
+    c = np.ascontiguousarray(colat, dtype=np.float64)
+    lib().orc_legendre_tables(_I(L), _I(nth), _ptr(c), _ptr(P), _ptr(dP))
+    return P, dP
+
+
+# ------------------------------------------------------------ transforms --
+
+class Grid:
+    """Single-domain transform geometry: truncation L, nth Gauss points, radii."""
+
+    def __init__(self, L: int, nth: int, radius: np.ndarray):
+        self.L = L
+        self.nth = nth
+        self.nr = int(radius.shape[0])
+        self.J = (L + 1) ** 2
+        self.M = 2 * L + 1
+        self.radius = np.ascontiguousarray(radius, dtype=np.float64)
+        self.point, self.weight, self.colat = gauss_points(nth)
+        self.P, self.dP = legendre_tables(L, self.colat)
+        self.jlm, self.lstack = rlm_index(L)
+        self.g = g_sph_rlm(L)
+
+    def n_rlm(self, ncomp):
+        return ncomp * self.J * self.nr
+
+    def n_rtm(self, ncomp):
+        return ncomp * self.nth * self.nr * self.M
+
+
+def leg_bwd(grid: Grid, ncomp, nvec, nscl, sp: np.ndarray) -> np.ndarray:
+    sp = np.ascontiguousarray(sp, dtype=np.float64).ravel()
+    assert sp.size == grid.n_rlm(ncomp)
+    vr = np.zeros(grid.n_rtm(ncomp))
+    lib().orc_leg_bwd(_I(grid.L), _I(grid.nth), _I(grid.nr), _I(ncomp),
+                      _I(nvec), _I(nscl), _ptr(grid.radius), _ptr(grid.colat),
+                      _ptr(grid.P), _ptr(grid.dP), _ptr(sp), _ptr(vr))
+    return vr
+
+
+def leg_fwd(grid: Grid, ncomp, nvec, nscl, vr: np.ndarray) -> np.ndarray:
+    vr = np.ascontiguousarray(vr, dtype=np.float64).ravel()
+    assert vr.size == grid.n_rtm(ncomp)
+    sp = np.zeros(grid.n_rlm(ncomp))
+    lib().orc_leg_fwd(_I(grid.L), _I(grid.nth), _I(grid.nr), _I(ncomp),
+                      _I(nvec), _I(nscl), _ptr(grid.radius), _ptr(grid.colat),
+                      _ptr(grid.weight), _ptr(grid.P), _ptr(grid.dP), _ptr(vr),
+                      _ptr(sp))
+    return sp
+
+
+class RefLegendre:
+    """The reference's own org-loop transforms (compiled from its sources)."""
+
+    def __init__(self, grid: Grid, nthreads: int = 1):
+        self.grid = grid
+        ref().ref_leg_setup(_I(grid.L), _I(grid.nth), _I(grid.nr),
+                            _ptr(grid.jlm), _ptr(grid.lstack),
+                            _ptr(grid.radius), _ptr(grid.colat),
+                            _ptr(grid.weight), _ptr(grid.P), _ptr(grid.dP),
+                            _ptr(grid.g), _I(nthreads))
+
+    def bwd(self, ncomp, nvec, nscl, sp):
+        sp = np.ascontiguousarray(sp, dtype=np.float64).ravel()
+        vr = np.zeros(self.grid.n_rtm(ncomp))
+        ref().ref_leg_bwd(_I(ncomp), _I(nvec), _I(nscl), _ptr(sp), _ptr(vr))
+        return vr
+
+    def fwd(self, ncomp, nvec, nscl, vr):
+        vr = np.ascontiguousarray(vr, dtype=np.float64).ravel()
+        sp = np.zeros(self.grid.n_rlm(ncomp))
+        ref().ref_leg_fwd(_I(ncomp), _I(nvec), _I(nscl), _ptr(vr), _ptr(sp))
+        return sp

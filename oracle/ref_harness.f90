@@ -1,0 +1,201 @@
+!> ref_harness.f90 -- TEST INFRASTRUCTURE.  A bind(C) driver that links the
+!! reference's own compiled Fortran routines (built from their sources under
+!! the reference tree by oracle/Makefile, output in oracle/_ref/) so the tests
+!! can check the C oracle against the reference arithmetic itself.
+!!
+!! It only sets module state and forwards calls; every computation below is
+!! done by a reference routine:
+!!   construct_gauss_coefs, set_gauss_colatitude  (m_gauss_points.f90:117-195)
+!!   schmidt_legendres_m                          (schmidt_fix_m.f90:62-85)
+!!   set_trans_legendre_rtm                       (set_legendre_matrices.f90:80-97)
+!!   legendre_b_trans_vector_org / _scalar_org    (legendre_bwd_trans_org.f90:45-197)
+!!   legendre_f_trans_vector_org / _scalar_org    (legendre_fwd_trans_org.f90:45-160)
+!!   ludcmp_3band, ludcmp_band, lubksb_*band_mul  (direct_solver/)
+!!   rfftmi, rfftmf, rfftmb                       (fftpack5.1d.f90)
+      module calypso_ref_harness
+!
+      use iso_c_binding
+      implicit none
+!
+      real(kind = 8), allocatable, save :: sp_work(:), vr_work(:)
+!
+      contains
+!
+! -----------------------------------------------------------------------
+!
+      subroutine ref_gauss_points(n, point, weight, colat)              &
+     &          bind(C, name="ref_gauss_points")
+      use m_gauss_points
+      integer(c_int), value :: n
+      real(c_double), intent(inout) :: point(n), weight(n), colat(n)
+!
+      call allocate_gauss_points(n)
+      call allocate_gauss_colatitude
+      call construct_gauss_coefs
+      call set_gauss_colatitude
+      point(1:n) = w_point(1:n)
+      weight(1:n) = w_coefs(1:n)
+      colat(1:n) = w_colat(1:n)
+      call deallocate_gauss_colatitude
+      call deallocate_gauss_points
+      end subroutine ref_gauss_points
+!
+! -----------------------------------------------------------------------
+!
+      subroutine ref_schmidt_legendres_m(ltr, m, theta, p, dp)          &
+     &          bind(C, name="ref_schmidt_legendres_m")
+      use schmidt_fix_m
+      integer(c_int), value :: ltr, m
+      real(c_double), value :: theta
+      real(c_double), intent(inout) :: p(0:ltr), dp(0:ltr)
+      real(kind = 8) :: pmn1(0:ltr), pmp1(0:ltr), df(0:ltr+2)
+!
+      pmn1 = 0.0d0
+      pmp1 = 0.0d0
+      df = 0.0d0
+      call schmidt_legendres_m(ltr, m, theta, p, dp, pmn1, pmp1, df)
+      end subroutine ref_schmidt_legendres_m
+!
+! -----------------------------------------------------------------------
+!> Set the single-domain module state the org-loop transforms read.
+!! jlm(3,J): (l(l+1)+m, l, m) in rlm order; lstack(0:2L+1); P, dP(nth,J);
+!! g(17,J); nthreads = np_smp (radial blocks split evenly).
+      subroutine ref_leg_setup(ltr, nth, nr, jlm, lstack, radius,       &
+     &          colat, weight, P, dP, g, nthreads)                      &
+     &          bind(C, name="ref_leg_setup")
+      use m_machine_parameter
+      use m_spheric_parameter
+      use m_spheric_param_smp
+      use m_schmidt_poly_on_rtm
+      use m_work_4_sph_trans
+      use set_legendre_matrices
+      integer(c_int), value :: ltr, nth, nr, nthreads
+      integer(c_int), intent(in) :: jlm(3,(ltr+1)**2)
+      integer(c_int), intent(in) :: lstack(0:2*ltr+1)
+      real(c_double), intent(in) :: radius(nr), colat(nth), weight(nth)
+      real(c_double), intent(in) :: P(nth,(ltr+1)**2)
+      real(c_double), intent(in) :: dP(nth,(ltr+1)**2)
+      real(c_double), intent(in) :: g(17,(ltr+1)**2)
+      integer :: j, m, ip
+!
+      l_truncation = ltr
+      nidx_rlm(1) = nr
+      nidx_rlm(2) = (ltr+1)**2
+      nidx_rtm(1) = nr
+      nidx_rtm(2) = nth
+      nidx_rtm(3) = 2*ltr + 1
+      nidx_rtp(1) = nr
+      nidx_rtp(2) = nth
+      nidx_rtp(3) = 0
+      nidx_rj(1) = nr
+      nidx_rj(2) = nidx_rlm(2)
+      nnod_rlm = nidx_rlm(1) * nidx_rlm(2)
+      nnod_rtm = nidx_rtm(1) * nidx_rtm(2) * nidx_rtm(3)
+!     strides of set_global_sph_4_rlm / set_global_sph_4_rtm
+      istep_rlm(1) = nidx_rlm(2)
+      istep_rlm(2) = 1
+      istep_rtm(1) = nidx_rtm(2)
+      istep_rtm(2) = 1
+      istep_rtm(3) = nidx_rtm(1) * nidx_rtm(2)
+!
+      if(allocated(idx_gl_1d_rlm_j)) then
+        call deallocate_sph_1d_index_rlm
+        call deallocate_sph_1d_index_rtm
+        call deallocate_work_4_sph_trans
+        deallocate(P_rtm, dPdt_rtm, g_sph_rlm, g_sph_rj)
+        call deallocate_trans_schmidt_rtm
+        call deallocate_gauss_colat_rtm
+        call deallocate_sph_param_smp
+      end if
+      np_smp = max(1, nthreads)
+      call allocate_sph_1d_index_rlm
+      call allocate_sph_1d_index_rtm
+      call allocate_work_4_sph_trans
+      call allocate_schmidt_poly_rtm
+      call allocate_trans_schmidt_rtm
+      call allocate_gauss_colat_rtm
+      call allocate_sph_param_smp
+!
+      do j = 1, nidx_rlm(2)
+        idx_gl_1d_rlm_j(j,1:3) = jlm(1:3,j)
+      end do
+      do m = 1, nidx_rtm(3)
+        idx_gl_1d_rtm_m(m,1) = m
+        idx_gl_1d_rtm_m(m,2) = m - 1 - ltr
+      end do
+      lstack_rlm(0:nidx_rtm(3)) = lstack(0:nidx_rtm(3))
+      do j = 1, nidx_rlm(2)
+        mdx_p_rlm_rtm(j) =  jlm(3,j) + ltr + 1
+        mdx_n_rlm_rtm(j) = -jlm(3,j) + ltr + 1
+      end do
+      radius_1d_rlm_r(1:nr) = radius(1:nr)
+      a_r_1d_rlm_r(1:nr) = 1.0d0 / radius(1:nr)
+      g_colat_rtm(1:nth) = colat(1:nth)
+      weight_rtm(1:nth) = weight(1:nth)
+      do j = 1, nth
+        asin_theta_1d_rtm(j) = 1.0d0 / sin(colat(j))
+      end do
+      P_rtm(1:nth,1:nidx_rlm(2)) = P(1:nth,1:nidx_rlm(2))
+      dPdt_rtm(1:nth,1:nidx_rlm(2)) = dP(1:nth,1:nidx_rlm(2))
+      g_sph_rlm(1:nidx_rlm(2),1:17) = transpose(g(1:17,1:nidx_rlm(2)))
+      call set_trans_legendre_rtm
+!
+      idx_rtm_smp_stack(0,1:3) = 0
+      idx_rlm_smp_stack(0,1:2) = 0
+      do ip = 1, np_smp
+        idx_rtm_smp_stack(ip,1) = (nr * ip) / np_smp
+        idx_rlm_smp_stack(ip,1) = (nr * ip) / np_smp
+      end do
+!
+      if(allocated(sp_work)) deallocate(sp_work, vr_work)
+      allocate(sp_work(1), vr_work(1))
+      end subroutine ref_leg_setup
+!
+! -----------------------------------------------------------------------
+!
+      subroutine ref_leg_bwd(ncomp, nvector, nscalar, sp_in, vr_out)    &
+     &          bind(C, name="ref_leg_bwd")
+      use m_spheric_parameter
+      use legendre_bwd_trans_org
+      integer(c_int), value :: ncomp, nvector, nscalar
+      real(c_double), intent(in) :: sp_in(ncomp*nnod_rlm)
+      real(c_double), intent(inout) :: vr_out(ncomp*nnod_rtm)
+!
+      if(size(sp_work) .ne. ncomp*nnod_rlm) then
+        deallocate(sp_work)
+        allocate(sp_work(ncomp*nnod_rlm))
+      end if
+      sp_work(1:ncomp*nnod_rlm) = sp_in(1:ncomp*nnod_rlm)
+      vr_out(1:ncomp*nnod_rtm) = 0.0d0
+      if(nvector .gt. 0) then
+        call legendre_b_trans_vector_org(ncomp, nvector, sp_work, vr_out)
+      end if
+      if(nscalar .gt. 0) then
+        call legendre_b_trans_scalar_org(ncomp, nvector, nscalar,       &
+     &      sp_work, vr_out)
+      end if
+      end subroutine ref_leg_bwd
+!
+! -----------------------------------------------------------------------
+!
+      subroutine ref_leg_fwd(ncomp, nvector, nscalar, vr_in, sp_out)    &
+     &          bind(C, name="ref_leg_fwd")
+      use m_spheric_parameter
+      use legendre_fwd_trans_org
+      integer(c_int), value :: ncomp, nvector, nscalar
+      real(c_double), intent(in) :: vr_in(ncomp*nnod_rtm)
+      real(c_double), intent(inout) :: sp_out(ncomp*nnod_rlm)
+!
+      sp_out(1:ncomp*nnod_rlm) = 0.0d0
+      if(nvector .gt. 0) then
+        call legendre_f_trans_vector_org(ncomp, nvector, vr_in, sp_out)
+      end if
+      if(nscalar .gt. 0) then
+        call legendre_f_trans_scalar_org(ncomp, nvector, nscalar,       &
+     &      vr_in, sp_out)
+      end if
+      end subroutine ref_leg_fwd
+!
+! -----------------------------------------------------------------------
+!
+      end module calypso_ref_harness

@@ -1,0 +1,167 @@
+"""GPU parity of the HIP Legendre transforms against the oracle and the
+reference golden vectors (FP64: 1e-12 normwise per field; the mixed modes are
+held to their own documented bounds)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import calypso_amd
+import oracle
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "legendre_ref.npz")
+
+FP64_TOL = 1e-12
+MIXED_TOL = {"hip_fp32": 2e-5, "hip_bf16x3": 2e-5}
+
+
+def field_err(got, ref, ncomp):
+    """max over components of max|got-ref| / max|ref| (normwise per field)."""
+    g = np.asarray(got).reshape(-1, ncomp)
+    r = np.asarray(ref).reshape(-1, ncomp)
+    errs = []
+    for nd in range(ncomp):
+        s = np.max(np.abs(r[:, nd]))
+        errs.append(np.max(np.abs(g[:, nd] - r[:, nd])) / (s if s > 0 else 1.0))
+    return max(errs)
+
+
+def run_dev(torch, lt, sp=None, vr=None):
+    dev = torch.device("cuda:0")
+    if sp is not None:
+        d_sp = torch.from_numpy(np.ascontiguousarray(sp)).to(dev)
+        d_vr = torch.full((lt.n_rtm(),), float("nan"), dtype=torch.float64, device=dev)
+        lt.backward(d_sp, d_vr)
+        torch.cuda.synchronize()
+        return d_vr.cpu().numpy()
+    d_vr = torch.from_numpy(np.ascontiguousarray(vr)).to(dev)
+    d_sp = torch.full((lt.n_rlm(),), float("nan"), dtype=torch.float64, device=dev)
+    lt.forward(d_vr, d_sp)
+    torch.cuda.synchronize()
+    return d_sp.cpu().numpy()
+
+
+@pytest.mark.parametrize("name", ["l7", "l10_odd"])
+def test_tables_fp64_vs_reference_golden(torch_cuda, name):
+    g = np.load(GOLD)
+    L, nth, nr, nv, ns = (int(x) for x in g[f"{name}_meta"])
+    print(f"[tables] case={name} L={L} nth={nth}", flush=True)
+    lt = calypso_amd.LegendreTransform(L, nth, g[f"{name}_radius"], 3 * nv + ns, nv, ns, "hip_fp64")
+    P, dP = lt.tables()
+    assert np.max(np.abs(P - g[f"{name}_P"])) < 1e-13
+    assert np.max(np.abs(dP - g[f"{name}_dP"])) < 1e-12
+    lt.close()
+
+
+@pytest.mark.parametrize("name", ["l7", "l10_odd"])
+def test_fp64_vs_reference_golden(torch_cuda, name):
+    g = np.load(GOLD)
+    L, nth, nr, nv, ns = (int(x) for x in g[f"{name}_meta"])
+    nc = 3 * nv + ns
+    print(f"[golden] case={name} L={L} nth={nth} nr={nr} nv={nv} ns={ns}", flush=True)
+    lt = calypso_amd.LegendreTransform(L, nth, g[f"{name}_radius"], nc, nv, ns, "hip_fp64")
+    vr = run_dev(torch_cuda, lt, sp=g[f"{name}_sp_in"])
+    assert field_err(vr, g[f"{name}_vr_out"], nc) < FP64_TOL
+    sp = run_dev(torch_cuda, lt, vr=g[f"{name}_vr_in"])
+    assert field_err(sp, g[f"{name}_sp_out"], nc) < FP64_TOL
+    lt.close()
+
+
+CASES = [
+    # L, nth, nr, nv, ns : the MHD field sets (13 bwd / 19 fwd) at L=31,
+    # the 12-component benchmark set at a reduced grid, scalars only, odd nth
+    (31, 48, 5, 4, 1),
+    (31, 48, 4, 6, 1),
+    (63, 96, 3, 4, 0),
+    (20, 33, 2, 0, 3),
+    (47, 72, 2, 2, 2),
+]
+
+
+@pytest.mark.parametrize("mode", ["hip_fp64", "hip_fp32", "hip_bf16x3"])
+@pytest.mark.parametrize("L,nth,nr,nv,ns", CASES)
+def test_vs_oracle(torch_cuda, mode, L, nth, nr, nv, ns):
+    nc = 3 * nv + ns
+    print(f"[oracle] mode={mode} L={L} nth={nth} nr={nr} nv={nv} ns={ns}", flush=True)
+    rng = np.random.default_rng(L * 7 + nr)
+    radius = np.linspace(7.0 / 13.0, 20.0 / 13.0, nr)
+    grid = oracle.Grid(L, nth, radius)
+    l = grid.jlm[:, 1].astype(float)
+    sp = (rng.standard_normal((nr, grid.J, nc)) * ((l + 1.0) ** -1.5)[None, :, None]).ravel()
+    vr = rng.standard_normal(grid.n_rtm(nc))
+    lt = calypso_amd.LegendreTransform(L, nth, radius, nc, nv, ns, mode)
+    tol = FP64_TOL if mode == "hip_fp64" else MIXED_TOL[mode]
+    e_b = field_err(run_dev(torch_cuda, lt, sp=sp), oracle.leg_bwd(grid, nc, nv, ns, sp), nc)
+    e_f = field_err(run_dev(torch_cuda, lt, vr=vr), oracle.leg_fwd(grid, nc, nv, ns, vr), nc)
+    print(f"   err bwd={e_b:.3e} fwd={e_f:.3e}", flush=True)
+    assert e_b < tol and e_f < tol
+    lt.close()
+
+
+def test_host_selector_api(torch_cuda):
+    g = np.load(GOLD)
+    name = "l7"
+    L, nth, nr, nv, ns = (int(x) for x in g[f"{name}_meta"])
+    nc = 3 * nv + ns
+    calypso_amd.sel_init_legendre_trans("hip_fp64", L, nth, g[f"{name}_radius"],
+                                        g[f"{name}_colat"], g[f"{name}_weight"], nc, nv, ns)
+    WR = np.array(g[f"{name}_sp_in"])
+    WS = np.zeros(g[f"{name}_vr_out"].size)
+    calypso_amd.sel_backward_legendre_trans(nc, nv, ns, WR.size, WS.size, WR, WS)
+    assert field_err(WS, g[f"{name}_vr_out"], nc) < FP64_TOL
+    WR2 = np.array(g[f"{name}_vr_in"])
+    WS2 = np.zeros(g[f"{name}_sp_out"].size)
+    calypso_amd.sel_forward_legendre_trans(nc, nv, ns, WR2.size, WS2.size, WR2, WS2)
+    assert field_err(WS2, g[f"{name}_sp_out"], nc) < FP64_TOL
+    calypso_amd.sel_finalize_legendre_trans()
+
+
+def test_fortran_module_driver(torch_cuda, tmp_path):
+    g = np.load(GOLD)
+    name = "l7"
+    L, nth, nr, nv, ns = (int(x) for x in g[f"{name}_meta"])
+    nc = 3 * nv + ns
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        np.array([L, nth, nr, nv, ns], dtype=np.int32).tofile(f)
+        for key in ("radius", "colat", "weight", "sp_in"):
+            np.asarray(g[f"{name}_{key}"], dtype=np.float64).tofile(f)
+    drv = os.path.join(calypso_amd._lib.LIB_DIR, "leg_driver")
+    print(f"[fortran] {drv} hip_fp64", flush=True)
+    r = subprocess.run([drv, "hip_fp64", str(fin), str(fout)], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    out = np.fromfile(fout, dtype=np.float64)
+    n_rtm = g[f"{name}_vr_out"].size
+    vr, sp2 = out[:n_rtm], out[n_rtm:]
+    assert field_err(vr, g[f"{name}_vr_out"], nc) < FP64_TOL
+    ref2 = oracle.leg_fwd(oracle.Grid(L, nth, g[f"{name}_radius"]), nc, nv, ns, vr)
+    assert field_err(sp2, ref2, nc) < FP64_TOL
+
+
+def test_full_size_roundtrip_fp64(torch_cuda):
+    """Benchmark shape (L=255, nth=384, nr=96, 4 vectors): fwd(bwd(x)) = x."""
+    torch = torch_cuda
+    L, nth, nr, nv, ns = 255, 384, 96, 4, 0
+    nc = 12
+    print(f"[roundtrip] L={L} nth={nth} nr={nr} comps={nc}", flush=True)
+    radius = np.linspace(7.0 / 13.0, 20.0 / 13.0, nr)
+    lt = calypso_amd.LegendreTransform(L, nth, radius, nc, nv, ns, "hip_fp64")
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    J = (L + 1) ** 2
+    sp = torch.randn((nr, J, nc), generator=gen, device="cuda", dtype=torch.float64)
+    sp[:, 0, :] = 0.0                       # l = 0 vector parts carry no field
+    sp = sp.reshape(-1).contiguous()
+    vr = torch.empty(lt.n_rtm(), dtype=torch.float64, device="cuda")
+    sp2 = torch.empty_like(sp)
+    lt.backward(sp, vr)
+    lt.forward(vr, sp2)
+    torch.cuda.synchronize()
+    err = (sp2 - sp).abs().max().item() / sp.abs().max().item()
+    print(f"   roundtrip err={err:.3e}", flush=True)
+    assert err < 1e-9
+    lt.close()
