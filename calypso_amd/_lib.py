@@ -34,6 +34,12 @@ SIGNATURES = [
     ("calypso_sel_backward_legendre_trans", _I, [_I, _I, _I, _I, _I, _P, _P]),
     ("calypso_sel_forward_legendre_trans", _I, [_I, _I, _I, _I, _I, _P, _P]),
     ("calypso_sel_finalize_legendre_trans", _I, []),
+    ("calypso_fft_create", _I, [_I, _I, _I, _I, ctypes.POINTER(_P)]),
+    ("calypso_fft_destroy", _I, [_P]),
+    ("calypso_fft_backward_device", _I, [_P, _I, _P, _P, _P]),
+    ("calypso_fft_forward_device", _I, [_P, _I, _P, _P, _P]),
+    ("calypso_nonlinear_products_device", _I, [ctypes.c_longlong, _P, _P, _P, _P]),
+    ("calypso_fft_mhd_fused_device", _I, [_P, _P, _P, _P, _P]),
 ]
 
 _lib = None

@@ -8,12 +8,17 @@
 #include <new>
 #include <string>
 
+#include "fft_internal.hpp"
 #include "leg_internal.hpp"
 
 using calypso::LegContext;
 
 struct calypso_leg {
     LegContext c;
+};
+
+struct calypso_fft {
+    calypso::FftContext f;
 };
 
 namespace {
@@ -215,6 +220,70 @@ int calypso_sel_finalize_legendre_trans(void)
     int rc = calypso_leg_destroy(g_sel);
     g_sel = nullptr;
     return rc;
+}
+
+int calypso_fft_create(int l_truncation, int nth, int nr, int nphi, calypso_fft_t **out)
+{
+    if (!out) return fail("calypso_fft_create: null argument");
+    *out = nullptr;
+    calypso_fft *h = new (std::nothrow) calypso_fft();
+    if (!h) return fail("calypso_fft_create: out of host memory");
+    std::string err;
+    if (calypso::fft_context_build(&h->f, l_truncation, nth, nr, nphi, &err)) {
+        calypso::fft_context_free(&h->f);
+        delete h;
+        return fail(err);
+    }
+    *out = h;
+    return 0;
+}
+
+int calypso_fft_destroy(calypso_fft_t *fft)
+{
+    if (!fft) return 0;
+    calypso::fft_context_free(&fft->f);
+    delete fft;
+    return 0;
+}
+
+int calypso_fft_backward_device(calypso_fft_t *fft, int ncomp, const double *d_vr_rtm,
+                                double *d_fld_rtp, void *stream)
+{
+    if (!fft || !d_vr_rtm || !d_fld_rtp || ncomp < 1) return fail("calypso_fft_backward_device: bad argument");
+    std::string err;
+    if (calypso::fft_backward_dev(&fft->f, ncomp, d_vr_rtm, d_fld_rtp, (hipStream_t)stream, &err))
+        return fail(err);
+    return 0;
+}
+
+int calypso_fft_forward_device(calypso_fft_t *fft, int ncomp, const double *d_frc_rtp,
+                               double *d_vr_rtm, void *stream)
+{
+    if (!fft || !d_vr_rtm || !d_frc_rtp || ncomp < 1) return fail("calypso_fft_forward_device: bad argument");
+    std::string err;
+    if (calypso::fft_forward_dev(&fft->f, ncomp, d_frc_rtp, d_vr_rtm, (hipStream_t)stream, &err))
+        return fail(err);
+    return 0;
+}
+
+int calypso_nonlinear_products_device(long long nnod, const double *d_fld, double *d_frc,
+                                      const double *coef, void *stream)
+{
+    if (nnod < 1 || !d_fld || !d_frc || !coef) return fail("calypso_nonlinear_products_device: bad argument");
+    std::string err;
+    if (calypso::products_mhd_dev((size_t)nnod, d_fld, d_frc, coef, (hipStream_t)stream, &err))
+        return fail(err);
+    return 0;
+}
+
+int calypso_fft_mhd_fused_device(calypso_fft_t *fft, const double *d_bwd_rtm, double *d_fwd_rtm,
+                                 const double *coef, void *stream)
+{
+    if (!fft || !d_bwd_rtm || !d_fwd_rtm || !coef) return fail("calypso_fft_mhd_fused_device: bad argument");
+    std::string err;
+    if (calypso::fft_mhd_fused_dev(&fft->f, d_bwd_rtm, d_fwd_rtm, coef, (hipStream_t)stream, &err))
+        return fail(err);
+    return 0;
 }
 
 }  // extern "C"

@@ -16,6 +16,10 @@
 //   sp_rlm[nd + ncomp*(j + J*k)]              j in rlm order (m-major)
 //   vr_rtm[nd + ncomp*(t + nth*k + nth*nr*mp)] mp = m + L
 //
+// MFMA accumulator layouts differ by instruction: row = 4*(lane/16)+v for
+// the f32 16x16x4 and bf16 16x16x32 forms, row = lane/16 + 4v for the f64
+// 16x16x4 form (column = lane%16 for all); T::row encodes it.
+//
 // Kernels (one work-group = one |m|, a tile of KT radial points, a group of
 // 16-row tiles; the +m and -m orders share every P tile):
 //   backward: rows = colatitude, K = degree;  out(t) = S+A, out(pi-t) = S-A
@@ -67,6 +71,9 @@ struct TF64 {
         c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], c, 0, 0, 0);
     }
     __device__ static inline double get(const acc &c, int v) { return c[v]; }
+    // accumulator row of value v in lane `lane` (column = lane % 16):
+    // the f64 16x16x4 MFMA interleaves rows across the four lane groups
+    __device__ static inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
 
 struct TF32 {
@@ -81,6 +88,7 @@ struct TF32 {
         c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
     }
     __device__ static inline double get(const acc &c, int v) { return (double)c[v]; }
+    __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 __device__ inline unsigned short bf16_bits(double x) {
@@ -125,6 +133,7 @@ struct TBF3 {
         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], c, 0, 0, 0);
     }
     __device__ static inline double get(const acc &c, int v) { return (double)c[v]; }
+    __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
 // unit = one 16 x KSTEP operand tile in lane order: NS slices x 64 lanes x EPL
@@ -485,7 +494,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
             double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int t = rt * 16 + 4 * (lane >> 4) + v;
+                int t = rt * 16 + T::row(lane, v);
                 if (t >= a.th_h) continue;
                 double S = T::get(a1[r][ct][0], v), An = T::get(a1[r][ct][1], v);
                 int tm = a.nth - 1 - t;
@@ -507,7 +516,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
             double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                int t = rt * 16 + 4 * (lane >> 4) + v;
+                int t = rt * 16 + T::row(lane, v);
                 if (t >= a.th_h) continue;
                 double S = T::get(a2[r][ct][0], v), An = T::get(a2[r][ct][1], v);
                 int tm = a.nth - 1 - t;
@@ -715,7 +724,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
             const int j0 = sg ? j0n : j0p;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int i = rt * 16 + 4 * (lane >> 4) + v;
+                const int i = rt * 16 + T::row(lane, v);
                 if (i >= Kp) continue;
                 const int l = m + par + 2 * i;
                 double fac;
@@ -1035,6 +1044,11 @@ static int launch_t(LegContext *c, const LegPlan &p, bool fwd, const double *in,
     LegArgs a = make_args(c, p, fwd, in, out);
     size_t lds = lds_bytes<T>(p, fwd);
     if (lds > 160 * 1024) { *err = "LDS tile too large for this component count"; return -1; }
+    if (lds > 64 * 1024) {
+        const void *fn = fwd ? reinterpret_cast<const void *>(&k_leg_fwd<T, RPW, MCT1, MCT2>)
+                             : reinterpret_cast<const void *>(&k_leg_bwd<T, RPW, MCT1, MCT2>);
+        HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
     if (fwd)
         hipLaunchKernelGGL((k_leg_fwd<T, RPW, MCT1, MCT2>), dim3(p.n_items), dim3(kThreads), lds, s, a);
     else

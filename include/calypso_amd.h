@@ -103,6 +103,44 @@ int calypso_sel_forward_legendre_trans(int ncomp, int nvector, int nscalar,
                                        double *WS);
 int calypso_sel_finalize_legendre_trans(void);
 
+
+/* ---- longitude FFT, grid products, rtm <-> rtp exchange ----------------- */
+typedef struct calypso_fft calypso_fft_t;
+
+/* FFT context for nphi = 2^a 3^b longitudes (init_MHD_FFT_select,
+ * MHD_FFT_selector.F90:86-114). */
+int calypso_fft_create(int l_truncation, int nth, int nr, int nphi,
+                       calypso_fft_t **out);
+int calypso_fft_destroy(calypso_fft_t *fft);
+
+/* rtm spectra (a_m in slot L+m, b_m in slot L-m) -> grid
+ * fld_rtp[(k + nr*t + nr*nth*i) + nnod_rtp*nd]: back_MHD_FFT_sel_from_recv
+ * (MHD_FFT_selector.F90:219-249) with the rtm->rtp exchange folded in. */
+int calypso_fft_backward_device(calypso_fft_t *fft, int ncomp,
+                                const double *d_vr_rtm, double *d_fld_rtp,
+                                void *stream);
+/* grid -> rtm spectra: fwd_MHD_FFT_sel_from_recv (MHD_FFT_selector.F90:
+ * 185-215) with the rtp->rtm exchange folded in. */
+int calypso_fft_forward_device(calypso_fft_t *fft, int ncomp,
+                               const double *d_frc_rtp, double *d_vr_rtm,
+                               void *stream);
+
+/* s_cal_nonlinear_sph_MHD (cal_nonlinear_sph_MHD.f90:35-75) on the grid:
+ * fld(nnod, 13: u, w, B, J, T) -> frc(nnod, 19: w x u, Coriolis(0),
+ * rot Coriolis(0), J x B, u x B, u T, div Coriolis(0)).
+ * coef[4] = coef_velo, coef_lor, coef_induct, coef_temp. */
+int calypso_nonlinear_products_device(long long nnod, const double *d_fld,
+                                      double *d_frc, const double *coef,
+                                      void *stream);
+
+/* Fused pencil step: rtm spectra of the 13 backward fields -> inverse FFT ->
+ * products -> forward FFT -> rtm spectra of the 19 forward fields, the grid
+ * kept on chip (back_MHD_FFT_sel_from_recv + s_cal_nonlinear_sph_MHD +
+ * fwd_MHD_FFT_sel_from_recv). */
+int calypso_fft_mhd_fused_device(calypso_fft_t *fft, const double *d_bwd_rtm,
+                                 double *d_fwd_rtm, const double *coef,
+                                 void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -171,3 +171,49 @@ class RefLegendre:
         sp = np.zeros(self.grid.n_rlm(ncomp))
         ref().ref_leg_fwd(_I(ncomp), _I(nvec), _I(nscl), _ptr(vr), _ptr(sp))
         return sp
+
+
+# ------------------------------------------------------- FFT, products --
+
+def fft_bwd(L, nth, nr, nphi, ncomp, vr_rtm):
+    vr = np.ascontiguousarray(vr_rtm, dtype=np.float64).ravel()
+    out = np.zeros(ncomp * nr * nth * nphi)
+    lib().orc_fft_bwd(_I(L), _I(nth), _I(nr), _I(nphi), _I(ncomp), _ptr(vr), _ptr(out))
+    return out
+
+
+def fft_fwd(L, nth, nr, nphi, ncomp, fld_rtp):
+    x = np.ascontiguousarray(fld_rtp, dtype=np.float64).ravel()
+    out = np.zeros(ncomp * nth * nr * (2 * L + 1))
+    lib().orc_fft_fwd(_I(L), _I(nth), _I(nr), _I(nphi), _I(ncomp), _ptr(x), _ptr(out))
+    return out
+
+
+def products_mhd(nnod, fld, coef_velo, coef_lor, coef_induct, coef_temp):
+    f = np.ascontiguousarray(fld, dtype=np.float64).ravel()
+    assert f.size == 13 * nnod
+    out = np.zeros(19 * nnod)
+    lib().orc_products_mhd(ctypes.c_size_t(nnod), _ptr(f), _ptr(out), _D(coef_velo),
+                           _D(coef_lor), _D(coef_induct), _D(coef_temp))
+    return out
+
+
+def ref_rfftm(x, forward=True):
+    """FFTPACK5 RFFTMF/RFFTMB of the compiled reference on rows of x (lot x N)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    lot, n = x.shape
+    r = ref()
+    I = lambda v: ctypes.byref(ctypes.c_int(v))
+    lensav = 2 * n + 64
+    wsave = np.zeros(lensav)
+    ier = ctypes.c_int(0)
+    r.rfftmi_(I(n), _ptr(wsave), I(lensav), ctypes.byref(ier))
+    # FFTPACK's multi-vector layout: element i of sequence s at s + i*lot
+    buf = np.ascontiguousarray(x.T).ravel()
+    work = np.zeros(lot * n)
+    f = r.rfftmf_ if forward else r.rfftmb_
+    f(I(lot), I(1), I(n), I(lot), _ptr(buf), I(lot * n), _ptr(wsave), I(lensav), _ptr(work),
+      I(lot * n), ctypes.byref(ier))
+    if ier.value != 0:
+        raise RuntimeError(f"FFTPACK error {ier.value}")
+    return buf.reshape(n, lot).T.copy()

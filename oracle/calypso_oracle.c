@@ -397,3 +397,104 @@ void orc_leg_fwd(int L, int nth, int nr, int ncomp, int nvec, int nscl,
     free(dpvw);
     free(pgvw);
 }
+
+/* Longitude transforms with FFTPACK5 RFFTMF/RFFTMB semantics as used by
+ * sph_RFFTMF_to_send_t / sph_RFFTMB_from_recv_t (t_sph_FFTPACK5.f90:183-331):
+ *   forward  a0 = (1/N) sum x,  a_m = (2/N) sum x cos(m phi),
+ *            b_m = (2/N) sum x sin(m phi),  phi_i = 2 pi i / N
+ *   backward x = a0 + sum_{m=1..L} (a_m cos(m phi) + b_m sin(m phi))
+ * and the exchange ordering of set_wavenumber_4_ispack_fft /
+ * set_zonal_wavenum_4_legendre (set_sph_tranform_ordering.f90:43-159):
+ * a_m goes to rtm slot L+m, b_m to slot L-m, a0 to slot L; m > L dropped.
+ * Grid layout fld_rtp[(k + nr*t + nr*nth*i) + nnod_rtp*nd]
+ * (set_local_sphere_param.f90:66-81), spectra in the rtm layout.
+ * Evaluated as direct FP64 sums (the DFT definition FFTPACK computes). */
+void orc_fft_bwd(int L, int nth, int nr, int nphi, int ncomp,
+                 const double *vr_rtm, double *fld_rtp)
+{
+    const double pi = 4.0 * atan(1.0);
+    const size_t nnod = (size_t)nr * nth * nphi;
+    const size_t sl = (size_t)nth * nr;
+    for (int nd = 0; nd < ncomp; ++nd)
+        for (int t = 0; t < nth; ++t)
+            for (int k = 0; k < nr; ++k) {
+                const size_t base = (size_t)t + (size_t)nth * k;
+                for (int i = 0; i < nphi; ++i) {
+                    double phi = 2.0 * pi * (double)i / (double)nphi;
+                    double x = vr_rtm[nd + (size_t)ncomp * (base + sl * L)];
+                    for (int m = 1; m <= L; ++m) {
+                        double a = vr_rtm[nd + (size_t)ncomp * (base + sl * (L + m))];
+                        double b = vr_rtm[nd + (size_t)ncomp * (base + sl * (L - m))];
+                        x += a * cos((double)m * phi) + b * sin((double)m * phi);
+                    }
+                    fld_rtp[(size_t)k + (size_t)nr * t + (size_t)nr * nth * i + nnod * nd] = x;
+                }
+            }
+}
+
+void orc_fft_fwd(int L, int nth, int nr, int nphi, int ncomp,
+                 const double *fld_rtp, double *vr_rtm)
+{
+    const double pi = 4.0 * atan(1.0);
+    const size_t nnod = (size_t)nr * nth * nphi;
+    const size_t sl = (size_t)nth * nr;
+    for (int nd = 0; nd < ncomp; ++nd)
+        for (int t = 0; t < nth; ++t)
+            for (int k = 0; k < nr; ++k) {
+                const size_t base = (size_t)t + (size_t)nth * k;
+                const double *x = fld_rtp + (size_t)k + (size_t)nr * t + nnod * nd;
+                for (int m = 0; m <= L; ++m) {
+                    double sc = 0.0, ss = 0.0;
+                    for (int i = 0; i < nphi; ++i) {
+                        double phi = 2.0 * pi * (double)i / (double)nphi;
+                        double xi = x[(size_t)nr * nth * i];
+                        sc += xi * cos((double)m * phi);
+                        ss += xi * sin((double)m * phi);
+                    }
+                    if (m == 0) {
+                        vr_rtm[nd + (size_t)ncomp * (base + sl * L)] = sc / (double)nphi;
+                    } else {
+                        vr_rtm[nd + (size_t)ncomp * (base + sl * (L + m))] = 2.0 * sc / (double)nphi;
+                        vr_rtm[nd + (size_t)ncomp * (base + sl * (L - m))] = 2.0 * ss / (double)nphi;
+                    }
+                }
+            }
+}
+
+/* Grid-space nonlinear terms of s_cal_nonlinear_sph_MHD
+ * (cal_nonlinear_sph_MHD.f90:35-75) with cal_cross_prod_w_coef_smp and
+ * cal_vec_scalar_prod_w_coef_smp (cal_products_smp.f90:204-235,302-330):
+ *   fld comps (b_trns): u 0-2, w 3-5, B 6-8, J 9-11, T 12
+ *   frc comps (f_trns): w x u 0-2, Coriolis 3-5, rot Coriolis 6-8,
+ *                       J x B 9-11, u x B 12-14, u T 15-17, div Coriolis 18
+ * The Coriolis slots are filled in rlm space, so they leave the grid as 0. */
+void orc_products_mhd(size_t nnod, const double *fld, double *frc,
+                      double coef_velo, double coef_lor, double coef_induct,
+                      double coef_temp)
+{
+#define F(c) (fld + (size_t)(c) * nnod)
+#define G(c) (frc + (size_t)(c) * nnod)
+    for (size_t i = 0; i < nnod; ++i) {
+        const double u1 = F(0)[i], u2 = F(1)[i], u3 = F(2)[i];
+        const double w1 = F(3)[i], w2 = F(4)[i], w3 = F(5)[i];
+        const double b1 = F(6)[i], b2 = F(7)[i], b3 = F(8)[i];
+        const double j1 = F(9)[i], j2 = F(10)[i], j3 = F(11)[i];
+        const double T = F(12)[i];
+        G(0)[i] = (w2 * u3 - w3 * u2) * coef_velo;
+        G(1)[i] = (w3 * u1 - w1 * u3) * coef_velo;
+        G(2)[i] = (w1 * u2 - w2 * u1) * coef_velo;
+        for (int c = 3; c < 9; ++c) G(c)[i] = 0.0;
+        G(9)[i] = (j2 * b3 - j3 * b2) * coef_lor;
+        G(10)[i] = (j3 * b1 - j1 * b3) * coef_lor;
+        G(11)[i] = (j1 * b2 - j2 * b1) * coef_lor;
+        G(12)[i] = (u2 * b3 - u3 * b2) * coef_induct;
+        G(13)[i] = (u3 * b1 - u1 * b3) * coef_induct;
+        G(14)[i] = (u1 * b2 - u2 * b1) * coef_induct;
+        G(15)[i] = u1 * T * coef_temp;
+        G(16)[i] = u2 * T * coef_temp;
+        G(17)[i] = u3 * T * coef_temp;
+        G(18)[i] = 0.0;
+    }
+#undef F
+#undef G
+}

@@ -17,6 +17,8 @@
 #ifndef CALYPSO_ORACLE_H
 #define CALYPSO_ORACLE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -37,6 +39,14 @@ void orc_leg_fwd(int L, int nth, int nr, int ncomp, int nvec, int nscl,
                  const double *radius, const double *colat,
                  const double *weight, const double *P, const double *dP,
                  const double *vr_in, double *sp_out);
+
+void orc_fft_bwd(int L, int nth, int nr, int nphi, int ncomp,
+                 const double *vr_rtm, double *fld_rtp);
+void orc_fft_fwd(int L, int nth, int nr, int nphi, int ncomp,
+                 const double *fld_rtp, double *vr_rtm);
+void orc_products_mhd(size_t nnod, const double *fld, double *frc,
+                      double coef_velo, double coef_lor, double coef_induct,
+                      double coef_temp);
 
 #ifdef __cplusplus
 }
