@@ -38,8 +38,13 @@ def legendre_flops(L, nth, nr, nvec, nscl):
     return 2.0 * th_h * nr * (L + 1) ** 2 * (5 * nvec + nscl)
 
 
+def rlm_degrees(L):
+    """Degree l of each mode in rlm order (m = -L..L, then l = |m|..L)."""
+    return np.concatenate([np.arange(abs(m), L + 1) for m in range(-L, L + 1)]).astype(np.float64)
+
+
 def make_spectrum(torch, J, seed=1234):
-    l = torch.floor(torch.sqrt(torch.arange(J, dtype=torch.float64, device="cuda")))
+    l = torch.from_numpy(rlm_degrees(L)).cuda()
     gen = torch.Generator(device="cuda").manual_seed(seed)
     x = torch.randn((NR, J, NCOMP), generator=gen, dtype=torch.float64, device="cuda")
     return (x * ((l + 1.0) ** -1.5)[None, :, None]).reshape(-1).contiguous()

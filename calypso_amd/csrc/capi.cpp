@@ -130,7 +130,7 @@ int calypso_leg_backward_device(calypso_leg_t *ctx, int ncomp, int nvector, int 
 {
     if (!ctx || !d_sp_rlm || !d_vr_rtm) return fail("calypso_leg_backward_device: null argument");
     std::string err;
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->c.stream;
+    hipStream_t s = (hipStream_t)stream;
     if (calypso::leg_backward_dev(&ctx->c, ncomp, nvector, nscalar, d_sp_rlm, d_vr_rtm, s, &err))
         return fail(err);
     return 0;
@@ -141,7 +141,7 @@ int calypso_leg_forward_device(calypso_leg_t *ctx, int ncomp, int nvector, int n
 {
     if (!ctx || !d_sp_rlm || !d_vr_rtm) return fail("calypso_leg_forward_device: null argument");
     std::string err;
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->c.stream;
+    hipStream_t s = (hipStream_t)stream;
     if (calypso::leg_forward_dev(&ctx->c, ncomp, nvector, nscalar, d_vr_rtm, d_sp_rlm, s, &err))
         return fail(err);
     return 0;

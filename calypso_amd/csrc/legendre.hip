@@ -428,48 +428,112 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
         __syncthreads();
 
         const int ks_end = min(KS, (c0 + KC) / KS_);
-        for (int ks = c0 / KS_; ks < ks_end; ++ks) {
-            const int kl = (ks * KS_ - c0) + (KS_ / 4) * (lane >> 4);
+        if constexpr (NS == 1) {
+            // single-slice operands: B fragments loaded once per K step and
+            // shared by the wave's row tiles; the next step's P fragments
+            // are fetched while this step's MFMAs run
+            frag Acur[RPW][6];
+            auto load_a = [&](frag (&A)[RPW][6], int ks) {
 #pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-                const int rt = (it.rgroup * 4 + wave) * RPW + r;
-                if (rt >= a.RT) continue;
-                frag A[6][NS];
-                const elem *ap = tab + (ubase + ((long long)rt * KS + ks) * 6) * UE + lane * EPL;
+                for (int r = 0; r < RPW; ++r) {
+                    const int rt = (it.rgroup * 4 + wave) * RPW + r;
+                    if (rt >= a.RT) continue;
+                    const elem *ap = tab + (ubase + ((long long)rt * KS + ks) * 6) * UE + lane * EPL;
 #pragma unroll
-                for (int w = 0; w < 6; ++w)
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) A[w][s] = T::load(ap + w * UE + s * 64 * EPL);
+                    for (int w = 0; w < 6; ++w) A[r][w] = T::load(ap + w * UE);
+                }
+            };
+            load_a(Acur, c0 / KS_);
+            for (int ks = c0 / KS_; ks < ks_end; ++ks) {
+                const int kl = (ks * KS_ - c0) + (KS_ / 4) * (lane >> 4);
+                frag Anx[RPW][6];
+                if (ks + 1 < ks_end) load_a(Anx, ks + 1);
+                frag be[MCT1], bo[MCT1], xe[MCT2], xo[MCT2], ye[MCT2], yo[MCT2];
 #pragma unroll
                 for (int ct = 0; ct < MCT1; ++ct) {
-                    if (ct >= nct1) break;
-                    frag be[NS], bo[NS];
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        be[s] = T::load(X + s * slice + (0 * ncol + b1col[ct]) * KCP + kl);
-                        bo[s] = T::load(X + s * slice + (1 * ncol + b1col[ct]) * KCP + kl);
-                    }
-                    T::mma(a1[r][ct][0], A[0], be);
-                    T::mma(a1[r][ct][1], A[1], bo);
+                    be[ct] = T::load(X + (0 * ncol + b1col[ct]) * KCP + kl);
+                    bo[ct] = T::load(X + (1 * ncol + b1col[ct]) * KCP + kl);
                 }
 #pragma unroll
                 for (int ct = 0; ct < MCT2; ++ct) {
-                    if (ct >= nct2) break;
-                    frag xe[NS], xo[NS], ye[NS], yo[NS];
+                    xe[ct] = T::load(X + (0 * ncol + b2x[ct]) * KCP + kl);
+                    xo[ct] = T::load(X + (1 * ncol + b2x[ct]) * KCP + kl);
+                    ye[ct] = T::load(X + (0 * ncol + b2y[ct]) * KCP + kl);
+                    yo[ct] = T::load(X + (1 * ncol + b2y[ct]) * KCP + kl);
+                    if (b2xn[ct]) { xe[ct] = T::neg(xe[ct]); xo[ct] = T::neg(xo[ct]); }
+                    if (b2yn[ct]) { ye[ct] = T::neg(ye[ct]); yo[ct] = T::neg(yo[ct]); }
+                }
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        xe[s] = T::load(X + s * slice + (0 * ncol + b2x[ct]) * KCP + kl);
-                        xo[s] = T::load(X + s * slice + (1 * ncol + b2x[ct]) * KCP + kl);
-                        ye[s] = T::load(X + s * slice + (0 * ncol + b2y[ct]) * KCP + kl);
-                        yo[s] = T::load(X + s * slice + (1 * ncol + b2y[ct]) * KCP + kl);
-                        if (b2xn[ct]) { xe[s] = T::neg(xe[s]); xo[s] = T::neg(xo[s]); }
-                        if (b2yn[ct]) { ye[s] = T::neg(ye[s]); yo[s] = T::neg(yo[s]); }
+                for (int r = 0; r < RPW; ++r) {
+                    const int rt = (it.rgroup * 4 + wave) * RPW + r;
+                    if (rt >= a.RT) continue;
+#pragma unroll
+                    for (int ct = 0; ct < MCT1; ++ct) {
+                        if (ct >= nct1) break;
+                        T::mma(a1[r][ct][0], &Acur[r][0], &be[ct]);
+                        T::mma(a1[r][ct][1], &Acur[r][1], &bo[ct]);
                     }
-                    // symmetric set: dP odd, PM even; antisymmetric: dP even, PM odd
-                    T::mma(a2[r][ct][0], A[3], xo);
-                    T::mma(a2[r][ct][0], A[4], ye);
-                    T::mma(a2[r][ct][1], A[2], xe);
-                    T::mma(a2[r][ct][1], A[5], yo);
+#pragma unroll
+                    for (int ct = 0; ct < MCT2; ++ct) {
+                        if (ct >= nct2) break;
+                        // symmetric set: dP odd, PM even; antisymmetric: dP even, PM odd
+                        T::mma(a2[r][ct][0], &Acur[r][3], &xo[ct]);
+                        T::mma(a2[r][ct][0], &Acur[r][4], &ye[ct]);
+                        T::mma(a2[r][ct][1], &Acur[r][2], &xe[ct]);
+                        T::mma(a2[r][ct][1], &Acur[r][5], &yo[ct]);
+                    }
+                }
+                if (ks + 1 < ks_end) {
+#pragma unroll
+                    for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                        for (int w = 0; w < 6; ++w) Acur[r][w] = Anx[r][w];
+                }
+            }
+        } else {
+            for (int ks = c0 / KS_; ks < ks_end; ++ks) {
+                const int kl = (ks * KS_ - c0) + (KS_ / 4) * (lane >> 4);
+    #pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    const int rt = (it.rgroup * 4 + wave) * RPW + r;
+                    if (rt >= a.RT) continue;
+                    frag A[6][NS];
+                    const elem *ap = tab + (ubase + ((long long)rt * KS + ks) * 6) * UE + lane * EPL;
+    #pragma unroll
+                    for (int w = 0; w < 6; ++w)
+    #pragma unroll
+                        for (int s = 0; s < NS; ++s) A[w][s] = T::load(ap + w * UE + s * 64 * EPL);
+    #pragma unroll
+                    for (int ct = 0; ct < MCT1; ++ct) {
+                        if (ct >= nct1) break;
+                        frag be[NS], bo[NS];
+    #pragma unroll
+                        for (int s = 0; s < NS; ++s) {
+                            be[s] = T::load(X + s * slice + (0 * ncol + b1col[ct]) * KCP + kl);
+                            bo[s] = T::load(X + s * slice + (1 * ncol + b1col[ct]) * KCP + kl);
+                        }
+                        T::mma(a1[r][ct][0], A[0], be);
+                        T::mma(a1[r][ct][1], A[1], bo);
+                    }
+    #pragma unroll
+                    for (int ct = 0; ct < MCT2; ++ct) {
+                        if (ct >= nct2) break;
+                        frag xe[NS], xo[NS], ye[NS], yo[NS];
+    #pragma unroll
+                        for (int s = 0; s < NS; ++s) {
+                            xe[s] = T::load(X + s * slice + (0 * ncol + b2x[ct]) * KCP + kl);
+                            xo[s] = T::load(X + s * slice + (1 * ncol + b2x[ct]) * KCP + kl);
+                            ye[s] = T::load(X + s * slice + (0 * ncol + b2y[ct]) * KCP + kl);
+                            yo[s] = T::load(X + s * slice + (1 * ncol + b2y[ct]) * KCP + kl);
+                            if (b2xn[ct]) { xe[s] = T::neg(xe[s]); xo[s] = T::neg(xo[s]); }
+                            if (b2yn[ct]) { ye[s] = T::neg(ye[s]); yo[s] = T::neg(yo[s]); }
+                        }
+                        // symmetric set: dP odd, PM even; antisymmetric: dP even, PM odd
+                        T::mma(a2[r][ct][0], A[3], xo);
+                        T::mma(a2[r][ct][0], A[4], ye);
+                        T::mma(a2[r][ct][1], A[2], xe);
+                        T::mma(a2[r][ct][1], A[5], yo);
+                    }
                 }
             }
         }

@@ -55,8 +55,9 @@ int calypso_leg_create(int l_truncation, int nth, int nr, const double *radius,
 /* sel_finalize_legendre_trans (legendre_transform_select.f90:269-304). */
 int calypso_leg_destroy(calypso_leg_t *ctx);
 
-/* Device-resident transforms on HIP stream `stream` (NULL: the context's own
- * stream).  Asynchronous.  d_sp_rlm / d_vr_rtm are device pointers.
+/* Device-resident transforms on HIP stream `stream` (a hipStream_t; NULL is
+ * the null stream; calypso_leg_stream() gives the context's own stream).
+ * Asynchronous.  d_sp_rlm / d_vr_rtm are device pointers.
  * backward = sel_backward_legendre_trans (legendre_transform_select.f90:309-371)
  * forward  = sel_forward_legendre_trans  (legendre_transform_select.f90:375-439)
  * The output array is fully overwritten. */

@@ -78,6 +78,10 @@ CASES = [
     (63, 96, 3, 4, 0),
     (20, 33, 2, 0, 3),
     (47, 72, 2, 2, 2),
+    # several LDS chunks along degree (bwd) and colatitude (fwd)
+    (127, 192, 2, 1, 1),
+    # more than one group of row tiles per order (fwd, m small)
+    (191, 288, 1, 1, 0),
 ]
 
 
@@ -144,7 +148,15 @@ def test_fortran_module_driver(torch_cuda, tmp_path):
 
 
 def test_full_size_roundtrip_fp64(torch_cuda):
-    """Benchmark shape (L=255, nth=384, nr=96, 4 vectors): fwd(bwd(x)) = x."""
+    """Benchmark shape (L=255, nth=384, nr=96, 4 vectors): fwd(bwd(x)) = x.
+
+    Gauss quadrature with nth >= L+1 points makes the pair an identity on
+    band-limited data up to the error of the Gauss weights, which the
+    reference's Newton iteration leaves at ~3e-13 (it stops at 3e-12; the
+    weights sum to 2 - 8e-13).  Modes l >= 1 see that error directly; the
+    l = 0 poloidal output collects it times l(l+1) <= 6.6e4 from every m = 0
+    mode, so it gets its own bound.
+    """
     torch = torch_cuda
     L, nth, nr, nv, ns = 255, 384, 96, 4, 0
     nc = 12
@@ -154,14 +166,24 @@ def test_full_size_roundtrip_fp64(torch_cuda):
     gen = torch.Generator(device="cuda").manual_seed(5)
     J = (L + 1) ** 2
     sp = torch.randn((nr, J, nc), generator=gen, device="cuda", dtype=torch.float64)
-    sp[:, 0, :] = 0.0                       # l = 0 vector parts carry no field
+    j00 = L * (L + 1) // 2                  # (l, m) = (0, 0) in rlm order
+    sp[:, j00, :] = 0.0                     # l = 0 vector parts carry no field
     sp = sp.reshape(-1).contiguous()
     vr = torch.empty(lt.n_rtm(), dtype=torch.float64, device="cuda")
     sp2 = torch.empty_like(sp)
     lt.backward(sp, vr)
     lt.forward(vr, sp2)
     torch.cuda.synchronize()
-    err = (sp2 - sp).abs().max().item() / sp.abs().max().item()
-    print(f"   roundtrip err={err:.3e}", flush=True)
+    scale = sp.abs().max().item()
+    d = (sp2 - sp).abs().reshape(nr, J, nc)
+    err_l0 = d[:, j00, :].max().item() / scale
+    d[:, j00, :] = 0.0
+    err = d.max().item() / scale
+    k, j, c = np.unravel_index(int(d.argmax().item()), (nr, J, nc))
+    degs = np.concatenate([np.arange(abs(m), L + 1) for m in range(-L, L + 1)])
+    ords = np.concatenate([np.full(L - abs(m) + 1, m) for m in range(-L, L + 1)])
+    print(f"   roundtrip err(l>=1)={err:.3e} at k={k} l={degs[j]} m={ords[j]} comp={c}; "
+          f"err(l=0)={err_l0:.3e}", flush=True)
     assert err < 1e-9
+    assert err_l0 < 1e-6
     lt.close()
