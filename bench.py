@@ -80,6 +80,8 @@ def time_mode(torch, calypso_amd, mode, radius, sp, steps, warmup):
     bwd_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in evs]))
     fwd_ms = float(np.mean([b.elapsed_time(c) for _, b, c in evs]))
     info = lt.info()
+    info["plan_bwd"] = lt.plan_info(False)
+    info["plan_fwd"] = lt.plan_info(True)
     lt.close()
     return {"wall_s": wall, "bwd_ms": bwd_ms, "fwd_ms": fwd_ms, "vr": vr, "sp2": sp2,
             "info": info}
@@ -159,7 +161,7 @@ def main():
         e = {"steps_per_s": 1e3 / step_ms, "ms_per_step": step_ms,
              "bwd_ms": r["bwd_ms"], "fwd_ms": r["fwd_ms"],
              "legendre_tflops": 2 * flops / ((r["bwd_ms"] + r["fwd_ms"]) * 1e-3) / 1e12,
-             "kt": [r["info"]["kt_bwd"], r["info"]["kt_fwd"]]}
+             "plan_bwd": r["info"]["plan_bwd"], "plan_fwd": r["info"]["plan_fwd"]}
         if ref_vr is not None:
             e["err_vs_fp64_bwd"] = field_err(r["vr"], ref_vr, NCOMP)
             e["err_vs_fp64_fwd"] = field_err(r["sp2"], ref_sp, NCOMP)

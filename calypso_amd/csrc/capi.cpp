@@ -190,6 +190,31 @@ int calypso_leg_info(calypso_leg_t *ctx, long long *info)
 
 void *calypso_leg_stream(calypso_leg_t *ctx) { return ctx ? (void *)ctx->c.stream : nullptr; }
 
+int calypso_leg_plan_info(calypso_leg_t *ctx, int ncomp, int nvector, int nscalar, int fwd,
+                          long long *info)
+{
+    if (!ctx || !info) return fail("calypso_leg_plan_info: null argument");
+    std::string err;
+    const calypso::LegPlan *p = calypso::leg_plan(&ctx->c, ncomp, nvector, nscalar, fwd != 0, &err);
+    if (!p) return fail(err);
+    info[0] = p->KT;
+    info[1] = p->mct1;
+    info[2] = p->mct2;
+    info[3] = p->rpw;
+    info[4] = p->pair;
+    info[5] = p->tc;
+    info[6] = p->tuned_ms < 0 ? -1 : (long long)(p->tuned_ms * 1000.0f);
+    info[7] = p->n_items;
+    return 0;
+}
+
+int calypso_leg_set_autotune(calypso_leg_t *ctx, int on)
+{
+    if (!ctx) return fail("calypso_leg_set_autotune: null argument");
+    ctx->c.autotune = on != 0;
+    return 0;
+}
+
 int calypso_sel_init_legendre_trans(int mode, int l_truncation, int nth, int nr,
                                     const double *radius, const double *colat,
                                     const double *weight, int ncomp, int nvector, int nscalar)

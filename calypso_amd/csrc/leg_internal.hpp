@@ -34,6 +34,9 @@ struct LegPlan {
     int KT = 1;             // radial points per work item
     int rpw = 3;            // row tiles per wave
     int mct1 = 1, mct2 = 1; // column-tile capacity of the kernel instance
+    int pair = 0;           // fwd: orders m and L-m in one work item
+    int tc = 64;            // fwd: colatitudes per LDS chunk
+    float tuned_ms = -1.f;  // autotuner timing of the chosen plan
     int n_items = 0;
     WorkItem *d_items = nullptr;
 };
@@ -55,6 +58,7 @@ struct LegContext {
     size_t tab_bytes = 0;
 
     std::deque<LegPlan> plans;    // per (ncomp, nvec, nscl, direction)
+    bool autotune = true;         // time candidate plans on first use
 
     // staging buffers for the host-pointer entry points
     double *d_rlm = nullptr, *d_rtm = nullptr;

@@ -305,11 +305,13 @@ struct LegArgs {
     int L, nth, nr, th_h, J;
     int ncomp, nvec, nscl;
     int KT, RT, KSt;
+    int TC;     // colatitudes per LDS chunk (fwd), multiple of KSTEP
+    int pair;   // fwd: the item also carries order L-m
 };
 
 constexpr int kThreads = 256;
 constexpr int kChunkB = 32;   // degree pairs staged per LDS chunk (bwd)
-constexpr int kChunkF = 64;   // colatitudes staged per LDS chunk (fwd)
+constexpr int kChunkF = 64;   // largest colatitude chunk staged in LDS (fwd)
 
 template <class T> __device__ inline int lds_pad() { return T::EPL == 8 ? 8 : 1; }
 
@@ -602,6 +604,10 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
 // (legendre_fwd_trans_org.f90:70-115).  The sphere sum is folded onto the
 // hemisphere: P-terms of an even (l-m) row use V(t)+V(pi-t), dP-terms use
 // V(t)-V(pi-t); odd rows swap the two.
+//
+// A work-group takes the orders m and L-m together (the pairing of the
+// reference's order distribution, set_sph_tranform_ordering.f90:69-159), so
+// every group has ~(L+2)/16 degree rows whatever m is.
 template <class T, int RPW, int MCT1, int MCT2>
 __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
 {
@@ -609,20 +615,35 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
     using frag = typename T::frag;
     using acc = typename T::acc;
     constexpr int KS_ = T::KSTEP, NS = T::NS, EPL = T::EPL;
-    constexpr int TC = kChunkF;
+    const int TC = a.TC;
     const int TCP = TC + lds_pad<T>();
 
     const WorkItem it = a.items[blockIdx.x];
-    const int m = it.m, kt0 = it.ktile * a.KT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int K = a.L - m + 1, K0 = (K + 1) / 2, K1 = K / 2;
-    const int RTe = (K0 + 15) / 16, RTo = (K1 + 15) / 16;
+    // orders of this item: m_0 = it.m, m_1 = L - it.m (if different)
+    const int nm = (a.pair && a.L - it.m != it.m) ? 2 : 1;
+    int mm[2], K0v[2], K1v[2], RTe[2], RTo[2], j0p[2], j0n[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int m = (i == 0) ? it.m : a.L - it.m;
+        mm[i] = m;
+        const int K = a.L - m + 1;
+        K0v[i] = (K + 1) / 2;
+        K1v[i] = K / 2;
+        RTe[i] = (K0v[i] + 15) / 16;
+        RTo[i] = (K1v[i] + 15) / 16;
+        j0p[i] = a.lstack[a.L + m];
+        j0n[i] = a.lstack[a.L - m];
+    }
+    const int rows0 = RTe[0] + RTo[0];
+    const int nrows = rows0 + (nm == 2 ? RTe[1] + RTo[1] : 0);
+    const int kt0 = it.ktile * a.KT;
     const int nkk = min(a.KT, a.nr - kt0);
     const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
-    const int ncol = a.KT * 2 * ncomp;
+    const int ncolm = a.KT * 2 * ncomp;      // columns of one order
+    const int ncol = nm * ncolm;
     const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 2 * 2 * nv;
     const int nct1 = (c1n + 15) / 16, nct2 = (c2n + 15) / 16;
-    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     elem *V = reinterpret_cast<elem *>(smem_raw);  // [NS][2 fold][ncol][TCP]
@@ -662,6 +683,22 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
         }
     }
 
+    // row tile r of this wave -> (order slot i, parity, tile index)
+    int ri[RPW], rpar[RPW], rtile[RPW];
+    bool rok[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int rr = (it.rgroup * 4 + wave) * RPW + r;
+        rok[r] = rr < nrows;
+        int i = (rr < rows0) ? 0 : 1;
+        int x = (i == 0) ? rr : rr - rows0;
+        int p = (x < RTe[i]) ? 0 : 1;
+        ri[r] = i;
+        rpar[r] = p;
+        rtile[r] = p ? x - RTe[i] : x;
+        if (!rok[r]) { ri[r] = 0; rpar[r] = 0; rtile[r] = 0; }
+    }
+
     acc a1[RPW][MCT1], a2[RPW][MCT2];
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
@@ -675,15 +712,23 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
     constexpr int UE = unit_elems<T>();
     const size_t sl = (size_t)a.nth * a.nr;
 
+    auto aptr = [&](int r, int ks) -> const elem * {
+        const int m = mm[ri[r]];
+        return tab + (a.off[2 * m + rpar[r]] + ((long long)rtile[r] * a.KSt + ks) * 3) * UE
+               + lane * EPL;
+    };
+
     for (int t0 = 0; t0 < a.KSt * KS_; t0 += TC) {
         __syncthreads();
-        const int nstage = TC * 2 * a.KT * ncomp;
+        const int nstage = TC * ncol;
         for (int e = threadIdx.x; e < nstage; e += kThreads) {
             int comp = e % ncomp, rest = e / ncomp;
             int tl = rest % TC, rest2 = rest / TC;
-            int sg = rest2 & 1, kk = rest2 >> 1;
+            int sg = rest2 & 1, rest3 = rest2 >> 1;
+            int kk = rest3 % a.KT, i = rest3 / a.KT;
             int t = t0 + tl;
             double vs = 0.0, va = 0.0;
+            const int m = mm[i];
             if (t < a.th_h && kk < nkk && (sg == 0 || m > 0)) {
                 int mp = a.L + (sg ? -m : m), k = kt0 + kk;
                 const double *src = a.in + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
@@ -698,51 +743,68 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
                     vs = w * v1;
                 }
             }
-            int col = (kk * 2 + sg) * ncomp + comp;
+            int col = i * ncolm + (kk * 2 + sg) * ncomp + comp;
             T::store(V + (0 * ncol + col) * TCP + tl, vs, slice);
             T::store(V + (1 * ncol + col) * TCP + tl, va, slice);
         }
         __syncthreads();
 
         const int ks_end = min(a.KSt, (t0 + TC) / KS_);
-        for (int ks = t0 / KS_; ks < ks_end; ++ks) {
-            const int kl = (ks * KS_ - t0) + (KS_ / 4) * (lane >> 4);
+        frag Acur[RPW][3][NS];
+        auto load_a = [&](frag (&A)[RPW][3][NS], int ks) {
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
-                const int rr = (it.rgroup * 4 + wave) * RPW + r;
-                if (rr >= RTe + RTo) continue;
-                const int par = rr < RTe ? 0 : 1;
-                const int rt = par ? rr - RTe : rr;
-                frag A[3][NS];
-                const elem *ap = tab + (a.off[2 * m + par] + ((long long)rt * a.KSt + ks) * 3) * UE
-                               + lane * EPL;
+                if (!rok[r]) continue;
+                const elem *ap = aptr(r, ks);
 #pragma unroll
                 for (int w = 0; w < 3; ++w)
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) A[w][s] = T::load(ap + w * UE + s * 64 * EPL);
+                    for (int s2 = 0; s2 < NS; ++s2) A[r][w][s2] = T::load(ap + w * UE + s2 * 64 * EPL);
+            }
+        };
+        load_a(Acur, t0 / KS_);
+        for (int ks = t0 / KS_; ks < ks_end; ++ks) {
+            const int kl = (ks * KS_ - t0) + (KS_ / 4) * (lane >> 4);
+            frag Anx[RPW][3][NS];
+            if (NS == 1 && ks + 1 < ks_end) load_a(Anx, ks + 1);
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                if (!rok[r]) continue;
+                const int par = rpar[r];
+                const int cofs = ri[r] * ncolm;
                 const int fP = par, fD = 1 - par;  // fold index: 0 sym, 1 anti
 #pragma unroll
                 for (int ct = 0; ct < MCT1; ++ct) {
                     if (ct >= nct1) break;
                     frag b[NS];
 #pragma unroll
-                    for (int s = 0; s < NS; ++s)
-                        b[s] = T::load(V + s * slice + (fP * ncol + b1col[ct]) * TCP + kl);
-                    T::mma(a1[r][ct], A[0], b);
+                    for (int s2 = 0; s2 < NS; ++s2)
+                        b[s2] = T::load(V + s2 * slice + (fP * ncol + cofs + b1col[ct]) * TCP + kl);
+                    T::mma(a1[r][ct], Acur[r][0], b);
                 }
 #pragma unroll
                 for (int ct = 0; ct < MCT2; ++ct) {
                     if (ct >= nct2) break;
                     frag x[NS], y[NS];
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        x[s] = T::load(V + s * slice + (fD * ncol + b2x[ct]) * TCP + kl);
-                        y[s] = T::load(V + s * slice + (fP * ncol + b2y[ct]) * TCP + kl);
-                        if (b2xn[ct]) x[s] = T::neg(x[s]);
-                        if (b2yn[ct]) y[s] = T::neg(y[s]);
+                    for (int s2 = 0; s2 < NS; ++s2) {
+                        x[s2] = T::load(V + s2 * slice + (fD * ncol + cofs + b2x[ct]) * TCP + kl);
+                        y[s2] = T::load(V + s2 * slice + (fP * ncol + cofs + b2y[ct]) * TCP + kl);
+                        if (b2xn[ct]) x[s2] = T::neg(x[s2]);
+                        if (b2yn[ct]) y[s2] = T::neg(y[s2]);
                     }
-                    T::mma(a2[r][ct], A[1], x);
-                    T::mma(a2[r][ct], A[2], y);
+                    T::mma(a2[r][ct], Acur[r][1], x);
+                    T::mma(a2[r][ct], Acur[r][2], y);
+                }
+            }
+            if (ks + 1 < ks_end) {
+                if (NS == 1) {
+#pragma unroll
+                    for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                        for (int w = 0; w < 3; ++w) Acur[r][w][0] = Anx[r][w][0];
+                } else {
+                    load_a(Acur, ks + 1);
                 }
             }
         }
@@ -752,11 +814,10 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
     // spherical_harmonics.f90:121-139) and store in rlm order
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-        const int rr = (it.rgroup * 4 + wave) * RPW + r;
-        if (rr >= RTe + RTo) continue;
-        const int par = rr < RTe ? 0 : 1;
-        const int rt = par ? rr - RTe : rr;
-        const int Kp = par ? K1 : K0;
+        if (!rok[r]) continue;
+        const int i_m = ri[r], par = rpar[r], rt = rtile[r];
+        const int m = mm[i_m];
+        const int Kp = par ? K1v[i_m] : K0v[i_m];
 #pragma unroll
         for (int ct = 0; ct < MCT1 + MCT2; ++ct) {
             const bool g1 = ct < MCT1;
@@ -785,7 +846,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
             if (kk >= nkk || (sg == 1 && m == 0)) continue;
             const int k = kt0 + kk;
             const double rad = a.radius[k];
-            const int j0 = sg ? j0n : j0p;
+            const int j0 = sg ? j0n[i_m] : j0p[i_m];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = rt * 16 + T::row(lane, v);
@@ -900,31 +961,58 @@ static int build_tables(LegContext *c, const double *dP_raw_P, const double *dP_
     return 0;
 }
 
-// Radial tile: the largest KT whose column tiles fit one kernel instance.
-// Instances (mct1, mct2, rows-per-wave): (1,1,3), (1,2,3), (2,4,2|3).
-static void choose_plan(LegPlan *p, int nv, int ns, bool fwd)
+// Candidate tile plans for one field set and direction.  Kernel instances
+// (column tiles of the P GEMM, of the dP|PM GEMM): (1,1), (1,2), (2,4); the
+// radial tile KT is the largest that fits the instance; rows per wave 2, 3 or
+// 5; the forward kernel optionally carries orders m and L-m together.
+static size_t plan_lds(const LegPlan &p, bool fwd, int prec)
+{
+    const int ns = (prec == kBF16x3) ? 3 : 1;
+    const size_t es = (prec == kF64) ? 8 : (prec == kF32 ? 4 : 2);
+    const int pad = (prec == kBF16x3) ? 8 : 1;
+    const int ncol = p.KT * 2 * p.ncomp * ((fwd && p.pair) ? 2 : 1);
+    const int kc = fwd ? p.tc : 32;
+    return (size_t)ns * 2 * ncol * (kc + pad) * es;
+}
+
+static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd, int prec)
 {
     const int per1 = 2 * (nv + ns), per2 = 4 * nv;
-    static const int inst[3][2] = {{1, 1}, {1, 2}, {2, 4}};
-    int best_kt = 1, best_i = 2;
-    double best_eff = -1.0;
+    static const int inst[3][2] = {{1, 2}, {2, 4}, {1, 1}};
+    std::vector<LegPlan> out;
     for (int i = 0; i < 3; ++i) {
-        for (int kt = 1; kt <= 8; ++kt) {
-            int n1 = (per1 * kt + 15) / 16, n2 = (per2 * kt + 15) / 16;
-            if (n1 > inst[i][0] || n2 > inst[i][1]) continue;
-            double used = per1 * kt + per2 * kt;
-            double tiles = 16.0 * (n1 + n2);
-            // wider radial tiles re-read P fewer times; the widest instance
-            // needs fewer rows per wave in the backward kernel
-            double eff = (used / tiles) * std::min(1.0, (double)kt / 2.0);
-            if (i == 2 && !fwd) eff *= 0.9;
-            if (eff > best_eff + 1e-9) { best_eff = eff; best_kt = kt; best_i = i; }
+        int kt = 0;
+        for (int k = 8; k >= 1; --k) {
+            if ((per1 * k + 15) / 16 <= inst[i][0] && (per2 * k + 15) / 16 <= inst[i][1]) {
+                kt = k;
+                break;
+            }
         }
+        if (!kt) continue;
+        std::vector<int> rpws;
+        if (!fwd) rpws = {inst[i][0] == 2 ? 2 : 3};
+        else if (prec == kBF16x3 || inst[i][0] == 2) rpws = {3};
+        else rpws = {3, 5};
+        for (int rpw : rpws)
+            for (int pair = 0; pair < (fwd ? 2 : 1); ++pair) {
+                LegPlan p;
+                p.ncomp = ncomp;
+                p.nvec = nv;
+                p.nscl = ns;
+                p.fwd = fwd;
+                p.KT = kt;
+                p.mct1 = inst[i][0];
+                p.mct2 = inst[i][1];
+                p.rpw = rpw;
+                p.pair = pair;
+                const int kmin = (prec == kBF16x3) ? 32 : 4;
+                p.tc = 64;
+                while (fwd && p.tc > kmin && plan_lds(p, fwd, prec) > 160 * 1024) p.tc /= 2;
+                if (plan_lds(p, fwd, prec) > 160 * 1024) continue;
+                out.push_back(p);
+            }
     }
-    p->KT = best_kt;
-    p->mct1 = inst[best_i][0];
-    p->mct2 = inst[best_i][1];
-    p->rpw = (best_i == 2 && !fwd) ? 2 : 3;
+    return out;
 }
 
 static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
@@ -933,11 +1021,20 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
     const int nkt = (g.nr + p->KT - 1) / p->KT;
     std::vector<WorkItem> items;
     const int rows_per_group = 4 * p->rpw;
+    auto rows = [&](int mm) {
+        int K = g.L - mm + 1, K0 = (K + 1) / 2, K1 = K / 2;
+        return (K0 + 15) / 16 + (K1 + 15) / 16;
+    };
     for (int m = 0; m <= g.L; ++m) {
         int nrow;
         if (fwd) {
-            int K = g.L - m + 1, K0 = (K + 1) / 2, K1 = K / 2;
-            nrow = (K0 + 15) / 16 + (K1 + 15) / 16;
+            if (p->pair) {
+                // orders m and L-m share a work item
+                if (g.L - m < m) break;
+                nrow = rows(m) + (g.L - m != m ? rows(g.L - m) : 0);
+            } else {
+                nrow = rows(m);
+            }
         } else {
             nrow = c->rt_bwd;
         }
@@ -952,6 +1049,8 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
     return 0;
 }
 
+static int autotune_plan(LegContext *c, std::vector<LegPlan> &cand, bool fwd, std::string *err);
+
 const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
                         std::string *err)
 {
@@ -961,14 +1060,21 @@ const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
     }
     for (const LegPlan &p : c->plans)
         if (p.ncomp == ncomp && p.nvec == nvec && p.nscl == nscl && p.fwd == fwd) return &p;
-    LegPlan p;
-    p.ncomp = ncomp;
-    p.nvec = nvec;
-    p.nscl = nscl;
-    p.fwd = fwd;
-    choose_plan(&p, nvec, nscl, fwd);
-    if (build_items(c, &p, fwd, err)) return nullptr;
-    c->plans.push_back(p);
+    std::vector<LegPlan> cand = plan_candidates(ncomp, nvec, nscl, fwd, c->prec);
+    if (cand.empty()) {
+        *err = "no Legendre tile plan fits this component count in LDS";
+        return nullptr;
+    }
+    for (LegPlan &p : cand)
+        if (build_items(c, &p, fwd, err)) return nullptr;
+    int best = 0;
+    if (c->autotune && cand.size() > 1) {
+        best = autotune_plan(c, cand, fwd, err);
+        if (best < 0) return nullptr;
+    }
+    for (size_t i = 0; i < cand.size(); ++i)
+        if ((int)i != best) (void)hipFree(cand[i].d_items);
+    c->plans.push_back(cand[best]);
     return &c->plans.back();
 }
 
@@ -1089,16 +1195,15 @@ static LegArgs make_args(LegContext *c, const LegPlan &p, bool fwd, const double
     a.KT = p.KT;
     a.RT = c->rt_bwd;
     a.KSt = c->ks_fwd;
+    a.TC = p.tc;
+    a.pair = p.pair;
     return a;
 }
 
 template <class T>
-static size_t lds_bytes(const LegPlan &p, bool fwd)
+static size_t lds_bytes(const LegContext *c, const LegPlan &p, bool fwd)
 {
-    const int ncol = p.KT * 2 * p.ncomp;
-    const int pad = (T::EPL == 8) ? 8 : 1;
-    const int kc = fwd ? kChunkF : (kChunkB / T::KSTEP > 0 ? kChunkB : T::KSTEP);
-    return (size_t)T::NS * 2 * ncol * (kc + pad) * sizeof(typename T::elem);
+    return plan_lds(p, fwd, c->prec);
 }
 
 template <class T, int RPW, int MCT1, int MCT2>
@@ -1106,7 +1211,7 @@ static int launch_t(LegContext *c, const LegPlan &p, bool fwd, const double *in,
                     hipStream_t s, std::string *err)
 {
     LegArgs a = make_args(c, p, fwd, in, out);
-    size_t lds = lds_bytes<T>(p, fwd);
+    size_t lds = lds_bytes<T>(c, p, fwd);
     if (lds > 160 * 1024) { *err = "LDS tile too large for this component count"; return -1; }
     if (lds > 64 * 1024) {
         const void *fn = fwd ? reinterpret_cast<const void *>(&k_leg_fwd<T, RPW, MCT1, MCT2>)
@@ -1125,10 +1230,24 @@ template <class T>
 static int launch_p(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
                     hipStream_t s, std::string *err)
 {
-    if (p.mct1 == 1 && p.mct2 == 1) return launch_t<T, 3, 1, 1>(c, p, fwd, in, out, s, err);
-    if (p.mct1 == 1 && p.mct2 == 2) return launch_t<T, 3, 1, 2>(c, p, fwd, in, out, s, err);
+    if (p.mct1 == 1 && p.mct2 == 1) {
+        if (p.rpw == 5) return launch_t<T, 5, 1, 1>(c, p, fwd, in, out, s, err);
+        return launch_t<T, 3, 1, 1>(c, p, fwd, in, out, s, err);
+    }
+    if (p.mct1 == 1 && p.mct2 == 2) {
+        if (p.rpw == 5) return launch_t<T, 5, 1, 2>(c, p, fwd, in, out, s, err);
+        return launch_t<T, 3, 1, 2>(c, p, fwd, in, out, s, err);
+    }
     if (p.rpw == 2) return launch_t<T, 2, 2, 4>(c, p, fwd, in, out, s, err);
     return launch_t<T, 3, 2, 4>(c, p, fwd, in, out, s, err);
+}
+
+static int launch_plan(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
+                       hipStream_t s, std::string *err)
+{
+    if (c->prec == kF64) return launch_p<TF64>(c, p, fwd, in, out, s, err);
+    if (c->prec == kF32) return launch_p<TF32>(c, p, fwd, in, out, s, err);
+    return launch_p<TBF3>(c, p, fwd, in, out, s, err);
 }
 
 static int launch(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, const double *in,
@@ -1136,9 +1255,44 @@ static int launch(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, const 
 {
     const LegPlan *p = leg_plan(c, ncomp, nvec, nscl, fwd, err);
     if (!p) return -1;
-    if (c->prec == kF64) return launch_p<TF64>(c, *p, fwd, in, out, s, err);
-    if (c->prec == kF32) return launch_p<TF32>(c, *p, fwd, in, out, s, err);
-    return launch_p<TBF3>(c, *p, fwd, in, out, s, err);
+    return launch_plan(c, *p, fwd, in, out, s, err);
+}
+
+// Time every candidate plan (3 launches each after one warm-up, on the
+// context stream) and return the fastest: the GPU counterpart of the
+// reference's init-time selection of the Legendre loop variant
+// (select_legendre_transform, sph_transforms_4_MHD.f90:309-426).
+static int autotune_plan(LegContext *c, std::vector<LegPlan> &cand, bool fwd, std::string *err)
+{
+    const LegPlan &p0 = cand[0];
+    const size_t nrlm = (size_t)p0.ncomp * c->g.J * c->g.nr;
+    const size_t nrtm = (size_t)p0.ncomp * c->g.nth * c->g.nr * c->g.M;
+    double *din = nullptr, *dout = nullptr;
+    HIP_TRY(hipMalloc(&din, sizeof(double) * (fwd ? nrtm : nrlm)));
+    HIP_TRY(hipMalloc(&dout, sizeof(double) * (fwd ? nrlm : nrtm)));
+    HIP_TRY(hipMemsetAsync(din, 0, sizeof(double) * (fwd ? nrtm : nrlm), c->stream));
+    int best = -1;
+    float best_ms = 0.f;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        std::string e2;
+        if (launch_plan(c, cand[i], fwd, din, dout, c->stream, &e2)) continue;
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < 3; ++r)
+            if (launch_plan(c, cand[i], fwd, din, dout, c->stream, &e2)) break;
+        HIP_TRY(hipEventRecord(c->ev1, c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        cand[i].tuned_ms = ms / 3.f;
+        if (best < 0 || cand[i].tuned_ms < best_ms) {
+            best = (int)i;
+            best_ms = cand[i].tuned_ms;
+        }
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    if (best < 0) *err = "no Legendre tile plan could be launched";
+    return best;
 }
 
 int leg_backward_dev(LegContext *c, int ncomp, int nvec, int nscl, const double *d_sp,

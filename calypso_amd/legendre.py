@@ -148,6 +148,14 @@ class LegendreTransform:
         keys = ["L", "nth", "nr", "mode", "table_bytes", "kt_bwd", "kt_fwd", "wg_bwd"]
         return dict(zip(keys, (int(v) for v in a)))
 
+    def plan_info(self, fwd: bool, ncomp=None, nvector=None, nscalar=None):
+        nc, nv, ns = self._fs(ncomp, nvector, nscalar)
+        a = np.zeros(8, dtype=np.int64)
+        _lib.check(_lib.load().calypso_leg_plan_info(self._h, nc, nv, ns, int(fwd), a.ctypes.data),
+                   "calypso_leg_plan_info")
+        keys = ["KT", "mct1", "mct2", "rows_per_wave", "pair", "tc", "tuned_us", "work_groups"]
+        return dict(zip(keys, (int(v) for v in a)))
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().calypso_leg_destroy(self._h)
