@@ -42,6 +42,8 @@ SIGNATURES = [
     ("calypso_fft_forward_device", _I, [_P, _I, _P, _P, _P]),
     ("calypso_nonlinear_products_device", _I, [ctypes.c_longlong, _P, _P, _P, _P]),
     ("calypso_fft_mhd_fused_device", _I, [_P, _P, _P, _P, _P]),
+    ("calypso_ludcmp_band_device", _I, [_I, _I, _I, _P, _P, _P, _P, _P]),
+    ("calypso_lubksb_band_mul_device", _I, [_I, _I, _I, _P, _P, _P, _P]),
 ]
 
 _lib = None

@@ -10,6 +10,7 @@
 
 #include "fft_internal.hpp"
 #include "leg_internal.hpp"
+#include "radial_internal.hpp"
 
 using calypso::LegContext;
 
@@ -307,6 +308,26 @@ int calypso_fft_mhd_fused_device(calypso_fft_t *fft, const double *d_bwd_rtm, do
     if (!fft || !d_bwd_rtm || !d_fwd_rtm || !coef) return fail("calypso_fft_mhd_fused_device: bad argument");
     std::string err;
     if (calypso::fft_mhd_fused_dev(&fft->f, d_bwd_rtm, d_fwd_rtm, coef, (hipStream_t)stream, &err))
+        return fail(err);
+    return 0;
+}
+
+int calypso_ludcmp_band_device(int nb, int n, int mcomp, const double *d_band_a, double *d_band_lu,
+                               int *d_i_pivot, int *d_ierr, void *stream)
+{
+    if (!d_band_a || !d_band_lu || !d_i_pivot) return fail("calypso_ludcmp_band_device: null argument");
+    std::string err;
+    if (calypso::ludcmp_dev(nb, n, mcomp, d_band_a, d_band_lu, d_i_pivot, d_ierr, (hipStream_t)stream, &err))
+        return fail(err);
+    return 0;
+}
+
+int calypso_lubksb_band_mul_device(int nb, int n, int mcomp, const double *d_band_lu,
+                                   const int *d_i_pivot, double *d_x, void *stream)
+{
+    if (!d_band_lu || !d_i_pivot || !d_x) return fail("calypso_lubksb_band_mul_device: null argument");
+    std::string err;
+    if (calypso::lubksb_dev(nb, n, mcomp, d_band_lu, d_i_pivot, d_x, (hipStream_t)stream, &err))
         return fail(err);
     return 0;
 }

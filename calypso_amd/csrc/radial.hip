@@ -1,0 +1,269 @@
+// radial.hip -- FP64 radial-direction kernels of the spectral time step
+// (gfx950): banded LU factorisation and back substitution of the
+// Crank-Nicolson matrices, one thread per spherical-harmonic mode.
+//
+// The factorisations follow the reference's partial-pivoting rules
+//   ludcmp_3band  (direct_solver/m_ludcmp_3band.f90:76-173)
+//   ludcmp_band   (direct_solver/m_ludcmp_band.f90:83-207), nband = 5
+// and the solves
+//   lubksb_3band_mul / lubksb_5band_mul (direct_solver/lubksb_357band_mul.f90:73-195)
+// with the same operation order and no FMA contraction, so results are
+// bitwise those of the reference.  Arrays keep the reference shapes:
+// band_a(nb, n, mcomp), band_lu(2nb-1, n, mcomp), i_pivot(n, mcomp),
+// x(mcomp, n) (mode fastest, as d_rj stores a field).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "radial_internal.hpp"
+
+namespace calypso {
+
+#define HIP_TRY(expr)                                                         \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            if (err) *err = std::string(#expr " failed: ") + hipGetErrorString(e_); \
+            return -1;                                                        \
+        }                                                                     \
+    } while (0)
+
+#pragma clang fp contract(off)
+
+// 1-based accessors matching the reference indexing
+#define A3(k, i) ba[(size_t)((k) - 1) + 3 * (size_t)((i) - 1)]
+#define LU5(k, i) lu[(size_t)((k) - 1) + 5 * (size_t)((i) - 1)]
+
+__global__ void k_ludcmp_3band(int n, int mcomp, const double *band_a, double *band_lu,
+                               int *i_pivot, int *ierr, double *wk_all, int *iwk_all)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mcomp) return;
+    const double TINY = 1.0e-30;
+    const double *ba = band_a + (size_t)3 * n * m;
+    double *lu = band_lu + (size_t)5 * n * m;
+    int *ipiv = i_pivot + (size_t)n * m;
+    double *vwk = wk_all + (size_t)(n + 1) * m;   // 1-based
+    int *idx = iwk_all + (size_t)(n + 1) * m;
+    for (int k = 0; k < 5 * n; ++k) lu[k] = 0.0;
+    LU5(3, 1) = A3(2, 1);
+    LU5(4, 1) = A3(3, 1);
+    for (int j = 2; j <= n - 1; ++j) {
+        LU5(2, j) = A3(1, j);
+        LU5(3, j) = A3(2, j);
+        LU5(4, j) = A3(3, j);
+    }
+    LU5(2, n) = A3(1, n);
+    LU5(3, n) = A3(2, n);
+    for (int i = 1; i <= n; ++i) {
+        double aamax = fabs(LU5(3, i));
+        if (i < n) aamax = fmax(aamax, fabs(LU5(2, i + 1)));
+        if (i > 1) aamax = fmax(aamax, fabs(LU5(4, i - 1)));
+        if (aamax == 0.0) {
+            ierr[m] = 100;   // singular, as the reference reports it
+            return;
+        }
+        vwk[i] = 1.0 / aamax;
+        idx[i] = i;
+    }
+    for (int j = 1; j <= n; ++j) {
+        if (j == 2) {
+            LU5(3, j) = LU5(3, j) - LU5(4, j - 1) * LU5(2, j);
+        } else if (j > 2) {
+            LU5(2, j) = LU5(2, j) - LU5(4, j - 2) * LU5(1, j);
+            LU5(3, j) = LU5(3, j) - LU5(5, j - 2) * LU5(1, j) - LU5(4, j - 1) * LU5(2, j);
+        }
+        ipiv[j - 1] = j;
+        if (j < n) {
+            double aamax = vwk[j] * fabs(LU5(3, j));
+            if (j > 1) LU5(4, j) = LU5(4, j) - LU5(5, j - 1) * LU5(2, j);
+            double dum = vwk[j + 1] * fabs(LU5(4, j));
+            if (dum >= aamax && j <= idx[j]) {
+                int k = idx[j + 1];
+                idx[j + 1] = idx[j];
+                idx[j] = k;
+                ipiv[j - 1] = j + 1;
+                const int kst = j - 1 > 1 ? j - 1 : 1;
+                const int ked = j + 2 < n ? j + 2 : n;
+                for (k = kst; k <= ked; ++k) {
+                    double t = LU5(j + 4 - k, k);
+                    LU5(j + 4 - k, k) = LU5(j + 3 - k, k);
+                    LU5(j + 3 - k, k) = t;
+                }
+                vwk[j + 1] = vwk[j];
+            }
+        }
+        if (LU5(3, n) == 0.0) LU5(3, n) = TINY;
+        double dum = 1.0 / LU5(3, j);
+        if (j < n) LU5(4, j) = LU5(4, j) * dum;
+    }
+    ierr[m] = 0;
+}
+#undef A3
+#undef LU5
+
+#define BA(k, i) ba[(size_t)((k) - 1) + 5 * (size_t)((i) - 1)]
+#define BL(k, i) bl[(size_t)((k) - 1) + 9 * (size_t)((i) - 1)]
+
+__global__ void k_ludcmp_5band(int n, int mcomp, const double *band_a, double *band_lu,
+                               int *indx_all, double *wk_all, int *iwk_all)
+{
+    const int mc = blockIdx.x * blockDim.x + threadIdx.x;
+    if (mc >= mcomp) return;
+    const double TINY = 1.0e-30;
+    const int nb1 = 2, nb2 = 4;
+    const double *ba = band_a + (size_t)5 * n * mc;
+    double *bl = band_lu + (size_t)9 * n * mc;
+    int *indx = indx_all + (size_t)n * mc;
+    double *vv = wk_all + (size_t)(n + 1) * mc;
+    int *idx = iwk_all + (size_t)(n + 1) * mc;
+    for (int i = 1; i <= n; ++i) {
+        const int jst = i - nb1 > 1 ? i - nb1 : 1;
+        const int jed = i + nb1 < n ? i + nb1 : n;
+        double aamax = fabs(BA(i - jst + nb1 + 1, jst));
+        for (int j = jst + 1; j <= jed; ++j) aamax = fmax(aamax, fabs(BA(i - j + nb1 + 1, j)));
+        vv[i] = 1.0 / aamax;
+        idx[i] = i;
+    }
+    for (int k = 0; k < 9 * n; ++k) bl[k] = 0.0;
+    for (int i = 1; i <= n; ++i) {
+        const int jst = i - nb1 > 1 ? i - nb1 : 1;
+        const int jed = i + nb1 < n ? i + nb1 : n;
+        for (int j = jst; j <= jed; ++j) BL(i - j + nb2 + 1, j) = BA(i - j + nb1 + 1, j);
+    }
+    for (int j = 1; j <= n; ++j) {
+        const int ist = j - nb2 > 1 ? j - nb2 : 1;
+        const int ied = j + nb1 < n ? j + nb1 : n;
+        for (int i = ist; i <= j - 1; ++i) {
+            const int l = i - j + nb2 + 1;
+            for (int k = ist; k <= i - 1; ++k)
+                BL(l, j) = BL(l, j) - BL(i - k + nb2 + 1, k) * BL(k - j + nb2 + 1, j);
+        }
+        double aamax = 0.0;
+        int imax = j;
+        for (int i = j; i <= ied; ++i) {
+            const int l = i - j + nb2 + 1;
+            const int kst = i - nb2 > 1 ? i - nb2 : 1;
+            for (int k = kst; k <= j - 1; ++k)
+                BL(l, j) = BL(l, j) - BL(i - k + nb2 + 1, k) * BL(k - j + nb2 + 1, j);
+            const double dum = vv[i] * fabs(BL(l, j));
+            if (dum >= aamax && i <= (idx[j] + nb1)) {
+                imax = i;
+                aamax = dum;
+            }
+        }
+        if (j != imax) {
+            int k = idx[imax];
+            idx[imax] = idx[j];
+            idx[j] = k;
+            const int kst = j - nb2 > 1 ? j - nb2 : 1;
+            const int ked = imax + nb2 < n ? imax + nb2 : n;
+            for (k = kst; k <= ked; ++k) {
+                double t = BL(imax - k + nb2 + 1, k);
+                BL(imax - k + nb2 + 1, k) = BL(j - k + nb2 + 1, k);
+                BL(j - k + nb2 + 1, k) = t;
+            }
+            vv[imax] = vv[j];
+        }
+        indx[j - 1] = imax;
+        if (BL(nb2 + 1, j) == 0.0) BL(nb2 + 1, j) = TINY;
+        if (j != n) {
+            const double dum = 1.0 / BL(nb2 + 1, j);
+            const int ied2 = j + nb2 < n ? j + nb2 : n;
+            for (int i = j + 1; i <= ied2; ++i) BL(i - j + nb2 + 1, j) = BL(i - j + nb2 + 1, j) * dum;
+        }
+    }
+}
+#undef BA
+#undef BL
+
+// lubksb_3band_mul / lubksb_5band_mul for x(mcomp, n), one thread per mode;
+// x is read and written with the mode index fastest (coalesced).
+template <int NB>
+__global__ void k_lubksb_mul(int n, int mcomp, const double *band_lu, const int *i_pivot,
+                             double *x)
+{
+    constexpr int W = (NB == 3) ? 5 : 9;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mcomp) return;
+    const double *bl = band_lu + (size_t)W * n * m;
+#define BL(k, i) bl[(size_t)((k) - 1) + (size_t)W * ((i) - 1)]
+#define X(i) x[(size_t)m + (size_t)mcomp * ((i) - 1)]
+    for (int i = 1; i <= n; ++i) {
+        const int ll = i_pivot[(size_t)(i - 1) + (size_t)n * m];
+        const double s = X(ll);
+        X(ll) = X(i);
+        X(i) = s;
+    }
+    if (NB == 3) {
+        X(2) = X(2) - BL(4, 1) * X(1);
+        for (int i = 3; i <= n; ++i) X(i) = X(i) - BL(5, i - 2) * X(i - 2) - BL(4, i - 1) * X(i - 1);
+        X(n) = X(n) / BL(3, n);
+        X(n - 1) = (X(n - 1) - BL(2, n) * X(n)) / BL(3, n - 1);
+        for (int i = n - 2; i >= 1; --i)
+            X(i) = (X(i) - BL(2, i + 1) * X(i + 1) - BL(1, i + 2) * X(i + 2)) / BL(3, i);
+    } else {
+        X(2) = X(2) - BL(6, 1) * X(1);
+        X(3) = X(3) - BL(7, 1) * X(1) - BL(6, 2) * X(2);
+        X(4) = X(4) - BL(8, 1) * X(1) - BL(7, 2) * X(2) - BL(6, 3) * X(3);
+        for (int i = 5; i <= n; ++i)
+            X(i) = X(i) - BL(9, i - 4) * X(i - 4) - BL(8, i - 3) * X(i - 3)
+                 - BL(7, i - 2) * X(i - 2) - BL(6, i - 1) * X(i - 1);
+        X(n) = X(n) / BL(5, n);
+        X(n - 1) = (X(n - 1) - BL(4, n) * X(n)) / BL(5, n - 1);
+        X(n - 2) = (X(n - 2) - BL(4, n - 1) * X(n - 1) - BL(3, n) * X(n)) / BL(5, n - 2);
+        X(n - 3) = (X(n - 3) - BL(4, n - 2) * X(n - 2) - BL(3, n - 1) * X(n - 1)
+                    - BL(2, n) * X(n)) / BL(5, n - 3);
+        for (int i = n - 4; i >= 1; --i)
+            X(i) = (X(i) - BL(4, i + 1) * X(i + 1) - BL(3, i + 2) * X(i + 2)
+                    - BL(2, i + 3) * X(i + 3) - BL(1, i + 4) * X(i + 4)) / BL(5, i);
+    }
+#undef BL
+#undef X
+}
+
+// ----------------------------------------------------------------- host --
+
+int ludcmp_dev(int nb, int n, int mcomp, const double *d_band_a, double *d_band_lu,
+               int *d_ipiv, int *d_ierr, hipStream_t s, std::string *err)
+{
+    if ((nb != 3 && nb != 5) || n < 5 || mcomp < 1) {
+        *err = "ludcmp: nb must be 3 or 5, n >= 5";
+        return -1;
+    }
+    double *wk = nullptr;
+    int *iwk = nullptr;
+    HIP_TRY(hipMallocAsync(&wk, sizeof(double) * (size_t)(n + 1) * mcomp, s));
+    HIP_TRY(hipMallocAsync(&iwk, sizeof(int) * (size_t)(n + 1) * mcomp, s));
+    const int blocks = (mcomp + 127) / 128;
+    if (nb == 3) {
+        hipLaunchKernelGGL(k_ludcmp_3band, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_a,
+                           d_band_lu, d_ipiv, d_ierr, wk, iwk);
+    } else {
+        hipLaunchKernelGGL(k_ludcmp_5band, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_a,
+                           d_band_lu, d_ipiv, wk, iwk);
+        if (d_ierr) HIP_TRY(hipMemsetAsync(d_ierr, 0, sizeof(int) * mcomp, s));
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipFreeAsync(wk, s));
+    HIP_TRY(hipFreeAsync(iwk, s));
+    return 0;
+}
+
+int lubksb_dev(int nb, int n, int mcomp, const double *d_band_lu, const int *d_ipiv, double *d_x,
+               hipStream_t s, std::string *err)
+{
+    if ((nb != 3 && nb != 5) || n < 5 || mcomp < 1) {
+        *err = "lubksb: nb must be 3 or 5, n >= 5";
+        return -1;
+    }
+    const int blocks = (mcomp + 127) / 128;
+    if (nb == 3)
+        hipLaunchKernelGGL(k_lubksb_mul<3>, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_lu, d_ipiv, d_x);
+    else
+        hipLaunchKernelGGL(k_lubksb_mul<5>, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_lu, d_ipiv, d_x);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace calypso

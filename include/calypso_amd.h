@@ -153,6 +153,22 @@ int calypso_fft_mhd_fused_device(calypso_fft_t *fft, const double *d_bwd_rtm,
                                  double *d_fwd_rtm, const double *coef,
                                  void *stream);
 
+/* ---- banded LU of the Crank-Nicolson radial matrices (FP64) ------------ */
+/* Factor mcomp band matrices band_a(nb, n, mcomp) (nb = 3: ludcmp_3band,
+ * m_ludcmp_3band.f90:76-173; nb = 5: ludcmp_band with nband 5,
+ * m_ludcmp_band.f90:83-207) into band_lu(2nb-1, n, mcomp) and
+ * i_pivot(n, mcomp); ierr(mcomp) = 100 for a singular 3-band matrix.
+ * Same pivoting and operation order as the reference (bitwise results). */
+int calypso_ludcmp_band_device(int nb, int n, int mcomp, const double *d_band_a,
+                               double *d_band_lu, int *d_i_pivot, int *d_ierr,
+                               void *stream);
+/* Solve in place x(mcomp, n) (mode fastest, a d_rj field):
+ * lubksb_3band_mul / lubksb_5band_mul (lubksb_357band_mul.f90:73-195). */
+int calypso_lubksb_band_mul_device(int nb, int n, int mcomp,
+                                   const double *d_band_lu,
+                                   const int *d_i_pivot, double *d_x,
+                                   void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -217,3 +217,36 @@ def ref_rfftm(x, forward=True):
     if ier.value != 0:
         raise RuntimeError(f"FFTPACK error {ier.value}")
     return buf.reshape(n, lot).T.copy()
+
+
+# ------------------------------------------------------------ banded LU --
+
+def ludcmp(nb, band_a, use_ref=False):
+    """band_a: (mcomp, n, nb) C-order == Fortran band_a(nb, n, mcomp)."""
+    band_a = np.ascontiguousarray(band_a, dtype=np.float64)
+    mcomp, n, _ = band_a.shape
+    w = 5 if nb == 3 else 9
+    lu = np.zeros((mcomp, n, w))
+    piv = np.zeros((mcomp, n), dtype=np.int32)
+    if nb == 3:
+        ierr = np.zeros(mcomp, dtype=np.int32)
+        f = ref().ref_ludcmp_3band if use_ref else lib().orc_ludcmp_3band
+        if use_ref:
+            f(_I(n), _I(mcomp), _ptr(band_a), _ptr(lu), _ptr(piv), _ptr(ierr))
+        else:
+            f(_I(n), _I(mcomp), _ptr(band_a), _ptr(lu), _ptr(piv), _ptr(ierr))
+    else:
+        f = ref().ref_ludcmp_5band if use_ref else lib().orc_ludcmp_5band
+        f(_I(n), _I(mcomp), _ptr(band_a), _ptr(lu), _ptr(piv))
+    return lu, piv
+
+
+def lubksb(nb, lu, piv, x, use_ref=False):
+    """x: (n, mcomp) C-order == Fortran x(mcomp, n); returns the solution."""
+    lu = np.ascontiguousarray(lu, dtype=np.float64)
+    piv = np.ascontiguousarray(piv, dtype=np.int32)
+    x = np.array(x, dtype=np.float64, order="C")
+    mcomp, n, _ = lu.shape
+    f = ref().ref_lubksb_mul if use_ref else lib().orc_lubksb_mul
+    f(_I(nb), _I(n), _I(mcomp), _ptr(lu), _ptr(piv), _ptr(x))
+    return x

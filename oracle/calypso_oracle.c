@@ -498,3 +498,206 @@ void orc_products_mhd(size_t nnod, const double *fld, double *frc,
 #undef F
 #undef G
 }
+
+/* ---------------------------------------------------------------------
+ * Banded LU with the reference's partial-pivoting rule, one matrix per
+ * mode (mcomp modes, arrays in the reference's column-major shapes).
+ * BA(k,i) = band_a(k,i), 1-based like the reference.
+ * ------------------------------------------------------------------- */
+#define A3(k, i) band_a[(size_t)((k) - 1) + 3 * (size_t)((i) - 1)]
+#define LU5(k, i) lu[(size_t)((k) - 1) + 5 * (size_t)((i) - 1)]
+
+/* ludcmp_3band, m_ludcmp_3band.f90:76-173 */
+static int ludcmp_3band_one(int n, const double *band_a, int *ipiv, double *lu)
+{
+    const double TINY = 1.0e-30;
+    double *vwk = (double *)calloc((size_t)n + 1, sizeof(double));
+    int *idx = (int *)calloc((size_t)n + 1, sizeof(int));
+    for (int k = 0; k < 5 * n; ++k) lu[k] = 0.0;
+    LU5(3, 1) = A3(2, 1);
+    LU5(4, 1) = A3(3, 1);
+    for (int j = 2; j <= n - 1; ++j) {
+        LU5(2, j) = A3(1, j);
+        LU5(3, j) = A3(2, j);
+        LU5(4, j) = A3(3, j);
+    }
+    LU5(2, n) = A3(1, n);
+    LU5(3, n) = A3(2, n);
+    for (int i = 1; i <= n; ++i) {
+        double aamax = fabs(LU5(3, i));
+        if (i < n) aamax = fmax(aamax, fabs(LU5(2, i + 1)));
+        if (i > 1) aamax = fmax(aamax, fabs(LU5(4, i - 1)));
+        if (aamax == 0.0) {
+            free(vwk);
+            free(idx);
+            return 100;
+        }
+        vwk[i] = 1.0 / aamax;
+        idx[i] = i;
+    }
+    for (int j = 1; j <= n; ++j) {
+        if (j == 2) {
+            LU5(3, j) = LU5(3, j) - LU5(4, j - 1) * LU5(2, j);
+        } else if (j > 2) {
+            LU5(2, j) = LU5(2, j) - LU5(4, j - 2) * LU5(1, j);
+            LU5(3, j) = LU5(3, j) - LU5(5, j - 2) * LU5(1, j) - LU5(4, j - 1) * LU5(2, j);
+        }
+        ipiv[j - 1] = j;
+        if (j < n) {
+            double aamax = vwk[j] * fabs(LU5(3, j));
+            if (j > 1) LU5(4, j) = LU5(4, j) - LU5(5, j - 1) * LU5(2, j);
+            double dum = vwk[j + 1] * fabs(LU5(4, j));
+            if (dum >= aamax && j <= idx[j]) {
+                int k = idx[j + 1];
+                idx[j + 1] = idx[j];
+                idx[j] = k;
+                ipiv[j - 1] = j + 1;
+                int kst = j - 1 > 1 ? j - 1 : 1;
+                int ked = j + 2 < n ? j + 2 : n;
+                for (k = kst; k <= ked; ++k) {
+                    double t = LU5(j + 4 - k, k);
+                    LU5(j + 4 - k, k) = LU5(j + 3 - k, k);
+                    LU5(j + 3 - k, k) = t;
+                }
+                vwk[j + 1] = vwk[j];
+            }
+        }
+        if (LU5(3, n) == 0.0) LU5(3, n) = TINY;
+        double dum = 1.0 / LU5(3, j);
+        if (j < n) LU5(4, j) = LU5(4, j) * dum;
+    }
+    free(vwk);
+    free(idx);
+    return 0;
+}
+#undef A3
+#undef LU5
+
+void orc_ludcmp_3band(int n, int mcomp, const double *band_a, double *band_lu,
+                      int *i_pivot, int *ierr)
+{
+    for (int m = 0; m < mcomp; ++m)
+        ierr[m] = ludcmp_3band_one(n, band_a + (size_t)3 * n * m, i_pivot + (size_t)n * m,
+                                   band_lu + (size_t)5 * n * m);
+}
+
+/* ludcmp_band with nband = 5 (nb1 = 2, nb2 = 4), m_ludcmp_band.f90:83-207 */
+void orc_ludcmp_5band(int n, int mcomp, const double *band_a_all, double *band_lu_all,
+                      int *indx_all)
+{
+    const double TINY = 1.0e-30;
+    const int nband = 5, nb1 = 2, nb2 = 4;
+#define BA(k, i) ba[(size_t)((k) - 1) + (size_t)nband * ((i) - 1)]
+#define BL(k, i) bl[(size_t)((k) - 1) + 9 * (size_t)((i) - 1)]
+    double *vv = (double *)calloc((size_t)n + 1, sizeof(double));
+    int *idx = (int *)calloc((size_t)n + 1, sizeof(int));
+    for (int mc = 0; mc < mcomp; ++mc) {
+        const double *ba = band_a_all + (size_t)nband * n * mc;
+        double *bl = band_lu_all + (size_t)9 * n * mc;
+        int *indx = indx_all + (size_t)n * mc;
+        for (int i = 1; i <= n; ++i) {
+            int jst = i - nb1 > 1 ? i - nb1 : 1;
+            int jed = i + nb1 < n ? i + nb1 : n;
+            double aamax = fabs(BA(i - jst + nb1 + 1, jst));
+            for (int j = jst + 1; j <= jed; ++j) aamax = fmax(aamax, fabs(BA(i - j + nb1 + 1, j)));
+            vv[i] = 1.0 / aamax;
+            idx[i] = i;
+        }
+        for (int k = 0; k < 9 * n; ++k) bl[k] = 0.0;
+        for (int i = 1; i <= n; ++i) {
+            int jst = i - nb1 > 1 ? i - nb1 : 1;
+            int jed = i + nb1 < n ? i + nb1 : n;
+            for (int j = jst; j <= jed; ++j) BL(i - j + nb2 + 1, j) = BA(i - j + nb1 + 1, j);
+        }
+        for (int j = 1; j <= n; ++j) {
+            int ist = j - nb2 > 1 ? j - nb2 : 1;
+            int ied = j + nb1 < n ? j + nb1 : n;
+            for (int i = ist; i <= j - 1; ++i) {
+                int l = i - j + nb2 + 1;
+                for (int k = ist; k <= i - 1; ++k)
+                    BL(l, j) = BL(l, j) - BL(i - k + nb2 + 1, k) * BL(k - j + nb2 + 1, j);
+            }
+            double aamax = 0.0;
+            int imax = j;
+            for (int i = j; i <= ied; ++i) {
+                int l = i - j + nb2 + 1;
+                int kst = i - nb2 > 1 ? i - nb2 : 1;
+                for (int k = kst; k <= j - 1; ++k)
+                    BL(l, j) = BL(l, j) - BL(i - k + nb2 + 1, k) * BL(k - j + nb2 + 1, j);
+                double dum = vv[i] * fabs(BL(l, j));
+                if (dum >= aamax && i <= (idx[j] + nb1)) {
+                    imax = i;
+                    aamax = dum;
+                }
+            }
+            if (j != imax) {
+                int k = idx[imax];
+                idx[imax] = idx[j];
+                idx[j] = k;
+                int kst = j - nb2 > 1 ? j - nb2 : 1;
+                int ked = imax + nb2 < n ? imax + nb2 : n;
+                for (k = kst; k <= ked; ++k) {
+                    double t = BL(imax - k + nb2 + 1, k);
+                    BL(imax - k + nb2 + 1, k) = BL(j - k + nb2 + 1, k);
+                    BL(j - k + nb2 + 1, k) = t;
+                }
+                vv[imax] = vv[j];
+            }
+            indx[j - 1] = imax;
+            if (BL(nb2 + 1, j) == 0.0) BL(nb2 + 1, j) = TINY;
+            if (j != n) {
+                double dum = 1.0 / BL(nb2 + 1, j);
+                int ied2 = j + nb2 < n ? j + nb2 : n;
+                for (int i = j + 1; i <= ied2; ++i) BL(i - j + nb2 + 1, j) = BL(i - j + nb2 + 1, j) * dum;
+            }
+        }
+    }
+    free(vv);
+    free(idx);
+#undef BA
+#undef BL
+}
+
+/* lubksb_3band_mul / lubksb_5band_mul, lubksb_357band_mul.f90:73-195;
+ * x(mcomp, n) mode-fastest, solved in place. */
+void orc_lubksb_mul(int nb, int n, int mcomp, const double *band_lu, const int *i_pivot,
+                    double *x)
+{
+    const int w = (nb == 3) ? 5 : 9;
+#define BL(k, i) bl[(size_t)((k) - 1) + (size_t)w * ((i) - 1)]
+#define X(i) x[(size_t)m + (size_t)mcomp * ((i) - 1)]
+    for (int m = 0; m < mcomp; ++m) {
+        const double *bl = band_lu + (size_t)w * n * m;
+        for (int i = 1; i <= n; ++i) {
+            int ll = i_pivot[(size_t)(i - 1) + (size_t)n * m];
+            double s = X(ll);
+            X(ll) = X(i);
+            X(i) = s;
+        }
+        if (nb == 3) {
+            X(2) = X(2) - BL(4, 1) * X(1);
+            for (int i = 3; i <= n; ++i) X(i) = X(i) - BL(5, i - 2) * X(i - 2) - BL(4, i - 1) * X(i - 1);
+            X(n) = X(n) / BL(3, n);
+            X(n - 1) = (X(n - 1) - BL(2, n) * X(n)) / BL(3, n - 1);
+            for (int i = n - 2; i >= 1; --i)
+                X(i) = (X(i) - BL(2, i + 1) * X(i + 1) - BL(1, i + 2) * X(i + 2)) / BL(3, i);
+        } else {
+            X(2) = X(2) - BL(6, 1) * X(1);
+            X(3) = X(3) - BL(7, 1) * X(1) - BL(6, 2) * X(2);
+            X(4) = X(4) - BL(8, 1) * X(1) - BL(7, 2) * X(2) - BL(6, 3) * X(3);
+            for (int i = 5; i <= n; ++i)
+                X(i) = X(i) - BL(9, i - 4) * X(i - 4) - BL(8, i - 3) * X(i - 3)
+                     - BL(7, i - 2) * X(i - 2) - BL(6, i - 1) * X(i - 1);
+            X(n) = X(n) / BL(5, n);
+            X(n - 1) = (X(n - 1) - BL(4, n) * X(n)) / BL(5, n - 1);
+            X(n - 2) = (X(n - 2) - BL(4, n - 1) * X(n - 1) - BL(3, n) * X(n)) / BL(5, n - 2);
+            X(n - 3) = (X(n - 3) - BL(4, n - 2) * X(n - 2) - BL(3, n - 1) * X(n - 1)
+                        - BL(2, n) * X(n)) / BL(5, n - 3);
+            for (int i = n - 4; i >= 1; --i)
+                X(i) = (X(i) - BL(4, i + 1) * X(i + 1) - BL(3, i + 2) * X(i + 2)
+                        - BL(2, i + 3) * X(i + 3) - BL(1, i + 4) * X(i + 4)) / BL(5, i);
+        }
+    }
+#undef BL
+#undef X
+}

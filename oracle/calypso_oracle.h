@@ -47,6 +47,12 @@ void orc_fft_fwd(int L, int nth, int nr, int nphi, int ncomp,
 void orc_products_mhd(size_t nnod, const double *fld, double *frc,
                       double coef_velo, double coef_lor, double coef_induct,
                       double coef_temp);
+void orc_ludcmp_3band(int n, int mcomp, const double *band_a, double *band_lu,
+                      int *i_pivot, int *ierr);
+void orc_ludcmp_5band(int n, int mcomp, const double *band_a, double *band_lu,
+                      int *indx);
+void orc_lubksb_mul(int nb, int n, int mcomp, const double *band_lu,
+                    const int *i_pivot, double *x);
 
 #ifdef __cplusplus
 }

@@ -197,5 +197,64 @@
       end subroutine ref_leg_fwd
 !
 ! -----------------------------------------------------------------------
+!> ludcmp_3band (m_ludcmp_3band.f90:76-173) on each of mcomp matrices:
+!! band_a(3,n,mcomp) -> band_lu(5,n,mcomp), i_pivot(n,mcomp), ierr(mcomp)
+      subroutine ref_ludcmp_3band(n, mcomp, band_a, band_lu, i_pivot,   &
+     &          ierr) bind(C, name="ref_ludcmp_3band")
+      use m_ludcmp_3band
+      integer(c_int), value :: n, mcomp
+      real(c_double), intent(in) :: band_a(3,n,mcomp)
+      real(c_double), intent(inout) :: band_lu(5,n,mcomp)
+      integer(c_int), intent(inout) :: i_pivot(n,mcomp), ierr(mcomp)
+      real(kind = 8) :: d
+      integer :: j
+!
+      do j = 1, mcomp
+        call ludcmp_3band(n, band_a(1,1,j), i_pivot(1,j), ierr(j),      &
+     &      band_lu(1,1,j), d)
+      end do
+      end subroutine ref_ludcmp_3band
+!
+! -----------------------------------------------------------------------
+!> ludcmp_band (m_ludcmp_band.f90:83-207) with nband = 5:
+!! band_a(5,n,mcomp) -> band_lu(9,n,mcomp), indx(n,mcomp)
+      subroutine ref_ludcmp_5band(n, mcomp, band_a, band_lu, indx)      &
+     &          bind(C, name="ref_ludcmp_5band")
+      use m_ludcmp_band
+      integer(c_int), value :: n, mcomp
+      real(c_double), intent(in) :: band_a(5,n,mcomp)
+      real(c_double), intent(inout) :: band_lu(9,n,mcomp)
+      integer(c_int), intent(inout) :: indx(n,mcomp)
+      real(kind = 8) :: d
+      integer :: j
+!
+      do j = 1, mcomp
+        call ludcmp_band(n, 5, band_a(1,1,j), band_lu(1,1,j),           &
+     &      indx(1,j), d)
+      end do
+      end subroutine ref_ludcmp_5band
+!
+! -----------------------------------------------------------------------
+!> lubksb_3band_mul / lubksb_5band_mul (lubksb_357band_mul.f90:73-195):
+!! x(mcomp,n) solved in place, one SMP block
+      subroutine ref_lubksb_mul(nb, n, mcomp, band_lu, i_pivot, x)      &
+     &          bind(C, name="ref_lubksb_mul")
+      use lubksb_357band_mul
+      integer(c_int), value :: nb, n, mcomp
+      real(c_double), intent(in) :: band_lu(*)
+      integer(c_int), intent(in) :: i_pivot(n,mcomp)
+      real(c_double), intent(inout) :: x(mcomp,n)
+      integer(kind = 4) :: stack(0:1)
+!
+      stack(0) = 0
+      stack(1) = mcomp
+      if(nb .eq. 3) then
+        call lubksb_3band_mul(1, stack, mcomp, n, band_lu, i_pivot, x)
+      else
+        call lubksb_5band_mul(1, stack, mcomp, n, band_lu, i_pivot, x)
+      end if
+      end subroutine ref_lubksb_mul
+!
+! -----------------------------------------------------------------------
 !
       end module calypso_ref_harness
