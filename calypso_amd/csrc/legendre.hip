@@ -990,9 +990,9 @@ static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd,
         }
         if (!kt) continue;
         std::vector<int> rpws;
-        if (!fwd) rpws = {inst[i][0] == 2 ? 2 : 3};
-        else if (prec == kBF16x3 || inst[i][0] == 2) rpws = {3};
-        else rpws = {3, 5};
+        if (!fwd) rpws = {1, inst[i][0] == 2 ? 2 : 3};
+        else if (prec == kBF16x3 || inst[i][0] == 2) rpws = {1, 3};
+        else rpws = {1, 3, 5};
         for (int rpw : rpws)
             for (int pair = 0; pair < (fwd ? 2 : 1); ++pair) {
                 LegPlan p;
@@ -1232,12 +1232,15 @@ static int launch_p(LegContext *c, const LegPlan &p, bool fwd, const double *in,
 {
     if (p.mct1 == 1 && p.mct2 == 1) {
         if (p.rpw == 5) return launch_t<T, 5, 1, 1>(c, p, fwd, in, out, s, err);
+        if (p.rpw == 1) return launch_t<T, 1, 1, 1>(c, p, fwd, in, out, s, err);
         return launch_t<T, 3, 1, 1>(c, p, fwd, in, out, s, err);
     }
     if (p.mct1 == 1 && p.mct2 == 2) {
         if (p.rpw == 5) return launch_t<T, 5, 1, 2>(c, p, fwd, in, out, s, err);
+        if (p.rpw == 1) return launch_t<T, 1, 1, 2>(c, p, fwd, in, out, s, err);
         return launch_t<T, 3, 1, 2>(c, p, fwd, in, out, s, err);
     }
+    if (p.rpw == 1) return launch_t<T, 1, 2, 4>(c, p, fwd, in, out, s, err);
     if (p.rpw == 2) return launch_t<T, 2, 2, 4>(c, p, fwd, in, out, s, err);
     return launch_t<T, 3, 2, 4>(c, p, fwd, in, out, s, err);
 }
