@@ -44,6 +44,8 @@ SIGNATURES = [
     ("calypso_fft_mhd_fused_device", _I, [_P, _P, _P, _P, _P]),
     ("calypso_ludcmp_band_device", _I, [_I, _I, _I, _P, _P, _P, _P, _P]),
     ("calypso_lubksb_band_mul_device", _I, [_I, _I, _I, _P, _P, _P, _P]),
+    ("calypso_explicit_update_device", _I, [_I, _I, ctypes.c_longlong, _D, _D, _P, _P, _P, _P, _P]),
+    ("calypso_sph_mean_square", _I, [_I, _I, _I, _P, _P, _I, _I, _P, _P, _P]),
 ]
 
 _lib = None

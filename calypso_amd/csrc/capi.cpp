@@ -332,4 +332,48 @@ int calypso_lubksb_band_mul_device(int nb, int n, int mcomp, const double *d_ban
     return 0;
 }
 
+int calypso_explicit_update_device(int kind, int scheme, long long n, double dt, double coef_exp,
+                                   double *d_x, const double *d_diffuse, const double *d_force,
+                                   double *d_pre, void *stream)
+{
+    if (n < 1 || !d_x || !d_diffuse || !d_force || (!d_pre && scheme != 1))
+        return fail("calypso_explicit_update_device: bad argument");
+    std::string err;
+    if (calypso::explicit_update_dev(kind, scheme, (size_t)n, dt, coef_exp, d_x, d_diffuse, d_force,
+                                     d_pre, (hipStream_t)stream, &err))
+        return fail(err);
+    return 0;
+}
+
+int calypso_sph_mean_square(int kind, int nr, int J, const int *l_of_j, const double *radius,
+                            int kr_st, int kr_ed, const double *d_f, double *out, void *stream)
+{
+    if (nr < 2 || J < 1 || !l_of_j || !radius || !d_f || !out)
+        return fail("calypso_sph_mean_square: bad argument");
+    int lmax = 0;
+    for (int j = 0; j < J; ++j) lmax = l_of_j[j] > lmax ? l_of_j[j] : lmax;
+    hipStream_t s = (hipStream_t)stream;
+    int *d_l = nullptr;
+    double *d_r = nullptr, *d_vol = nullptr, *d_out = nullptr;
+    CHECK_HIP(hipMallocAsync(&d_l, sizeof(int) * J, s));
+    CHECK_HIP(hipMallocAsync(&d_r, sizeof(double) * nr, s));
+    CHECK_HIP(hipMallocAsync(&d_vol, sizeof(double) * 3 * (size_t)J, s));
+    CHECK_HIP(hipMallocAsync(&d_out, sizeof(double) * 3, s));
+    CHECK_HIP(hipMemcpyAsync(d_l, l_of_j, sizeof(int) * J, hipMemcpyHostToDevice, s));
+    CHECK_HIP(hipMemcpyAsync(d_r, radius, sizeof(double) * nr, hipMemcpyHostToDevice, s));
+    std::string err;
+    int rc = calypso::mean_square_dev(kind, nr, J, lmax, d_l, d_r, radius, kr_st, kr_ed, d_f, d_vol,
+                                      d_out, s, &err);
+    double h[3] = {0, 0, 0};
+    if (!rc) CHECK_HIP(hipMemcpyAsync(h, d_out, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+    (void)hipFreeAsync(d_l, s);
+    (void)hipFreeAsync(d_r, s);
+    (void)hipFreeAsync(d_vol, s);
+    (void)hipFreeAsync(d_out, s);
+    CHECK_HIP(hipStreamSynchronize(s));
+    if (rc) return fail(err);
+    for (int c = 0; c < 3; ++c) out[c] = h[c];
+    return 0;
+}
+
 }  // extern "C"

@@ -13,6 +13,7 @@
 // x(mcomp, n) (mode fastest, as d_rj stores a field).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "radial_internal.hpp"
@@ -262,6 +263,153 @@ int lubksb_dev(int nb, int n, int mcomp, const double *d_band_lu, const int *d_i
         hipLaunchKernelGGL(k_lubksb_mul<3>, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_lu, d_ipiv, d_x);
     else
         hipLaunchKernelGGL(k_lubksb_mul<5>, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_lu, d_ipiv, d_x);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------------
+// Explicit integration (pointwise over d_rj), reference operation order:
+// kind 0 x += dt (c D + 3/2 F - 1/2 P), P <- F      (cal_vorticity_eq_adams,
+//        cal_vorticity_terms_adams.f90:31-51; cal_diff_induction_MHD_adams,
+//        cal_explicit_terms.f90:22-42)
+// kind 1 x += dt (c D - 3/2 A - 1/2 P), P <- -A     (scalar_diff_advect_adams,
+//        cal_diff_adv_src_explicit.f90:129-154)
+// scheme 0 Adams-Bashforth, 1 Euler, 2 Euler + start of the history (step 1).
+__global__ void k_explicit_update(int kind, int scheme, size_t n, double dt, double coef_exp,
+                                  double *x, const double *diff, const double *force,
+                                  double *pre)
+{
+    const double adam_0 = 3.0 / 2.0, adam_1 = -1.0 / 2.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double D = diff[i], F = force[i];
+        if (kind == 0) {
+            if (scheme == 0) {
+                x[i] = x[i] + dt * (coef_exp * D + adam_0 * F + adam_1 * pre[i]);
+                pre[i] = F;
+            } else {
+                x[i] = x[i] + dt * (coef_exp * D + F);
+                if (scheme == 2) pre[i] = F;
+            }
+        } else {
+            if (scheme == 0) {
+                x[i] = x[i] + dt * (coef_exp * D - adam_0 * F + adam_1 * pre[i]);
+                pre[i] = -F;
+            } else {
+                x[i] = x[i] + dt * (coef_exp * D - F);
+                if (scheme == 2) pre[i] = -F;
+            }
+        }
+    }
+}
+
+// Volume mean square, stage 1: per mode, the radial trapezoid integral of
+// the layer mean square over kr_st..kr_ed (cal_rms_each_vector_sph_spec /
+// _scalar_sph_spec, cal_rms_by_sph_spectr.f90:63-154;
+// radial_int_by_trapezoid, radial_int_for_sph_spec.f90:66-94).
+__global__ void k_ms_modes(int kind, int nr, int J, const int *l_of_j, const double *radius,
+                           int kr_st, int kr_ed, const double *f, double *vol)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= J) return;
+    const int l = l_of_j[j];
+    const int ncp = (kind == 2) ? 1 : 3;
+    double s[3] = {0.0, 0.0, 0.0};
+    double prev[3] = {0.0, 0.0, 0.0};
+    for (int k = kr_st; k <= kr_ed; ++k) {
+        const size_t in = (size_t)j + (size_t)J * (k - 1);
+        const double r = radius[k - 1], ar = 1.0 / r;
+        double cur[3];
+        if (kind == 2) {
+            const double g11 = 1.0 / (double)(2 * l + 1);
+            const double d = f[in];
+            cur[0] = d * d * g11 * r * r;
+            cur[1] = cur[2] = 0.0;
+        } else {
+            const double S = f[in], dS = f[in + (size_t)J * nr], T = f[in + 2 * (size_t)J * nr];
+            double r1, r2;
+            if (l == 0) {
+                r1 = (0.5 * S) * (0.5 * S) * ar * ar;
+                r2 = 0.0;
+            } else {
+                const double g3 = (double)(l * (l + 1));
+                const double g12 = (double)(l * (l + 1)) / (double)(2 * l + 1);
+                r1 = g12 * (g3 * ar * ar * S * S + dS * dS);
+                r2 = g12 * T * T;
+            }
+            double r3 = r1 + r2;
+            if (kind == 1) {
+                r1 = 0.5 * r1;
+                r2 = 0.5 * r2;
+                r3 = 0.5 * r3;
+            }
+            cur[0] = r1;
+            cur[1] = r2;
+            cur[2] = r3;
+        }
+        if (k > kr_st) {
+            const double dr1 = radius[k - 1] - radius[k - 2];
+            for (int c = 0; c < ncp; ++c) s[c] = s[c] + 0.5 * dr1 * (prev[c] + cur[c]);
+        }
+        for (int c = 0; c < 3; ++c) prev[c] = cur[c];
+    }
+    for (int c = 0; c < ncp; ++c) vol[(size_t)j + (size_t)J * c] = s[c];
+}
+
+// stage 2 (one work-group): sum the modes of each degree in mode order, then
+// the degrees in order, times 3 / (r_ed^3 - r_st^3)
+__global__ void k_ms_reduce(int kind, int J, int lmax, const int *l_of_j, const double *vol,
+                            double rs, double re, double *out)
+{
+    extern __shared__ double sl[];   // [ncp][lmax+1]
+    const int ncp = (kind == 2) ? 1 : 3;
+    for (int e = threadIdx.x; e < ncp * (lmax + 1); e += blockDim.x) {
+        const int c = e / (lmax + 1), l = e % (lmax + 1);
+        double s = 0.0;
+        for (int j = 0; j < J; ++j)
+            if (l_of_j[j] == l) s = s + vol[(size_t)j + (size_t)J * c];
+        sl[e] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < ncp) {
+        const int c = threadIdx.x;
+        double tot = sl[c * (lmax + 1)];
+        for (int l = 1; l <= lmax; ++l) tot = tot + sl[c * (lmax + 1) + l];
+        const double avol = 3.0 / (re * re * re - rs * rs * rs);
+        out[c] = avol * tot;
+    }
+}
+
+int explicit_update_dev(int kind, int scheme, size_t n, double dt, double coef_exp, double *d_x,
+                        const double *d_diff, const double *d_force, double *d_pre,
+                        hipStream_t s, std::string *err)
+{
+    if (kind < 0 || kind > 1 || scheme < 0 || scheme > 2) {
+        *err = "explicit update: kind 0|1, scheme 0|1|2";
+        return -1;
+    }
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_explicit_update, dim3(blocks), dim3(256), 0, s, kind, scheme, n, dt,
+                       coef_exp, d_x, d_diff, d_force, d_pre);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int mean_square_dev(int kind, int nr, int J, int lmax, const int *d_l_of_j,
+                    const double *d_radius, const double *h_radius, int kr_st, int kr_ed,
+                    const double *d_f, double *d_vol, double *d_out, hipStream_t s,
+                    std::string *err)
+{
+    if (kind < 0 || kind > 2 || kr_st < 1 || kr_ed > nr || kr_ed <= kr_st) {
+        *err = "mean square: kind 0|1|2 and 1 <= kr_st < kr_ed <= nr";
+        return -1;
+    }
+    hipLaunchKernelGGL(k_ms_modes, dim3((J + 127) / 128), dim3(128), 0, s, kind, nr, J, d_l_of_j,
+                       d_radius, kr_st, kr_ed, d_f, d_vol);
+    HIP_TRY(hipGetLastError());
+    const int ncp = (kind == 2) ? 1 : 3;
+    hipLaunchKernelGGL(k_ms_reduce, dim3(1), dim3(256), sizeof(double) * ncp * (lmax + 1), s, kind,
+                       J, lmax, d_l_of_j, d_vol, h_radius[kr_st - 1], h_radius[kr_ed - 1], d_out);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <string>
 
 namespace calypso {
@@ -11,5 +12,13 @@ int ludcmp_dev(int nb, int n, int mcomp, const double *d_band_a, double *d_band_
                int *d_ierr, hipStream_t s, std::string *err);
 int lubksb_dev(int nb, int n, int mcomp, const double *d_band_lu, const int *d_ipiv, double *d_x,
                hipStream_t s, std::string *err);
+
+int explicit_update_dev(int kind, int scheme, size_t n, double dt, double coef_exp, double *d_x,
+                        const double *d_diff, const double *d_force, double *d_pre,
+                        hipStream_t s, std::string *err);
+int mean_square_dev(int kind, int nr, int J, int lmax, const int *d_l_of_j,
+                    const double *d_radius, const double *h_radius, int kr_st, int kr_ed,
+                    const double *d_f, double *d_vol, double *d_out, hipStream_t s,
+                    std::string *err);
 
 }  // namespace calypso

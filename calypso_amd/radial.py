@@ -1,6 +1,7 @@
-"""Python face of the radial solvers: the reference's direct band solver
+"""Python face of the radial kernels: the reference's direct band solver
 (SOLVER_src/direct_solver: ludcmp_3band, ludcmp_band, lubksb_3band_mul,
-lubksb_5band_mul) on device tensors, batched over modes."""
+lubksb_5band_mul) on device tensors, batched over modes; the explicit
+Adams-Bashforth/Euler update; the volume mean-square (energy) diagnostics."""
 from __future__ import annotations
 
 import ctypes
@@ -34,3 +35,25 @@ def lubksb_band_mul(nb, band_lu, i_pivot, x, stream=None):
         nb, n, mcomp, _dev(band_lu, band_lu.numel(), "band_lu"), _P(i_pivot.data_ptr()),
         _dev(x, x.numel(), "x"), _stream(stream)), "calypso_lubksb_band_mul_device")
     return x
+
+
+def explicit_update(kind, scheme, dt, coef_exp, x, diffuse, force, pre=None, stream=None):
+    """In-place explicit update of one d_rj component (see include/calypso_amd.h)."""
+    n = x.numel()
+    _lib.check(_lib.load().calypso_explicit_update_device(
+        kind, scheme, n, dt, coef_exp, _dev(x, n, "x"), _dev(diffuse, n, "diffuse"),
+        _dev(force, n, "force"), _dev(pre, n, "pre") if pre is not None else None,
+        _stream(stream)), "calypso_explicit_update_device")
+
+
+def sph_mean_square(kind, radius, l_of_j, kr_st, kr_ed, f, stream=None):
+    """Volume mean square of one field; f is a (ncomp, nr, J) device tensor."""
+    import numpy as np
+    radius = np.ascontiguousarray(radius, dtype=np.float64)
+    l_of_j = np.ascontiguousarray(l_of_j, dtype=np.int32)
+    out = np.zeros(3)
+    _lib.check(_lib.load().calypso_sph_mean_square(
+        kind, radius.size, l_of_j.size, l_of_j.ctypes.data_as(_P), radius.ctypes.data_as(_P),
+        kr_st, kr_ed, _dev(f, f.numel(), "f"), out.ctypes.data_as(_P), _stream(stream)),
+        "calypso_sph_mean_square")
+    return out

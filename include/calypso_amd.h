@@ -169,6 +169,32 @@ int calypso_lubksb_band_mul_device(int nb, int n, int mcomp,
                                    const int *d_i_pivot, double *d_x,
                                    void *stream);
 
+/* ---- explicit time integration and energy diagnostics (FP64) ----------- */
+/* Pointwise update of n d_rj values (one field component):
+ *  kind 0 (vorticity, magnetic field): x += dt (coef_exp D + 3/2 F - 1/2 P),
+ *         P <- F  (cal_vorticity_eq_adams, cal_vorticity_terms_adams.f90:31-51;
+ *         cal_diff_induction_MHD_adams, cal_explicit_terms.f90:22-42)
+ *  kind 1 (scalar): x += dt (coef_exp D - 3/2 A - 1/2 P), P <- -A
+ *         (scalar_diff_advect_adams, cal_diff_adv_src_explicit.f90:129-154)
+ *  scheme 0 Adams-Bashforth, 1 Euler, 2 Euler and start of the history
+ *  (cal_expricit_sph_euler, cal_momentum_eq_explicit.f90:82-128). */
+int calypso_explicit_update_device(int kind, int scheme, long long n,
+                                   double dt, double coef_exp, double *d_x,
+                                   const double *d_diffuse,
+                                   const double *d_force, double *d_pre,
+                                   void *stream);
+
+/* Volume mean square of one field over radial points kr_st..kr_ed (1-based):
+ * the numbers of sph_pwr_volume_s.dat (cal_rms_by_sph_spectr.f90:63-154,
+ * radial_int_for_sph_spec.f90:66-94, cal_ave_4_rms_vector_sph.f90:47-148).
+ * d_f: the field's d_rj columns (pol, dpol/dr, tor or one scalar), each
+ * J*nr with the mode fastest; l_of_j[J] degrees of the rj modes (host);
+ * radius[nr] (host).  kind 0 vector, 1 vector energy (x 1/2), 2 scalar.
+ * out[3] (host): pol, tor, total (vector) or out[0] (scalar). */
+int calypso_sph_mean_square(int kind, int nr, int J, const int *l_of_j,
+                            const double *radius, int kr_st, int kr_ed,
+                            const double *d_f, double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -250,3 +250,27 @@ def lubksb(nb, lu, piv, x, use_ref=False):
     f = ref().ref_lubksb_mul if use_ref else lib().orc_lubksb_mul
     f(_I(nb), _I(n), _I(mcomp), _ptr(lu), _ptr(piv), _ptr(x))
     return x
+
+
+# ------------------------------------------------- explicit step, energies --
+
+def explicit_update(kind, scheme, dt, coef_exp, x, diff, force, pre):
+    x = np.array(x, dtype=np.float64)
+    pre = np.array(pre, dtype=np.float64)
+    diff = np.ascontiguousarray(diff, dtype=np.float64)
+    force = np.ascontiguousarray(force, dtype=np.float64)
+    lib().orc_explicit_update(_I(kind), _I(scheme), ctypes.c_size_t(x.size), _D(dt),
+                              _D(coef_exp), _ptr(x), _ptr(diff), _ptr(force), _ptr(pre))
+    return x, pre
+
+
+def sph_mean_square(kind, radius, l_of_j, kr_st, kr_ed, f):
+    """f: (ncomp, nr, J) C-order d_rj slice of one field."""
+    radius = np.ascontiguousarray(radius, dtype=np.float64)
+    l_of_j = np.ascontiguousarray(l_of_j, dtype=np.int32)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(3)
+    lib().orc_sph_mean_square(_I(kind), _I(radius.size), _I(l_of_j.size), _ptr(l_of_j),
+                              _I(int(l_of_j.max())), _ptr(radius), _I(kr_st), _I(kr_ed),
+                              _ptr(f), _ptr(out))
+    return out

@@ -701,3 +701,115 @@ void orc_lubksb_mul(int nb, int n, int mcomp, const double *band_lu, const int *
 #undef BL
 #undef X
 }
+
+/* ---------------------------------------------------------------------
+ * Explicit time integration of the spectral fields (pointwise over d_rj):
+ *   kind 0 (vorticity, induction; cal_vorticity_eq_adams,
+ *     cal_vorticity_terms_adams.f90:31-51; cal_diff_induction_MHD_adams,
+ *     cal_explicit_terms.f90:22-42):
+ *       adams: x += dt (c D + 3/2 F - 1/2 P),  P <- F
+ *       euler: x += dt (c D + F)               (and P <- F on step 1)
+ *   kind 1 (scalar; scalar_diff_advect_adams/_euler,
+ *     cal_diff_adv_src_explicit.f90:129-200):
+ *       adams: x += dt (c D - 3/2 A - 1/2 P),  P <- -A
+ *       euler: x += dt (c D - A)               (and P <- -A on step 1)
+ * scheme: 0 adams, 1 euler, 2 euler + start of the Adams history.
+ * ------------------------------------------------------------------- */
+void orc_explicit_update(int kind, int scheme, size_t n, double dt, double coef_exp,
+                         double *x, const double *diff, const double *force, double *pre)
+{
+    const double adam_0 = 3.0 / 2.0, adam_1 = -1.0 / 2.0;
+    for (size_t i = 0; i < n; ++i) {
+        if (kind == 0) {
+            if (scheme == 0) {
+                x[i] = x[i] + dt * (coef_exp * diff[i] + adam_0 * force[i] + adam_1 * pre[i]);
+                pre[i] = force[i];
+            } else {
+                x[i] = x[i] + dt * (coef_exp * diff[i] + force[i]);
+                if (scheme == 2) pre[i] = force[i];
+            }
+        } else {
+            if (scheme == 0) {
+                x[i] = x[i] + dt * (coef_exp * diff[i] - adam_0 * force[i] + adam_1 * pre[i]);
+                pre[i] = -force[i];
+            } else {
+                x[i] = x[i] + dt * (coef_exp * diff[i] - force[i]);
+                if (scheme == 2) pre[i] = -force[i];
+            }
+        }
+    }
+}
+
+/* Volume mean square of one field in the shell kr_st..kr_ed (1-based):
+ * cal_rms_each_vector_sph_spec / cal_rms_each_scalar_sph_spec
+ * (cal_rms_by_sph_spectr.f90:63-154), set_sph_energies_by_rms (x 1/2 for
+ * velocity and magnetic field), radial_int_by_trapezoid
+ * (radial_int_for_sph_spec.f90:66-94), sum over modes of each degree, then
+ * over degrees (sum_sph_rms_data.f90:290-320, cal_ave_4_rms_vector_sph.f90:
+ * 127-148), times 3/(r_ed^3 - r_st^3) (cal_one_over_volume, :47-62).
+ * f is d_rj of the field: f[j + J*k + J*nr*c], c = pol, dpol/dr, tor
+ * (vector) or the scalar.  kind 0 vector, 1 vector energy, 2 scalar.
+ * out[0..2] = pol, tor, total (vector) or out[0] (scalar). */
+void orc_sph_mean_square(int kind, int nr, int J, const int *l_of_j, int lmax,
+                         const double *radius, int kr_st, int kr_ed, const double *f,
+                         double *out)
+{
+    const int ncp = (kind == 2) ? 1 : 3;
+    double *vol = (double *)calloc((size_t)J * ncp, sizeof(double));
+    double *lay = (double *)calloc((size_t)(nr + 1) * ncp, sizeof(double));
+    for (int j = 0; j < J; ++j) {
+        const int l = l_of_j[j];
+        for (int k = 1; k <= nr; ++k) {
+            const size_t in = (size_t)j + (size_t)J * (k - 1);
+            const double r = radius[k - 1], ar = 1.0 / r;
+            if (kind == 2) {
+                const double g11 = 1.0 / (double)(2 * l + 1);
+                const double d = f[in];
+                lay[k] = d * d * g11 * r * r;
+            } else {
+                const double S = f[in], dS = f[in + (size_t)J * nr], T = f[in + 2 * (size_t)J * nr];
+                double r1, r2;
+                if (l == 0) {
+                    r1 = (0.5 * S) * (0.5 * S) * ar * ar;
+                    r2 = 0.0;
+                } else {
+                    const double g3 = (double)(l * (l + 1));
+                    const double g12 = (double)(l * (l + 1)) / (double)(2 * l + 1);
+                    r1 = g12 * (g3 * ar * ar * S * S + dS * dS);
+                    r2 = g12 * T * T;
+                }
+                double r3 = r1 + r2;
+                if (kind == 1) {
+                    r1 = 0.5 * r1;
+                    r2 = 0.5 * r2;
+                    r3 = 0.5 * r3;
+                }
+                lay[k] = r1;
+                lay[k + (nr + 1)] = r2;
+                lay[k + 2 * (nr + 1)] = r3;
+            }
+        }
+        for (int c = 0; c < ncp; ++c) {
+            double s = 0.0;
+            for (int kr = kr_st; kr <= kr_ed - 1; ++kr) {
+                const double dr1 = radius[kr] - radius[kr - 1];
+                s = s + 0.5 * dr1 * (lay[kr + c * (nr + 1)] + lay[kr + 1 + c * (nr + 1)]);
+            }
+            vol[(size_t)j + (size_t)J * c] = s;
+        }
+    }
+    const double rs = radius[kr_st - 1], re = radius[kr_ed - 1];
+    const double avol = 3.0 / (re * re * re - rs * rs * rs);
+    for (int c = 0; c < ncp; ++c) {
+        double tot = 0.0;
+        for (int l = 0; l <= lmax; ++l) {
+            double sl = 0.0;
+            for (int j = 0; j < J; ++j)
+                if (l_of_j[j] == l) sl = sl + vol[(size_t)j + (size_t)J * c];
+            tot = (l == 0) ? sl : tot + sl;
+        }
+        out[c] = avol * tot;
+    }
+    free(vol);
+    free(lay);
+}

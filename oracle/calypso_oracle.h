@@ -53,6 +53,11 @@ void orc_ludcmp_5band(int n, int mcomp, const double *band_a, double *band_lu,
                       int *indx);
 void orc_lubksb_mul(int nb, int n, int mcomp, const double *band_lu,
                     const int *i_pivot, double *x);
+void orc_explicit_update(int kind, int scheme, size_t n, double dt, double coef_exp,
+                         double *x, const double *diff, const double *force, double *pre);
+void orc_sph_mean_square(int kind, int nr, int J, const int *l_of_j, int lmax,
+                         const double *radius, int kr_st, int kr_ed, const double *f,
+                         double *out);
 
 #ifdef __cplusplus
 }
