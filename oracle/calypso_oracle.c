@@ -415,21 +415,32 @@ void orc_fft_bwd(int L, int nth, int nr, int nphi, int ncomp,
     const double pi = 4.0 * atan(1.0);
     const size_t nnod = (size_t)nr * nth * nphi;
     const size_t sl = (size_t)nth * nr;
+    /* cos(m phi_i), sin(m phi_i) tabulated once: same values, same sums */
+    double *ct = (double *)malloc(sizeof(double) * (size_t)(L + 1) * nphi);
+    double *st = (double *)malloc(sizeof(double) * (size_t)(L + 1) * nphi);
+    for (int i = 0; i < nphi; ++i) {
+        double phi = 2.0 * pi * (double)i / (double)nphi;
+        for (int m = 0; m <= L; ++m) {
+            ct[(size_t)m * nphi + i] = cos((double)m * phi);
+            st[(size_t)m * nphi + i] = sin((double)m * phi);
+        }
+    }
     for (int nd = 0; nd < ncomp; ++nd)
         for (int t = 0; t < nth; ++t)
             for (int k = 0; k < nr; ++k) {
                 const size_t base = (size_t)t + (size_t)nth * k;
                 for (int i = 0; i < nphi; ++i) {
-                    double phi = 2.0 * pi * (double)i / (double)nphi;
                     double x = vr_rtm[nd + (size_t)ncomp * (base + sl * L)];
                     for (int m = 1; m <= L; ++m) {
                         double a = vr_rtm[nd + (size_t)ncomp * (base + sl * (L + m))];
                         double b = vr_rtm[nd + (size_t)ncomp * (base + sl * (L - m))];
-                        x += a * cos((double)m * phi) + b * sin((double)m * phi);
+                        x += a * ct[(size_t)m * nphi + i] + b * st[(size_t)m * nphi + i];
                     }
                     fld_rtp[(size_t)k + (size_t)nr * t + (size_t)nr * nth * i + nnod * nd] = x;
                 }
             }
+    free(ct);
+    free(st);
 }
 
 void orc_fft_fwd(int L, int nth, int nr, int nphi, int ncomp,
@@ -438,6 +449,15 @@ void orc_fft_fwd(int L, int nth, int nr, int nphi, int ncomp,
     const double pi = 4.0 * atan(1.0);
     const size_t nnod = (size_t)nr * nth * nphi;
     const size_t sl = (size_t)nth * nr;
+    double *ct = (double *)malloc(sizeof(double) * (size_t)(L + 1) * nphi);
+    double *st = (double *)malloc(sizeof(double) * (size_t)(L + 1) * nphi);
+    for (int i = 0; i < nphi; ++i) {
+        double phi = 2.0 * pi * (double)i / (double)nphi;
+        for (int m = 0; m <= L; ++m) {
+            ct[(size_t)m * nphi + i] = cos((double)m * phi);
+            st[(size_t)m * nphi + i] = sin((double)m * phi);
+        }
+    }
     for (int nd = 0; nd < ncomp; ++nd)
         for (int t = 0; t < nth; ++t)
             for (int k = 0; k < nr; ++k) {
@@ -446,10 +466,9 @@ void orc_fft_fwd(int L, int nth, int nr, int nphi, int ncomp,
                 for (int m = 0; m <= L; ++m) {
                     double sc = 0.0, ss = 0.0;
                     for (int i = 0; i < nphi; ++i) {
-                        double phi = 2.0 * pi * (double)i / (double)nphi;
                         double xi = x[(size_t)nr * nth * i];
-                        sc += xi * cos((double)m * phi);
-                        ss += xi * sin((double)m * phi);
+                        sc += xi * ct[(size_t)m * nphi + i];
+                        ss += xi * st[(size_t)m * nphi + i];
                     }
                     if (m == 0) {
                         vr_rtm[nd + (size_t)ncomp * (base + sl * L)] = sc / (double)nphi;
@@ -459,6 +478,8 @@ void orc_fft_fwd(int L, int nth, int nr, int nphi, int ncomp,
                     }
                 }
             }
+    free(ct);
+    free(st);
 }
 
 /* Grid-space nonlinear terms of s_cal_nonlinear_sph_MHD

@@ -1,0 +1,938 @@
+"""Test-only CPU restatement of the reference's full MHD time step
+(SPH_analyze_MHD, programs/SPH_MHD/SPH_analyzer_MHD.f90:115-202) for a
+rotating Boussinesq shell with rigid walls, insulating magnetic boundaries,
+fixed-temperature boundaries and the spherical-shell reference temperature --
+the model of the reference's Dynamobench_case1 test.
+
+It is the checker for the GPU step, never the product.  Each routine cites the
+reference routine it restates; numpy arrays are indexed [k, j] with k the
+radial point (0-based) and j the rj mode, j = l(l+1) + m.
+
+Composition, free-slip / rotating-inner-core walls, the centre point, and the
+diagnostic-only fields (pressure, grad T, viscous diffusion of u) are not part
+of this model and are not restated.
+"""
+from __future__ import annotations
+
+import math
+from fractions import Fraction
+
+import numpy as np
+
+from . import (Grid, fft_bwd, fft_fwd, leg_bwd, leg_fwd, ludcmp, lubksb,
+               products_mhd, rlm_index, sph_mean_square, sph_normalizations)
+
+
+# --------------------------------------------------------------- helpers --
+
+def _det22(a11, a12, a21, a22):
+    """cal_det_22_matrix (cal_inverse_small_matrix.f90): a11 a22 - a21 a12."""
+    return a11 * a22 - a21 * a12
+
+
+def inverse33(a):
+    """cal_inverse_33_matrix (SERIAL_src/BASE/cal_inverse_small_matrix.f90):
+    cofactor expansion along the first row, adjugate / det.  a is a 3x3 list
+    of rows (a[i][j] = Fortran a(i+1, j+1))."""
+    det = 0.0
+    for k in range(3):
+        cols = [c for c in range(3) if c != k]
+        d = _det22(a[1][cols[0]], a[1][cols[1]], a[2][cols[0]], a[2][cols[1]])
+        det = det + (-1) ** k * d * a[0][k]
+    a_det = 1.0 / det
+    inv = [[0.0] * 3 for _ in range(3)]
+    for kj in range(3):
+        for ki in range(3):
+            rows = [r for r in range(3) if r != ki]
+            cols = [c for c in range(3) if c != kj]
+            d = _det22(a[rows[0]][cols[0]], a[rows[0]][cols[1]],
+                       a[rows[1]][cols[0]], a[rows[1]][cols[1]])
+            inv[kj][ki] = (-1) ** (ki + kj) * d * a_det
+    return inv
+
+
+def chebyshev_radius(nri, r_icb, r_cmb):
+    """set_chebyshev_distance_shell with no points outside the shell
+    (SERIAL_src/SPH_SPECTR_src/set_radial_grid_sph_shell.f90:30-70)."""
+    pi = 4.0 * math.atan(1.0)
+    shell = r_cmb - r_icb
+    return np.array([r_icb + 0.5 * shell * (1.0 - math.cos(pi * float(k) / float(nri)))
+                     for k in range(nri + 1)])
+
+
+def fdm_coefs(radius):
+    """d1nod_mat_fdm_2 / d2nod_mat_fdm_2: 3-point Taylor inverse at every
+    point (set_dr_for_nonequi, nod_r_2nd_fdm_coefs_nonequi,
+    SERIAL_src/SPH_SPECTR_src/set_radius_func_noequi.f90; copy order
+    copy_fdm_nod_coefs_from_mat, m_fdm_coefs.f90).  Returns (d1, d2), each
+    (nr, 3) for offsets -1, 0, +1."""
+    nr = radius.size
+    dr = np.zeros(nr)
+    for kr in range(1, nr - 1):
+        dr[kr] = radius[kr + 1] - radius[kr]
+    dr[nr - 1] = radius[nr - 1] - radius[nr - 2]
+    dr[0] = radius[1] - radius[0]
+    d1 = np.zeros((nr, 3))
+    d2 = np.zeros((nr, 3))
+    for k in range(nr):
+        if k == 0:
+            dp, dn = dr[0], dr[0]
+        elif k == nr - 1:
+            dp, dn = dr[k], dr[k]
+        else:
+            dp, dn = dr[k], dr[k - 1]
+        m = inverse33([[1.0, 0.0, 0.0],
+                       [1.0, dp, 0.5 * dp * dp],
+                       [1.0, -dn, 0.5 * dn * dn]])
+        d1[k] = (m[1][2], m[1][0], m[1][1])
+        d2[k] = (m[2][2], m[2][0], m[2][1])
+    return d1, d2
+
+
+class BoundaryFdm:
+    """cal_fdm_coefs_4_BCs (MHD_src/sph_MHD/t_boundary_params_sph_MHD.f90) with
+    the one-sided formulas of cal_fdm_coefs_4_boundaries.f90."""
+
+    def __init__(self, radius, kr_in, kr_out):
+        r = radius
+        i0 = kr_in - 1
+        dp1, dp2 = r[i0 + 1] - r[i0], r[i0 + 2] - r[i0]
+        m = inverse33([[1.0, 0.0, 0.0], [1.0, dp1, dp1 * dp1 / 2.0],
+                       [1.0, dp2, dp2 * dp2 / 2.0]])
+        # fix_fld_icb[i][c]: point kr_in+i, c = 0 value, 1 d/dr, 2 d2/dr2
+        self.fix_fld_icb = [[m[c][i] for c in range(3)] for i in range(3)]
+        m = inverse33([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [dp1, 1.0, dp1 * dp1 / 2.0]])
+        # fix_dr_icb[i+1][c]: i = -1 the given d/dr, 0 point kr_in, 1 kr_in+1
+        self.fix_dr_icb = [[m[c][i] for c in range(3)] for i in range(3)]
+        o = kr_out - 1
+        dn1, dn2 = r[o] - r[o - 1], r[o] - r[o - 2]
+        m = inverse33([[1.0, 0.0, 0.0], [1.0, -dn1, dn1 * dn1 / 2.0],
+                       [1.0, -dn2, dn2 * dn2 / 2.0]])
+        # fix_fld_cmb[i][c]: point kr_out-i
+        self.fix_fld_cmb = [[m[c][i] for c in range(3)] for i in range(3)]
+        m = inverse33([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [1.0, -dn1, dn1 * dn1 / 2.0]])
+        # fix_dr_cmb[i+1][c]: i = -1 point kr_out-1, 0 point kr_out, 1 given d/dr
+        self.fix_dr_cmb = [[m[c][2] for c in range(3)], [m[c][0] for c in range(3)],
+                           [m[c][1] for c in range(3)]]
+        self.r_icb = (r[i0], 1.0 / r[i0], (1.0 / r[i0]) ** 2)
+        self.r_cmb = (r[o], 1.0 / r[o], (1.0 / r[o]) ** 2)
+
+
+# ------------------------------------------------------------ Gaunt ints --
+
+def _fact(n):
+    return math.factorial(n)
+
+
+def gaunt_gs(m1, l1, m2, l2, m3, l3):
+    """Adams-Gaunt integral of three Schmidt harmonics, exact rational form of
+    gs_mul_fact / gs_prime (SERIAL_src/spherical_harmonics/normgi_by_fact.f90,
+    normgi_by_primes.f90): the same closed form, products of factorials."""
+    s = (l1 + l2 + l3) // 2
+    itmax = min(l1 - m1, l3 - m3, l2 + l3 - m1)
+    itmin = -min(l2 - l3 + m1, l1 + m1, 0)
+    pre = Fraction(_fact(2 * s - 2 * l3) * _fact(s),
+                   _fact(s - l1) * _fact(s - l2) * _fact(s - l3) * _fact(2 * s + 1))
+    sq = Fraction(_fact(l1 - m1) * _fact(l2 + m2) * _fact(l3 + m3) * _fact(l3 - m3),
+                  _fact(l1 + m1) * _fact(l2 - m2))
+    tsum = Fraction(0)
+    for it in range(itmin, itmax + 1):
+        tsum += Fraction((-1) ** it * _fact(l1 + m1 + it) * _fact(l2 + l3 - m1 - it),
+                         _fact(l1 - m1 - it) * _fact(l2 - l3 + m1 + it)
+                         * _fact(l3 - m3 - it) * _fact(it))
+    val = float((-1) ** (s - l2 - m3) * 2 * pre * tsum) * math.sqrt(float(sq))
+    for m in (m1, m2, m3):
+        if m != 0:
+            val *= math.sqrt(2.0)
+    return val
+
+
+def _normki(gi, m1, m2, m3):
+    if m1 * m2 * m3 > 0:
+        return -gi * 0.5 if (m2 < 0 and m3 < 0) else gi * 0.5
+    if m2 == 0 and m1 * m3 > 0:
+        return gi
+    if m3 == 0 and m2 * m1 > 0:
+        return gi
+    if m3 == 0 and m2 == 0 and m1 == 0:
+        return 2.0 * gi
+    return 0.0
+
+
+def leadki(l1, m1, l2, m2, l3, m3):
+    """leadki (SERIAL_src/spherical_harmonics/cal_gaunt_itgs.f90)."""
+    if not (l1 + l2 >= l3 and l2 + l3 >= l1 and l3 + l1 >= l2):
+        return 0.0
+    if (l1 + l2 + l3) % 2 != 0:
+        return 0.0
+    if not (m1 + m2 + m3 == 0 or m1 + m2 - m3 == 0 or m1 - m2 + m3 == 0 or -m1 + m2 + m3 == 0):
+        return 0.0
+    a1, a2, a3 = abs(m1), abs(m2), abs(m3)
+    if a1 == a2 + a3:
+        return _normki(gaunt_gs(a1, l1, a2, l2, a3, l3), m1, m2, m3)
+    if a2 == a1 + a3:
+        return _normki(gaunt_gs(a2, l2, a1, l1, a3, l3), m2, m1, m3)
+    if a3 == a2 + a1:
+        return _normki(gaunt_gs(a3, l3, a2, l2, a1, l1), m3, m2, m1)
+    return 0.0
+
+
+def _leades(m2, l2, m3, l3, gg):
+    a1 = float((l2 + m2) * (l3 - m3))
+    a2 = float((l3 + m3) * (l2 - m2))
+    a3 = float((l2 + m2 - 1) * (l3 + m3 + 1))
+    a4 = float((l3 + m3 - 1) * (l2 + m2 + 1))
+    a5 = float((l2 - m2) * (l3 - m3))
+    if m2 - 1 == 0:
+        a3 *= 2.0
+    if m3 == 0:
+        a3 *= 0.5
+        a4 *= 0.5
+    if m2 == 0:
+        a3 *= 0.5
+        a4 *= 0.5
+    if m3 - 1 == 0:
+        a4 *= 2.0
+    return -(math.sqrt(a1) * (math.sqrt(a3) * gg[0] + math.sqrt(a5) * gg[1])
+             - math.sqrt(a2) * (math.sqrt(a4) * gg[2] + math.sqrt(a5) * gg[3])) * 0.5
+
+
+def _setei(l1, m1, l2, m2, l3, m3):
+    """setei (SERIAL_src/spherical_harmonics/normei.f90)."""
+    if l2 == l3 and m2 == m3:
+        return 0.0
+    gg = [0.0] * 4
+    if m3 + 1 > l3 or abs(m2 - 1) > l2 - 1:
+        gg[0] = 0.0
+    elif m2 == 0:
+        gg[0] = -gaunt_gs(m3 + 1, l3, 1, l2 - 1, m1, l1)
+    else:
+        gg[0] = gaunt_gs(m1, l1, m2 - 1, l2 - 1, m3 + 1, l3)
+    gg[1] = 0.0 if m2 > l2 - 1 else gaunt_gs(m1, l1, m2, l2 - 1, m3, l3)
+    if m2 + 1 > l2 or abs(m3 - 1) > l3 - 1:
+        gg[2] = 0.0
+    elif m3 == 0:
+        gg[2] = -gaunt_gs(m2 + 1, l2, 1, l3 - 1, m1, l1)
+    else:
+        gg[2] = gaunt_gs(m1, l1, m2 + 1, l2, m3 - 1, l3 - 1)
+    gg[3] = 0.0 if m3 > l3 - 1 else gaunt_gs(m1, l1, m2, l2, m3, l3 - 1)
+    return _leades(m2, l2, m3, l3, gg)
+
+
+def _normli(ei, m1, m2, m3, mm1):
+    if m1 * m2 * m3 < 0:
+        return ei * 0.5 if (m3 > 0 and m2 > 0) else -ei * 0.5
+    if m2 == 0 and m1 * m3 < 0:
+        return -float(int(m1 / mm1)) * ei
+    if m3 == 0 and m2 * m1 < 0:
+        return -float(int(m1 / mm1)) * ei
+    return 0.0
+
+
+def leadli(l1, m1, l2, m2, l3, m3):
+    """leadli, the Elsasser integral (cal_gaunt_itgs.f90)."""
+    if not (l1 + l2 >= l3 and l2 + l3 >= l1 and l3 + l1 >= l2):
+        return 0.0
+    if (l1 + l2 + l3) % 2 != 1:
+        return 0.0
+    if not (m1 + m2 + m3 == 0 or m1 + m2 - m3 == 0 or m1 - m2 + m3 == 0 or -m1 + m2 + m3 == 0):
+        return 0.0
+    a1, a2, a3 = abs(m1), abs(m2), abs(m3)
+    if a1 == a2 + a3:
+        return _normli(_setei(l1, a1, l2, a2, l3, a3), m1, m2, m3, a1)
+    if a2 == a1 + a3:
+        return -_normli(_setei(l2, a2, l1, a1, l3, a3), m2, m1, m3, a2)
+    if a3 == a2 + a1:
+        return -_normli(_setei(l3, a3, l2, a2, l1, a1), m3, m2, m1, a3)
+    return 0.0
+
+
+class CoriolisRlm:
+    """Coupling coefficients of the curl of the Coriolis force:
+    cal_gaunt_coriolis_rlm / interact_rot_coriolis_rlm
+    (MHD_src/sph_MHD/interact_coriolis_rlm.f90), indexed by rj mode
+    (the coupling only pairs (l+-1, m) and (l, -m) of one rj mode)."""
+
+    def __init__(self, L):
+        J = (L + 1) ** 2
+        self.jk1 = np.zeros(J, dtype=np.int64)   # (l-1, m)
+        self.jk2 = np.zeros(J, dtype=np.int64)   # (l+1, m)
+        self.jl1 = np.zeros(J, dtype=np.int64)   # (l, -m)
+        gi = np.zeros((J, 2))
+        ei = np.zeros(J)
+        for j in range(J):
+            l3 = int(math.isqrt(j))
+            m3 = j - l3 * (l3 + 1)
+            l2 = l3 - 1
+            if 1 <= l2 <= L and abs(m3) <= l2:
+                gi[j, 0] = leadki(1, 0, l2, m3, l3, m3)
+                k1 = (l2, m3)
+            else:
+                k1 = (l3, m3)
+            l2 = l3 + 1
+            if 1 <= l2 <= L and abs(m3) <= l2:
+                gi[j, 1] = leadki(1, 0, l2, m3, l3, m3)
+                k2 = (l2, m3)
+            else:
+                k2 = (l3, m3)
+            e1 = (l3, m3)
+            if l3 >= 1:
+                ei[j] = leadli(1, 0, l3, -m3, l3, m3)
+                e1 = (l3, -m3)
+            if j == 0:
+                k1 = k2 = e1 = (0, 0)
+                gi[j, 1] = 0.0
+                ei[j] = 0.0
+            self.jk1[j] = k1[0] * (k1[0] + 1) + k1[1]
+            self.jk2[j] = k2[0] * (k2[0] + 1) + k2[1]
+            self.jl1[j] = e1[0] * (e1[0] + 1) + e1[1]
+        g = np.array([sph_normalizations(int(math.isqrt(j)), j - int(math.isqrt(j)) *
+                                         (int(math.isqrt(j)) + 1)) for j in range(J)])
+        g[0, 2] = 0.5                                   # g_sph_rlm(3) at l = m = 0
+        g3, g4, g5, g17 = g[:, 2], g[:, 3], g[:, 4], g[:, 16]
+        self.sw11 = (2.0 - g4 - g3) * gi[:, 0] * g17
+        self.sw21 = (2.0 - g5 - g3) * gi[:, 1] * g17
+        self.sw12 = g4 * (2.0 - g4 + g3) * gi[:, 0] * g17 * 0.5
+        self.sw22 = g5 * (2.0 - g5 + g3) * gi[:, 1] * g17 * 0.5
+        self.sw13 = 2.0 * ei * g17
+        self.tw13 = -(g3 * (2.0 + g4 - g3) + 2.0 * (-2.0 + g4 + g3)) * gi[:, 0] * g17 * 0.5
+        self.tw23 = -(g3 * (2.0 + g5 - g3) + 2.0 * (-2.0 + g5 + g3)) * gi[:, 1] * g17 * 0.5
+        self.tw14 = -(-2.0 + g4 + g3) * gi[:, 0] * g17
+        self.tw24 = -(-2.0 + g5 + g3) * gi[:, 1] * g17
+        self.tw11 = -2.0 * ei * g17
+        self.tw12 = -g3 * ei * g17
+        self.g3rlm = g3
+        self.gi, self.ei = gi, ei
+
+
+# ------------------------------------------------------------- the model --
+
+class DynamoBenchmarkCase1:
+    """Parameters of the reference's tests/Dynamobench_case1 (control_MHD,
+    control_sph_shell): L = 31, 48 x 96 grid, 48 Chebyshev intervals,
+    r_i/r_o = 0.35 with r_o - r_i = 1, E = 1e-3, modified Ra = 100, Pr = 1,
+    Pm = 5, Crank-Nicolson weight 0.6, dt = 5e-5."""
+    L, nth, nphi, nri = 31, 48, 96, 48
+    ratio, shell = 0.35, 1.0
+    dt = 5.0e-5
+    coef_imp = 0.6
+    Pr, Ra_mod, E, Pm = 1.0, 1.0e2, 1.0e-3, 5.0
+    depth_high_t, depth_low_t = 0.5384615384615384, 1.5384615384615384
+    high_temp, low_temp = 1.0, 0.0
+    isig = 400
+
+
+class MhdStep:
+    """State d_rj and the reference's step sequence (see module docstring)."""
+
+    def __init__(self, case=DynamoBenchmarkCase1):
+        c = case
+        self.c = c
+        L = self.L = c.L
+        self.J = J = (L + 1) ** 2
+        r_cmb = c.shell / (1.0 - c.ratio)
+        r_icb = r_cmb * c.ratio
+        self.r_icb, self.r_cmb = r_icb, r_cmb
+        self.radius = chebyshev_radius(c.nri, r_icb, r_cmb)
+        nr = self.nr = self.radius.size
+        self.kr_in, self.kr_out = 1, nr
+        self.ar1 = 1.0 / self.radius
+        self.ar2 = self.ar1 ** 2
+        self.d1, self.d2 = fdm_coefs(self.radius)
+        self.bc = BoundaryFdm(self.radius, self.kr_in, self.kr_out)
+        # rj mode degrees and normalisations (copy_sph_normalization_2_rj: no
+        # l = 0 override in rj)
+        self.l_of_j = np.array([int(math.isqrt(j)) for j in range(J)])
+        self.m_of_j = np.array([j - l * (l + 1) for j, l in enumerate(self.l_of_j)])
+        self.g1 = self.l_of_j.astype(np.float64)
+        self.g3 = (self.l_of_j * (self.l_of_j + 1)).astype(np.float64)
+        # coefficients (material_property.f90, construct_MHD_coefficient.f90)
+        self.coef_temp = 1.0
+        self.coef_d_temp = 1.0 * c.Pr ** -1.0
+        self.coef_velo = 1.0
+        self.coef_d_velo = 1.0
+        self.coef_buo = 1.0 * (1.0 - 0.35) * c.Ra_mod ** 1.0 * c.E ** -1.0
+        self.coef_cor = 1.0 * 2.0 ** 1.0 * c.E ** -1.0
+        self.coef_lor = 1.0 * c.Pm ** -1.0 * c.E ** -1.0
+        self.coef_magne = 1.0
+        self.coef_d_magne = 1.0 * c.Pm ** -1.0
+        self.coef_induct = 1.0
+        self.coef_imp_v = self.coef_imp_t = self.coef_imp_b = c.coef_imp
+        self.coef_exp_v = 1.0 - c.coef_imp
+        self.coef_exp_t = 1.0 - c.coef_imp
+        self.coef_exp_b = 1.0 - c.coef_imp
+        self.dt = c.dt
+        # reference temperature (set_ref_temp_sph_mhd, set_reference_sph_mhd.f90)
+        dh, dl, hi, lo = c.depth_high_t, c.depth_low_t, c.high_temp, c.low_temp
+        self.reft0 = (dh * dl * self.ar1 * (hi - lo) - dh * hi + dl * lo) / (dl - dh)
+        self.reft1 = -dh * dl * self.ar2 * (hi - lo) / (dl - dh)
+        # temperature boundary values after adjust_sph_temp_bc_by_reftemp
+        self.T_icb = np.zeros(J)
+        self.T_cmb = np.zeros(J)
+        self.T_icb[0] = hi - self.reft0[self.kr_in - 1]
+        self.T_cmb[0] = lo - self.reft0[self.kr_out - 1]
+        # transform geometry and Coriolis couplings
+        self.grid = Grid(L, c.nth, self.radius)
+        jlm, _ = rlm_index(L)
+        self.rj_of_rlm = jlm[:, 0].astype(np.int64)      # rlm j -> rj j
+        self.cor = CoriolisRlm(L)
+        self.f = {}
+        self._matrices()
+
+    # ---------------------------------------------------------- fields --
+    def zeros(self, n=1):
+        return np.zeros((n, self.nr, self.J)) if n > 1 else np.zeros((self.nr, self.J))
+
+    # --------------------------------------------------------- matrices --
+    def _matrices(self):
+        """const_radial_mat_vort_2step, const_radial_mat_4_temp_sph,
+        const_radial_mat_4_magne_sph (const_r_mat_4_vector_sph.f90,
+        const_r_mat_4_scalar_sph.f90) for rigid walls, insulating and
+        fixed-temperature boundaries; band storage mat3[j, k, 0..2] =
+        Fortran mat3(1..3, k, j)."""
+        nr, J = self.nr, self.J
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        bc = self.bc
+        d1, d2, ar1, ar2, g3 = self.d1, self.d2, self.ar1, self.ar2, self.g3
+
+        def unit_evo():
+            m = np.zeros((J, nr, 3))
+            m[:, :, 1] = 1.0
+            return m
+
+        def unit_poisson():
+            m = np.zeros((J, nr, 3))
+            m[:, :ki, 1] = 1.0
+            m[:, ko + 1:, 1] = 1.0
+            return m
+
+        def add_vector_poisson(m, coef):
+            for k in range(ki + 1, ko):
+                m[:, k - 1, 2] = m[:, k - 1, 2] - coef * d2[k, 0]
+                m[:, k, 1] = m[:, k, 1] - coef * (d2[k, 1] - g3 * ar2[k])
+                m[:, k + 1, 0] = m[:, k + 1, 0] - coef * d2[k, 2]
+
+        def add_scalar_poisson(m, coef):
+            for k in range(ki + 1, ko):
+                m[:, k - 1, 2] = m[:, k - 1, 2] - coef * (d2[k, 0] + 2.0 * ar1[k] * d1[k, 0])
+                m[:, k, 1] = m[:, k, 1] - coef * (d2[k, 1] + 2.0 * ar1[k] * d1[k, 1]
+                                                  - g3 * ar2[k])
+                m[:, k + 1, 0] = m[:, k + 1, 0] - coef * (d2[k, 2] + 2.0 * ar1[k] * d1[k, 2])
+
+        def add_fix_flux_icb(m, coef):
+            m[:, ki, 1] = m[:, ki, 1] - coef * (bc.fix_dr_icb[1][2] - g3 * bc.r_icb[2])
+            m[:, ki + 1, 0] = m[:, ki + 1, 0] - coef * bc.fix_dr_icb[2][2]
+
+        def add_fix_flux_cmb(m, coef):
+            m[:, ko - 1, 2] = m[:, ko - 1, 2] - coef * bc.fix_dr_cmb[0][2]
+            m[:, ko, 1] = m[:, ko, 1] - coef * (bc.fix_dr_cmb[1][2] - g3 * bc.r_cmb[2])
+
+        def set_fix_fld_icb(m):
+            m[:, ki, 1] = 1.0
+            m[:, ki + 1, 0] = 0.0
+
+        def set_fix_fld_cmb(m):
+            m[:, ko - 1, 2] = 0.0
+            m[:, ko, 1] = 1.0
+
+        # velocity (vorticity two-step)
+        coef_dvt = self.coef_imp_v * self.coef_d_velo * self.dt
+        vt, wt = unit_evo(), unit_evo()
+        vs = unit_poisson()
+        add_vector_poisson(vt, coef_dvt)
+        add_vector_poisson(wt, coef_dvt)
+        add_vector_poisson(vs, 1.0)
+        add_fix_flux_icb(wt, coef_dvt)
+        set_fix_fld_icb(vt)
+        add_fix_flux_icb(vs, 1.0)
+        add_fix_flux_cmb(wt, coef_dvt)
+        set_fix_fld_cmb(vt)
+        add_fix_flux_cmb(vs, 1.0)
+        vp = self._mat_product_3band(wt, vs)
+        # temperature
+        coef = self.coef_imp_t * self.coef_d_temp * self.dt
+        tm = unit_evo()
+        add_scalar_poisson(tm, coef)
+        set_fix_fld_icb(tm)
+        set_fix_fld_cmb(tm)
+        # magnetic field
+        coef_dbt = self.coef_imp_b * self.coef_d_magne * self.dt
+        bs, bt = unit_evo(), unit_evo()
+        add_vector_poisson(bs, coef_dbt)
+        add_vector_poisson(bt, coef_dbt)
+        g1 = self.g1
+        bs[:, ki, 1] = bs[:, ki, 1] - coef_dbt * (bc.fix_dr_icb[1][2] - g3 * bc.r_icb[2]
+                                                  + (g1 + 1.0) * bc.r_icb[1] * bc.fix_dr_icb[0][2])
+        bs[:, ki + 1, 0] = bs[:, ki + 1, 0] - coef_dbt * bc.fix_dr_icb[2][2]
+        set_fix_fld_icb(bt)
+        bs[:, ko - 1, 2] = bs[:, ko - 1, 2] - coef_dbt * bc.fix_dr_cmb[0][2]
+        bs[:, ko, 1] = bs[:, ko, 1] - coef_dbt * (bc.fix_dr_cmb[1][2] - g3 * bc.r_cmb[2]
+                                                  - g1 * bc.r_cmb[1] * bc.fix_dr_cmb[2][2])
+        set_fix_fld_cmb(bt)
+        self.mats = {"vp": vp, "vt": vt, "temp": tm, "bs": bs, "bt": bt}
+        self.lu = {}
+        for name, m in self.mats.items():
+            nb = 5 if name == "vp" else 3
+            self.lu[name] = ludcmp(nb, m)[:2]
+
+    def _mat_product_3band(self, a_l, a_r):
+        """cal_mat_product_3band_mul (SOLVER_src/.../mat_product_3band_mul.f90)
+        into 5-band storage p[j, k, 0..4] = Fortran a_prod(1..5, k, j)."""
+        nr, J = self.nr, self.J
+        st, ed = self.kr_in - 1, self.kr_out - 1
+        L_, R_ = a_l, a_r
+        p = np.zeros((J, nr, 5))
+
+        def rows(k, skip5=False, skip1=False):
+            if not skip5:
+                p[:, k - 2, 4] = L_[:, k - 1, 2] * R_[:, k - 2, 2]
+            p[:, k - 1, 3] = L_[:, k - 1, 2] * R_[:, k - 1, 1] + L_[:, k, 1] * R_[:, k - 1, 2]
+            p[:, k, 2] = (L_[:, k - 1, 2] * R_[:, k, 0] + L_[:, k, 1] * R_[:, k, 1]
+                          + L_[:, k + 1, 0] * R_[:, k, 2])
+            p[:, k + 1, 1] = L_[:, k, 1] * R_[:, k + 1, 0] + L_[:, k + 1, 0] * R_[:, k + 1, 1]
+            if not skip1:
+                p[:, k + 2, 0] = L_[:, k + 1, 0] * R_[:, k + 2, 0]
+
+        for k in range(st + 2, ed - 1):
+            rows(k)
+        p[:, st, 2] = 1.0
+        p[:, st + 1, 1] = 0.0
+        p[:, st + 2, 0] = 0.0
+        rows(st + 1, skip5=True)
+        rows(ed - 1, skip1=True)
+        p[:, ed - 2, 4] = 0.0
+        p[:, ed - 1, 3] = 0.0
+        p[:, ed, 2] = 1.0
+        return p
+
+    def _solve(self, name, x):
+        nb = 5 if name == "vp" else 3
+        lu, piv = self.lu[name]
+        return lubksb(nb, lu, piv, x)
+
+    # ------------------------------------------------- radial operators --
+    def _interior(self):
+        return range(self.kr_in, self.kr_out - 1)   # 0-based k = kr_in+1..kr_out-1
+
+    def _dr(self, f, k, which=1):
+        d = self.d1 if which == 1 else self.d2
+        return d[k, 0] * f[k - 1] + d[k, 1] * f[k] + d[k, 2] * f[k + 1]
+
+    def grad_vp_and_vorticity(self):
+        """const_grad_vp_and_vorticity for rigid walls: cal_sph_nod_icb_rigid_velo2,
+        cal_sph_nod_icb_rigid_rot2, cal_sph_nod_cmb_rigid_v_and_w
+        (set_sph_exp_rigid_ICB.f90 / _CMB.f90), cal_sph_diff_pol_and_rot2
+        (cal_sph_exp_rotation.f90)."""
+        u, w = self.f["velo"], self.f["vort"]
+        bc, g3 = self.bc, self.g3
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        u[0, :ki + 1] = 0.0
+        u[1, :ki + 1] = 0.0
+        u[2, :ki + 1] = 0.0          # vt_ICB_bc = 0
+        fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
+        d2s = fdr[1][2] * u[0, ki] + fdr[2][2] * u[0, ki + 1]
+        d1t = ffl[0][1] * u[2, ki] + ffl[1][1] * u[2, ki + 1] + ffl[2][1] * u[2, ki + 2]
+        w[0, ki] = u[2, ki]
+        w[1, ki] = d1t
+        w[2, ki] = -(d2s - g3 * bc.r_icb[2] * u[0, ki])
+        u[0, ko] = 0.0
+        u[1, ko] = 0.0
+        u[2, ko] = 0.0
+        fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
+        d2s = fdr[0][2] * u[0, ko - 1]
+        d1t = ffl[2][1] * u[2, ko - 2] + ffl[1][1] * u[2, ko - 1] + ffl[0][1] * u[2, ko]
+        w[0, ko] = u[2, ko]
+        w[1, ko] = d1t
+        w[2, ko] = -d2s
+        self._diff_pol_and_rot(u, w)
+
+    def _diff_pol_and_rot(self, f, rot):
+        g3 = self.g3
+        for k in self._interior():
+            d1s = self._dr(f[0], k, 1)
+            d2s = self._dr(f[0], k, 2)
+            d1t = self._dr(f[2], k, 1)
+            f[1, k] = d1s
+            rot[0, k] = f[2, k]
+            rot[1, k] = d1t
+            rot[2, k] = -d2s + g3 * self.ar2[k] * f[0, k]
+
+    def grad_bp_and_current(self):
+        """const_grad_bp_and_current with insulating boundaries:
+        cal_sph_nod_icb_ins_b_and_j / cal_sph_nod_cmb_ins_b_and_j
+        (cal_sph_exp_nod_icb_ins.f90, _cmb_ins.f90) then
+        cal_sph_diff_pol_and_rot2 (no points outside the shell to extend)."""
+        b, jc = self.f["magne"], self.f["current"]
+        bc, g1, g3 = self.bc, self.g1, self.g3
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
+        d1s = (g1 + 1.0) * bc.r_icb[1] * b[0, ki]
+        d2s = (((g1 + 1.0) * bc.r_icb[1] * fdr[0][2] + fdr[1][2]) * b[0, ki]
+               + fdr[2][2] * b[0, ki + 1])
+        d1t = ffl[1][1] * b[2, ki + 1] + ffl[2][1] * b[2, ki + 2]
+        b[1, ki] = d1s
+        b[2, ki] = 0.0
+        jc[0, ki] = 0.0
+        jc[1, ki] = d1t
+        jc[2, ki] = -(d2s - g3 * bc.r_icb[2] * b[0, ki])
+        fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
+        d1s = -g1 * bc.r_cmb[1] * b[0, ko]
+        d2s = (fdr[0][2] * b[0, ko - 1]
+               + (fdr[1][2] - g1 * bc.r_cmb[1] * fdr[2][2]) * b[0, ko])
+        d1t = ffl[2][1] * b[2, ko - 2] + ffl[1][1] * b[2, ko - 1]
+        b[1, ko] = d1s
+        b[2, ko] = 0.0
+        jc[0, ko] = 0.0
+        jc[1, ko] = d1t
+        jc[2, ko] = -(d2s - g3 * bc.r_cmb[2] * b[0, ko])
+        self._diff_pol_and_rot(b, jc)
+
+    def vorticity_diffusion(self):
+        """const_sph_vorticirty_diffusion (const_sph_diffusion.f90) with
+        cal_sph_nod_icb_rgd_w_diffuse2 / cal_sph_nod_cmb_rgd_w_diffuse2; the
+        d/dr component is not used by the step and is not formed."""
+        w, out = self.f["vort"], self.f["w_diffuse"]
+        coef, g3, bc = self.coef_d_velo, self.g3, self.bc
+        for k in self._interior():
+            d2s = self._dr(w[0], k, 2)
+            d2t = self._dr(w[2], k, 2)
+            out[0, k] = coef * (d2s - g3 * self.ar2[k] * w[0, k])
+            out[2, k] = coef * (d2t - g3 * self.ar2[k] * w[2, k])
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        ffl = bc.fix_fld_icb
+        for c in (0, 2):
+            d2 = ffl[0][2] * w[c, ki] + ffl[1][2] * w[c, ki + 1] + ffl[2][2] * w[c, ki + 2]
+            out[c, ki] = coef * (d2 - g3 * bc.r_icb[2] * w[c, ki])
+        ffl = bc.fix_fld_cmb
+        for c in (0, 2):
+            d2 = ffl[2][2] * w[c, ko - 2] + ffl[1][2] * w[c, ko - 1] + ffl[0][2] * w[c, ko]
+            out[c, ko] = coef * (d2 - g3 * bc.r_cmb[2] * w[c, ko])
+
+    def magnetic_diffusion(self):
+        """const_sph_mag_diffuse_by_j (const_sph_rotation.f90):
+        cal_sph_nod_diffuse_by_rot2 on the current density, then
+        cal_sph_nod_icb_ins_diffuse2 / cal_sph_nod_cmb_ins_diffuse2 on B."""
+        b, jc, out = self.f["magne"], self.f["current"], self.f["b_diffuse"]
+        coef, g1, g3, bc = self.coef_d_magne, self.g1, self.g3, self.bc
+        for k in self._interior():
+            d2s = self._dr(jc[0], k, 2)
+            d1t = self._dr(jc[2], k, 1)
+            out[0, k] = -coef * jc[2, k]
+            out[1, k] = -coef * d1t
+            out[2, k] = coef * (d2s - g3 * self.ar2[k] * jc[0, k])
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
+        d2s = (((g1 + 1.0) * bc.r_icb[1] * fdr[0][2] + fdr[1][2]) * b[0, ki]
+               + fdr[2][2] * b[0, ki + 1])
+        d2t = ffl[0][2] * b[2, ki] + ffl[1][2] * b[2, ki + 1] + ffl[2][2] * b[2, ki + 2]
+        out[0, ki] = coef * (d2s - g3 * bc.r_icb[2] * b[0, ki])
+        out[2, ki] = coef * (d2t - g3 * bc.r_icb[2] * b[2, ki])
+        fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
+        d2s = (fdr[0][2] * b[0, ko - 1]
+               + (fdr[1][2] - g1 * bc.r_cmb[1] * fdr[2][2]) * b[0, ko])
+        d2t = ffl[2][2] * b[2, ko - 2] + ffl[1][2] * b[2, ko - 1] + ffl[0][2] * b[2, ko]
+        out[0, ko] = coef * (d2s - g3 * bc.r_cmb[2] * b[0, ko])
+        out[2, ko] = coef * (d2t - g3 * bc.r_cmb[2] * b[2, ko])
+
+    def scalar_diffusion(self):
+        """const_sph_scalar_diffusion: cal_sph_nod_scalar_diffuse2
+        (cal_sph_exp_diffusion.f90), cal_sph_fix_scalar_in_diffuse2 /
+        cal_sph_out_fix_scalar_diffuse2 (cal_sph_exp_fixed_scalar.f90)."""
+        t, out = self.f["temp"], self.f["t_diffuse"]
+        coef, g3, bc = self.coef_d_temp, self.g3, self.bc
+        for k in self._interior():
+            d1s = self._dr(t, k, 1)
+            d2s = self._dr(t, k, 2)
+            out[k] = coef * (d2s + 2.0 * self.ar1[k] * d1s - g3 * self.ar2[k] * t[k])
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        ffl = bc.fix_fld_icb
+        d1t = ffl[0][1] * self.T_icb + ffl[1][1] * t[ki + 1] + ffl[2][1] * t[ki + 2]
+        d2t = ffl[0][2] * self.T_icb + ffl[1][2] * t[ki + 1] + ffl[2][2] * t[ki + 2]
+        t[ki] = self.T_icb
+        out[ki] = coef * (d2t + 2.0 * bc.r_icb[1] * d1t - g3 * bc.r_icb[2] * t[ki])
+        ffl = bc.fix_fld_cmb
+        d1t = ffl[2][1] * t[ko - 2] + ffl[1][1] * t[ko - 1] + ffl[0][1] * self.T_cmb
+        d2t = ffl[2][2] * t[ko - 2] + ffl[1][2] * t[ko - 1] + ffl[0][2] * self.T_cmb
+        t[ko] = self.T_cmb
+        out[ko] = coef * (d2t + 2.0 * bc.r_cmb[1] * d1t - g3 * bc.r_cmb[2] * t[ko])
+
+    def rot_buoyancy(self):
+        """cal_rot_buoyancy_sph_MHD (cal_rot_buoyancies_sph_MHD.f90)."""
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        rb = self.f["rot_buoyancy"]
+        rb[ki:ko + 1] = self.coef_buo * self.f["temp"][ki:ko + 1] * self.radius[ki:ko + 1, None]
+
+    def _force_rot(self, src, rot, rigid):
+        """const_sph_force_rot2 / const_sph_rotation_uxb: boundary rows
+        (cal_sph_nod_icb_rigid_rot2 + cal_sph_nod_cmb_rigid_rot2 for the
+        momentum terms, cal_sph_nod_icb_ins_vp_rot2 +
+        cal_sph_nod_cmb_ins_vp_rot2 for the induction) and
+        cal_sph_nod_vect_w_div_rot2 inside."""
+        bc, g1, g3 = self.bc, self.g1, self.g3
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        if rigid:
+            fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
+            d2s = fdr[1][2] * src[0, ki] + fdr[2][2] * src[0, ki + 1]
+            d1t = ffl[0][1] * src[2, ki] + ffl[1][1] * src[2, ki + 1] + ffl[2][1] * src[2, ki + 2]
+            rot[0, ki] = src[2, ki]
+            rot[1, ki] = d1t
+            rot[2, ki] = -(d2s - g3 * bc.r_icb[2] * src[0, ki])
+            fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
+            d2s = fdr[0][2] * src[0, ko - 1] + fdr[1][2] * src[0, ko]
+            d1t = ffl[2][1] * src[2, ko - 2] + ffl[1][1] * src[2, ko - 1] + ffl[0][1] * src[2, ko]
+            rot[0, ko] = src[2, ko]
+            rot[1, ko] = d1t
+            rot[2, ko] = -(d2s - g3 * bc.r_cmb[2] * src[0, ko])
+        else:
+            rot[0, ki] = src[2, ki]
+            rot[1, ki] = (g1 + 1.0) * bc.r_icb[1] * src[2, ki]
+            rot[2, ki] = 0.0
+            rot[0, ko] = src[2, ko]
+            rot[1, ko] = -g1 * bc.r_cmb[1] * src[2, ko]
+            rot[2, ko] = 0.0
+        for k in self._interior():
+            d1d = self._dr(src[1], k, 1)
+            d1t = self._dr(src[2], k, 1)
+            rot[0, k] = src[2, k]
+            rot[1, k] = d1t
+            rot[2, k] = -d1d + g3 * self.ar2[k] * src[0, k]
+
+    def scalar_advect(self):
+        """const_sph_scalar_advect: cal_sph_nod_vect_div2 inside,
+        cal_sph_div_flux_4_fix_in / _out at fixed-value boundaries (the
+        boundary value of the scalar stands in for the flux there, as the
+        reference does)."""
+        flux, out = self.f["h_flux"], self.f["h_advect"]
+        bc, g3 = self.bc, self.g3
+        gg = np.maximum(g3, 0.5)
+        for k in self._interior():
+            d1s = self._dr(flux[0], k, 1)
+            out[k] = (d1s - flux[1, k]) * gg * self.ar2[k]
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        ffl = bc.fix_fld_icb
+        d1s = ffl[0][1] * self.T_icb + ffl[1][1] * flux[0, ki + 1] + ffl[2][1] * flux[0, ki + 2]
+        out[ki] = (d1s - flux[1, ki]) * gg * bc.r_icb[2]
+        ffl = bc.fix_fld_cmb
+        d1s = ffl[2][1] * flux[0, ko - 2] + ffl[1][1] * flux[0, ko - 1] + ffl[0][1] * self.T_cmb
+        out[ko] = (d1s - flux[1, ko]) * gg * bc.r_cmb[2]
+
+    # --------------------------------------------------------- nonlinear --
+    def coriolis(self):
+        """sum_rot_coriolis_rlm_10 (sum_coriolis_terms_rlm.f90:88-165) on the
+        rj fields: omega_rlm = (r^2/2, r, 1) (set_rotatio_spectr)."""
+        u, w, c = self.f["velo"], self.f["vort"], self.cor
+        r = self.radius[:, None]
+        ar = 1.0 / self.radius[:, None]
+        om0, om1, om2 = 0.5 * r ** 2, r, 1.0
+        k1, k2, l1 = c.jk1, c.jk2, c.jl1
+        sp_dvp_k1, sp_dvp_k2 = u[1][:, k1], u[1][:, k2]
+        sp_wp_l1 = w[0][:, l1]
+        sp_vp_k1, sp_vp_k2 = u[0][:, k1], u[0][:, k2]
+        sp_d2vp_l1 = ar * ar * c.g3rlm[None, :] * u[0][:, l1] - w[2][:, l1]
+        sp_vp_l1 = u[0][:, l1]
+        sp_wp_k1, sp_wp_k2 = w[0][:, k1], w[0][:, k2]
+        sp_dwp_k1 = w[1][:, k1] - 2.0 * ar * w[0][:, k1]
+        sp_dwp_k2 = w[1][:, k2] - 2.0 * ar * w[0][:, k2]
+        pol = (c.sw11 * om0 * sp_dvp_k1 + c.sw21 * om0 * sp_dvp_k2 + c.sw13 * om0 * sp_wp_l1
+               + c.sw12 * om1 * sp_vp_k1 + c.sw22 * om1 * sp_vp_k2)
+        tor = (c.tw11 * om0 * sp_d2vp_l1 - c.tw12 * om2 * sp_vp_l1 + c.tw13 * om1 * sp_wp_k1
+               + c.tw23 * om1 * sp_wp_k2 + c.tw14 * om0 * sp_dwp_k1 + c.tw24 * om0 * sp_dwp_k2)
+        pol = -self.coef_cor * pol * ar * ar
+        tor = -self.coef_cor * tor * ar * ar
+        return pol, tor
+
+    def nonlinear(self):
+        """nonlinear (cal_nonlinear.f90): backward transform of u, w, B, J, T,
+        grid products, forward transform, Coriolis in spectral space, curl /
+        divergence of the forces, reference-temperature advection, force sum."""
+        f, J, nr = self.f, self.J, self.nr
+        perm = self.rj_of_rlm
+        sp = np.zeros((nr, J, 13))
+        for c0, name in ((0, "velo"), (3, "vort"), (6, "magne"), (9, "current")):
+            for c in range(3):
+                sp[:, :, c0 + c] = f[name][c][:, perm]
+        sp[:, :, 12] = f["temp"][:, perm]
+        vr = leg_bwd(self.grid, 13, 4, 1, sp.ravel())
+        nphi = self.c.nphi
+        fld = fft_bwd(self.L, self.c.nth, nr, nphi, 13, vr)
+        nnod = nr * self.c.nth * nphi
+        frc = products_mhd(nnod, fld, self.coef_velo, self.coef_lor, self.coef_induct,
+                           self.coef_temp)
+        vr2 = fft_fwd(self.L, self.c.nth, nr, nphi, 19, frc)
+        sp2 = leg_fwd(self.grid, 19, 6, 1, vr2).reshape(nr, J, 19)
+        inv = np.empty(J, dtype=np.int64)
+        inv[perm] = np.arange(J)
+
+        def take(c0):
+            out = np.zeros((3, nr, J))
+            for c in range(3):
+                out[c] = sp2[:, inv, c0 + c]
+            return out
+
+        f["m_advect"] = take(0)
+        rc = take(6)
+        rc[0], rc[2] = self.coriolis()
+        f["rot_Coriolis"] = rc
+        f["lorentz"] = take(9)
+        f["vp_induct"] = take(12)
+        f["h_flux"] = take(15)
+        self._force_rot(f["m_advect"], f["rot_inertia"], rigid=True)
+        self._force_rot(f["lorentz"], f["rot_Lorentz"], rigid=True)
+        self._force_rot(f["vp_induct"], f["induction"], rigid=False)
+        self.scalar_advect()
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        ha = f["h_advect"]
+        for k in range(ki, ko + 1):
+            ha[k] = ha[k] + self.coef_temp * self.g3 * self.ar2[k] * self.reft1[k] * f["velo"][0, k]
+        fo = f["forces"]
+        fo[0] = -f["rot_inertia"][0] + f["rot_Coriolis"][0] + f["rot_Lorentz"][0]
+        fo[2] = (-f["rot_inertia"][2] + f["rot_Coriolis"][2] + f["rot_Lorentz"][2]
+                 + f["rot_buoyancy"])
+
+    # ------------------------------------------------------ initialisation --
+    def initialize(self):
+        """SPH_initialize_MHD: dynamo_benchmark_1 initial fields
+        (set_initial_sph_dynamo.f90, initial_magne_dynamobench.f90), the
+        perturbation temperature, the fields derived from them
+        (set_sph_field_to_start) and the first nonlinear terms."""
+        nr, J = self.nr, self.J
+        f = self.f
+        for name in ("velo", "vort", "magne", "current", "w_diffuse", "b_diffuse",
+                     "forces", "pre_mom", "rot_inertia", "rot_Coriolis", "rot_Lorentz",
+                     "induction", "pre_uxb", "m_advect", "lorentz", "vp_induct", "h_flux"):
+            f[name] = np.zeros((3, nr, J))
+        for name in ("temp", "t_diffuse", "rot_buoyancy", "h_advect", "pre_heat"):
+            f[name] = np.zeros((nr, J))
+        pi = 4.0 * math.atan(1.0)
+        r = self.radius
+        shell = self.r_cmb - self.r_icb
+        f["temp"][:, 0] = self.reft0
+        m = int((self.c.isig % 1000) / 100)
+        jj = m * (m + 1) + m
+        xr = 2.0 * r - 1.0 * (self.r_cmb + self.r_icb) / shell
+        f["temp"][:, jj] = ((1.0 - 3.0 * xr ** 2 + 3.0 * xr ** 4 - xr ** 6)
+                            * 0.1 * 3.0 / math.sqrt(2.0 * pi))
+        js, jt = 2, 6                       # (l, m) = (1, 0), (2, 0)
+        f["magne"][0][:, js] = (5.0 / 8.0) * (-3.0 * r ** 3 + 4.0 * self.r_cmb * r ** 2
+                                              - self.r_icb ** 4 / r)
+        f["magne"][1][:, js] = (5.0 / 8.0) * (-9.0 * r ** 2 + 8.0 * self.r_cmb * r
+                                              + self.r_icb ** 4 / r ** 2)
+        f["current"][2][:, js] = (5.0 * 3.0 / 2.0) * r
+        f["magne"][2][:, jt] = (10.0 / 3.0) * r * np.sin(pi * (r - self.r_icb))
+        f["current"][0][:, jt] = f["magne"][2][:, jt]
+        f["current"][1][:, jt] = (10.0 / 3.0) * (np.sin(pi * (r - self.r_icb))
+                                                 + pi * r * np.cos(pi * (r - self.r_icb)))
+        self.sync_temp_by_per_temp()
+        # set_sph_field_to_start
+        self.grad_vp_and_vorticity()
+        self.vorticity_diffusion()
+        self.rot_buoyancy()
+        self.scalar_diffusion()
+        self.grad_bp_and_current()
+        self.magnetic_diffusion()
+        self.nonlinear()
+        self.i_step = 0
+
+    def sync_temp_by_per_temp(self):
+        self.f["temp"][:, 0] = self.f["temp"][:, 0] - self.reft0
+
+    def trans_per_temp_to_temp(self):
+        self.f["temp"][:, 0] = self.f["temp"][:, 0] + self.reft0
+
+    # --------------------------------------------------------------- step --
+    def explicit(self, euler, first):
+        """cal_expricit_sph_euler / cal_expricit_sph_adams
+        (cal_momentum_eq_explicit.f90:54-128)."""
+        f, dt = self.f, self.dt
+        a0, a1 = 3.0 / 2.0, -1.0 / 2.0
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        sl = slice(ki, ko + 1)
+        w, wd, fo, pm = f["vort"], f["w_diffuse"], f["forces"], f["pre_mom"]
+        b, bd, ind, pu = f["magne"], f["b_diffuse"], f["induction"], f["pre_uxb"]
+        t, td, ha, ph = f["temp"], f["t_diffuse"], f["h_advect"], f["pre_heat"]
+        for c in (0, 2):
+            if euler:
+                w[c, sl] = w[c, sl] + dt * (self.coef_exp_v * wd[c, sl] + fo[c, sl])
+                b[c] = b[c] + dt * (self.coef_exp_b * bd[c] + ind[c])
+            else:
+                w[c, sl] = w[c, sl] + dt * (self.coef_exp_v * wd[c, sl] + a0 * fo[c, sl]
+                                            + a1 * pm[c, sl])
+                pm[c, sl] = fo[c, sl]
+                b[c] = b[c] + dt * (self.coef_exp_b * bd[c] + a0 * ind[c] + a1 * pu[c])
+                pu[c] = ind[c]
+        if euler:
+            t[sl] = t[sl] + dt * (self.coef_exp_t * td[sl] - ha[sl])
+        else:
+            t[sl] = t[sl] + dt * (self.coef_exp_t * td[sl] - a0 * ha[sl] + a1 * ph[sl])
+            ph[sl] = -ha[sl]
+        if first:
+            for c in (0, 2):
+                pm[c] = fo[c]
+                pu[c] = ind[c]
+            ph[sl] = -ha[sl]
+
+    def crank_nicolson(self):
+        """s_cal_sol_sph_MHD_crank (cal_sol_sph_MHD_crank.f90:44-101) with
+        cal_sol_velo_by_vort_sph_crank, cal_sol_temperature_sph_crank,
+        cal_sol_magne_sph_crank (cal_sol_sph_fluid_crank.f90)."""
+        f = self.f
+        u, w = f["velo"], f["vort"]
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        u[0] = w[2].copy()
+        u[2] = w[0].copy()
+        u[0][:, 0] = 0.0
+        u[2][:, 0] = 0.0
+        u[0, :ki + 1] = 0.0
+        u[1, :ki + 1] = 0.0
+        u[2, :ki + 1] = 0.0
+        u[0, ko] = 0.0
+        u[1, ko] = 0.0
+        u[2, ko] = 0.0
+        u[0] = self._solve("vp", u[0])
+        u[2] = self._solve("vt", u[2])
+        self.grad_vp_and_vorticity()
+        # temperature
+        t = f["temp"]
+        t[ko:] = self.T_cmb
+        t[:ki + 1] = self.T_icb
+        f["temp"] = t = self._solve("temp", t)
+        # magnetic field
+        b = f["magne"]
+        b[0][:, 0] = 0.0
+        b[2][:, 0] = 0.0
+        b[1, ki] = (self.g1 + 1.0) * self.bc.r_icb[1] * b[0, ki]
+        b[2, ki] = 0.0
+        b[1, ko] = -self.g1 * self.bc.r_cmb[1] * b[0, ko]
+        b[2, ko] = 0.0
+        b[0] = self._solve("bs", b[0])
+        b[2] = self._solve("bt", b[2])
+        self.grad_bp_and_current()
+        # update_after_*
+        self.vorticity_diffusion()
+        self.rot_buoyancy()
+        self.scalar_diffusion()
+        self.magnetic_diffusion()
+
+    def step(self):
+        """SPH_analyze_MHD for one step; returns nothing, advances i_step."""
+        self.i_step += 1
+        first = self.i_step == 1
+        self.explicit(euler=first, first=first)
+        self.crank_nicolson()
+        self.nonlinear()
+        self.trans_per_temp_to_temp()
+        self._last_energy = None
+        self.sync_temp_by_per_temp()
+
+    def energies(self):
+        """The sph_pwr_volume_s.dat columns K_ene_pol, K_ene_tor, K_ene,
+        temperature, M_ene_pol, M_ene_tor, M_ene over the whole shell, with the
+        full (reference + perturbation) temperature."""
+        f = self.f
+        t = f["temp"].copy()
+        t[:, 0] = t[:, 0] + self.reft0
+        lj = self.l_of_j.astype(np.int32)
+        ke = sph_mean_square(1, self.radius, lj, self.kr_in, self.kr_out, f["velo"])
+        ts = sph_mean_square(2, self.radius, lj, self.kr_in, self.kr_out, t[None])
+        me = sph_mean_square(1, self.radius, lj, self.kr_in, self.kr_out, f["magne"])
+        return np.array([ke[0], ke[1], ke[2], ts[0], me[0], me[1], me[2]])

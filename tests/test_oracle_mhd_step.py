@@ -1,0 +1,78 @@
+"""The oracle's full MHD step against the reference's own Dynamobench_case1
+goldens (sph_pwr_volume_s.dat at steps 50 and 100, compared at the 1e-9
+relative bound of the reference's compare_sph_mean_square.f90:96-122), plus
+closed-form pins of its radial finite differences and Gaunt couplings."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mhd_step
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "dynamobench_case1_energies.json")
+
+
+def test_fdm_coefs_exact_for_quadratics():
+    r = mhd_step.chebyshev_radius(48, 0.35 / 0.65, 1.0 / 0.65)
+    d1, d2 = mhd_step.fdm_coefs(r)
+    f = 3.0 * r ** 2 - 2.0 * r + 0.5
+    for k in range(1, r.size - 1):
+        df = d1[k, 0] * f[k - 1] + d1[k, 1] * f[k] + d1[k, 2] * f[k + 1]
+        ddf = d2[k, 0] * f[k - 1] + d2[k, 1] * f[k] + d2[k, 2] * f[k + 1]
+        assert abs(df - (6.0 * r[k] - 2.0)) < 1e-10
+        assert abs(ddf - 6.0) < 1e-8
+
+
+def test_boundary_fdm_exact_for_quadratics():
+    r = mhd_step.chebyshev_radius(48, 0.35 / 0.65, 1.0 / 0.65)
+    bc = mhd_step.BoundaryFdm(r, 1, r.size)
+    f = lambda x: 3.0 * x ** 2 - 2.0 * x + 0.5
+    df = lambda x: 6.0 * x - 2.0
+    a, b, c = r[0], r[1], r[2]
+    ffl = bc.fix_fld_icb
+    assert abs(ffl[0][1] * f(a) + ffl[1][1] * f(b) + ffl[2][1] * f(c) - df(a)) < 1e-8
+    assert abs(ffl[0][2] * f(a) + ffl[1][2] * f(b) + ffl[2][2] * f(c) - 6.0) < 1e-6
+    fdr = bc.fix_dr_icb          # data: f'(a), f(a), f(b)
+    assert abs(fdr[0][2] * df(a) + fdr[1][2] * f(a) + fdr[2][2] * f(b) - 6.0) < 1e-6
+    a, b, c = r[-1], r[-2], r[-3]
+    ffl = bc.fix_fld_cmb
+    assert abs(ffl[0][1] * f(a) + ffl[1][1] * f(b) + ffl[2][1] * f(c) - df(a)) < 1e-8
+    fdr = bc.fix_dr_cmb          # data: f(b), f(a), f'(a)
+    assert abs(fdr[0][2] * f(b) + fdr[1][2] * f(a) + fdr[2][2] * df(a) - 6.0) < 1e-6
+
+
+def test_gaunt_couplings_match_quadrature():
+    """leadki(1,0,l2,m,l3,m) is the Adams-Gaunt integral of
+    P_1 Y_l2^m Y_l3^m over the sphere divided by 4 pi times a constant; the
+    ratios for different (l, m) must match a direct Gauss quadrature."""
+    nth = 40
+    _, w, colat = oracle.gauss_points(nth)
+    x = np.cos(colat)
+    vals, quad = [], []
+    for l3, m in [(2, 0), (3, 1), (5, 2), (7, 3), (4, 4), (9, 0)]:
+        l2 = l3 - 1
+        if m > l2:
+            continue
+        p2 = np.array([oracle.schmidt_legendres_m(l2, m, t)[0][l2] for t in colat])
+        p3 = np.array([oracle.schmidt_legendres_m(l3, m, t)[0][l3] for t in colat])
+        q = np.sum(w * x * p2 * p3) * (1.0 if m == 0 else 0.5)
+        vals.append(mhd_step.leadki(1, 0, l2, m, l3, m))
+        quad.append(q)
+    ratio = np.array(vals) / np.array(quad)
+    assert np.allclose(ratio, ratio[0], rtol=1e-12)
+
+
+def test_dynamobench_case1_energies():
+    gold = json.load(open(GOLD))
+    s = mhd_step.MhdStep()
+    s.initialize()
+    got = {}
+    for _ in range(100):
+        s.step()
+        if s.i_step in (50, 100):
+            got[str(s.i_step)] = s.energies()
+    for step, want in gold["steps"].items():
+        np.testing.assert_allclose(got[step], np.array(want), rtol=gold["rtol"], atol=0)
