@@ -10,6 +10,7 @@
 
 #include "fft_internal.hpp"
 #include "leg_internal.hpp"
+#include "mhd_internal.hpp"
 #include "radial_internal.hpp"
 
 using calypso::LegContext;
@@ -20,6 +21,10 @@ struct calypso_leg {
 
 struct calypso_fft {
     calypso::FftContext f;
+};
+
+struct calypso_mhd {
+    calypso::MhdContext c;
 };
 
 namespace {
@@ -373,6 +378,119 @@ int calypso_sph_mean_square(int kind, int nr, int J, const int *l_of_j, const do
     CHECK_HIP(hipStreamSynchronize(s));
     if (rc) return fail(err);
     for (int c = 0; c < 3; ++c) out[c] = h[c];
+    return 0;
+}
+
+int calypso_mhd_create(const calypso_mhd_params *pp, const double *radius, void *stream,
+                       calypso_mhd_t **out)
+{
+    if (!pp || !radius || !out) return fail("calypso_mhd_create: null argument");
+    calypso::MhdCoefs co;
+    co.dt = pp->dt;
+    co.coef_imp_v = pp->coef_imp_v;
+    co.coef_imp_t = pp->coef_imp_t;
+    co.coef_imp_b = pp->coef_imp_b;
+    co.coef_velo = pp->coef_velo;
+    co.coef_d_velo = pp->coef_d_velo;
+    co.coef_buo = pp->coef_buo;
+    co.coef_cor = pp->coef_cor;
+    co.coef_lor = pp->coef_lor;
+    co.coef_temp = pp->coef_temp;
+    co.coef_d_temp = pp->coef_d_temp;
+    co.coef_magne = pp->coef_magne;
+    co.coef_d_magne = pp->coef_d_magne;
+    co.coef_induct = pp->coef_induct;
+    co.depth_high_t = pp->depth_high_t;
+    co.depth_low_t = pp->depth_low_t;
+    co.high_temp = pp->high_temp;
+    co.low_temp = pp->low_temp;
+    if (co.dt <= 0.0) return fail("calypso_mhd_create: dt must be positive");
+    if (pp->legendre_mode < 1 || pp->legendre_mode > 3)
+        return fail("calypso_mhd_create: unknown Legendre mode");
+    calypso_mhd *h = new (std::nothrow) calypso_mhd;
+    if (!h) return fail("calypso_mhd_create: out of host memory");
+    std::string err;
+    if (calypso::mhd_create(&h->c, pp->l_truncation, pp->nth, pp->nphi, pp->nr, radius, co,
+                            pp->legendre_mode, (hipStream_t)stream, &err)) {
+        calypso::mhd_free(&h->c);
+        delete h;
+        return fail(err);
+    }
+    *out = h;
+    return 0;
+}
+
+int calypso_mhd_destroy(calypso_mhd_t *mhd)
+{
+    if (!mhd) return 0;
+    calypso::mhd_free(&mhd->c);
+    delete mhd;
+    return 0;
+}
+
+int calypso_mhd_set_field(calypso_mhd_t *mhd, int comp, const double *host)
+{
+    if (!mhd || !host) return fail("calypso_mhd_set_field: null argument");
+    std::string err;
+    if (calypso::mhd_field_io(&mhd->c, comp, const_cast<double *>(host), false, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_get_field(calypso_mhd_t *mhd, int comp, double *host)
+{
+    if (!mhd || !host) return fail("calypso_mhd_get_field: null argument");
+    std::string err;
+    if (calypso::mhd_field_io(&mhd->c, comp, host, true, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_set_dynamobench_ic(calypso_mhd_t *mhd, int isig)
+{
+    if (!mhd) return fail("calypso_mhd_set_dynamobench_ic: null handle");
+    std::string err;
+    if (calypso::mhd_set_dynamobench_ic(&mhd->c, isig, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_initialize(calypso_mhd_t *mhd)
+{
+    if (!mhd) return fail("calypso_mhd_initialize: null handle");
+    std::string err;
+    if (calypso::mhd_initialize(&mhd->c, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_step(calypso_mhd_t *mhd, int nsteps)
+{
+    if (!mhd || nsteps < 0) return fail("calypso_mhd_step: bad argument");
+    std::string err;
+    if (calypso::mhd_steps(&mhd->c, nsteps, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_energies(calypso_mhd_t *mhd, double *out)
+{
+    if (!mhd || !out) return fail("calypso_mhd_energies: null argument");
+    std::string err;
+    if (calypso::mhd_energies(&mhd->c, out, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_info(calypso_mhd_t *mhd, long long *info)
+{
+    if (!mhd || !info) return fail("calypso_mhd_info: null argument");
+    info[0] = mhd->c.i_step;
+    info[1] = mhd->c.nr;
+    info[2] = mhd->c.J;
+    info[3] = calypso::kMhdNumComp;
+    return 0;
+}
+
+int calypso_mhd_coriolis_table(int l_truncation, double *cor, int *idx)
+{
+    if (l_truncation < 1 || !cor || !idx) return fail("calypso_mhd_coriolis_table: bad argument");
+    const int J = (l_truncation + 1) * (l_truncation + 1);
+    calypso::coriolis_table(l_truncation, cor, idx, idx + J, idx + 2 * J);
     return 0;
 }
 

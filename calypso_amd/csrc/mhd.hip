@@ -1,0 +1,1086 @@
+// mhd.hip -- the spectral MHD time step on the GPU (gfx950), FP64.
+//
+// One step follows SPH_analyze_MHD (programs/SPH_MHD/SPH_analyzer_MHD.f90:
+// 115-202) for a rotating Boussinesq shell with rigid walls, insulating
+// magnetic boundaries and fixed-temperature boundaries around the
+// spherical-shell reference temperature:
+//
+//   k_explicit      cal_expricit_sph_euler / _adams
+//                   (cal_momentum_eq_explicit.f90:54-128)
+//   k_cn_rhs        right-hand sides and boundary rows of the Crank-Nicolson
+//                   solves (cal_sol_sph_fluid_crank.f90:53-391)
+//   lubksb x 5      banded solves, poloidal velocity 5-band, the rest 3-band
+//   k_bc_rows       wall rows of grad / curl (set_sph_exp_rigid_*.f90,
+//   k_grad_interior cal_sph_exp_nod_*_ins.f90, cal_sph_exp_rotation.f90)
+//   k_diffusion     diffusion terms and buoyancy (const_sph_diffusion.f90,
+//                   const_sph_rotation.f90, cal_rot_buoyancies_sph_MHD.f90)
+//   k_pack          rj -> rlm, 13 components (u, w, B, J, T)
+//   Legendre bwd -> FFT, grid products, FFT (one pencil kernel) -> Legendre fwd
+//   k_unpack        rlm -> rj of the 19 forward components and the curl of the
+//                   Coriolis force in spectral space
+//                   (sum_coriolis_terms_rlm.f90:88-165)
+//   k_rot_forces    curl of the forces, curl of u x B, divergence of u T with
+//                   the reference-temperature advection, the force sum
+//                   (cal_sph_field_by_rotation.f90, cal_nonlinear.f90)
+//
+// The whole state stays in HBM as d_rj (kMhdNumComp components of nr*J
+// doubles, mode fastest, like the reference's d_rj); one thread per (k, j)
+// node, so every access to a component is coalesced over j.  Pointwise and
+// stencil kernels are HBM-bound; FMA contraction is off so their arithmetic
+// follows the reference's operation order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mhd_internal.hpp"
+#include "radial_internal.hpp"
+
+namespace calypso {
+
+#define HIP_TRY(expr)                                                         \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            if (err) *err = std::string(#expr " failed: ") + hipGetErrorString(e_); \
+            return -1;                                                        \
+        }                                                                     \
+    } while (0)
+
+#pragma clang fp contract(off)
+
+#define FLD(c) (p.d + (size_t)(c) * p.nnod)
+
+__device__ __forceinline__ double dr1(const MhdDev &p, const double *f, int k, size_t i)
+{
+    return p.d1[3 * k + 0] * f[i - p.J] + p.d1[3 * k + 1] * f[i] + p.d1[3 * k + 2] * f[i + p.J];
+}
+
+__device__ __forceinline__ double dr2(const MhdDev &p, const double *f, int k, size_t i)
+{
+    return p.d2[3 * k + 0] * f[i - p.J] + p.d2[3 * k + 1] * f[i] + p.d2[3 * k + 2] * f[i + p.J];
+}
+
+// ---------------------------------------------------------- explicit ----
+__global__ void k_explicit(MhdDev p, int euler, int first)
+{
+    const double a0 = 3.0 / 2.0, a1 = -1.0 / 2.0, dt = p.dt;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / p.J);
+        const bool in_shell = (k >= p.ki && k <= p.ko);
+        for (int c = 0; c <= 2; c += 2) {
+            double *w = FLD(kVort + c), *pm = FLD(kPreMom + c);
+            const double wd = FLD(kWDiffuse + c)[i], fo = FLD(kForces + c)[i];
+            if (in_shell) {
+                if (euler) {
+                    w[i] = w[i] + dt * (p.coef_exp_v * wd + fo);
+                } else {
+                    w[i] = w[i] + dt * (p.coef_exp_v * wd + a0 * fo + a1 * pm[i]);
+                    pm[i] = fo;
+                }
+            }
+            double *b = FLD(kMagne + c), *pu = FLD(kPreUxB + c);
+            const double bd = FLD(kBDiffuse + c)[i], ind = FLD(kInduction + c)[i];
+            if (euler) {
+                b[i] = b[i] + dt * (p.coef_exp_b * bd + ind);
+            } else {
+                b[i] = b[i] + dt * (p.coef_exp_b * bd + a0 * ind + a1 * pu[i]);
+                pu[i] = ind;
+            }
+            if (first) {
+                pm[i] = fo;
+                pu[i] = ind;
+            }
+        }
+        if (in_shell) {
+            double *t = FLD(kTemp), *ph = FLD(kPreHeat);
+            const double td = FLD(kTDiffuse)[i], ha = FLD(kHAdvect)[i];
+            if (euler) {
+                t[i] = t[i] + dt * (p.coef_exp_t * td - ha);
+            } else {
+                t[i] = t[i] + dt * (p.coef_exp_t * td - a0 * ha + a1 * ph[i]);
+                ph[i] = -ha;
+            }
+            if (first) ph[i] = -ha;
+        }
+    }
+}
+
+// ------------------------------------------- Crank-Nicolson right sides --
+// cal_sol_velo_by_vort_sph_crank: u_pol <- w_tor, u_tor <- w_pol,
+// delete_zero_degree_comp, cal_sph_nod_icb/cmb_rigid_velo2;
+// cal_sol_scalar_sph_crank: set_fixed_scalar_sph at both walls;
+// cal_sol_magne_sph_crank: delete_zero_degree_comp,
+// cal_sph_nod_icb/cmb_ins_mag2.
+__global__ void k_cn_rhs(MhdDev p)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        double up = FLD(kVort + 2)[i], ut = FLD(kVort)[i];
+        if (j == 0) {
+            up = 0.0;
+            ut = 0.0;
+        }
+        if (k <= p.ki || k == p.ko) {
+            up = 0.0;
+            ut = 0.0;
+            FLD(kVelo + 1)[i] = 0.0;
+        }
+        FLD(kVelo)[i] = up;
+        FLD(kVelo + 2)[i] = ut;
+        if (k >= p.ko) FLD(kTemp)[i] = p.t_cmb[j];
+        if (k <= p.ki) FLD(kTemp)[i] = p.t_icb[j];
+        double *bp = FLD(kMagne), *bdp = FLD(kMagne + 1), *bt = FLD(kMagne + 2);
+        if (j == 0) {
+            bp[i] = 0.0;
+            bt[i] = 0.0;
+        }
+        if (k == p.ki) {
+            bdp[i] = (p.g1[j] + 1.0) * p.ri[1] * bp[i];
+            bt[i] = 0.0;
+        }
+        if (k == p.ko) {
+            bdp[i] = -p.g1[j] * p.rc[1] * bp[i];
+            bt[i] = 0.0;
+        }
+    }
+}
+
+// ------------------------------------------------ wall rows of grad/curl --
+// cal_sph_nod_icb_rigid_velo2 + cal_sph_nod_icb_rigid_rot2,
+// cal_sph_nod_cmb_rigid_v_and_w, cal_sph_nod_icb_ins_b_and_j,
+// cal_sph_nod_cmb_ins_b_and_j.  One thread per mode.
+__global__ void k_bc_rows(MhdDev p)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p.J) return;
+    const double g1 = p.g1[j], g3 = p.g3[j];
+    double *us = FLD(kVelo), *ud = FLD(kVelo + 1), *ut = FLD(kVelo + 2);
+    double *ws = FLD(kVort), *wd = FLD(kVort + 1), *wt = FLD(kVort + 2);
+    const size_t J = p.J;
+    {   // ICB velocity and vorticity
+        for (int k = 0; k <= p.ki; ++k) {
+            const size_t i = j + J * k;
+            us[i] = 0.0;
+            ud[i] = 0.0;
+            ut[i] = 0.0;
+        }
+        const size_t i0 = j + J * p.ki, i1 = i0 + J, i2 = i1 + J;
+        const double d2s = p.fdr_i[1][2] * us[i0] + p.fdr_i[2][2] * us[i1];
+        const double d1t = p.ffl_i[0][1] * ut[i0] + p.ffl_i[1][1] * ut[i1] + p.ffl_i[2][1] * ut[i2];
+        ws[i0] = ut[i0];
+        wd[i0] = d1t;
+        wt[i0] = -(d2s - g3 * p.ri[2] * us[i0]);
+    }
+    {   // CMB velocity and vorticity
+        const size_t i0 = j + J * p.ko, n1 = i0 - J, n2 = n1 - J;
+        us[i0] = 0.0;
+        ud[i0] = 0.0;
+        ut[i0] = 0.0;
+        const double d2s = p.fdr_c[0][2] * us[n1];
+        const double d1t = p.ffl_c[2][1] * ut[n2] + p.ffl_c[1][1] * ut[n1] + p.ffl_c[0][1] * ut[i0];
+        ws[i0] = ut[i0];
+        wd[i0] = d1t;
+        wt[i0] = -d2s;
+    }
+    double *bs = FLD(kMagne), *bd = FLD(kMagne + 1), *bt = FLD(kMagne + 2);
+    double *js = FLD(kCurrent), *jd = FLD(kCurrent + 1), *jt = FLD(kCurrent + 2);
+    {   // ICB insulator
+        const size_t i0 = j + J * p.ki, i1 = i0 + J, i2 = i1 + J;
+        const double d1s = (g1 + 1.0) * p.ri[1] * bs[i0];
+        const double d2s = ((g1 + 1.0) * p.ri[1] * p.fdr_i[0][2] + p.fdr_i[1][2]) * bs[i0]
+                         + p.fdr_i[2][2] * bs[i1];
+        const double d1t = p.ffl_i[1][1] * bt[i1] + p.ffl_i[2][1] * bt[i2];
+        bd[i0] = d1s;
+        bt[i0] = 0.0;
+        js[i0] = 0.0;
+        jd[i0] = d1t;
+        jt[i0] = -(d2s - g3 * p.ri[2] * bs[i0]);
+    }
+    {   // CMB insulator
+        const size_t i0 = j + J * p.ko, n1 = i0 - J, n2 = n1 - J;
+        const double d1s = -g1 * p.rc[1] * bs[i0];
+        const double d2s = p.fdr_c[0][2] * bs[n1]
+                         + (p.fdr_c[1][2] - g1 * p.rc[1] * p.fdr_c[2][2]) * bs[i0];
+        const double d1t = p.ffl_c[2][1] * bt[n2] + p.ffl_c[1][1] * bt[n1];
+        bd[i0] = d1s;
+        bt[i0] = 0.0;
+        js[i0] = 0.0;
+        jd[i0] = d1t;
+        jt[i0] = -(d2s - g3 * p.rc[2] * bs[i0]);
+    }
+}
+
+// cal_sph_diff_pol_and_rot2 for (u, w) and (B, J) inside the shell.
+__global__ void k_grad_interior(MhdDev p)
+{
+    const size_t ist = (size_t)p.J * (p.ki + 1), ied = (size_t)p.J * p.ko;
+    for (size_t i = ist + blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ied;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const double g3a = p.g3[j] * p.ar2[k];
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            const int src = f == 0 ? kVelo : kMagne, dst = f == 0 ? kVort : kCurrent;
+            const double *s = FLD(src), *t = FLD(src + 2);
+            const double d1s = dr1(p, s, k, i), d2s = dr2(p, s, k, i), d1t = dr1(p, t, k, i);
+            FLD(src + 1)[i] = d1s;
+            FLD(dst)[i] = t[i];
+            FLD(dst + 1)[i] = d1t;
+            FLD(dst + 2)[i] = -d2s + g3a * s[i];
+        }
+    }
+}
+
+// ------------------------------------------------ diffusion, buoyancy ----
+// const_sph_vorticirty_diffusion (cal_sph_nod_vect_diffuse2 +
+// cal_sph_nod_icb/cmb_rgd_w_diffuse2), const_sph_mag_diffuse_by_j
+// (cal_sph_nod_diffuse_by_rot2 + cal_sph_nod_icb/cmb_ins_diffuse2),
+// const_sph_scalar_diffusion (cal_sph_nod_scalar_diffuse2 +
+// cal_sph_fix_scalar_in/out_diffuse2), cal_rot_buoyancy_sph_MHD.
+__global__ void k_diffusion(MhdDev p)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const size_t J = p.J;
+        const double g1 = p.g1[j], g3 = p.g3[j];
+        const double *w0 = FLD(kVort), *w2 = FLD(kVort + 2);
+        const double *b0 = FLD(kMagne), *b2 = FLD(kMagne + 2);
+        const double *c0 = FLD(kCurrent), *c2 = FLD(kCurrent + 2);
+        const double *t = FLD(kTemp);
+        double *wd0 = FLD(kWDiffuse), *wd2 = FLD(kWDiffuse + 2);
+        double *bd0 = FLD(kBDiffuse), *bd1 = FLD(kBDiffuse + 1), *bd2 = FLD(kBDiffuse + 2);
+        double *td = FLD(kTDiffuse);
+        if (k > p.ki && k < p.ko) {
+            const double g3a = g3 * p.ar2[k];
+            wd0[i] = p.coef_d_velo * (dr2(p, w0, k, i) - g3a * w0[i]);
+            wd2[i] = p.coef_d_velo * (dr2(p, w2, k, i) - g3a * w2[i]);
+            const double d2s = dr2(p, c0, k, i), d1t = dr1(p, c2, k, i);
+            bd0[i] = -p.coef_d_magne * c2[i];
+            bd1[i] = -p.coef_d_magne * d1t;
+            bd2[i] = p.coef_d_magne * (d2s - g3a * c0[i]);
+            const double d1s = dr1(p, t, k, i), d2t = dr2(p, t, k, i);
+            td[i] = p.coef_d_temp * (d2t + 2.0 * p.ar1[k] * d1s - g3a * t[i]);
+        } else if (k == p.ki) {
+            const size_t i1 = i + J, i2 = i1 + J;
+            const double dw0 = p.ffl_i[0][2] * w0[i] + p.ffl_i[1][2] * w0[i1] + p.ffl_i[2][2] * w0[i2];
+            const double dw2 = p.ffl_i[0][2] * w2[i] + p.ffl_i[1][2] * w2[i1] + p.ffl_i[2][2] * w2[i2];
+            wd0[i] = p.coef_d_velo * (dw0 - g3 * p.ri[2] * w0[i]);
+            wd2[i] = p.coef_d_velo * (dw2 - g3 * p.ri[2] * w2[i]);
+            const double d2s = ((g1 + 1.0) * p.ri[1] * p.fdr_i[0][2] + p.fdr_i[1][2]) * b0[i]
+                             + p.fdr_i[2][2] * b0[i1];
+            const double d2t = p.ffl_i[0][2] * b2[i] + p.ffl_i[1][2] * b2[i1] + p.ffl_i[2][2] * b2[i2];
+            bd0[i] = p.coef_d_magne * (d2s - g3 * p.ri[2] * b0[i]);
+            bd2[i] = p.coef_d_magne * (d2t - g3 * p.ri[2] * b2[i]);
+            const double fix = p.t_icb[j];
+            const double d1t = p.ffl_i[0][1] * fix + p.ffl_i[1][1] * t[i1] + p.ffl_i[2][1] * t[i2];
+            const double d2tt = p.ffl_i[0][2] * fix + p.ffl_i[1][2] * t[i1] + p.ffl_i[2][2] * t[i2];
+            td[i] = p.coef_d_temp * (d2tt + 2.0 * p.ri[1] * d1t - g3 * p.ri[2] * fix);
+        } else if (k == p.ko) {
+            const size_t n1 = i - J, n2 = n1 - J;
+            const double dw0 = p.ffl_c[2][2] * w0[n2] + p.ffl_c[1][2] * w0[n1] + p.ffl_c[0][2] * w0[i];
+            const double dw2 = p.ffl_c[2][2] * w2[n2] + p.ffl_c[1][2] * w2[n1] + p.ffl_c[0][2] * w2[i];
+            wd0[i] = p.coef_d_velo * (dw0 - g3 * p.rc[2] * w0[i]);
+            wd2[i] = p.coef_d_velo * (dw2 - g3 * p.rc[2] * w2[i]);
+            const double d2s = p.fdr_c[0][2] * b0[n1]
+                             + (p.fdr_c[1][2] - g1 * p.rc[1] * p.fdr_c[2][2]) * b0[i];
+            const double d2t = p.ffl_c[2][2] * b2[n2] + p.ffl_c[1][2] * b2[n1] + p.ffl_c[0][2] * b2[i];
+            bd0[i] = p.coef_d_magne * (d2s - g3 * p.rc[2] * b0[i]);
+            bd2[i] = p.coef_d_magne * (d2t - g3 * p.rc[2] * b2[i]);
+            const double fix = p.t_cmb[j];
+            const double d1t = p.ffl_c[2][1] * t[n2] + p.ffl_c[1][1] * t[n1] + p.ffl_c[0][1] * fix;
+            const double d2tt = p.ffl_c[2][2] * t[n2] + p.ffl_c[1][2] * t[n1] + p.ffl_c[0][2] * fix;
+            td[i] = p.coef_d_temp * (d2tt + 2.0 * p.rc[1] * d1t - g3 * p.rc[2] * fix);
+        }
+        if (k >= p.ki && k <= p.ko) FLD(kRotBuoyancy)[i] = p.coef_buo * t[i] * p.r[k];
+    }
+}
+
+// cal_sph_fix_scalar_in/out_diffuse2 leave the wall values in place.
+__global__ void k_temp_bc(MhdDev p)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p.J) return;
+    FLD(kTemp)[j + (size_t)p.J * p.ki] = p.t_icb[j];
+    FLD(kTemp)[j + (size_t)p.J * p.ko] = p.t_cmb[j];
+}
+
+// ------------------------------------------------------- rj <-> rlm -----
+__global__ void k_pack(MhdDev p)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / p.J), jl = (int)(i % p.J);
+        const size_t src = (size_t)p.rj_of_rlm[jl] + (size_t)p.J * k;
+        double *o = p.sp_b + 13 * i;
+#pragma unroll
+        for (int c = 0; c < 12; ++c) o[c] = FLD(kVelo + c)[src];
+        o[12] = FLD(kTemp)[src];
+    }
+}
+
+// Unpack the forward transform into the rj fields and form the curl of the
+// Coriolis force from u and w (sum_rot_coriolis_rlm_10; omega = (r^2/2, r, 1)).
+__global__ void k_unpack(MhdDev p)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const double *s = p.sp_f + 19 * ((size_t)p.rlm_of_rj[j] + (size_t)p.J * k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            FLD(kMAdvect + c)[i] = s[c];
+            FLD(kLorentz + c)[i] = s[9 + c];
+            FLD(kVpInduct + c)[i] = s[12 + c];
+            FLD(kHFlux + c)[i] = s[15 + c];
+        }
+        const size_t base = (size_t)p.J * k;
+        const size_t k1 = base + p.jk1[j], k2 = base + p.jk2[j], l1 = base + p.jl1[j];
+        const double *us = FLD(kVelo), *ud = FLD(kVelo + 1);
+        const double *ws = FLD(kVort), *wd = FLD(kVort + 1), *wt = FLD(kVort + 2);
+        const double ar = 1.0 / p.r[k], rr = p.r[k];
+        const double om0 = 0.5 * rr * rr, om1 = rr, om2 = 1.0;
+        const double *C = p.cor;
+        const size_t J = p.J;
+        const double sw11 = C[0 * J + j], sw21 = C[1 * J + j], sw12 = C[2 * J + j];
+        const double sw22 = C[3 * J + j], sw13 = C[4 * J + j], tw13 = C[5 * J + j];
+        const double tw23 = C[6 * J + j], tw14 = C[7 * J + j], tw24 = C[8 * J + j];
+        const double tw11 = C[9 * J + j], tw12 = C[10 * J + j], g3r = C[11 * J + j];
+        const double sp_d2vp_l1 = ar * ar * g3r * us[l1] - wt[l1];
+        const double sp_dwp_k1 = wd[k1] - 2.0 * ar * ws[k1];
+        const double sp_dwp_k2 = wd[k2] - 2.0 * ar * ws[k2];
+        double pol = sw11 * om0 * ud[k1] + sw21 * om0 * ud[k2] + sw13 * om0 * ws[l1]
+                   + sw12 * om1 * us[k1] + sw22 * om1 * us[k2];
+        double tor = tw11 * om0 * sp_d2vp_l1 - tw12 * om2 * us[l1] + tw13 * om1 * ws[k1]
+                   + tw23 * om1 * ws[k2] + tw14 * om0 * sp_dwp_k1 + tw24 * om0 * sp_dwp_k2;
+        pol = -p.coef_cor * pol * ar * ar;
+        tor = -p.coef_cor * tor * ar * ar;
+        FLD(kRotCoriolis)[i] = pol;
+        FLD(kRotCoriolis + 1)[i] = s[7];
+        FLD(kRotCoriolis + 2)[i] = tor;
+    }
+}
+
+// Curl of a force with rigid-wall rows (const_sph_force_rot2) -> (pol, tor).
+__device__ __forceinline__ void rot_rigid(const MhdDev &p, int src, int k, int j, size_t i,
+                                          double *pol, double *tor)
+{
+    const double *s0 = FLD(src), *s1 = FLD(src + 1), *s2 = FLD(src + 2);
+    const double g3 = p.g3[j];
+    const size_t J = p.J;
+    *pol = s2[i];
+    if (k == p.ki) {
+        const double d2s = p.fdr_i[1][2] * s0[i] + p.fdr_i[2][2] * s0[i + J];
+        *tor = -(d2s - g3 * p.ri[2] * s0[i]);
+    } else if (k == p.ko) {
+        const double d2s = p.fdr_c[0][2] * s0[i - J] + p.fdr_c[1][2] * s0[i];
+        *tor = -(d2s - g3 * p.rc[2] * s0[i]);
+    } else {
+        *tor = -dr1(p, s1, k, i) + g3 * p.ar2[k] * s0[i];
+    }
+}
+
+__global__ void k_rot_forces(MhdDev p, int temp_roundtrip)
+{
+    for (size_t i = (size_t)p.J * p.ki + blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+         i < (size_t)p.J * (p.ko + 1); i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const size_t J = p.J;
+        const double g1 = p.g1[j], g3 = p.g3[j];
+        double ri_p, ri_t, rl_p, rl_t;
+        rot_rigid(p, kMAdvect, k, j, i, &ri_p, &ri_t);
+        rot_rigid(p, kLorentz, k, j, i, &rl_p, &rl_t);
+        FLD(kRotInertia)[i] = ri_p;
+        FLD(kRotInertia + 2)[i] = ri_t;
+        FLD(kRotLorentz)[i] = rl_p;
+        FLD(kRotLorentz + 2)[i] = rl_t;
+        // curl of u x B with insulating rows (const_sph_rotation_uxb)
+        const double *v0 = FLD(kVpInduct), *v1 = FLD(kVpInduct + 1), *v2 = FLD(kVpInduct + 2);
+        double in0 = v2[i], in1, in2;
+        if (k == p.ki) {
+            in1 = (g1 + 1.0) * p.ri[1] * v2[i];
+            in2 = 0.0;
+        } else if (k == p.ko) {
+            in1 = -g1 * p.rc[1] * v2[i];
+            in2 = 0.0;
+        } else {
+            in1 = dr1(p, v2, k, i);
+            in2 = -dr1(p, v1, k, i) + g3 * p.ar2[k] * v0[i];
+        }
+        FLD(kInduction)[i] = in0;
+        FLD(kInduction + 1)[i] = in1;
+        FLD(kInduction + 2)[i] = in2;
+        // divergence of the heat flux (const_sph_scalar_advect)
+        const double *h0 = FLD(kHFlux), *h1 = FLD(kHFlux + 1);
+        const double gg = fmax(g3, 0.5);
+        double ha;
+        if (k == p.ki) {
+            const double d1s = p.ffl_i[0][1] * p.t_icb[j] + p.ffl_i[1][1] * h0[i + J]
+                             + p.ffl_i[2][1] * h0[i + 2 * J];
+            ha = (d1s - h1[i]) * gg * p.ri[2];
+        } else if (k == p.ko) {
+            const double d1s = p.ffl_c[2][1] * h0[i - 2 * J] + p.ffl_c[1][1] * h0[i - J]
+                             + p.ffl_c[0][1] * p.t_cmb[j];
+            ha = (d1s - h1[i]) * gg * p.rc[2];
+        } else {
+            ha = (dr1(p, h0, k, i) - h1[i]) * gg * p.ar2[k];
+        }
+        // add_reftemp_advect_sph_MHD
+        ha = ha + p.coef_temp * g3 * p.ar2[k] * p.reft1[k] * FLD(kVelo)[i];
+        FLD(kHAdvect)[i] = ha;
+        // set_MHD_terms_to_force
+        FLD(kForces)[i] = -ri_p + FLD(kRotCoriolis)[i] + rl_p;
+        FLD(kForces + 2)[i] = -ri_t + FLD(kRotCoriolis + 2)[i] + rl_t + FLD(kRotBuoyancy)[i];
+        // trans_per_temp_to_temp_sph then sync_temp_by_per_temp_sph
+        if (temp_roundtrip && j == 0) {
+            double *t = FLD(kTemp);
+            t[i] = t[i] + p.reft0[k];
+            t[i] = t[i] - p.reft0[k];
+        }
+    }
+}
+
+__global__ void k_temp_full(MhdDev p, double *out)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        double t = FLD(kTemp)[i];
+        if (j == 0) t = t + p.reft0[k];
+        out[i] = t;
+    }
+}
+
+#undef FLD
+
+// ------------------------------------------------------------ host ------
+
+namespace {
+
+double det22(double a11, double a12, double a21, double a22) { return a11 * a22 - a21 * a12; }
+
+// cal_inverse_33_matrix: first-row cofactor determinant, adjugate / det.
+void inverse33(const double a[3][3], double inv[3][3])
+{
+    double det = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        int c0 = k == 0 ? 1 : 0, c1 = k == 2 ? 1 : 2;
+        const double d = det22(a[1][c0], a[1][c1], a[2][c0], a[2][c1]);
+        det = det + ((k % 2) ? -1.0 : 1.0) * d * a[0][k];
+    }
+    const double a_det = 1.0 / det;
+    for (int kj = 0; kj < 3; ++kj)
+        for (int ki = 0; ki < 3; ++ki) {
+            int r0 = ki == 0 ? 1 : 0, r1 = ki == 2 ? 1 : 2;
+            int c0 = kj == 0 ? 1 : 0, c1 = kj == 2 ? 1 : 2;
+            const double d = det22(a[r0][c0], a[r0][c1], a[r1][c0], a[r1][c1]);
+            inv[kj][ki] = (((ki + kj) % 2) ? -1.0 : 1.0) * d * a_det;
+        }
+}
+
+int isqrt_int(int j)
+{
+    int l = (int)std::sqrt((double)j);
+    while ((l + 1) * (l + 1) <= j) ++l;
+    while (l * l > j) --l;
+    return l;
+}
+
+template <typename T>
+int upload(T **d, const std::vector<T> &h, std::string *err)
+{
+    HIP_TRY(hipMalloc(d, sizeof(T) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) HIP_TRY(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int blocks_for(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 8192); }
+
+}  // namespace
+
+double gaunt_p1(int l, int m)
+{
+    // leadki(1,0, l-1,m, l,m) = 4 sqrt(l^2 - m^2) / ((2l+1)(2l-1)) (Schmidt
+    // harmonics; the recurrence x P_l^m = (sqrt(l^2-m^2) P_{l-1}^m +
+    // sqrt((l+1)^2-m^2) P_{l+1}^m) / (2l+1) and int (P_l^m)^2 = 2(2-d_m0)/(2l+1)).
+    if (l < 1 || std::abs(m) > l - 1) return 0.0;
+    return 4.0 * std::sqrt((double)(l * l - m * m)) / ((double)(2 * l + 1) * (double)(2 * l - 1));
+}
+
+double elsasser_p1(int l, int m)
+{
+    // leadli(1,0, l,-m, l,m) = -4 m / (2l+1): the phi integral leaves
+    // -m pi int cos(t) d(P^2)/dt dt = -m pi int (P_l^m)^2 dx.
+    if (l < 1) return 0.0;
+    return -4.0 * (double)m / (double)(2 * l + 1);
+}
+
+// Coupling table of the curl of the Coriolis force (init_sum_coriolis_rlm:
+// cal_gaunt_coriolis_rlm and interact_rot_coriolis_rlm,
+// interact_coriolis_rlm.f90), indexed by rj mode: cor[q*J + j] for
+// q = sw11 sw21 sw12 sw22 sw13 tw13 tw23 tw14 tw24 tw11 tw12 g3; the
+// neighbours (l-1,m), (l+1,m), (l,-m) as rj indices.
+void coriolis_table(int L, double *cor, int *jk1, int *jk2, int *jl1)
+{
+    const int J = (L + 1) * (L + 1);
+    for (int j = 0; j < J; ++j) {
+        const int l3 = isqrt_int(j), m3 = j - l3 * (l3 + 1);
+        double gi1 = 0.0, gi2 = 0.0, ei = 0.0;
+        int k1l = l3, k1m = m3, k2l = l3, k2m = m3, e1l = l3, e1m = m3;
+        if (l3 - 1 >= 1 && std::abs(m3) <= l3 - 1) {
+            gi1 = gaunt_p1(l3, m3);
+            k1l = l3 - 1;
+        }
+        if (l3 + 1 <= L) {
+            gi2 = gaunt_p1(l3 + 1, m3);
+            k2l = l3 + 1;
+        }
+        if (l3 >= 1) {
+            ei = elsasser_p1(l3, m3);
+            e1m = -m3;
+        }
+        if (j == 0) {
+            k1l = k1m = k2l = k2m = e1l = e1m = 0;
+            gi2 = 0.0;
+            ei = 0.0;
+        }
+        jk1[j] = k1l * (k1l + 1) + k1m;
+        jk2[j] = k2l * (k2l + 1) + k2m;
+        jl1[j] = e1l * (e1l + 1) + e1m;
+        const double G3 = (j == 0) ? 0.5 : (double)(l3 * (l3 + 1));
+        const double G4 = (double)((l3 - 1) * l3), G5 = (double)((l3 + 1) * (l3 + 2));
+        const double G17 = (j == 0) ? 0.0 : (double)(2 * l3 + 1) / (double)(4 * l3 * (l3 + 1));
+        double *C = cor;
+        C[0 * J + j] = (2.0 - G4 - G3) * gi1 * G17;
+        C[1 * J + j] = (2.0 - G5 - G3) * gi2 * G17;
+        C[2 * J + j] = G4 * (2.0 - G4 + G3) * gi1 * G17 * 0.5;
+        C[3 * J + j] = G5 * (2.0 - G5 + G3) * gi2 * G17 * 0.5;
+        C[4 * J + j] = 2.0 * ei * G17;
+        C[5 * J + j] = -(G3 * (2.0 + G4 - G3) + 2.0 * (-2.0 + G4 + G3)) * gi1 * G17 * 0.5;
+        C[6 * J + j] = -(G3 * (2.0 + G5 - G3) + 2.0 * (-2.0 + G5 + G3)) * gi2 * G17 * 0.5;
+        C[7 * J + j] = -(-2.0 + G4 + G3) * gi1 * G17;
+        C[8 * J + j] = -(-2.0 + G5 + G3) * gi2 * G17;
+        C[9 * J + j] = -2.0 * ei * G17;
+        C[10 * J + j] = -G3 * ei * G17;
+        C[11 * J + j] = G3;
+    }
+}
+
+int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *radius,
+               const MhdCoefs &co, int leg_mode, hipStream_t s, std::string *err)
+{
+    if (L < 2 || nth < L + 1 || nphi < 2 * L + 2 || nr < 5) {
+        *err = "mhd_create: need L >= 2, nth >= L+1, nphi >= 2L+2, nr >= 5";
+        return -1;
+    }
+    c->L = L;
+    c->nth = nth;
+    c->nphi = nphi;
+    c->nr = nr;
+    c->J = (L + 1) * (L + 1);
+    c->M = 2 * L + 1;
+    c->co = co;
+    c->leg_mode = leg_mode;
+    c->stream = s;
+    c->radius.assign(radius, radius + nr);
+    const int J = c->J;
+    const size_t nnod = (size_t)nr * J;
+    const int ki = 0, ko = nr - 1;
+    std::vector<double> r(radius, radius + nr), ar1(nr), ar2(nr);
+    for (int k = 0; k < nr; ++k) {
+        ar1[k] = 1.0 / r[k];
+        ar2[k] = ar1[k] * ar1[k];
+    }
+    // finite differences (set_dr_for_nonequi, nod_r_2nd_fdm_coefs_nonequi)
+    std::vector<double> dr(nr), d1(3 * nr), d2(3 * nr);
+    for (int k = 1; k < nr - 1; ++k) dr[k] = r[k + 1] - r[k];
+    dr[nr - 1] = r[nr - 1] - r[nr - 2];
+    dr[0] = r[1] - r[0];
+    for (int k = 0; k < nr; ++k) {
+        double dp, dn;
+        if (k == 0) { dp = dr[0]; dn = dr[0]; }
+        else if (k == nr - 1) { dp = dr[k]; dn = dr[k]; }
+        else { dp = dr[k]; dn = dr[k - 1]; }
+        const double a[3][3] = {{1.0, 0.0, 0.0}, {1.0, dp, 0.5 * dp * dp}, {1.0, -dn, 0.5 * dn * dn}};
+        double m[3][3];
+        inverse33(a, m);
+        d1[3 * k + 0] = m[1][2]; d1[3 * k + 1] = m[1][0]; d1[3 * k + 2] = m[1][1];
+        d2[3 * k + 0] = m[2][2]; d2[3 * k + 1] = m[2][0]; d2[3 * k + 2] = m[2][1];
+    }
+    MhdDev &p = c->p;
+    p = MhdDev();
+    p.nr = nr;
+    p.J = J;
+    p.ki = ki;
+    p.ko = ko;
+    p.nnod = nnod;
+    {   // one-sided wall formulas (cal_fdm_coefs_4_boundaries.f90)
+        const double dp1 = r[ki + 1] - r[ki], dp2 = r[ki + 2] - r[ki];
+        const double a1[3][3] = {{1, 0, 0}, {1, dp1, dp1 * dp1 / 2.0}, {1, dp2, dp2 * dp2 / 2.0}};
+        double m[3][3];
+        inverse33(a1, m);
+        for (int i = 0; i < 3; ++i)
+            for (int q = 0; q < 3; ++q) p.ffl_i[i][q] = m[q][i];
+        const double a2[3][3] = {{1, 0, 0}, {0, 1, 0}, {dp1, 1, dp1 * dp1 / 2.0}};
+        inverse33(a2, m);
+        for (int i = 0; i < 3; ++i)
+            for (int q = 0; q < 3; ++q) p.fdr_i[i][q] = m[q][i];
+        const double dn1 = r[ko] - r[ko - 1], dn2 = r[ko] - r[ko - 2];
+        const double a3[3][3] = {{1, 0, 0}, {1, -dn1, dn1 * dn1 / 2.0}, {1, -dn2, dn2 * dn2 / 2.0}};
+        inverse33(a3, m);
+        for (int i = 0; i < 3; ++i)
+            for (int q = 0; q < 3; ++q) p.ffl_c[i][q] = m[q][i];
+        const double a4[3][3] = {{1, 0, 0}, {0, 1, 0}, {1, -dn1, dn1 * dn1 / 2.0}};
+        inverse33(a4, m);
+        for (int q = 0; q < 3; ++q) {
+            p.fdr_c[0][q] = m[q][2];
+            p.fdr_c[1][q] = m[q][0];
+            p.fdr_c[2][q] = m[q][1];
+        }
+        p.ri[0] = r[ki]; p.ri[1] = 1.0 / r[ki]; p.ri[2] = p.ri[1] * p.ri[1];
+        p.rc[0] = r[ko]; p.rc[1] = 1.0 / r[ko]; p.rc[2] = p.rc[1] * p.rc[1];
+    }
+    p.dt = co.dt;
+    p.coef_exp_v = 1.0 - co.coef_imp_v;
+    p.coef_exp_t = 1.0 - co.coef_imp_t;
+    p.coef_exp_b = 1.0 - co.coef_imp_b;
+    p.coef_d_velo = co.coef_d_velo;
+    p.coef_d_magne = co.coef_d_magne;
+    p.coef_d_temp = co.coef_d_temp;
+    p.coef_buo = co.coef_buo;
+    p.coef_temp = co.coef_temp;
+    p.coef_cor = co.coef_cor;
+    // modes
+    std::vector<int> l_of_j(J), rj_of_rlm(J), rlm_of_rj(J);
+    std::vector<double> g1(J), g3(J);
+    for (int j = 0; j < J; ++j) {
+        const int l = isqrt_int(j);
+        l_of_j[j] = l;
+        g1[j] = (double)l;
+        g3[j] = (double)(l * (l + 1));
+    }
+    {
+        int jr = 0;
+        for (int m = -L; m <= L; ++m)
+            for (int l = std::abs(m); l <= L; ++l, ++jr) {
+                rj_of_rlm[jr] = l * (l + 1) + m;
+                rlm_of_rj[l * (l + 1) + m] = jr;
+            }
+    }
+    // reference temperature and wall values (set_ref_temp_sph_mhd,
+    // adjust_sph_temp_bc_by_reftemp)
+    std::vector<double> reft0(nr), reft1(nr), t_icb(J, 0.0), t_cmb(J, 0.0);
+    {
+        const double dh = co.depth_high_t, dl = co.depth_low_t, hi = co.high_temp, lo = co.low_temp;
+        for (int k = 0; k < nr; ++k) {
+            reft0[k] = (dh * dl * ar1[k] * (hi - lo) - dh * hi + dl * lo) / (dl - dh);
+            reft1[k] = -dh * dl * ar2[k] * (hi - lo) / (dl - dh);
+        }
+        t_icb[0] = hi - reft0[ki];
+        t_cmb[0] = lo - reft0[ko];
+    }
+    std::vector<int> jk1(J), jk2(J), jl1(J);
+    std::vector<double> cor(12 * (size_t)J, 0.0);
+    coriolis_table(L, cor.data(), jk1.data(), jk2.data(), jl1.data());
+    // constants to the device (one allocation each for doubles and ints)
+    std::vector<double> dc;
+    auto push = [&dc](const std::vector<double> &v) {
+        size_t off = dc.size();
+        dc.insert(dc.end(), v.begin(), v.end());
+        return off;
+    };
+    const size_t o_r = push(r), o_ar1 = push(ar1), o_ar2 = push(ar2), o_d1 = push(d1),
+                 o_d2 = push(d2), o_g1 = push(g1), o_g3 = push(g3), o_ti = push(t_icb),
+                 o_tc = push(t_cmb), o_r0 = push(reft0), o_r1 = push(reft1), o_cor = push(cor);
+    std::vector<int> ic;
+    auto pushi = [&ic](const std::vector<int> &v) {
+        size_t off = ic.size();
+        ic.insert(ic.end(), v.begin(), v.end());
+        return off;
+    };
+    const size_t o_k1 = pushi(jk1), o_k2 = pushi(jk2), o_l1 = pushi(jl1), o_rr = pushi(rj_of_rlm),
+                 o_rj = pushi(rlm_of_rj), o_lj = pushi(l_of_j);
+    if (upload(&c->d_consts, dc, err) || upload(&c->d_iconsts, ic, err)) return -1;
+    if (upload(&c->d_radius, r, err)) return -1;
+    p.r = c->d_consts + o_r;
+    p.ar1 = c->d_consts + o_ar1;
+    p.ar2 = c->d_consts + o_ar2;
+    p.d1 = c->d_consts + o_d1;
+    p.d2 = c->d_consts + o_d2;
+    p.g1 = c->d_consts + o_g1;
+    p.g3 = c->d_consts + o_g3;
+    p.t_icb = c->d_consts + o_ti;
+    p.t_cmb = c->d_consts + o_tc;
+    p.reft0 = c->d_consts + o_r0;
+    p.reft1 = c->d_consts + o_r1;
+    p.cor = c->d_consts + o_cor;
+    p.jk1 = c->d_iconsts + o_k1;
+    p.jk2 = c->d_iconsts + o_k2;
+    p.jl1 = c->d_iconsts + o_l1;
+    p.rj_of_rlm = c->d_iconsts + o_rr;
+    p.rlm_of_rj = c->d_iconsts + o_rj;
+    c->d_l_of_j = c->d_iconsts + o_lj;
+    // state and work buffers
+    const size_t nrtm = (size_t)nth * nr * c->M;
+    HIP_TRY(hipMalloc(&c->d_rj, sizeof(double) * kMhdNumComp * nnod));
+    HIP_TRY(hipMemset(c->d_rj, 0, sizeof(double) * kMhdNumComp * nnod));
+    HIP_TRY(hipMalloc(&c->d_sp_b, sizeof(double) * 13 * nnod));
+    HIP_TRY(hipMalloc(&c->d_sp_f, sizeof(double) * 19 * nnod));
+    HIP_TRY(hipMalloc(&c->d_vr_b, sizeof(double) * 13 * nrtm));
+    HIP_TRY(hipMalloc(&c->d_vr_f, sizeof(double) * 19 * nrtm));
+    HIP_TRY(hipMalloc(&c->d_scratch, sizeof(double) * (nnod + 3 * (size_t)J)));
+    HIP_TRY(hipMalloc(&c->d_ms_out, sizeof(double) * 4));
+    p.d = c->d_rj;
+    p.sp_b = c->d_sp_b;
+    p.sp_f = c->d_sp_f;
+    // Crank-Nicolson matrices (const_radial_mat_vort_2step,
+    // const_radial_mat_4_temp_sph, const_radial_mat_4_magne_sph), band storage
+    // m[(j*nr + k)*3 + b] = Fortran mat3(b+1, k+1, j+1)
+    auto M3 = [&](std::vector<double> &m, int j, int k, int b) -> double & {
+        return m[((size_t)j * nr + k) * 3 + b];
+    };
+    std::vector<double> vt(3 * nnod, 0.0), wt(3 * nnod, 0.0), vs(3 * nnod, 0.0),
+        tm(3 * nnod, 0.0), bs(3 * nnod, 0.0), bt(3 * nnod, 0.0);
+    for (int j = 0; j < J; ++j)
+        for (int k = 0; k < nr; ++k) {
+            M3(vt, j, k, 1) = 1.0;
+            M3(wt, j, k, 1) = 1.0;
+            M3(tm, j, k, 1) = 1.0;
+            M3(bs, j, k, 1) = 1.0;
+            M3(bt, j, k, 1) = 1.0;
+            M3(vs, j, k, 1) = (k < ki || k > ko) ? 1.0 : 0.0;
+        }
+    auto add_vector_poisson = [&](std::vector<double> &m, double coef) {
+        for (int k = ki + 1; k < ko; ++k)
+            for (int j = 0; j < J; ++j) {
+                M3(m, j, k - 1, 2) = M3(m, j, k - 1, 2) - coef * d2[3 * k + 0];
+                M3(m, j, k, 1) = M3(m, j, k, 1) - coef * (d2[3 * k + 1] - g3[j] * ar2[k]);
+                M3(m, j, k + 1, 0) = M3(m, j, k + 1, 0) - coef * d2[3 * k + 2];
+            }
+    };
+    auto add_scalar_poisson = [&](std::vector<double> &m, double coef) {
+        for (int k = ki + 1; k < ko; ++k)
+            for (int j = 0; j < J; ++j) {
+                M3(m, j, k - 1, 2) = M3(m, j, k - 1, 2)
+                                   - coef * (d2[3 * k + 0] + 2.0 * ar1[k] * d1[3 * k + 0]);
+                M3(m, j, k, 1) = M3(m, j, k, 1)
+                               - coef * (d2[3 * k + 1] + 2.0 * ar1[k] * d1[3 * k + 1] - g3[j] * ar2[k]);
+                M3(m, j, k + 1, 0) = M3(m, j, k + 1, 0)
+                                   - coef * (d2[3 * k + 2] + 2.0 * ar1[k] * d1[3 * k + 2]);
+            }
+    };
+    auto add_fix_flux_icb = [&](std::vector<double> &m, double coef) {
+        for (int j = 0; j < J; ++j) {
+            M3(m, j, ki, 1) = M3(m, j, ki, 1) - coef * (p.fdr_i[1][2] - g3[j] * p.ri[2]);
+            M3(m, j, ki + 1, 0) = M3(m, j, ki + 1, 0) - coef * p.fdr_i[2][2];
+        }
+    };
+    auto add_fix_flux_cmb = [&](std::vector<double> &m, double coef) {
+        for (int j = 0; j < J; ++j) {
+            M3(m, j, ko - 1, 2) = M3(m, j, ko - 1, 2) - coef * p.fdr_c[0][2];
+            M3(m, j, ko, 1) = M3(m, j, ko, 1) - coef * (p.fdr_c[1][2] - g3[j] * p.rc[2]);
+        }
+    };
+    auto set_fix_fld_icb = [&](std::vector<double> &m) {
+        for (int j = 0; j < J; ++j) {
+            M3(m, j, ki, 1) = 1.0;
+            M3(m, j, ki + 1, 0) = 0.0;
+        }
+    };
+    auto set_fix_fld_cmb = [&](std::vector<double> &m) {
+        for (int j = 0; j < J; ++j) {
+            M3(m, j, ko - 1, 2) = 0.0;
+            M3(m, j, ko, 1) = 1.0;
+        }
+    };
+    const double coef_dvt = co.coef_imp_v * co.coef_d_velo * co.dt;
+    add_vector_poisson(vt, coef_dvt);
+    add_vector_poisson(wt, coef_dvt);
+    add_vector_poisson(vs, 1.0);
+    add_fix_flux_icb(wt, coef_dvt);
+    set_fix_fld_icb(vt);
+    add_fix_flux_icb(vs, 1.0);
+    add_fix_flux_cmb(wt, coef_dvt);
+    set_fix_fld_cmb(vt);
+    add_fix_flux_cmb(vs, 1.0);
+    // poloidal velocity: 5-band product wt * vs (cal_mat_product_3band_mul)
+    std::vector<double> vp(5 * nnod, 0.0);
+    {
+        auto P5 = [&](int j, int k, int b) -> double & { return vp[((size_t)j * nr + k) * 5 + b]; };
+        auto Lm = [&](int j, int k, int b) { return M3(wt, j, k, b); };
+        auto Rm = [&](int j, int k, int b) { return M3(vs, j, k, b); };
+        auto rows = [&](int j, int k, bool skip5, bool skip1) {
+            if (!skip5) P5(j, k - 2, 4) = Lm(j, k - 1, 2) * Rm(j, k - 2, 2);
+            P5(j, k - 1, 3) = Lm(j, k - 1, 2) * Rm(j, k - 1, 1) + Lm(j, k, 1) * Rm(j, k - 1, 2);
+            P5(j, k, 2) = Lm(j, k - 1, 2) * Rm(j, k, 0) + Lm(j, k, 1) * Rm(j, k, 1)
+                        + Lm(j, k + 1, 0) * Rm(j, k, 2);
+            P5(j, k + 1, 1) = Lm(j, k, 1) * Rm(j, k + 1, 0) + Lm(j, k + 1, 0) * Rm(j, k + 1, 1);
+            if (!skip1) P5(j, k + 2, 0) = Lm(j, k + 1, 0) * Rm(j, k + 2, 0);
+        };
+        for (int k = ki + 2; k <= ko - 2; ++k)
+            for (int j = 0; j < J; ++j) rows(j, k, false, false);
+        for (int j = 0; j < J; ++j) {
+            P5(j, ki, 2) = 1.0;
+            P5(j, ki + 1, 1) = 0.0;
+            P5(j, ki + 2, 0) = 0.0;
+            rows(j, ki + 1, true, false);
+            rows(j, ko - 1, false, true);
+            P5(j, ko - 2, 4) = 0.0;
+            P5(j, ko - 1, 3) = 0.0;
+            P5(j, ko, 2) = 1.0;
+        }
+    }
+    const double coef_t = co.coef_imp_t * co.coef_d_temp * co.dt;
+    add_scalar_poisson(tm, coef_t);
+    set_fix_fld_icb(tm);
+    set_fix_fld_cmb(tm);
+    const double coef_dbt = co.coef_imp_b * co.coef_d_magne * co.dt;
+    add_vector_poisson(bs, coef_dbt);
+    add_vector_poisson(bt, coef_dbt);
+    for (int j = 0; j < J; ++j) {
+        M3(bs, j, ki, 1) = M3(bs, j, ki, 1)
+                         - coef_dbt * (p.fdr_i[1][2] - g3[j] * p.ri[2]
+                                       + (g1[j] + 1.0) * p.ri[1] * p.fdr_i[0][2]);
+        M3(bs, j, ki + 1, 0) = M3(bs, j, ki + 1, 0) - coef_dbt * p.fdr_i[2][2];
+        M3(bs, j, ko - 1, 2) = M3(bs, j, ko - 1, 2) - coef_dbt * p.fdr_c[0][2];
+        M3(bs, j, ko, 1) = M3(bs, j, ko, 1)
+                         - coef_dbt * (p.fdr_c[1][2] - g3[j] * p.rc[2]
+                                       - g1[j] * p.rc[1] * p.fdr_c[2][2]);
+    }
+    set_fix_fld_icb(bt);
+    set_fix_fld_cmb(bt);
+    // LU on the GPU (ludcmp_band / ludcmp_3band)
+    const std::vector<double> *mats[5] = {&vp, &vt, &tm, &bs, &bt};
+    const int nbs[5] = {5, 3, 3, 3, 3};
+    double *d_a = nullptr;
+    int *d_ierr = nullptr;
+    HIP_TRY(hipMalloc(&d_a, sizeof(double) * 5 * nnod));
+    HIP_TRY(hipMalloc(&d_ierr, sizeof(int) * J));
+    for (int q = 0; q < 5; ++q) {
+        const int nb = nbs[q];
+        HIP_TRY(hipMemcpy(d_a, mats[q]->data(), sizeof(double) * nb * nnod, hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&c->d_lu[q], sizeof(double) * (2 * nb - 1) * nnod));
+        HIP_TRY(hipMalloc(&c->d_piv[q], sizeof(int) * nnod));
+        if (ludcmp_dev(nb, nr, J, d_a, c->d_lu[q], c->d_piv[q], d_ierr, s, err)) return -1;
+        HIP_TRY(hipStreamSynchronize(s));
+        std::vector<int> ierr(J);
+        HIP_TRY(hipMemcpy(ierr.data(), d_ierr, sizeof(int) * J, hipMemcpyDeviceToHost));
+        for (int j = 0; j < J; ++j)
+            if (ierr[j]) {
+                *err = "mhd_create: singular radial matrix";
+                return -1;
+            }
+    }
+    (void)hipFree(d_a);
+    (void)hipFree(d_ierr);
+    // transforms
+    if (leg_context_build(&c->leg, L, nth, nr, radius, nullptr, nullptr, 19, 6, 1, leg_mode, err))
+        return -1;
+    if (fft_context_build(&c->fft, L, nth, nr, nphi, err)) return -1;
+    HIP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+void mhd_free(MhdContext *c)
+{
+    leg_context_free(&c->leg);
+    fft_context_free(&c->fft);
+    double *bufs[] = {c->d_rj, c->d_sp_b, c->d_sp_f, c->d_vr_b, c->d_vr_f, c->d_consts,
+                      c->d_radius, c->d_scratch, c->d_ms_out};
+    for (double *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->d_iconsts) (void)hipFree(c->d_iconsts);
+    for (int q = 0; q < 5; ++q) {
+        if (c->d_lu[q]) (void)hipFree(c->d_lu[q]);
+        if (c->d_piv[q]) (void)hipFree(c->d_piv[q]);
+    }
+}
+
+int mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err)
+{
+    if (comp < 0 || comp >= kMhdNumComp) {
+        *err = "field component out of range";
+        return -1;
+    }
+    const size_t n = c->p.nnod;
+    double *d = c->d_rj + (size_t)comp * n;
+    if (to_host)
+        HIP_TRY(hipMemcpyAsync(host, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    else
+        HIP_TRY(hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// dynamo_benchmark_1 initial fields (set_initial_velo_sph,
+// set_ini_reference_temp_sph, set_initial_temp_sph,
+// initial_b_dynamobench_1): full temperature, as a restart holds it.
+int mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err)
+{
+    const int nr = c->nr, J = c->J;
+    const size_t n = (size_t)nr * J;
+    const double pi = 4.0 * std::atan(1.0);
+    const double *r = c->radius.data();
+    const double r_icb = r[0], r_cmb = r[nr - 1], shell = r_cmb - r_icb;
+    const MhdCoefs &co = c->co;
+    std::vector<double> zero(n, 0.0), temp(n, 0.0), bp(n, 0.0), bd(n, 0.0), bt(n, 0.0),
+        jp(n, 0.0), jd(n, 0.0), jt(n, 0.0);
+    const double dh = co.depth_high_t, dl = co.depth_low_t, hi = co.high_temp, lo = co.low_temp;
+    const int m = (isig % 1000) / 100;
+    const int jj = m * (m + 1) + m;
+    if (jj >= J) {
+        *err = "initial temperature mode beyond the truncation";
+        return -1;
+    }
+    for (int k = 0; k < nr; ++k) {
+        const double ar1 = 1.0 / r[k];
+        temp[(size_t)J * k] = (dh * dl * ar1 * (hi - lo) - dh * hi + dl * lo) / (dl - dh);
+        const double xr = 2.0 * r[k] - 1.0 * (r_cmb + r_icb) / shell;
+        temp[jj + (size_t)J * k] = (1.0 - 3.0 * (xr * xr) + 3.0 * std::pow(xr, 4) - std::pow(xr, 6))
+                                 * 0.1 * 3.0 / std::sqrt(2.0 * pi);
+        const double rr = r[k];
+        const size_t is = 2 + (size_t)J * k, it = 6 + (size_t)J * k;
+        bp[is] = (5.0 / 8.0) * (-3.0 * std::pow(rr, 3) + (4.0 * r_cmb) * (rr * rr)
+                                - std::pow(r_icb, 4) / rr);
+        bd[is] = (5.0 / 8.0) * (-9.0 * (rr * rr) + 8.0 * r_cmb * rr + std::pow(r_icb, 4) / (rr * rr));
+        jt[is] = (5.0 * 3.0 / 2.0) * rr;
+        bt[it] = (10.0 / 3.0) * rr * std::sin(pi * (rr - r_icb));
+        jp[it] = bt[it];
+        jd[it] = (10.0 / 3.0) * (std::sin(pi * (rr - r_icb)) + pi * rr * std::cos(pi * (rr - r_icb)));
+    }
+    const std::pair<int, const std::vector<double> *> sets[] = {
+        {kVelo, &zero}, {kVelo + 1, &zero}, {kVelo + 2, &zero}, {kTemp, &temp},
+        {kMagne, &bp}, {kMagne + 1, &bd}, {kMagne + 2, &bt},
+        {kCurrent, &jp}, {kCurrent + 1, &jd}, {kCurrent + 2, &jt}};
+    for (const auto &sv : sets)
+        if (mhd_field_io(c, sv.first, const_cast<double *>(sv.second->data()), false, err)) return -1;
+    return 0;
+}
+
+namespace {
+
+int nonlinear(MhdContext *c, bool temp_roundtrip, std::string *err)
+{
+    MhdDev &p = c->p;
+    hipStream_t s = c->stream;
+    const int nb = blocks_for(p.nnod);
+    hipLaunchKernelGGL(k_pack, dim3(nb), dim3(256), 0, s, p);
+    HIP_TRY(hipGetLastError());
+    if (leg_backward_dev(&c->leg, 13, 4, 1, c->d_sp_b, c->d_vr_b, s, err)) return -1;
+    const double coef[4] = {c->co.coef_velo, c->co.coef_lor, c->co.coef_induct, c->co.coef_temp};
+    if (fft_mhd_fused_dev(&c->fft, c->d_vr_b, c->d_vr_f, coef, s, err)) return -1;
+    if (leg_forward_dev(&c->leg, 19, 6, 1, c->d_vr_f, c->d_sp_f, s, err)) return -1;
+    hipLaunchKernelGGL(k_unpack, dim3(nb), dim3(256), 0, s, p);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_rot_forces, dim3(nb), dim3(256), 0, s, p, temp_roundtrip ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// derived fields after new u, T, B (set_sph_field_to_start and the
+// update_after_* calls of s_cal_sol_sph_MHD_crank)
+int derived_fields(MhdContext *c, std::string *err)
+{
+    MhdDev &p = c->p;
+    hipStream_t s = c->stream;
+    const int nb = blocks_for(p.nnod), nj = (p.J + 127) / 128;
+    hipLaunchKernelGGL(k_bc_rows, dim3(nj), dim3(128), 0, s, p);
+    hipLaunchKernelGGL(k_grad_interior, dim3(nb), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_diffusion, dim3(nb), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_temp_bc, dim3(nj), dim3(128), 0, s, p);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+int mhd_initialize(MhdContext *c, std::string *err)
+{
+    MhdDev &p = c->p;
+    // sync_temp_by_per_temp_sph: the evolved temperature is the perturbation
+    std::vector<double> t(p.nnod);
+    if (mhd_field_io(c, kTemp, t.data(), true, err)) return -1;
+    const double dh = c->co.depth_high_t, dl = c->co.depth_low_t;
+    const double hi = c->co.high_temp, lo = c->co.low_temp;
+    for (int k = 0; k < p.nr; ++k) {
+        const double ar1 = 1.0 / c->radius[k];
+        const double ref = (dh * dl * ar1 * (hi - lo) - dh * hi + dl * lo) / (dl - dh);
+        t[(size_t)p.J * k] = t[(size_t)p.J * k] - ref;
+    }
+    if (mhd_field_io(c, kTemp, t.data(), false, err)) return -1;
+    if (derived_fields(c, err)) return -1;
+    if (nonlinear(c, false, err)) return -1;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->i_step = 0;
+    c->initialized = true;
+    return 0;
+}
+
+int mhd_steps(MhdContext *c, int nsteps, std::string *err)
+{
+    if (!c->initialized) {
+        *err = "mhd_steps: call the initialisation first";
+        return -1;
+    }
+    MhdDev &p = c->p;
+    hipStream_t s = c->stream;
+    const int nb = blocks_for(p.nnod);
+    const size_t J = p.J;
+    for (int it = 0; it < nsteps; ++it) {
+        c->i_step += 1;
+        const int first = c->i_step == 1;
+        hipLaunchKernelGGL(k_explicit, dim3(nb), dim3(256), 0, s, p, first, first);
+        hipLaunchKernelGGL(k_cn_rhs, dim3(nb), dim3(256), 0, s, p);
+        HIP_TRY(hipGetLastError());
+        double *d = c->d_rj;
+        if (lubksb_dev(5, p.nr, (int)J, c->d_lu[0], c->d_piv[0], d + kVelo * p.nnod, s, err) ||
+            lubksb_dev(3, p.nr, (int)J, c->d_lu[1], c->d_piv[1], d + (kVelo + 2) * p.nnod, s, err) ||
+            lubksb_dev(3, p.nr, (int)J, c->d_lu[2], c->d_piv[2], d + kTemp * p.nnod, s, err) ||
+            lubksb_dev(3, p.nr, (int)J, c->d_lu[3], c->d_piv[3], d + kMagne * p.nnod, s, err) ||
+            lubksb_dev(3, p.nr, (int)J, c->d_lu[4], c->d_piv[4], d + (kMagne + 2) * p.nnod, s, err))
+            return -1;
+        if (derived_fields(c, err)) return -1;
+        if (nonlinear(c, true, err)) return -1;
+    }
+    return 0;
+}
+
+int mhd_energies(MhdContext *c, double *out7, std::string *err)
+{
+    MhdDev &p = c->p;
+    hipStream_t s = c->stream;
+    const int nb = blocks_for(p.nnod);
+    double *tfull = c->d_scratch, *vol = c->d_scratch + p.nnod;
+    hipLaunchKernelGGL(k_temp_full, dim3(nb), dim3(256), 0, s, p, tfull);
+    HIP_TRY(hipGetLastError());
+    const double *h_r = c->radius.data();
+    const int kr_st = p.ki + 1, kr_ed = p.ko + 1;
+    double h[3];
+    const struct { int kind; const double *f; int o; } jobs[3] = {
+        {1, c->d_rj + (size_t)kVelo * p.nnod, 0},
+        {2, tfull, 3},
+        {1, c->d_rj + (size_t)kMagne * p.nnod, 4}};
+    for (const auto &jb : jobs) {
+        if (mean_square_dev(jb.kind, p.nr, p.J, c->L, c->d_l_of_j, p.r, h_r, kr_st, kr_ed, jb.f,
+                            vol, c->d_ms_out, s, err))
+            return -1;
+        HIP_TRY(hipMemcpyAsync(h, c->d_ms_out, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (jb.kind == 2) {
+            out7[3] = h[0];
+        } else {
+            out7[jb.o] = h[0];
+            out7[jb.o + 1] = h[1];
+            out7[jb.o + 2] = h[2];
+        }
+    }
+    return 0;
+}
+
+}  // namespace calypso

@@ -1,0 +1,96 @@
+// mhd_internal.hpp -- device-resident MHD time step (rotating Boussinesq
+// shell, rigid walls, insulating magnetic boundaries, fixed-temperature
+// boundaries with the spherical-shell reference temperature).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "fft_internal.hpp"
+#include "leg_internal.hpp"
+
+namespace calypso {
+
+// d_rj components, each nr*J doubles with the mode index fastest
+// (the reference's d_rj(inod, comp), inod = j + (k-1) J).
+enum MhdField : int {
+    kVelo = 0, kVort = 3, kMagne = 6, kCurrent = 9, kTemp = 12,
+    kWDiffuse = 13, kBDiffuse = 16, kTDiffuse = 19,
+    kForces = 20, kPreMom = 23, kInduction = 26, kPreUxB = 29,
+    kHAdvect = 32, kPreHeat = 33, kRotBuoyancy = 34,
+    kMAdvect = 35, kLorentz = 38, kVpInduct = 41, kHFlux = 44,
+    kRotCoriolis = 47, kRotInertia = 50, kRotLorentz = 53,
+    kMhdNumComp = 56
+};
+
+struct MhdCoefs {
+    double dt = 0;
+    double coef_imp_v = 0, coef_imp_t = 0, coef_imp_b = 0;
+    double coef_velo = 1, coef_d_velo = 1, coef_buo = 0, coef_cor = 0, coef_lor = 0;
+    double coef_temp = 1, coef_d_temp = 1;
+    double coef_magne = 1, coef_d_magne = 1, coef_induct = 1;
+    double depth_high_t = 0, depth_low_t = 0, high_temp = 0, low_temp = 0;
+};
+
+// Everything a kernel needs, passed by value.
+struct MhdDev {
+    int nr = 0, J = 0, ki = 0, ko = 0;
+    size_t nnod = 0;
+    double *d = nullptr;                       // d_rj, kMhdNumComp * nnod
+    const double *r = nullptr, *ar1 = nullptr, *ar2 = nullptr;
+    const double *d1 = nullptr, *d2 = nullptr; // [nr][3] offsets -1, 0, +1
+    const double *g1 = nullptr, *g3 = nullptr; // rj normalisations l, l(l+1)
+    const double *t_icb = nullptr, *t_cmb = nullptr;   // [J]
+    const double *reft0 = nullptr, *reft1 = nullptr;   // [nr]
+    const int *jk1 = nullptr, *jk2 = nullptr, *jl1 = nullptr;
+    const double *cor = nullptr;               // [12][J] Coriolis couplings
+    const int *rj_of_rlm = nullptr, *rlm_of_rj = nullptr;
+    double *sp_b = nullptr, *sp_f = nullptr;   // rlm buffers, 13 / 19 comps
+    double ffl_i[3][3], fdr_i[3][3], ffl_c[3][3], fdr_c[3][3];
+    double ri[3], rc[3];
+    double dt, coef_exp_v, coef_exp_t, coef_exp_b;
+    double coef_d_velo, coef_d_magne, coef_d_temp, coef_buo, coef_temp, coef_cor;
+};
+
+struct MhdContext {
+    int L = 0, nth = 0, nphi = 0, nr = 0, J = 0, M = 0;
+    int leg_mode = kF64;
+    int i_step = 0;
+    bool initialized = false;
+    MhdCoefs co;
+    std::vector<double> radius;
+    LegContext leg;
+    FftContext fft;
+    MhdDev p;
+    hipStream_t stream = nullptr;
+    // device allocations
+    double *d_rj = nullptr, *d_sp_b = nullptr, *d_sp_f = nullptr;
+    double *d_vr_b = nullptr, *d_vr_f = nullptr;
+    double *d_consts = nullptr;   // r, ar1, ar2, d1, d2, g1, g3, t_icb, t_cmb, reft0, reft1, cor
+    int *d_iconsts = nullptr;     // jk1, jk2, jl1, rj_of_rlm, rlm_of_rj, l_of_j
+    int *d_l_of_j = nullptr;
+    double *d_radius = nullptr;
+    double *d_lu[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // vp vt temp bs bt
+    int *d_piv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *d_scratch = nullptr;  // energies: full temperature + per-mode sums
+    double *d_ms_out = nullptr;
+};
+
+int  mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *radius,
+                const MhdCoefs &co, int leg_mode, hipStream_t s, std::string *err);
+void mhd_free(MhdContext *c);
+int  mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err);
+int  mhd_initialize(MhdContext *c, std::string *err);
+int  mhd_steps(MhdContext *c, int nsteps, std::string *err);
+int  mhd_energies(MhdContext *c, double *out7, std::string *err);
+int  mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err);
+
+// closed forms of the Coriolis couplings (exposed for the host tests)
+double gaunt_p1(int l, int m);   // K(1,0; l-1,m; l,m)
+double elsasser_p1(int l, int m); // E(1,0; l,-m; l,m)
+void coriolis_table(int L, double *cor, int *jk1, int *jk2, int *jl1);
+
+}  // namespace calypso

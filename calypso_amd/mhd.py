@@ -1,0 +1,165 @@
+"""Python face of the device-resident MHD time step (libcalypso_amd.so).
+
+Mirrors module SPH_analyzer_MHD (programs/SPH_MHD/SPH_analyzer_MHD.f90):
+``MhdSimulation.initialize`` is SPH_initialize_MHD after the initial fields,
+``MhdSimulation.step`` is SPH_analyze_MHD, ``energies`` the numbers of
+sph_pwr_volume_s.dat.  The coefficients are those m_physical_property holds;
+``dynamobench_case1`` builds them from the reference's
+tests/Dynamobench_case1 control files the way construct_coefficient
+(construct_MHD_coefficient.f90) does.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from .legendre import set_legendre_trans_mode_ctl
+
+_P = ctypes.c_void_p
+
+FIELDS = {"velocity": 0, "vorticity": 3, "magnetic_field": 6, "current_density": 9,
+          "temperature": 12, "vorticity_diffusion": 13, "magnetic_diffusion": 16,
+          "thermal_diffusion": 19, "forces": 20, "induction": 26, "heat_advect": 32,
+          "buoyancy": 34}
+ENERGY_COLUMNS = ["K_ene_pol", "K_ene_tor", "K_ene", "temperature",
+                  "M_ene_pol", "M_ene_tor", "M_ene"]
+
+
+class MhdParams(ctypes.Structure):
+    _fields_ = [("l_truncation", ctypes.c_int), ("nth", ctypes.c_int), ("nphi", ctypes.c_int),
+                ("nr", ctypes.c_int), ("legendre_mode", ctypes.c_int),
+                ("dt", ctypes.c_double),
+                ("coef_imp_v", ctypes.c_double), ("coef_imp_t", ctypes.c_double),
+                ("coef_imp_b", ctypes.c_double),
+                ("coef_velo", ctypes.c_double), ("coef_d_velo", ctypes.c_double),
+                ("coef_buo", ctypes.c_double), ("coef_cor", ctypes.c_double),
+                ("coef_lor", ctypes.c_double),
+                ("coef_temp", ctypes.c_double), ("coef_d_temp", ctypes.c_double),
+                ("coef_magne", ctypes.c_double), ("coef_d_magne", ctypes.c_double),
+                ("coef_induct", ctypes.c_double),
+                ("depth_high_t", ctypes.c_double), ("depth_low_t", ctypes.c_double),
+                ("high_temp", ctypes.c_double), ("low_temp", ctypes.c_double)]
+
+
+def chebyshev_radius(nri: int, r_icb: float, r_cmb: float) -> np.ndarray:
+    """set_chebyshev_distance_shell (set_radial_grid_sph_shell.f90) for a grid
+    with no points outside the shell: nri + 1 points from r_icb to r_cmb."""
+    pi = 4.0 * math.atan(1.0)
+    shell = r_cmb - r_icb
+    return np.array([r_icb + 0.5 * shell * (1.0 - math.cos(pi * float(k) / float(nri)))
+                     for k in range(nri + 1)])
+
+
+def dynamobench_case1(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6,
+                      legendre_mode="hip_fp64"):
+    """Parameters of tests/Dynamobench_case1 (control_MHD, control_sph_shell):
+    E = 1e-3, modified Ra = 100, Pr = 1, Pm = 5, r_i/r_o = 0.35 with
+    r_o - r_i = 1; returns (MhdParams, radius)."""
+    E, Ra, Pr, Pm = 1.0e-3, 1.0e2, 1.0, 5.0
+    r_cmb = 1.0 / (1.0 - 0.35)
+    r_icb = r_cmb * 0.35
+    p = MhdParams()
+    p.l_truncation, p.nth, p.nphi, p.nr = l_truncation, nth, nphi, nri + 1
+    p.legendre_mode = set_legendre_trans_mode_ctl(legendre_mode)
+    p.dt = dt
+    p.coef_imp_v = p.coef_imp_t = p.coef_imp_b = coef_imp
+    # construct_coefficient: 'One' multiplies by 1, 'Two' by 2**power,
+    # 'Radial_35' by (1 - 0.35), a dimensionless number by value**power
+    p.coef_velo = 1.0 * 1.0
+    p.coef_d_velo = 1.0 * 1.0
+    p.coef_buo = 1.0 * (1.0 - 0.35) * Ra ** 1.0 * E ** -1.0
+    p.coef_cor = 1.0 * 2.0 ** 1.0 * E ** -1.0
+    p.coef_lor = 1.0 * Pm ** -1.0 * E ** -1.0
+    p.coef_temp = 1.0 * 1.0
+    p.coef_d_temp = 1.0 * Pr ** -1.0
+    p.coef_magne = 1.0 * 1.0
+    p.coef_d_magne = 1.0 * Pm ** -1.0
+    p.coef_induct = 1.0 * 1.0
+    p.depth_high_t, p.depth_low_t = 0.5384615384615384, 1.5384615384615384
+    p.high_temp, p.low_temp = 1.0, 0.0
+    return p, chebyshev_radius(nri, r_icb, r_cmb)
+
+
+class MhdSimulation:
+    """One device-resident MHD state (d_rj) and its step."""
+
+    def __init__(self, params: MhdParams, radius, stream=None):
+        lib = _lib.load()
+        self.params = params
+        self.radius = np.ascontiguousarray(radius, dtype=np.float64)
+        if self.radius.size != params.nr:
+            raise _lib.CalypsoError("radius must have params.nr points")
+        self.L = params.l_truncation
+        self.J = (self.L + 1) ** 2
+        self.nr = params.nr
+        h = _P()
+        _lib.check(lib.calypso_mhd_create(ctypes.byref(params), self.radius.ctypes.data_as(_P),
+                                          _P(stream) if stream else None, ctypes.byref(h)),
+                   "calypso_mhd_create")
+        self._h = h
+
+    def _comp(self, field):
+        if isinstance(field, str):
+            base, _, c = field.partition(":")
+            return FIELDS[base] + (int(c) if c else 0)
+        return int(field)
+
+    def set_field(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(self.nr, self.J)
+        _lib.check(_lib.load().calypso_mhd_set_field(self._h, self._comp(field),
+                                                     v.ctypes.data_as(_P)),
+                   "calypso_mhd_set_field")
+
+    def get_field(self, field) -> np.ndarray:
+        out = np.zeros((self.nr, self.J))
+        _lib.check(_lib.load().calypso_mhd_get_field(self._h, self._comp(field),
+                                                     out.ctypes.data_as(_P)),
+                   "calypso_mhd_get_field")
+        return out
+
+    def set_dynamobench_ic(self, isig=400):
+        _lib.check(_lib.load().calypso_mhd_set_dynamobench_ic(self._h, isig),
+                   "calypso_mhd_set_dynamobench_ic")
+
+    def initialize(self):
+        _lib.check(_lib.load().calypso_mhd_initialize(self._h), "calypso_mhd_initialize")
+
+    def step(self, nsteps=1):
+        _lib.check(_lib.load().calypso_mhd_step(self._h, nsteps), "calypso_mhd_step")
+
+    def energies(self) -> np.ndarray:
+        out = np.zeros(7)
+        _lib.check(_lib.load().calypso_mhd_energies(self._h, out.ctypes.data_as(_P)),
+                   "calypso_mhd_energies")
+        return out
+
+    @property
+    def i_step(self) -> int:
+        a = np.zeros(4, dtype=np.int64)
+        _lib.check(_lib.load().calypso_mhd_info(self._h, a.ctypes.data_as(_P)), "calypso_mhd_info")
+        return int(a[0])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().calypso_mhd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def coriolis_table(l_truncation: int):
+    """Host-side coupling table of the Coriolis curl: (cor (12, J), idx (3, J))."""
+    J = (l_truncation + 1) ** 2
+    cor = np.zeros((12, J))
+    idx = np.zeros((3, J), dtype=np.int32)
+    _lib.check(_lib.load().calypso_mhd_coriolis_table(l_truncation, cor.ctypes.data_as(_P),
+                                                      idx.ctypes.data_as(_P)),
+               "calypso_mhd_coriolis_table")
+    return cor, idx

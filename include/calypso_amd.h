@@ -195,6 +195,58 @@ int calypso_sph_mean_square(int kind, int nr, int J, const int *l_of_j,
                             const double *radius, int kr_st, int kr_ed,
                             const double *d_f, double *out, void *stream);
 
+/* ---- the full MHD time step, device resident (FP64 state) ---------------
+ * Replaces the body of module SPH_analyzer_MHD: SPH_initialize_MHD and
+ * SPH_analyze_MHD(i_step, iflag_finish) (programs/SPH_MHD/
+ * SPH_analyzer_MHD.f90:31-202) for a rotating Boussinesq shell with rigid
+ * walls, insulating magnetic boundaries and fixed-temperature walls around
+ * the spherical-shell reference temperature (the model of the reference's
+ * Dynamobench_case1).  The coefficients are those module
+ * m_physical_property holds after set_material_property
+ * (material_property.f90); radius[nr] is the radial grid, the inner and
+ * outer walls at its ends. */
+typedef struct calypso_mhd calypso_mhd_t;
+typedef struct calypso_mhd_params {
+    int l_truncation, nth, nphi, nr;
+    int legendre_mode;            /* calypso_legendre_mode_id() value */
+    double dt;
+    double coef_imp_v, coef_imp_t, coef_imp_b;
+    double coef_velo, coef_d_velo, coef_buo, coef_cor, coef_lor;
+    double coef_temp, coef_d_temp;
+    double coef_magne, coef_d_magne, coef_induct;
+    double depth_high_t, depth_low_t, high_temp, low_temp;
+} calypso_mhd_params;
+
+int calypso_mhd_create(const calypso_mhd_params *params, const double *radius,
+                       void *stream, calypso_mhd_t **out);
+int calypso_mhd_destroy(calypso_mhd_t *mhd);
+/* One d_rj component (nr*J doubles, mode j = l(l+1)+m fastest), host copy.
+ * Components: 0-2 velocity, 3-5 vorticity, 6-8 magnetic field, 9-11 current
+ * density, 12 temperature (perturbation once initialised), 13-15 vorticity
+ * diffusion, 16-18 magnetic diffusion, 19 thermal diffusion, 20-22 forces,
+ * 26-28 induction, 32 heat advection, 34 buoyancy (m_sph_phys_address). */
+int calypso_mhd_set_field(calypso_mhd_t *mhd, int comp, const double *host);
+int calypso_mhd_get_field(calypso_mhd_t *mhd, int comp, double *host);
+/* dynamo_benchmark_1 initial fields (set_initial_sph_dynamo.f90:
+ * sph_initial_data_control, initial_magne_dynamobench.f90:
+ * initial_b_dynamobench_1); isig selects the (m, m) temperature mode. */
+int calypso_mhd_set_dynamobench_ic(calypso_mhd_t *mhd, int isig);
+/* SPH_initialize_MHD after the initial fields: perturbation temperature,
+ * set_sph_field_to_start and the first nonlinear terms. */
+int calypso_mhd_initialize(calypso_mhd_t *mhd);
+/* nsteps calls of SPH_analyze_MHD (Euler on step 1, then Adams-Bashforth). */
+int calypso_mhd_step(calypso_mhd_t *mhd, int nsteps);
+/* out[7]: K_ene_pol, K_ene_tor, K_ene, temperature, M_ene_pol, M_ene_tor,
+ * M_ene over the shell (sph_pwr_volume_s.dat, output_rms_sph_mhd_control). */
+int calypso_mhd_energies(calypso_mhd_t *mhd, double *out);
+/* info[4]: i_step, nr, J, number of d_rj components. */
+int calypso_mhd_info(calypso_mhd_t *mhd, long long *info);
+/* Host-only: coupling table of the curl of the Coriolis force for
+ * truncation L (init_sum_coriolis_rlm, const_coriolis_sph_rlm.f90:34-67).
+ * cor[12*J]: sw11 sw21 sw12 sw22 sw13 tw13 tw23 tw14 tw24 tw11 tw12 g3 per rj
+ * mode; idx[3*J]: rj indices of (l-1,m), (l+1,m), (l,-m). */
+int calypso_mhd_coriolis_table(int l_truncation, double *cor, int *idx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -332,8 +332,9 @@ class MhdStep:
         self.J = J = (L + 1) ** 2
         r_cmb = c.shell / (1.0 - c.ratio)
         r_icb = r_cmb * c.ratio
-        self.r_icb, self.r_cmb = r_icb, r_cmb
         self.radius = chebyshev_radius(c.nri, r_icb, r_cmb)
+        # set_radius_dat_4_sph_dynamo: r_ICB, r_CMB are the grid's end points
+        self.r_icb, self.r_cmb = float(self.radius[0]), float(self.radius[-1])
         nr = self.nr = self.radius.size
         self.kr_in, self.kr_out = 1, nr
         self.ar1 = 1.0 / self.radius
@@ -809,19 +810,24 @@ class MhdStep:
         f["temp"][:, 0] = self.reft0
         m = int((self.c.isig % 1000) / 100)
         jj = m * (m + 1) + m
-        xr = 2.0 * r - 1.0 * (self.r_cmb + self.r_icb) / shell
-        f["temp"][:, jj] = ((1.0 - 3.0 * xr ** 2 + 3.0 * xr ** 4 - xr ** 6)
-                            * 0.1 * 3.0 / math.sqrt(2.0 * pi))
+        # scalar libm arithmetic per radius (pow, sin, cos as the Fortran has them)
+        rc, ri = self.r_cmb, self.r_icb
+        for k, x in enumerate(float(v) for v in r):
+            xr = 2.0 * x - 1.0 * (rc + ri) / shell
+            f["temp"][k, jj] = ((1.0 - 3.0 * (xr * xr) + 3.0 * math.pow(xr, 4) - math.pow(xr, 6))
+                                * 0.1 * 3.0 / math.sqrt(2.0 * pi))
         js, jt = 2, 6                       # (l, m) = (1, 0), (2, 0)
-        f["magne"][0][:, js] = (5.0 / 8.0) * (-3.0 * r ** 3 + 4.0 * self.r_cmb * r ** 2
-                                              - self.r_icb ** 4 / r)
-        f["magne"][1][:, js] = (5.0 / 8.0) * (-9.0 * r ** 2 + 8.0 * self.r_cmb * r
-                                              + self.r_icb ** 4 / r ** 2)
+        for k, x in enumerate(float(v) for v in r):
+            f["magne"][0][k, js] = (5.0 / 8.0) * (-3.0 * math.pow(x, 3) + (4.0 * rc) * (x * x)
+                                                  - math.pow(ri, 4) / x)
+            f["magne"][1][k, js] = (5.0 / 8.0) * (-9.0 * (x * x) + 8.0 * rc * x
+                                                  + math.pow(ri, 4) / (x * x))
         f["current"][2][:, js] = (5.0 * 3.0 / 2.0) * r
-        f["magne"][2][:, jt] = (10.0 / 3.0) * r * np.sin(pi * (r - self.r_icb))
+        sin_ = np.array([math.sin(pi * (x - self.r_icb)) for x in r])   # libm, as Fortran
+        cos_ = np.array([math.cos(pi * (x - self.r_icb)) for x in r])
+        f["magne"][2][:, jt] = (10.0 / 3.0) * r * sin_
         f["current"][0][:, jt] = f["magne"][2][:, jt]
-        f["current"][1][:, jt] = (10.0 / 3.0) * (np.sin(pi * (r - self.r_icb))
-                                                 + pi * r * np.cos(pi * (r - self.r_icb)))
+        f["current"][1][:, jt] = (10.0 / 3.0) * (sin_ + pi * r * cos_)
         self.sync_temp_by_per_temp()
         # set_sph_field_to_start
         self.grad_vp_and_vorticity()

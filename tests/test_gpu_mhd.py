@@ -1,0 +1,85 @@
+"""The device-resident MHD step against the oracle's restatement (stage by
+stage, after the initialisation and after a few steps) and against the
+reference's own Dynamobench_case1 energies (sph_pwr_volume_s.dat, 1e-9
+relative, compare_sph_mean_square.f90:96-122)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from calypso_amd import mhd
+from oracle import mhd_step
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "dynamobench_case1_energies.json")
+
+# (oracle field, component, product component id)
+COMPARED = [("velo", 0, 0), ("velo", 1, 1), ("velo", 2, 2), ("vort", 0, 3), ("vort", 1, 4),
+            ("vort", 2, 5), ("magne", 0, 6), ("magne", 1, 7), ("magne", 2, 8),
+            ("current", 0, 9), ("current", 1, 10), ("current", 2, 11), ("temp", None, 12),
+            ("w_diffuse", 0, 13), ("w_diffuse", 2, 15), ("b_diffuse", 0, 16),
+            ("b_diffuse", 2, 18), ("t_diffuse", None, 19), ("forces", 0, 20),
+            ("forces", 2, 22), ("induction", 0, 26), ("induction", 2, 28),
+            ("h_advect", None, 32), ("rot_buoyancy", None, 34)]
+
+
+def _compare(sim, ref, tol, label):
+    errs = {}
+    for name, c, comp in COMPARED:
+        want = ref.f[name] if c is None else ref.f[name][c]
+        got = sim.get_field(comp)
+        scale = max(np.abs(want).max(), 1e-300)
+        errs[f"{name}[{c}]"] = np.abs(got - want).max() / scale
+    print(label, " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
+    bad = {k: v for k, v in errs.items() if not v < tol}
+    assert not bad, f"{label}: relative errors above {tol}: {bad}"
+    return max(errs.values())
+
+
+@pytest.fixture(scope="module")
+def pair(torch_cuda):
+    print("[mhd] Dynamobench case 1 grid, FP64", flush=True)
+    p, r = mhd.dynamobench_case1()
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_dynamobench_ic(400)
+    sim.initialize()
+    ref = mhd_step.MhdStep()
+    ref.initialize()
+    yield sim, ref
+    sim.close()
+
+
+def test_initialize_matches_oracle(pair):
+    sim, ref = pair
+    print("[mhd] after initialisation", flush=True)
+    _compare(sim, ref, 1e-11, "init")
+
+
+def test_steps_match_oracle(pair):
+    sim, ref = pair
+    for n in (1, 2, 3):
+        print(f"[mhd] step {n}", flush=True)
+        sim.step(1)
+        ref.step()
+        _compare(sim, ref, 1e-10, f"step {n}")
+        assert sim.i_step == n
+    np.testing.assert_allclose(sim.energies(), ref.energies(), rtol=1e-11)
+
+
+def test_dynamobench_case1_energies(torch_cuda):
+    gold = json.load(open(GOLD))
+    print("[mhd] 100 steps of Dynamobench case 1", flush=True)
+    p, r = mhd.dynamobench_case1()
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_dynamobench_ic(400)
+    sim.initialize()
+    got = {}
+    for _ in range(2):
+        sim.step(50)
+        got[str(sim.i_step)] = sim.energies()
+    sim.close()
+    for step, want in gold["steps"].items():
+        print(f"[mhd] step {step}: {got[step]}", flush=True)
+        np.testing.assert_allclose(got[step], np.array(want), rtol=gold["rtol"], atol=0)
