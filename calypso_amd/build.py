@@ -57,16 +57,17 @@ def build_fortran():
         return None
     obj = os.path.join(LIB, "fobj")
     os.makedirs(obj, exist_ok=True)
-    srcs = ["calypso_amd_capi.f90", "legendre_trans_hip.f90"]
+    srcs = ["calypso_amd_capi.f90", "legendre_trans_hip.f90", "sph_analyzer_mhd_hip.f90"]
     for s in srcs:
         _run([fc, "-O2", "-fPIC", "-c", os.path.join(FSRC, s), "-module-dir", LIB,
               "-o", os.path.join(obj, s.replace(".f90", ".o"))])
     so = os.path.join(LIB, "libcalypso_amd_f.so")
     _run([fc, "-shared", "-o", so, *[os.path.join(obj, s.replace(".f90", ".o")) for s in srcs],
           "-L", LIB, "-lcalypso_amd", "-Wl,-rpath,$ORIGIN"])
-    drv = os.path.join(LIB, "leg_driver")
-    _run([fc, "-O2", os.path.join(FSRC, "leg_driver.f90"), "-I", LIB, "-o", drv,
-          "-L", LIB, "-lcalypso_amd_f", "-lcalypso_amd", "-Wl,-rpath,$ORIGIN"])
+    for prog in ("leg_driver", "mhd_driver"):
+        _run([fc, "-O2", os.path.join(FSRC, prog + ".f90"), "-I", LIB,
+              "-o", os.path.join(LIB, prog), "-L", LIB, "-lcalypso_amd_f", "-lcalypso_amd",
+              "-Wl,-rpath,$ORIGIN"])
     return so
 
 

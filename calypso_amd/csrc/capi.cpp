@@ -486,6 +486,14 @@ int calypso_mhd_info(calypso_mhd_t *mhd, long long *info)
     return 0;
 }
 
+int calypso_mhd_set_option(calypso_mhd_t *mhd, const char *name, long long value)
+{
+    if (!mhd || !name) return fail("calypso_mhd_set_option: null argument");
+    std::string err;
+    if (calypso::mhd_set_option(&mhd->c, name, value, &err)) return fail(err);
+    return 0;
+}
+
 int calypso_mhd_coriolis_table(int l_truncation, double *cor, int *idx)
 {
     if (l_truncation < 1 || !cor || !idx) return fail("calypso_mhd_coriolis_table: bad argument");

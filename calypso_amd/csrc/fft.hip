@@ -441,6 +441,11 @@ int fft_forward_dev(const FftContext *f, int ncomp, const double *d_frc_rtp, dou
     return launch_pencil(f, kPencilFwd, ncomp, ncomp, d_frc_rtp, d_vr_rtm, nullptr, s, err);
 }
 
+bool fft_mhd_fused_fits(const FftContext *f)
+{
+    return pencil_lds(f, 19, 7) <= 160 * 1024;
+}
+
 int fft_mhd_fused_dev(const FftContext *f, const double *d_bwd_rtm, double *d_fwd_rtm,
                       const double *coef, hipStream_t s, std::string *err)
 {

@@ -883,6 +883,12 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     if (leg_context_build(&c->leg, L, nth, nr, radius, nullptr, nullptr, 19, 6, 1, leg_mode, err))
         return -1;
     if (fft_context_build(&c->fft, L, nth, nr, nphi, err)) return -1;
+    c->fused_fft = fft_mhd_fused_fits(&c->fft);
+    if (!c->fused_fft) {   // long pencils: grids in HBM between the three stages
+        const size_t nrtp = (size_t)nr * nth * nphi;
+        HIP_TRY(hipMalloc(&c->d_grid_b, sizeof(double) * 13 * nrtp));
+        HIP_TRY(hipMalloc(&c->d_grid_f, sizeof(double) * 19 * nrtp));
+    }
     HIP_TRY(hipDeviceSynchronize());
     return 0;
 }
@@ -892,7 +898,7 @@ void mhd_free(MhdContext *c)
     leg_context_free(&c->leg);
     fft_context_free(&c->fft);
     double *bufs[] = {c->d_rj, c->d_sp_b, c->d_sp_f, c->d_vr_b, c->d_vr_f, c->d_consts,
-                      c->d_radius, c->d_scratch, c->d_ms_out};
+                      c->d_radius, c->d_scratch, c->d_ms_out, c->d_grid_b, c->d_grid_f};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (c->d_iconsts) (void)hipFree(c->d_iconsts);
@@ -900,6 +906,29 @@ void mhd_free(MhdContext *c)
         if (c->d_lu[q]) (void)hipFree(c->d_lu[q]);
         if (c->d_piv[q]) (void)hipFree(c->d_piv[q]);
     }
+}
+
+int mhd_set_option(MhdContext *c, const std::string &name, long long value, std::string *err)
+{
+    if (name == "fused_fft") {
+        if (value && !fft_mhd_fused_fits(&c->fft)) {
+            *err = "fused_fft: the pencil does not fit in LDS at this size";
+            return -1;
+        }
+        if (!value && !c->d_grid_b) {
+            const size_t nrtp = (size_t)c->nr * c->nth * c->nphi;
+            HIP_TRY(hipMalloc(&c->d_grid_b, sizeof(double) * 13 * nrtp));
+            HIP_TRY(hipMalloc(&c->d_grid_f, sizeof(double) * 19 * nrtp));
+        }
+        c->fused_fft = value != 0;
+        return 0;
+    }
+    if (name == "legendre_autotune") {
+        c->leg.autotune = value != 0;
+        return 0;
+    }
+    *err = "unknown option " + name;
+    return -1;
 }
 
 int mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err)
@@ -974,7 +1003,15 @@ int nonlinear(MhdContext *c, bool temp_roundtrip, std::string *err)
     HIP_TRY(hipGetLastError());
     if (leg_backward_dev(&c->leg, 13, 4, 1, c->d_sp_b, c->d_vr_b, s, err)) return -1;
     const double coef[4] = {c->co.coef_velo, c->co.coef_lor, c->co.coef_induct, c->co.coef_temp};
-    if (fft_mhd_fused_dev(&c->fft, c->d_vr_b, c->d_vr_f, coef, s, err)) return -1;
+    if (c->fused_fft) {
+        if (fft_mhd_fused_dev(&c->fft, c->d_vr_b, c->d_vr_f, coef, s, err)) return -1;
+    } else {
+        const size_t nrtp = (size_t)c->nr * c->nth * c->nphi;
+        if (fft_backward_dev(&c->fft, 13, c->d_vr_b, c->d_grid_b, s, err) ||
+            products_mhd_dev(nrtp, c->d_grid_b, c->d_grid_f, coef, s, err) ||
+            fft_forward_dev(&c->fft, 19, c->d_grid_f, c->d_vr_f, s, err))
+            return -1;
+    }
     if (leg_forward_dev(&c->leg, 19, 6, 1, c->d_vr_f, c->d_sp_f, s, err)) return -1;
     hipLaunchKernelGGL(k_unpack, dim3(nb), dim3(256), 0, s, p);
     HIP_TRY(hipGetLastError());

@@ -69,6 +69,8 @@ struct MhdContext {
     // device allocations
     double *d_rj = nullptr, *d_sp_b = nullptr, *d_sp_f = nullptr;
     double *d_vr_b = nullptr, *d_vr_f = nullptr;
+    bool fused_fft = true;        // one pencil kernel: FFT, products, FFT
+    double *d_grid_b = nullptr, *d_grid_f = nullptr;   // grids when not fused
     double *d_consts = nullptr;   // r, ar1, ar2, d1, d2, g1, g3, t_icb, t_cmb, reft0, reft1, cor
     int *d_iconsts = nullptr;     // jk1, jk2, jl1, rj_of_rlm, rlm_of_rj, l_of_j
     int *d_l_of_j = nullptr;
@@ -87,6 +89,7 @@ int  mhd_initialize(MhdContext *c, std::string *err);
 int  mhd_steps(MhdContext *c, int nsteps, std::string *err);
 int  mhd_energies(MhdContext *c, double *out7, std::string *err);
 int  mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err);
+int  mhd_set_option(MhdContext *c, const std::string &name, long long value, std::string *err);
 
 // closed forms of the Coriolis couplings (exposed for the host tests)
 double gaunt_p1(int l, int m);   // K(1,0; l-1,m; l,m)

@@ -10,6 +10,20 @@
       integer(c_int), parameter :: CALYPSO_LEG_FP32 =   2
       integer(c_int), parameter :: CALYPSO_LEG_BF16X3 = 3
 !
+!>    calypso_mhd_params of include/calypso_amd.h (same field order)
+      type, bind(C) :: calypso_mhd_params
+        integer(c_int) :: l_truncation, nth, nphi, nr
+        integer(c_int) :: legendre_mode
+        real(c_double) :: dt
+        real(c_double) :: coef_imp_v, coef_imp_t, coef_imp_b
+        real(c_double) :: coef_velo, coef_d_velo, coef_buo, coef_cor
+        real(c_double) :: coef_lor
+        real(c_double) :: coef_temp, coef_d_temp
+        real(c_double) :: coef_magne, coef_d_magne, coef_induct
+        real(c_double) :: depth_high_t, depth_low_t
+        real(c_double) :: high_temp, low_temp
+      end type calypso_mhd_params
+!
       interface
 !
         function calypso_last_error() bind(C, name="calypso_last_error")
@@ -55,6 +69,64 @@
      &      bind(C, name="calypso_sel_finalize_legendre_trans")
           import :: c_int
         end function calypso_sel_finalize_legendre_trans
+!
+        integer(c_int) function calypso_mhd_create(params, radius,      &
+     &      stream, mhd) bind(C, name="calypso_mhd_create")
+          import :: c_int, c_double, c_ptr, calypso_mhd_params
+          type(calypso_mhd_params), intent(in) :: params
+          real(c_double), intent(in) :: radius(*)
+          type(c_ptr), value :: stream
+          type(c_ptr), intent(out) :: mhd
+        end function calypso_mhd_create
+!
+        integer(c_int) function calypso_mhd_destroy(mhd)                &
+     &      bind(C, name="calypso_mhd_destroy")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: mhd
+        end function calypso_mhd_destroy
+!
+        integer(c_int) function calypso_mhd_set_field(mhd, comp, d)     &
+     &      bind(C, name="calypso_mhd_set_field")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: mhd
+          integer(c_int), value :: comp
+          real(c_double), intent(in) :: d(*)
+        end function calypso_mhd_set_field
+!
+        integer(c_int) function calypso_mhd_get_field(mhd, comp, d)     &
+     &      bind(C, name="calypso_mhd_get_field")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: mhd
+          integer(c_int), value :: comp
+          real(c_double), intent(inout) :: d(*)
+        end function calypso_mhd_get_field
+!
+        integer(c_int) function calypso_mhd_set_dynamobench_ic(mhd,     &
+     &      isig) bind(C, name="calypso_mhd_set_dynamobench_ic")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: mhd
+          integer(c_int), value :: isig
+        end function calypso_mhd_set_dynamobench_ic
+!
+        integer(c_int) function calypso_mhd_initialize(mhd)             &
+     &      bind(C, name="calypso_mhd_initialize")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: mhd
+        end function calypso_mhd_initialize
+!
+        integer(c_int) function calypso_mhd_step(mhd, nsteps)           &
+     &      bind(C, name="calypso_mhd_step")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: mhd
+          integer(c_int), value :: nsteps
+        end function calypso_mhd_step
+!
+        integer(c_int) function calypso_mhd_energies(mhd, ene)          &
+     &      bind(C, name="calypso_mhd_energies")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: mhd
+          real(c_double), intent(inout) :: ene(7)
+        end function calypso_mhd_energies
 !
       end interface
 !

@@ -130,6 +130,11 @@ class MhdSimulation:
     def step(self, nsteps=1):
         _lib.check(_lib.load().calypso_mhd_step(self._h, nsteps), "calypso_mhd_step")
 
+    def set_option(self, name: str, value: int):
+        """"fused_fft" (0/1) or "legendre_autotune" (0/1)."""
+        _lib.check(_lib.load().calypso_mhd_set_option(self._h, name.encode(), int(value)),
+                   "calypso_mhd_set_option")
+
     def energies(self) -> np.ndarray:
         out = np.zeros(7)
         _lib.check(_lib.load().calypso_mhd_energies(self._h, out.ctypes.data_as(_P)),

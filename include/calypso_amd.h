@@ -241,6 +241,10 @@ int calypso_mhd_step(calypso_mhd_t *mhd, int nsteps);
 int calypso_mhd_energies(calypso_mhd_t *mhd, double *out);
 /* info[4]: i_step, nr, J, number of d_rj components. */
 int calypso_mhd_info(calypso_mhd_t *mhd, long long *info);
+/* Execution options: "fused_fft" (1: FFT, products and FFT in one pencil
+ * kernel, the default where the pencil fits in LDS; 0: three kernels with
+ * the grids in HBM), "legendre_autotune" (0/1). */
+int calypso_mhd_set_option(calypso_mhd_t *mhd, const char *name, long long value);
 /* Host-only: coupling table of the curl of the Coriolis force for
  * truncation L (init_sum_coriolis_rlm, const_coriolis_sph_rlm.f90:34-67).
  * cor[12*J]: sw11 sw21 sw12 sw22 sw13 tw13 tw23 tw14 tw24 tw11 tw12 g3 per rj
