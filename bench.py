@@ -7,7 +7,8 @@ transform, inputs resident in HBM.  Every precision mode runs; each mode's
 output is compared with the FP64 GPU path, and the FP64 path with the
 reference's own org-loop transforms (oracle/_ref, compiled from the reference
 sources) on a bounded sample of radial levels, timed on the host cores as the
-CPU baseline.
+CPU baseline.  The full MHD step (Dynamobench case 1 physics) is timed as
+well at three sizes and reported under "mhd_step".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes ...]
 Prints one JSON line.
@@ -123,15 +124,47 @@ def cpu_baseline(calypso_amd, radius, sp_host, vr_gpu_host, nsample, threads):
             "gpu_fp64_err_vs_cpu": float(max(errs))}
 
 
+MHD_CONFIGS = [   # (name, Lmax, nth, nphi, radial intervals, timed steps)
+    ("dynamobench_case1_L31", 31, 48, 96, 48, 200),
+    ("shell_L127_Nr65", 127, 192, 384, 64, 40),
+    ("shell_L255_Nr97", 255, 384, 768, 96, 8),
+]
+
+
+def mhd_bench(torch):
+    """Full MHD step (Dynamobench case 1 physics) at three sizes, FP64."""
+    from calypso_amd import mhd
+    out = {}
+    for name, L_, nth, nphi, nri, steps in MHD_CONFIGS:
+        p, r = mhd.dynamobench_case1(l_truncation=L_, nth=nth, nphi=nphi, nri=nri)
+        sim = mhd.MhdSimulation(p, r, stream=torch.cuda.current_stream().cuda_stream)
+        sim.set_dynamobench_ic(400)
+        sim.initialize()
+        sim.step(3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        e = sim.energies()
+        sim.close()
+        out[name] = {"Lmax": L_, "nth": nth, "nphi": nphi, "Nr": nri + 1, "steps": steps,
+                     "ms_per_step": ms, "steps_per_s": 1e3 / ms,
+                     "energies_finite": bool(np.all(np.isfinite(e)))}
+        print(f"# mhd {name}: {ms:.3f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--modes", default="hip_fp64,hip_fp32,hip_bf16x3")
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=96)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mhd", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded "
@@ -190,6 +223,8 @@ def main():
                      "peak": PEAK[head], "unit": "TFLOP/s", "frac": achieved / PEAK[head],
                      "traffic": None},
     }
+    if not args.no_mhd:
+        line["mhd_step"] = mhd_bench(torch)
     if not args.no_cpu:
         vr_host = ref_vr.cpu().numpy() if ref_vr is not None else res[modes[0]]["vr"].cpu().numpy()
         line["cpu_baseline"] = cpu_baseline(calypso_amd, radius, sp.cpu().numpy(), vr_host,
