@@ -32,6 +32,8 @@ SIGNATURES = [
     ("calypso_leg_stream", _P, [_P]),
     ("calypso_leg_plan_info", _I, [_P, _I, _I, _I, _I, _P]),
     ("calypso_leg_set_autotune", _I, [_P, _I]),
+    ("calypso_leg_plan_count", _I, [_P, _I, _I, _I, _I]),
+    ("calypso_leg_force_plan", _I, [_P, _I, _I, _I, _I, _I]),
     ("calypso_sel_init_legendre_trans", _I, [_I, _I, _I, _I, _P, _P, _P, _I, _I, _I]),
     ("calypso_sel_backward_legendre_trans", _I, [_I, _I, _I, _I, _I, _P, _P]),
     ("calypso_sel_forward_legendre_trans", _I, [_I, _I, _I, _I, _I, _P, _P]),

@@ -211,6 +211,28 @@ int calypso_leg_plan_info(calypso_leg_t *ctx, int ncomp, int nvector, int nscala
     info[5] = p->tc;
     info[6] = p->tuned_ms < 0 ? -1 : (long long)(p->tuned_ms * 1000.0f);
     info[7] = p->n_items;
+    info[8] = p->ver;
+    info[9] = p->nw;
+    info[10] = p->pd;
+    return 0;
+}
+
+int calypso_leg_plan_count(calypso_leg_t *ctx, int ncomp, int nvector, int nscalar, int fwd)
+{
+    if (!ctx) return fail("calypso_leg_plan_count: null argument");
+    std::string err;
+    const int n = calypso::leg_plan_count(&ctx->c, ncomp, nvector, nscalar, fwd != 0, &err);
+    if (n < 0) return fail(err);
+    return n;
+}
+
+int calypso_leg_force_plan(calypso_leg_t *ctx, int ncomp, int nvector, int nscalar, int fwd,
+                           int idx)
+{
+    if (!ctx) return fail("calypso_leg_force_plan: null argument");
+    std::string err;
+    if (calypso::leg_force_plan(&ctx->c, ncomp, nvector, nscalar, fwd != 0, idx, &err))
+        return fail(err);
     return 0;
 }
 

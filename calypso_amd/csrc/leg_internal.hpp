@@ -36,6 +36,10 @@ struct LegPlan {
     int mct1 = 1, mct2 = 1; // column-tile capacity of the kernel instance
     int pair = 0;           // fwd: orders m and L-m in one work item
     int tc = 64;            // fwd: colatitudes per LDS chunk
+    int ver = 1;            // kernel form (2: one work-group per order pair,
+                            // double-buffered staging)
+    int nw = 4;             // waves per work-group
+    int pd = 2;             // K steps the P fragments are fetched ahead
     float tuned_ms = -1.f;  // autotuner timing of the chosen plan
     int n_items = 0;
     WorkItem *d_items = nullptr;
@@ -83,6 +87,10 @@ int  leg_forward_dev(LegContext *c, int ncomp, int nvec, int nscl,
                      std::string *err);
 const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
                         std::string *err);
+int  leg_plan_count(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
+                    std::string *err);
+int  leg_force_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, int idx,
+                    std::string *err);
 int  leg_tables_to_host(LegContext *c, double *P, double *dP,
                         std::string *err);
 

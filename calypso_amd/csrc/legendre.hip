@@ -313,7 +313,13 @@ constexpr int kThreads = 256;
 constexpr int kChunkB = 32;   // degree pairs staged per LDS chunk (bwd)
 constexpr int kChunkF = 64;   // largest colatitude chunk staged in LDS (fwd)
 
-template <class T> __device__ inline int lds_pad() { return T::EPL == 8 ? 8 : 1; }
+// Row padding of the LDS operand tiles.  A B fragment read has 16 columns x 4
+// K values per wavefront; a row stride of 2 (mod 32) doubles or 4 (mod 64)
+// floats puts the 64 lanes of one read on distinct banks.
+template <class T> __host__ __device__ constexpr int lds_pad()
+{
+    return T::EPL == 8 ? 8 : (sizeof(typename T::elem) == 8 ? 2 : 4);
+}
 
 // ------------------------------------------------------- backward ------
 
@@ -325,7 +331,7 @@ template <class T> __device__ inline int lds_pad() { return T::EPL == 8 ? 8 : 1;
 //      v_theta(sm) = sum dP dS(sm)/r + (sm/sin) sum P T(-sm)/r
 //      v_phi(sm)   = -sum dP T(sm)/r + (sm/sin) sum P dS(-sm)/r
 // (legendre_bwd_trans_org.f90:71-151 with the +-m partner mn = M-mp+1).
-template <class T, int RPW, int MCT1, int MCT2>
+template <class T, int RPW, int MCT1, int MCT2, int PD>
 __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
 {
     using elem = typename T::elem;
@@ -336,6 +342,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
     const int KCP = KC + lds_pad<T>();
 
     const WorkItem it = a.items[blockIdx.x];
+    if (it.m < 0) return;   // padding of the XCD interleave
     const int m = it.m, kt0 = it.ktile * a.KT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int K = a.L - m + 1, K0 = (K + 1) / 2;
@@ -350,6 +357,9 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     elem *X = reinterpret_cast<elem *>(smem_raw);  // [NS][2 par][ncol][KCP]
     const int slice = 2 * ncol * KCP;
+    __shared__ double rinv[8];                     // 1/r of the item's radii
+    if (threadIdx.x < a.KT && threadIdx.x < 8)
+        rinv[threadIdx.x] = (kt0 + (int)threadIdx.x < a.nr) ? 1.0 / a.radius[kt0 + threadIdx.x] : 0.0;
 
     // per-lane B column descriptors
     int b1col[MCT1];
@@ -399,97 +409,97 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
     const long long ubase = a.off[m];
     constexpr int UE = unit_elems<T>();
 
+    // P fragments run PD K steps ahead of the MFMAs in a register ring
+    // (slot = step % PD; chunks start on multiples of PD)
+    frag Aq[NS == 1 ? PD : 1][RPW][6];
+    auto load_a = [&](frag (&A)[RPW][6], int ks) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int rt = (it.rgroup * 4 + wave) * RPW + r;
+            if (rt >= a.RT) continue;
+            const elem *ap = tab + (ubase + ((long long)rt * KS + ks) * 6) * UE + lane * EPL;
+#pragma unroll
+            for (int w = 0; w < 6; ++w) A[r][w] = T::load(ap + w * UE);
+        }
+    };
+    if constexpr (NS == 1) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d)
+            if (d < KS) load_a(Aq[d], d);
+    }
+
     for (int c0 = 0; c0 < KS * KS_; c0 += KC) {
         __syncthreads();
         // stage degree pairs [c0, c0+KC) of both parities, both +-m, KT radii
-        const int nstage = 2 * KC * ncol;
-        for (int e = threadIdx.x; e < nstage; e += kThreads) {
-            int comp = e % ncomp, rest = e / ncomp;
-            int li = rest % (2 * KC), rest2 = rest / (2 * KC);
-            int sg = rest2 & 1, kk = rest2 >> 1;
-            int l = m + 2 * c0 + li;
-            double v = 0.0;
-            if (l <= a.L && kk < nkk && (sg == 0 || m > 0)) {
-                int k = kt0 + kk;
-                int j = (sg ? j0n : j0p) + (l - m);
-                v = a.in[comp + (size_t)ncomp * (j + (size_t)a.J * k)];
-                double a1r = 1.0 / a.radius[k];
-                if (comp < 3 * nv) {
-                    if (comp % 3 == 0) {
-                        double g3 = (l == 0) ? 0.5 : (double)(l * (l + 1));
-                        v = v * (a1r * a1r) * g3;
-                    } else {
-                        v = v * a1r;
-                    }
+        // (degree, component) pairs over the threads; the (radius, sign)
+        // columns in an inner loop, so each thread divides once per pair
+        for (int q = threadIdx.x; q < 2 * KC * ncomp; q += kThreads) {
+            const int comp = q % ncomp, li = q / ncomp;
+            const int l = m + 2 * c0 + li;
+            const int par = li & 1, il = li >> 1;
+            const bool lok = l <= a.L;
+            const double g3 = (l == 0) ? 0.5 : (double)(l * (l + 1));
+            const int kind = comp < 3 * nv ? (comp % 3 == 0 ? 0 : 1) : 2;
+            for (int g = 0; g < 2 * a.KT; ++g) {
+                const int sg = g & 1, kk = g >> 1;
+                double v = 0.0;
+                if (lok && kk < nkk && (sg == 0 || m > 0)) {
+                    const int k = kt0 + kk;
+                    const int j = (sg ? j0n : j0p) + (l - m);
+                    v = a.in[comp + (size_t)ncomp * (j + (size_t)a.J * k)];
+                    const double a1r = rinv[kk];
+                    if (kind == 0) v = v * (a1r * a1r) * g3;
+                    else if (kind == 1) v = v * a1r;
                 }
+                T::store(X + (par * ncol + g * ncomp + comp) * KCP + il, v, slice);
             }
-            int par = li & 1, il = li >> 1;
-            int col = (kk * 2 + sg) * ncomp + comp;
-            T::store(X + (par * ncol + col) * KCP + il, v, slice);
         }
         __syncthreads();
 
         const int ks_end = min(KS, (c0 + KC) / KS_);
         if constexpr (NS == 1) {
-            // single-slice operands: B fragments loaded once per K step and
-            // shared by the wave's row tiles; the next step's P fragments
-            // are fetched while this step's MFMAs run
-            frag Acur[RPW][6];
-            auto load_a = [&](frag (&A)[RPW][6], int ks) {
+            for (int kb = c0 / KS_; kb < ks_end; kb += PD) {
 #pragma unroll
-                for (int r = 0; r < RPW; ++r) {
-                    const int rt = (it.rgroup * 4 + wave) * RPW + r;
-                    if (rt >= a.RT) continue;
-                    const elem *ap = tab + (ubase + ((long long)rt * KS + ks) * 6) * UE + lane * EPL;
-#pragma unroll
-                    for (int w = 0; w < 6; ++w) A[r][w] = T::load(ap + w * UE);
-                }
-            };
-            load_a(Acur, c0 / KS_);
-            for (int ks = c0 / KS_; ks < ks_end; ++ks) {
-                const int kl = (ks * KS_ - c0) + (KS_ / 4) * (lane >> 4);
-                frag Anx[RPW][6];
-                if (ks + 1 < ks_end) load_a(Anx, ks + 1);
-                frag be[MCT1], bo[MCT1], xe[MCT2], xo[MCT2], ye[MCT2], yo[MCT2];
-#pragma unroll
-                for (int ct = 0; ct < MCT1; ++ct) {
-                    be[ct] = T::load(X + (0 * ncol + b1col[ct]) * KCP + kl);
-                    bo[ct] = T::load(X + (1 * ncol + b1col[ct]) * KCP + kl);
-                }
-#pragma unroll
-                for (int ct = 0; ct < MCT2; ++ct) {
-                    xe[ct] = T::load(X + (0 * ncol + b2x[ct]) * KCP + kl);
-                    xo[ct] = T::load(X + (1 * ncol + b2x[ct]) * KCP + kl);
-                    ye[ct] = T::load(X + (0 * ncol + b2y[ct]) * KCP + kl);
-                    yo[ct] = T::load(X + (1 * ncol + b2y[ct]) * KCP + kl);
-                    if (b2xn[ct]) { xe[ct] = T::neg(xe[ct]); xo[ct] = T::neg(xo[ct]); }
-                    if (b2yn[ct]) { ye[ct] = T::neg(ye[ct]); yo[ct] = T::neg(yo[ct]); }
-                }
-#pragma unroll
-                for (int r = 0; r < RPW; ++r) {
-                    const int rt = (it.rgroup * 4 + wave) * RPW + r;
-                    if (rt >= a.RT) continue;
+                for (int d = 0; d < PD; ++d) {
+                    const int ks = kb + d;
+                    if (ks >= ks_end) break;
+                    const int kl = (ks * KS_ - c0) + (KS_ / 4) * (lane >> 4);
+                    frag be[MCT1], bo[MCT1], xe[MCT2], xo[MCT2], ye[MCT2], yo[MCT2];
 #pragma unroll
                     for (int ct = 0; ct < MCT1; ++ct) {
-                        if (ct >= nct1) break;
-                        T::mma(a1[r][ct][0], &Acur[r][0], &be[ct]);
-                        T::mma(a1[r][ct][1], &Acur[r][1], &bo[ct]);
+                        be[ct] = T::load(X + (0 * ncol + b1col[ct]) * KCP + kl);
+                        bo[ct] = T::load(X + (1 * ncol + b1col[ct]) * KCP + kl);
                     }
 #pragma unroll
                     for (int ct = 0; ct < MCT2; ++ct) {
-                        if (ct >= nct2) break;
-                        // symmetric set: dP odd, PM even; antisymmetric: dP even, PM odd
-                        T::mma(a2[r][ct][0], &Acur[r][3], &xo[ct]);
-                        T::mma(a2[r][ct][0], &Acur[r][4], &ye[ct]);
-                        T::mma(a2[r][ct][1], &Acur[r][2], &xe[ct]);
-                        T::mma(a2[r][ct][1], &Acur[r][5], &yo[ct]);
+                        xe[ct] = T::load(X + (0 * ncol + b2x[ct]) * KCP + kl);
+                        xo[ct] = T::load(X + (1 * ncol + b2x[ct]) * KCP + kl);
+                        ye[ct] = T::load(X + (0 * ncol + b2y[ct]) * KCP + kl);
+                        yo[ct] = T::load(X + (1 * ncol + b2y[ct]) * KCP + kl);
+                        if (b2xn[ct]) { xe[ct] = T::neg(xe[ct]); xo[ct] = T::neg(xo[ct]); }
+                        if (b2yn[ct]) { ye[ct] = T::neg(ye[ct]); yo[ct] = T::neg(yo[ct]); }
                     }
-                }
-                if (ks + 1 < ks_end) {
 #pragma unroll
-                    for (int r = 0; r < RPW; ++r)
+                    for (int r = 0; r < RPW; ++r) {
+                        const int rt = (it.rgroup * 4 + wave) * RPW + r;
+                        if (rt >= a.RT) continue;
 #pragma unroll
-                        for (int w = 0; w < 6; ++w) Acur[r][w] = Anx[r][w];
+                        for (int ct = 0; ct < MCT1; ++ct) {
+                            if (ct >= nct1) break;
+                            T::mma(a1[r][ct][0], &Aq[d][r][0], &be[ct]);
+                            T::mma(a1[r][ct][1], &Aq[d][r][1], &bo[ct]);
+                        }
+#pragma unroll
+                        for (int ct = 0; ct < MCT2; ++ct) {
+                            if (ct >= nct2) break;
+                            // symmetric set: dP odd, PM even; antisymmetric: dP even, PM odd
+                            T::mma(a2[r][ct][0], &Aq[d][r][3], &xo[ct]);
+                            T::mma(a2[r][ct][0], &Aq[d][r][4], &ye[ct]);
+                            T::mma(a2[r][ct][1], &Aq[d][r][2], &xe[ct]);
+                            T::mma(a2[r][ct][1], &Aq[d][r][5], &yo[ct]);
+                        }
+                    }
+                    if (ks + PD < KS) load_a(Aq[d], ks + PD);
                 }
             }
         } else {
@@ -608,7 +618,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd(LegArgs a)
 // A work-group takes the orders m and L-m together (the pairing of the
 // reference's order distribution, set_sph_tranform_ordering.f90:69-159), so
 // every group has ~(L+2)/16 degree rows whatever m is.
-template <class T, int RPW, int MCT1, int MCT2>
+template <class T, int RPW, int MCT1, int MCT2, int PD>
 __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
 {
     using elem = typename T::elem;
@@ -619,6 +629,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
     const int TCP = TC + lds_pad<T>();
 
     const WorkItem it = a.items[blockIdx.x];
+    if (it.m < 0) return;   // padding of the XCD interleave
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // orders of this item: m_0 = it.m, m_1 = L - it.m (if different)
     const int nm = (a.pair && a.L - it.m != it.m) ? 2 : 1;
@@ -718,61 +729,376 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
                + lane * EPL;
     };
 
+    auto load_a = [&](frag (&A)[RPW][3][NS], int ks) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            if (!rok[r]) continue;
+            const elem *ap = aptr(r, ks);
+#pragma unroll
+            for (int w = 0; w < 3; ++w)
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2) A[r][w][s2] = T::load(ap + w * UE + s2 * 64 * EPL);
+        }
+    };
+    auto step = [&](const frag (&A)[RPW][3][NS], int ks, int t0) {
+        const int kl = (ks * KS_ - t0) + (KS_ / 4) * (lane >> 4);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            if (!rok[r]) continue;
+            const int par = rpar[r];
+            const int cofs = ri[r] * ncolm;
+            const int fP = par, fD = 1 - par;  // fold index: 0 sym, 1 anti
+#pragma unroll
+            for (int ct = 0; ct < MCT1; ++ct) {
+                if (ct >= nct1) break;
+                frag b[NS];
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2)
+                    b[s2] = T::load(V + s2 * slice + (fP * ncol + cofs + b1col[ct]) * TCP + kl);
+                T::mma(a1[r][ct], A[r][0], b);
+            }
+#pragma unroll
+            for (int ct = 0; ct < MCT2; ++ct) {
+                if (ct >= nct2) break;
+                frag x[NS], y[NS];
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2) {
+                    x[s2] = T::load(V + s2 * slice + (fD * ncol + cofs + b2x[ct]) * TCP + kl);
+                    y[s2] = T::load(V + s2 * slice + (fP * ncol + cofs + b2y[ct]) * TCP + kl);
+                    if (b2xn[ct]) x[s2] = T::neg(x[s2]);
+                    if (b2yn[ct]) y[s2] = T::neg(y[s2]);
+                }
+                T::mma(a2[r][ct], A[r][1], x);
+                T::mma(a2[r][ct], A[r][2], y);
+            }
+        }
+    };
+    // single-slice P fragments run PD K steps ahead in a register ring
+    // (slot = step % PD; chunks start on multiples of PD)
+    frag Aq[NS == 1 ? PD : 1][RPW][3][NS];
+    if constexpr (NS == 1) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d)
+            if (d < a.KSt) load_a(Aq[d], d);
+    }
+
     for (int t0 = 0; t0 < a.KSt * KS_; t0 += TC) {
         __syncthreads();
-        const int nstage = TC * ncol;
-        for (int e = threadIdx.x; e < nstage; e += kThreads) {
-            int comp = e % ncomp, rest = e / ncomp;
-            int tl = rest % TC, rest2 = rest / TC;
-            int sg = rest2 & 1, rest3 = rest2 >> 1;
-            int kk = rest3 % a.KT, i = rest3 / a.KT;
-            int t = t0 + tl;
-            double vs = 0.0, va = 0.0;
-            const int m = mm[i];
-            if (t < a.th_h && kk < nkk && (sg == 0 || m > 0)) {
-                int mp = a.L + (sg ? -m : m), k = kt0 + kk;
-                const double *src = a.in + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
-                int tm = a.nth - 1 - t;
-                double w = a.weight[t];
-                double v1 = src[(size_t)ncomp * t];
-                if (tm != t) {
-                    double v2 = src[(size_t)ncomp * tm];
-                    vs = w * (v1 + v2);
-                    va = w * (v1 - v2);
-                } else {
-                    vs = w * v1;
-                }
+        // (colatitude, component) pairs over the threads; the (order,
+        // radius, sign) columns in inner loops
+        for (int q = threadIdx.x; q < TC * ncomp; q += kThreads) {
+            const int comp = q % ncomp, tl = q / ncomp;
+            const int t = t0 + tl;
+            const bool tok = t < a.th_h;
+            const int tm = a.nth - 1 - t;
+            const double w = tok ? a.weight[t] : 0.0;
+            for (int i = 0; i < nm; ++i) {
+                const int m = mm[i];
+                for (int kk = 0; kk < a.KT; ++kk)
+                    for (int sg = 0; sg < 2; ++sg) {
+                        double vs = 0.0, va = 0.0;
+                        if (tok && kk < nkk && (sg == 0 || m > 0)) {
+                            const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
+                            const double *src = a.in + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
+                            const double v1 = src[(size_t)ncomp * t];
+                            if (tm != t) {
+                                const double v2 = src[(size_t)ncomp * tm];
+                                vs = w * (v1 + v2);
+                                va = w * (v1 - v2);
+                            } else {
+                                vs = w * v1;
+                            }
+                        }
+                        const int col = i * ncolm + (kk * 2 + sg) * ncomp + comp;
+                        T::store(V + (0 * ncol + col) * TCP + tl, vs, slice);
+                        T::store(V + (1 * ncol + col) * TCP + tl, va, slice);
+                    }
             }
-            int col = i * ncolm + (kk * 2 + sg) * ncomp + comp;
-            T::store(V + (0 * ncol + col) * TCP + tl, vs, slice);
-            T::store(V + (1 * ncol + col) * TCP + tl, va, slice);
         }
         __syncthreads();
 
         const int ks_end = min(a.KSt, (t0 + TC) / KS_);
-        frag Acur[RPW][3][NS];
-        auto load_a = [&](frag (&A)[RPW][3][NS], int ks) {
+        if constexpr (NS == 1) {
+            for (int kb = t0 / KS_; kb < ks_end; kb += PD) {
 #pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-                if (!rok[r]) continue;
-                const elem *ap = aptr(r, ks);
-#pragma unroll
-                for (int w = 0; w < 3; ++w)
-#pragma unroll
-                    for (int s2 = 0; s2 < NS; ++s2) A[r][w][s2] = T::load(ap + w * UE + s2 * 64 * EPL);
+                for (int d = 0; d < PD; ++d) {
+                    const int ks = kb + d;
+                    if (ks >= ks_end) break;
+                    step(Aq[d], ks, t0);
+                    if (ks + PD < a.KSt) load_a(Aq[d], ks + PD);
+                }
             }
-        };
-        load_a(Acur, t0 / KS_);
+        } else {
+            for (int ks = t0 / KS_; ks < ks_end; ++ks) {
+                frag A[RPW][3][NS];
+                load_a(A, ks);
+                step(A, ks, t0);
+            }
+        }
+    }
+
+    // epilogue: scale by r^2 g7, r g7 or g6 (sph_normalizations,
+    // spherical_harmonics.f90:121-139) and store in rlm order
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        if (!rok[r]) continue;
+        const int i_m = ri[r], par = rpar[r], rt = rtile[r];
+        const int m = mm[i_m];
+        const int Kp = par ? K1v[i_m] : K0v[i_m];
+#pragma unroll
+        for (int ct = 0; ct < MCT1 + MCT2; ++ct) {
+            const bool g1 = ct < MCT1;
+            const int cc = g1 ? ct : ct - MCT1;
+            if (g1 ? (cc >= nct1) : (cc >= nct2)) continue;
+            int c = cc * 16 + (lane & 15);
+            int kk, sg, comp, kind;  // kind 0: r^2 g7, 1: r g7, 2: g6
+            if (g1) {
+                if (c >= c1n) continue;
+                int per = nv + ns;
+                kk = c / (2 * per);
+                int rem = c % (2 * per);
+                sg = rem / per;
+                int q = rem % per;
+                comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+                kind = q < nv ? 0 : 2;
+            } else {
+                if (c >= c2n) continue;
+                kk = c / (4 * nv);
+                int rem = c % (4 * nv);
+                sg = rem / (2 * nv);
+                int r2 = rem % (2 * nv);
+                comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
+                kind = 1;
+            }
+            if (kk >= nkk || (sg == 1 && m == 0)) continue;
+            const int k = kt0 + kk;
+            const double rad = a.radius[k];
+            const int j0 = sg ? j0n[i_m] : j0p[i_m];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + T::row(lane, v);
+                if (i >= Kp) continue;
+                const int l = m + par + 2 * i;
+                double fac;
+                if (kind == 2) {
+                    fac = (l == 0) ? 0.5 : (m == 0 ? (double)(2 * l + 1) / 2.0
+                                                   : (double)(2 * l + 1) / 4.0);
+                } else {
+                    double g7 = (l == 0) ? 1.0
+                              : (m == 0 ? (double)(2 * l + 1) / (double)(2 * l * (l + 1))
+                                        : (double)(2 * l + 1) / (double)(4 * l * (l + 1)));
+                    fac = (kind == 0) ? rad * rad * g7 : rad * g7;
+                }
+                double val = (g1 ? T::get(a1[r][cc], v) : T::get(a2[r][cc], v)) * fac;
+                a.out[comp + (size_t)ncomp * ((j0 + (l - m)) + (size_t)a.J * k)] = val;
+            }
+        }
+    }
+}
+
+// Forward kernel, second form: one work-group holds every degree row of the
+// order pair (m, L-m) for KT radial points, so the folded input is staged
+// once per (pair, radial tile); the staging of colatitude chunk c+1 is
+// loaded into registers while chunk c feeds the MFMAs and written to the
+// other half of a double LDS buffer (one barrier per chunk).
+constexpr int kStageMax = 8;   // staged elements per thread per chunk
+
+template <class T, int RPW, int MCT1, int MCT2, int NW>
+__global__ __launch_bounds__(NW * 64) void k_leg_fwd2(LegArgs a)
+{
+    using elem = typename T::elem;
+    using frag = typename T::frag;
+    using acc = typename T::acc;
+    constexpr int KS_ = T::KSTEP, NS = T::NS, EPL = T::EPL, NT = NW * 64;
+    const int TC = a.TC;
+    const int TCP = TC + lds_pad<T>();
+
+    const WorkItem it = a.items[blockIdx.x];
+    if (it.m < 0) return;   // padding of the XCD interleave
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nm = (a.L - it.m != it.m) ? 2 : 1;
+    int mm[2], K0v[2], K1v[2], RTe[2], RTo[2], j0p[2], j0n[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int m = (i == 0) ? it.m : a.L - it.m;
+        mm[i] = m;
+        const int K = a.L - m + 1;
+        K0v[i] = (K + 1) / 2;
+        K1v[i] = K / 2;
+        RTe[i] = (K0v[i] + 15) / 16;
+        RTo[i] = (K1v[i] + 15) / 16;
+        j0p[i] = a.lstack[a.L + m];
+        j0n[i] = a.lstack[a.L - m];
+    }
+    const int rows0 = RTe[0] + RTo[0];
+    const int nrows = rows0 + (nm == 2 ? RTe[1] + RTo[1] : 0);
+    const int kt0 = it.ktile * a.KT;
+    const int nkk = min(a.KT, a.nr - kt0);
+    const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
+    const int ncolm = a.KT * 2 * ncomp;
+    const int ncol = 2 * ncolm;                 // room for both orders
+    const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 2 * 2 * nv;
+    const int nct1 = (c1n + 15) / 16, nct2 = (c2n + 15) / 16;
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    elem *Vb = reinterpret_cast<elem *>(smem_raw);  // [2 buf][NS][2 fold][ncol][TCP]
+    const int slice = 2 * ncol * TCP;
+    const int bufsz = NS * slice;
+
+    int b1col[MCT1];
+    int b2x[MCT2], b2y[MCT2];
+    bool b2xn[MCT2], b2yn[MCT2];
+#pragma unroll
+    for (int ct = 0; ct < MCT1; ++ct) {
+        int c = ct * 16 + (lane & 15);
+        b1col[ct] = 0;
+        if (c < c1n) {
+            int per = nv + ns, kk = c / (2 * per), rem = c % (2 * per);
+            int sg = rem / per, q = rem % per;
+            int comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+            b1col[ct] = (kk * 2 + sg) * ncomp + comp;
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < MCT2; ++ct) {
+        int c = ct * 16 + (lane & 15);
+        b2x[ct] = b2y[ct] = 0;
+        b2xn[ct] = b2yn[ct] = false;
+        if (c < c2n) {
+            int kk = c / (4 * nv), rem = c % (4 * nv);
+            int sg = rem / (2 * nv), r2 = rem % (2 * nv);
+            int q = r2 >> 1, ax = r2 & 1;
+            int xs = (kk * 2 + sg) * ncomp, ys = (kk * 2 + (1 - sg)) * ncomp;
+            if (ax == 0) {
+                b2x[ct] = xs + 3 * q + 1; b2xn[ct] = false;
+                b2y[ct] = ys + 3 * q + 2; b2yn[ct] = (sg == 0);
+            } else {
+                b2x[ct] = xs + 3 * q + 2; b2xn[ct] = true;
+                b2y[ct] = ys + 3 * q + 1; b2yn[ct] = (sg == 0);
+            }
+        }
+    }
+
+    int ri[RPW], rpar[RPW], rtile[RPW];
+    bool rok[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int rr = wave * RPW + r;
+        rok[r] = rr < nrows;
+        int i = (rr < rows0) ? 0 : 1;
+        int x = (i == 0) ? rr : rr - rows0;
+        int p = (x < RTe[i]) ? 0 : 1;
+        ri[r] = i;
+        rpar[r] = p;
+        rtile[r] = p ? x - RTe[i] : x;
+        if (!rok[r]) { ri[r] = 0; rpar[r] = 0; rtile[r] = 0; }
+    }
+
+    // staging descriptors: element e = comp + ncomp*(tl + TC*g), g = column
+    // group (order i, radial kk, sign sg); consecutive threads read
+    // consecutive addresses of one (k, mp) line
+    const size_t sl = (size_t)a.nth * a.nr;
+    const double *ssrc[kStageMax];
+    int sdst[kStageMax], stl[kStageMax];
+    bool sok[kStageMax];
+    const int nstage = TC * ncomp * 2 * a.KT * nm;
+#pragma unroll
+    for (int s2 = 0; s2 < kStageMax; ++s2) {
+        const int e = threadIdx.x + s2 * NT;
+        ssrc[s2] = a.in;
+        sdst[s2] = 0;
+        stl[s2] = 0;
+        sok[s2] = false;
+        if (e < nstage) {
+            const int comp = e % ncomp, q = e / ncomp;
+            const int tl = q % TC, g = q / TC;
+            const int sg = g & 1, kk = (g >> 1) % a.KT, i = (g >> 1) / a.KT;
+            const int m = mm[i];
+            stl[s2] = tl;
+            sdst[s2] = (i * ncolm + (kk * 2 + sg) * ncomp + comp) * TCP + tl;
+            if (kk < nkk && (sg == 0 || m > 0)) {
+                const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
+                ssrc[s2] = a.in + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
+                sok[s2] = true;
+            }
+        }
+    }
+    double sv1[kStageMax], sv2[kStageMax];
+    auto stage_load = [&](int t0) {
+#pragma unroll
+        for (int s2 = 0; s2 < kStageMax; ++s2) {
+            sv1[s2] = 0.0;
+            sv2[s2] = 0.0;
+            const int t = t0 + stl[s2];
+            if (sok[s2] && t < a.th_h) {
+                const int tm = a.nth - 1 - t;
+                sv1[s2] = ssrc[s2][(size_t)ncomp * t];
+                sv2[s2] = (tm != t) ? ssrc[s2][(size_t)ncomp * tm] : 0.0;
+            }
+        }
+    };
+    auto stage_store = [&](int t0, int buf) {
+        elem *V = Vb + buf * bufsz;
+#pragma unroll
+        for (int s2 = 0; s2 < kStageMax; ++s2) {
+            if (threadIdx.x + s2 * NT >= nstage) continue;
+            const int t = t0 + stl[s2];
+            const double w = (t < a.th_h) ? a.weight[t] : 0.0;
+            T::store(V + sdst[s2], w * (sv1[s2] + sv2[s2]), slice);
+            T::store(V + ncol * TCP + sdst[s2], w * (sv1[s2] - sv2[s2]), slice);
+        }
+    };
+
+    acc a1[RPW][MCT1], a2[RPW][MCT2];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+        for (int ct = 0; ct < MCT1; ++ct) a1[r][ct] = acc{};
+#pragma unroll
+        for (int ct = 0; ct < MCT2; ++ct) a2[r][ct] = acc{};
+    }
+
+    const elem *tab = reinterpret_cast<const elem *>(a.tab);
+    constexpr int UE = unit_elems<T>();
+    auto aptr = [&](int r, int ks) -> const elem * {
+        const int m = mm[ri[r]];
+        return tab + (a.off[2 * m + rpar[r]] + ((long long)rtile[r] * a.KSt + ks) * 3) * UE
+               + lane * EPL;
+    };
+    frag Acur[RPW][3][NS];
+    auto load_a = [&](frag (&A)[RPW][3][NS], int ks) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            if (!rok[r]) continue;
+            const elem *ap = aptr(r, ks);
+#pragma unroll
+            for (int w = 0; w < 3; ++w)
+#pragma unroll
+                for (int s2 = 0; s2 < NS; ++s2) A[r][w][s2] = T::load(ap + w * UE + s2 * 64 * EPL);
+        }
+    };
+
+    const int nchunk = (a.KSt * KS_ + TC - 1) / TC;
+    stage_load(0);
+    stage_store(0, 0);
+    load_a(Acur, 0);
+    __syncthreads();
+    for (int c = 0; c < nchunk; ++c) {
+        const int t0 = c * TC;
+        const elem *V = Vb + (c & 1) * bufsz;
+        if (c + 1 < nchunk) stage_load(t0 + TC);
+        const int ks_end = min(a.KSt, (t0 + TC) / KS_);
         for (int ks = t0 / KS_; ks < ks_end; ++ks) {
             const int kl = (ks * KS_ - t0) + (KS_ / 4) * (lane >> 4);
             frag Anx[RPW][3][NS];
-            if (NS == 1 && ks + 1 < ks_end) load_a(Anx, ks + 1);
+            const bool more = ks + 1 < a.KSt;
+            if (more) load_a(Anx, ks + 1);
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
                 if (!rok[r]) continue;
                 const int par = rpar[r];
                 const int cofs = ri[r] * ncolm;
-                const int fP = par, fD = 1 - par;  // fold index: 0 sym, 1 anti
+                const int fP = par, fD = 1 - par;
 #pragma unroll
                 for (int ct = 0; ct < MCT1; ++ct) {
                     if (ct >= nct1) break;
@@ -797,21 +1123,19 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
                     T::mma(a2[r][ct], Acur[r][2], y);
                 }
             }
-            if (ks + 1 < ks_end) {
-                if (NS == 1) {
+            if (more) {
 #pragma unroll
-                    for (int r = 0; r < RPW; ++r)
+                for (int r = 0; r < RPW; ++r)
 #pragma unroll
-                        for (int w = 0; w < 3; ++w) Acur[r][w][0] = Anx[r][w][0];
-                } else {
-                    load_a(Acur, ks + 1);
-                }
+                    for (int w = 0; w < 3; ++w)
+#pragma unroll
+                        for (int s2 = 0; s2 < NS; ++s2) Acur[r][w][s2] = Anx[r][w][s2];
             }
         }
+        if (c + 1 < nchunk) stage_store(t0 + TC, (c + 1) & 1);
+        __syncthreads();
     }
 
-    // epilogue: scale by r^2 g7, r g7 or g6 (sph_normalizations,
-    // spherical_harmonics.f90:121-139) and store in rlm order
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
         if (!rok[r]) continue;
@@ -824,7 +1148,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
             const int cc = g1 ? ct : ct - MCT1;
             if (g1 ? (cc >= nct1) : (cc >= nct2)) continue;
             int c = cc * 16 + (lane & 15);
-            int kk, sg, comp, kind;  // kind 0: r^2 g7, 1: r g7, 2: g6
+            int kk, sg, comp, kind;
             if (g1) {
                 if (c >= c1n) continue;
                 int per = nv + ns;
@@ -969,7 +1293,11 @@ static size_t plan_lds(const LegPlan &p, bool fwd, int prec)
 {
     const int ns = (prec == kBF16x3) ? 3 : 1;
     const size_t es = (prec == kF64) ? 8 : (prec == kF32 ? 4 : 2);
-    const int pad = (prec == kBF16x3) ? 8 : 1;
+    const int pad = (prec == kBF16x3) ? 8 : (prec == kF64 ? 2 : 4);   // lds_pad<T>()
+    if (p.ver == 2) {
+        const int ncol = 2 * p.KT * 2 * p.ncomp;
+        return (size_t)2 * ns * 2 * ncol * (p.tc + pad) * es;
+    }
     const int ncol = p.KT * 2 * p.ncomp * ((fwd && p.pair) ? 2 : 1);
     const int kc = fwd ? p.tc : 32;
     return (size_t)ns * 2 * ncol * (kc + pad) * es;
@@ -1012,7 +1340,80 @@ static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd,
                 out.push_back(p);
             }
     }
+    if (prec != kBF16x3) {
+        // deeper P prefetch for the small-tile plans (chunk steps must be a
+        // multiple of the depth)
+        const size_t n1 = out.size();
+        for (size_t i = 0; i < n1; ++i) {
+            LegPlan p = out[i];
+            if (p.mct1 != 1 || (p.rpw != 1 && p.rpw != 3)) continue;
+            const int steps = fwd ? p.tc / 4 : 32 / 4;
+            if (steps % 4) continue;
+            p.pd = 4;
+            out.push_back(p);
+        }
+    }
+    if (fwd) {
+        // second form: (rows per wave, waves) pairs; the row capacity is
+        // checked against the order pairs when the items are built
+        static const int f2[6][2] = {{1, 4}, {3, 8}, {5, 4}, {2, 8}, {1, 8}, {3, 4}};
+        for (int i = 0; i < 2; ++i) {
+            const int mc1 = (i == 0) ? 1 : 2, mc2 = (i == 0) ? 2 : 4;
+            int kt = 0;
+            for (int k = 8; k >= 1; --k)
+                if ((per1 * k + 15) / 16 <= mc1 && (per2 * k + 15) / 16 <= mc2) { kt = k; break; }
+            if (!kt) continue;
+            for (const auto &f : f2) {
+                // instances that would spill registers are not offered
+                if (prec == kBF16x3 && (f[0] > 3 || (f[1] == 8 && f[0] > 1) ||
+                                        (mc1 == 2 && f[0] > 1))) continue;
+                if (prec == kF64 && mc1 == 2 && ((f[1] == 8 && f[0] > 2) || f[0] > 3)) continue;
+                LegPlan p;
+                p.ncomp = ncomp;
+                p.nvec = nv;
+                p.nscl = ns;
+                p.fwd = true;
+                p.KT = kt;
+                p.mct1 = mc1;
+                p.mct2 = mc2;
+                p.rpw = f[0];
+                p.nw = f[1];
+                p.pair = 1;
+                p.ver = 2;
+                const int kmin = (prec == kBF16x3) ? 32 : 4;
+                const int ncol = 2 * kt * 2 * ncomp;
+                p.tc = 64;
+                while (p.tc > kmin && (plan_lds(p, true, prec) > 160 * 1024 ||
+                                       p.tc * ncol > kStageMax * 64 * p.nw))
+                    p.tc /= 2;
+                if (plan_lds(p, true, prec) > 160 * 1024) continue;
+                if (p.tc * ncol > kStageMax * 64 * p.nw) continue;
+                out.push_back(p);
+            }
+        }
+    }
     return out;
+}
+
+// Order the work items for the 8 XCDs: work-group b runs on XCD b % 8, so
+// the list is interleaved such that every item of one order pair (m, L-m)
+// lands on the same XCD (its P tiles are then shared through that XCD's L2)
+// while the pairs, which carry equal work, are dealt round-robin over the
+// XCDs.  Shorter XCD lists are padded with empty items (m = -1).
+static void xcd_interleave(int L, std::vector<WorkItem> *items)
+{
+    constexpr int NX = 8;
+    std::vector<WorkItem> per[NX];
+    for (const WorkItem &w : *items) {
+        const int pr = std::min(w.m, L - w.m);
+        per[pr % NX].push_back(w);
+    }
+    size_t len = 0;
+    for (int x = 0; x < NX; ++x) len = std::max(len, per[x].size());
+    std::vector<WorkItem> out(len * NX, WorkItem{-1, 0, 0, 0});
+    for (int x = 0; x < NX; ++x)
+        for (size_t i = 0; i < per[x].size(); ++i) out[i * NX + x] = per[x][i];
+    items->swap(out);
 }
 
 static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
@@ -1025,6 +1426,25 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
         int K = g.L - mm + 1, K0 = (K + 1) / 2, K1 = K / 2;
         return (K0 + 15) / 16 + (K1 + 15) / 16;
     };
+    if (p->ver == 2) {
+        int maxrows = 0;
+        for (int m = 0; m <= g.L - m; ++m) {
+            const int nrow = rows(m) + (g.L - m != m ? rows(g.L - m) : 0);
+            maxrows = std::max(maxrows, nrow);
+            for (int kt = 0; kt < nkt; ++kt) items.push_back({m, kt, 0, 0});
+        }
+        if (maxrows > p->nw * p->rpw) {
+            p->n_items = 0;
+            p->d_items = nullptr;
+            return 1;   // this plan cannot hold every row of an order pair
+        }
+        xcd_interleave(g.L, &items);
+        p->n_items = (int)items.size();
+        HIP_TRY(hipMalloc(&p->d_items, sizeof(WorkItem) * items.size()));
+        HIP_TRY(hipMemcpy(p->d_items, items.data(), sizeof(WorkItem) * items.size(),
+                          hipMemcpyHostToDevice));
+        return 0;
+    }
     for (int m = 0; m <= g.L; ++m) {
         int nrow;
         if (fwd) {
@@ -1042,6 +1462,7 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
         for (int rg = 0; rg < ngroup; ++rg)
             for (int kt = 0; kt < nkt; ++kt) items.push_back({m, kt, rg, 0});
     }
+    xcd_interleave(g.L, &items);
     p->n_items = (int)items.size();
     HIP_TRY(hipMalloc(&p->d_items, sizeof(WorkItem) * items.size()));
     HIP_TRY(hipMemcpy(p->d_items, items.data(), sizeof(WorkItem) * items.size(),
@@ -1065,8 +1486,19 @@ const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
         *err = "no Legendre tile plan fits this component count in LDS";
         return nullptr;
     }
-    for (LegPlan &p : cand)
-        if (build_items(c, &p, fwd, err)) return nullptr;
+    {
+        std::vector<LegPlan> ok;
+        for (LegPlan &p : cand) {
+            const int rc = build_items(c, &p, fwd, err);
+            if (rc < 0) return nullptr;
+            if (rc == 0) ok.push_back(p);
+        }
+        cand.swap(ok);
+    }
+    if (cand.empty()) {
+        *err = "no Legendre tile plan fits this grid";
+        return nullptr;
+    }
     int best = 0;
     if (c->autotune && cand.size() > 1) {
         best = autotune_plan(c, cand, fwd, err);
@@ -1076,6 +1508,58 @@ const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
         if ((int)i != best) (void)hipFree(cand[i].d_items);
     c->plans.push_back(cand[best]);
     return &c->plans.back();
+}
+
+// Candidate plans of a field set, in the autotuner's order, after the items
+// are built (plans that cannot cover the grid are dropped).
+static int valid_candidates(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
+                            std::vector<LegPlan> *out, std::string *err)
+{
+    std::vector<LegPlan> cand = plan_candidates(ncomp, nvec, nscl, fwd, c->prec);
+    for (LegPlan &p : cand) {
+        const int rc = build_items(c, &p, fwd, err);
+        if (rc < 0) return -1;
+        if (rc == 0) out->push_back(p);
+    }
+    return 0;
+}
+
+int leg_plan_count(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, std::string *err)
+{
+    if (ncomp != 3 * nvec + nscl || nvec < 0 || nscl < 0 || ncomp < 1) {
+        *err = "ncomp must equal 3*nvector + nscalar";
+        return -1;
+    }
+    std::vector<LegPlan> v;
+    if (valid_candidates(c, ncomp, nvec, nscl, fwd, &v, err)) return -1;
+    for (LegPlan &p : v) (void)hipFree(p.d_items);
+    return (int)v.size();
+}
+
+int leg_force_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, int idx,
+                   std::string *err)
+{
+    if (ncomp != 3 * nvec + nscl || nvec < 0 || nscl < 0 || ncomp < 1) {
+        *err = "ncomp must equal 3*nvector + nscalar";
+        return -1;
+    }
+    std::vector<LegPlan> v;
+    if (valid_candidates(c, ncomp, nvec, nscl, fwd, &v, err)) return -1;
+    if (idx < 0 || idx >= (int)v.size()) {
+        for (LegPlan &p : v) (void)hipFree(p.d_items);
+        *err = "plan index out of range";
+        return -1;
+    }
+    for (int i = 0; i < (int)v.size(); ++i)
+        if (i != idx) (void)hipFree(v[i].d_items);
+    for (LegPlan &p : c->plans)
+        if (p.ncomp == ncomp && p.nvec == nvec && p.nscl == nscl && p.fwd == fwd) {
+            (void)hipFree(p.d_items);
+            p = v[idx];
+            return 0;
+        }
+    c->plans.push_back(v[idx]);
+    return 0;
 }
 
 int leg_context_build(LegContext *c, int L, int nth, int nr, const double *radius,
@@ -1206,7 +1690,7 @@ static size_t lds_bytes(const LegContext *c, const LegPlan &p, bool fwd)
     return plan_lds(p, fwd, c->prec);
 }
 
-template <class T, int RPW, int MCT1, int MCT2>
+template <class T, int RPW, int MCT1, int MCT2, int PD>
 static int launch_t(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
                     hipStream_t s, std::string *err)
 {
@@ -1214,35 +1698,87 @@ static int launch_t(LegContext *c, const LegPlan &p, bool fwd, const double *in,
     size_t lds = lds_bytes<T>(c, p, fwd);
     if (lds > 160 * 1024) { *err = "LDS tile too large for this component count"; return -1; }
     if (lds > 64 * 1024) {
-        const void *fn = fwd ? reinterpret_cast<const void *>(&k_leg_fwd<T, RPW, MCT1, MCT2>)
-                             : reinterpret_cast<const void *>(&k_leg_bwd<T, RPW, MCT1, MCT2>);
+        const void *fn = fwd ? reinterpret_cast<const void *>(&k_leg_fwd<T, RPW, MCT1, MCT2, PD>)
+                             : reinterpret_cast<const void *>(&k_leg_bwd<T, RPW, MCT1, MCT2, PD>);
         HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     if (fwd)
-        hipLaunchKernelGGL((k_leg_fwd<T, RPW, MCT1, MCT2>), dim3(p.n_items), dim3(kThreads), lds, s, a);
+        hipLaunchKernelGGL((k_leg_fwd<T, RPW, MCT1, MCT2, PD>), dim3(p.n_items), dim3(kThreads), lds, s, a);
     else
-        hipLaunchKernelGGL((k_leg_bwd<T, RPW, MCT1, MCT2>), dim3(p.n_items), dim3(kThreads), lds, s, a);
+        hipLaunchKernelGGL((k_leg_bwd<T, RPW, MCT1, MCT2, PD>), dim3(p.n_items), dim3(kThreads), lds, s, a);
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+template <class T, int RPW, int MCT1, int MCT2, int NW>
+static int launch_f2(LegContext *c, const LegPlan &p, const double *in, double *out,
+                     hipStream_t s, std::string *err)
+{
+    LegArgs a = make_args(c, p, true, in, out);
+    const size_t lds = plan_lds(p, true, c->prec);
+    if (lds > 160 * 1024) { *err = "LDS tile too large for this component count"; return -1; }
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_leg_fwd2<T, RPW, MCT1, MCT2, NW>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_leg_fwd2<T, RPW, MCT1, MCT2, NW>), dim3(p.n_items), dim3(NW * 64), lds,
+                       s, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <class T, int MCT1, int MCT2>
+static int launch_f2m(LegContext *c, const LegPlan &p, const double *in, double *out,
+                      hipStream_t s, std::string *err)
+{
+    if (p.nw == 4) {
+        if (p.rpw == 1) return launch_f2<T, 1, MCT1, MCT2, 4>(c, p, in, out, s, err);
+        if (p.rpw == 3) return launch_f2<T, 3, MCT1, MCT2, 4>(c, p, in, out, s, err);
+        if constexpr (T::NS == 1)
+            if (p.rpw == 5) return launch_f2<T, 5, MCT1, MCT2, 4>(c, p, in, out, s, err);
+    } else {
+        if (p.rpw == 1) return launch_f2<T, 1, MCT1, MCT2, 8>(c, p, in, out, s, err);
+        if (p.rpw == 2) return launch_f2<T, 2, MCT1, MCT2, 8>(c, p, in, out, s, err);
+        if (p.rpw == 3) return launch_f2<T, 3, MCT1, MCT2, 8>(c, p, in, out, s, err);
+    }
+    *err = "no forward kernel instance for this plan";
+    return -1;
 }
 
 template <class T>
 static int launch_p(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
                     hipStream_t s, std::string *err)
 {
+    if (fwd && p.ver == 2) {
+        if (p.mct1 == 1) return launch_f2m<T, 1, 2>(c, p, in, out, s, err);
+        return launch_f2m<T, 2, 4>(c, p, in, out, s, err);
+    }
+    if constexpr (T::NS == 1) {
+        if (p.pd == 4) {
+            if (p.mct1 == 1 && p.mct2 == 1) {
+                if (p.rpw == 1) return launch_t<T, 1, 1, 1, 4>(c, p, fwd, in, out, s, err);
+                if (p.rpw == 3) return launch_t<T, 3, 1, 1, 4>(c, p, fwd, in, out, s, err);
+            }
+            if (p.mct1 == 1 && p.mct2 == 2) {
+                if (p.rpw == 1) return launch_t<T, 1, 1, 2, 4>(c, p, fwd, in, out, s, err);
+                if (p.rpw == 3) return launch_t<T, 3, 1, 2, 4>(c, p, fwd, in, out, s, err);
+            }
+            *err = "no kernel instance with this prefetch depth";
+            return -1;
+        }
+    }
     if (p.mct1 == 1 && p.mct2 == 1) {
-        if (p.rpw == 5) return launch_t<T, 5, 1, 1>(c, p, fwd, in, out, s, err);
-        if (p.rpw == 1) return launch_t<T, 1, 1, 1>(c, p, fwd, in, out, s, err);
-        return launch_t<T, 3, 1, 1>(c, p, fwd, in, out, s, err);
+        if (p.rpw == 5) return launch_t<T, 5, 1, 1, 2>(c, p, fwd, in, out, s, err);
+        if (p.rpw == 1) return launch_t<T, 1, 1, 1, 2>(c, p, fwd, in, out, s, err);
+        return launch_t<T, 3, 1, 1, 2>(c, p, fwd, in, out, s, err);
     }
     if (p.mct1 == 1 && p.mct2 == 2) {
-        if (p.rpw == 5) return launch_t<T, 5, 1, 2>(c, p, fwd, in, out, s, err);
-        if (p.rpw == 1) return launch_t<T, 1, 1, 2>(c, p, fwd, in, out, s, err);
-        return launch_t<T, 3, 1, 2>(c, p, fwd, in, out, s, err);
+        if (p.rpw == 5) return launch_t<T, 5, 1, 2, 2>(c, p, fwd, in, out, s, err);
+        if (p.rpw == 1) return launch_t<T, 1, 1, 2, 2>(c, p, fwd, in, out, s, err);
+        return launch_t<T, 3, 1, 2, 2>(c, p, fwd, in, out, s, err);
     }
-    if (p.rpw == 1) return launch_t<T, 1, 2, 4>(c, p, fwd, in, out, s, err);
-    if (p.rpw == 2) return launch_t<T, 2, 2, 4>(c, p, fwd, in, out, s, err);
-    return launch_t<T, 3, 2, 4>(c, p, fwd, in, out, s, err);
+    if (p.rpw == 1) return launch_t<T, 1, 2, 4, 2>(c, p, fwd, in, out, s, err);
+    if (p.rpw == 2) return launch_t<T, 2, 2, 4, 2>(c, p, fwd, in, out, s, err);
+    return launch_t<T, 3, 2, 4, 2>(c, p, fwd, in, out, s, err);
 }
 
 static int launch_plan(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,

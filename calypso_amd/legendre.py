@@ -150,11 +150,24 @@ class LegendreTransform:
 
     def plan_info(self, fwd: bool, ncomp=None, nvector=None, nscalar=None):
         nc, nv, ns = self._fs(ncomp, nvector, nscalar)
-        a = np.zeros(8, dtype=np.int64)
+        a = np.zeros(11, dtype=np.int64)
         _lib.check(_lib.load().calypso_leg_plan_info(self._h, nc, nv, ns, int(fwd), a.ctypes.data),
                    "calypso_leg_plan_info")
-        keys = ["KT", "mct1", "mct2", "rows_per_wave", "pair", "tc", "tuned_us", "work_groups"]
+        keys = ["KT", "mct1", "mct2", "rows_per_wave", "pair", "tc", "tuned_us", "work_groups",
+                "kernel_form", "waves", "prefetch"]
         return dict(zip(keys, (int(v) for v in a)))
+
+    def plan_count(self, fwd: bool, ncomp=None, nvector=None, nscalar=None) -> int:
+        nc, nv, ns = self._fs(ncomp, nvector, nscalar)
+        n = _lib.load().calypso_leg_plan_count(self._h, nc, nv, ns, int(fwd))
+        if n < 0:
+            _lib.check(n, "calypso_leg_plan_count")
+        return n
+
+    def force_plan(self, fwd: bool, idx: int, ncomp=None, nvector=None, nscalar=None):
+        nc, nv, ns = self._fs(ncomp, nvector, nscalar)
+        _lib.check(_lib.load().calypso_leg_force_plan(self._h, nc, nv, ns, int(fwd), idx),
+                   "calypso_leg_force_plan")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -90,15 +90,22 @@ int calypso_leg_info(calypso_leg_t *ctx, long long *info);
 void *calypso_leg_stream(calypso_leg_t *ctx);
 
 /* Tile plan used for a field set and direction (fwd = 0 backward, 1
- * forward), planning it if needed: info[0..7] = radial tile, P-GEMM column
+ * forward), planning it if needed: info[0..10] = radial tile, P-GEMM column
  * tiles, dP-GEMM column tiles, row tiles per wave, m/L-m pairing, colatitude
  * chunk, autotuned time per launch in microseconds (-1 if not timed),
- * work-groups per launch.  Plans are chosen by timing the candidates on first
+ * work-groups per launch, kernel form (1 or 2), waves per work-group, P
+ * prefetch depth in K steps.  Plans are chosen by timing the candidates on first
  * use (the GPU counterpart of select_legendre_transform,
  * sph_transforms_4_MHD.f90:309-426) unless calypso_leg_set_autotune(ctx, 0). */
 int calypso_leg_plan_info(calypso_leg_t *ctx, int ncomp, int nvector,
                           int nscalar, int fwd, long long *info);
 int calypso_leg_set_autotune(calypso_leg_t *ctx, int on);
+/* Number of candidate tile plans for a field set and direction, and use of
+ * candidate idx instead of the autotuner's pick (for tests and tuning). */
+int calypso_leg_plan_count(calypso_leg_t *ctx, int ncomp, int nvector,
+                           int nscalar, int fwd);
+int calypso_leg_force_plan(calypso_leg_t *ctx, int ncomp, int nvector,
+                           int nscalar, int fwd, int idx);
 
 /* ---- process-wide selector, the shape the Fortran module binds ---------- */
 /* One active context, like the reference's module-level state

@@ -191,3 +191,35 @@ def test_full_size_roundtrip_fp64(torch_cuda):
     assert err < 1e-9
     assert err_l0 < 1e-6
     lt.close()
+
+
+PLAN_CASES = [(31, 48, 9, 4, 1), (31, 48, 5, 6, 1), (47, 72, 7, 4, 0), (64, 96, 6, 1, 1)]
+
+
+@pytest.mark.parametrize("mode", ["hip_fp64", "hip_fp32", "hip_bf16x3"])
+@pytest.mark.parametrize("L,nth,nr,nv,ns", PLAN_CASES)
+def test_every_tile_plan_vs_oracle(torch_cuda, mode, L, nth, nr, nv, ns):
+    """Every candidate kernel plan the autotuner may pick, forced one by one."""
+    nc = 3 * nv + ns
+    rng = np.random.default_rng(L + 31 * nr + nv)
+    radius = np.linspace(7.0 / 13.0, 20.0 / 13.0, nr)
+    grid = oracle.Grid(L, nth, radius)
+    l = grid.jlm[:, 1].astype(float)
+    sp = (rng.standard_normal((nr, grid.J, nc)) * ((l + 1.0) ** -1.5)[None, :, None]).ravel()
+    vr = rng.standard_normal(grid.n_rtm(nc))
+    ref_b = oracle.leg_bwd(grid, nc, nv, ns, sp)
+    ref_f = oracle.leg_fwd(grid, nc, nv, ns, vr)
+    lt = calypso_amd.LegendreTransform(L, nth, radius, nc, nv, ns, mode)
+    tol = FP64_TOL if mode == "hip_fp64" else MIXED_TOL[mode]
+    for fwd in (False, True):
+        n = lt.plan_count(fwd)
+        assert n >= 1
+        for idx in range(n):
+            lt.force_plan(fwd, idx)
+            info = lt.plan_info(fwd)
+            print(f"[plans] mode={mode} L={L} nr={nr} nv={nv} ns={ns} fwd={fwd} "
+                  f"plan {idx}/{n} {info}", flush=True)
+            e = field_err(run_dev(torch_cuda, lt, vr=vr), ref_f, nc) if fwd else \
+                field_err(run_dev(torch_cuda, lt, sp=sp), ref_b, nc)
+            assert e < tol, (fwd, idx, info, e)
+    lt.close()
