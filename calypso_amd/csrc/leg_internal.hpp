@@ -37,7 +37,8 @@ struct LegPlan {
     int pair = 0;           // fwd: orders m and L-m in one work item
     int tc = 64;            // fwd: colatitudes per LDS chunk
     int ver = 1;            // kernel form (2: one work-group per order pair,
-                            // double-buffered staging)
+                            // double-buffered staging; 3: two P tables,
+                            // GEMM-ready LDS columns)
     int nw = 4;             // waves per work-group
     int pd = 2;             // K steps the P fragments are fetched ahead
     float tuned_ms = -1.f;  // autotuner timing of the chosen plan
@@ -59,6 +60,11 @@ struct LegContext {
     void *d_tab_bwd = nullptr, *d_tab_fwd = nullptr;
     long long *d_off_bwd = nullptr;   // [L+2] unit offsets per m
     long long *d_off_fwd = nullptr;   // [2(L+1)+1] unit offsets per (m,par)
+    // third-form tables (P, dP only; K steps padded to the prefetch depth)
+    void *d_tab_bwd3 = nullptr, *d_tab_fwd3 = nullptr;
+    long long *d_off_bwd3 = nullptr, *d_off_fwd3 = nullptr;
+    int ks_fwd3 = 0;
+    double *d_rsin = nullptr;         // 1/sin(theta) at the hemisphere rows
     size_t tab_bytes = 0;
 
     std::deque<LegPlan> plans;    // per (ncomp, nvec, nscl, direction)

@@ -302,6 +302,7 @@ struct LegArgs {
     const double *radius;
     const double *weight;
     const WorkItem *items;
+    const double *rsin;     // 1/sin(theta) at the hemisphere rows
     int L, nth, nr, th_h, J;
     int ncomp, nvec, nscl;
     int KT, RT, KSt;
@@ -1193,6 +1194,646 @@ __global__ __launch_bounds__(NW * 64) void k_leg_fwd2(LegArgs a)
     }
 }
 
+// ====================================================== third form ====
+//
+// Both directions, restructured so that the K loop is nothing but operand
+// loads and MFMAs:
+//  * two P tables (P, dP) instead of three: the m/sin(theta) factor of the
+//    PM terms goes onto the staged input (forward, where theta is the K
+//    index) or onto the accumulated rows in the epilogue (backward, where
+//    theta is the row index), so no PM tile is ever read;
+//  * the LDS tile holds the B operand column by column exactly as the MFMA
+//    reads it (GEMM1 columns, then the dP columns X, then the P columns Y of
+//    the theta/phi GEMM), with signs and factors applied while staging, plus
+//    one zero column that the lanes beyond the last column read;
+//  * a wave holds row tiles of one parity only, so each B fragment feeds all
+//    of its row tiles; a row tile past the end repeats a valid one and is
+//    not stored, so there are no per-tile branches in the K loop;
+//  * table pointers are formed once; the K steps of every order are padded
+//    with zero tiles to a multiple of the prefetch depth;
+//  * the global loads of the next LDS chunk are issued before the MFMAs of
+//    the current one and written to LDS after them.
+
+constexpr int kSlotF = 6;    // (row, mirror row) pairs staged per thread per chunk (fwd)
+constexpr int kSlotB = 8;    // spectral values staged per thread per chunk (bwd)
+
+// K-step padding of the third-form tables (backward, forward): the
+// prefetch depth of a plan must divide it
+template <class T> __host__ __device__ constexpr int kpad3b() { return T::NS == 1 ? 2 : 1; }
+template <class T> __host__ __device__ constexpr int kpad3f() { return T::NS == 1 ? 4 : 1; }
+
+// fwd3 units: for m: for par: for rt < RT_par(m): for ks < KSt3: for w < 2
+// (P, dP); element (s, lane, e): degree index i = rt*16 + lane%16,
+// row t = ks*KSTEP + (KSTEP/4)*(lane/16) + e
+template <class T>
+__global__ void k_pack_fwd3(int L, int th_h, int KSt, const double *P, const double *dP,
+                            const long long *off, typename T::elem *tab)
+{
+    const int mp = blockIdx.y;  // 2*m + par
+    const int m = mp >> 1, par = mp & 1;
+    const long long u0 = off[mp], nu = off[mp + 1] - off[mp];
+    const long long pos = (long long)m * (L + 1) - (long long)m * (m - 1) / 2;
+    constexpr int UE = unit_elems<T>() / T::NS;
+    for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < nu * UE;
+         x += (long long)gridDim.x * blockDim.x) {
+        const long long u = x / UE;
+        const int r = (int)(x % UE);
+        const int lane = r / T::EPL, e = r % T::EPL;
+        const int w = (int)(u & 1);
+        const long long u2 = u >> 1;
+        const int ks = (int)(u2 % KSt), rt = (int)(u2 / KSt);
+        const int i = rt * 16 + (lane & 15);
+        const int t = ks * T::KSTEP + (T::KSTEP / 4) * (lane >> 4) + e;
+        const int l = m + par + 2 * i;
+        double v = 0.0;
+        if (t < th_h && l <= L) v = (w ? dP : P)[(pos + (l - m)) * th_h + t];
+        typename T::elem *dst = tab + (u0 + u) * unit_elems<T>() + lane * T::EPL + e;
+        T::store(dst, v, 64 * T::EPL);
+    }
+}
+
+// bwd3 units: for m: for rt < RT: for ks < KS3(m): for w < 4 (P even,
+// P odd, dP even, dP odd); element (s, lane, e): row t = rt*16 + lane%16,
+// degree index i = ks*KSTEP + (KSTEP/4)*(lane/16) + e
+template <class T>
+__global__ void k_pack_bwd3(int L, int th_h, int RT, const double *P, const double *dP,
+                            const long long *off, typename T::elem *tab)
+{
+    const int m = blockIdx.y;
+    const long long u0 = off[m], nu = off[m + 1] - off[m];
+    const int K0 = (L - m + 2) / 2;
+    const int KS = ((K0 + T::KSTEP - 1) / T::KSTEP + kpad3b<T>() - 1) / kpad3b<T>() * kpad3b<T>();
+    const long long pos = (long long)m * (L + 1) - (long long)m * (m - 1) / 2;
+    constexpr int UE = unit_elems<T>() / T::NS;
+    for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < nu * UE;
+         x += (long long)gridDim.x * blockDim.x) {
+        const long long u = x / UE;
+        const int r = (int)(x % UE);
+        const int lane = r / T::EPL, e = r % T::EPL;
+        const int w = (int)(u & 3);
+        const long long u2 = u >> 2;
+        const int ks = (int)(u2 % KS), rt = (int)(u2 / KS);
+        const int par = w & 1;
+        const int t = rt * 16 + (lane & 15);
+        const int i = ks * T::KSTEP + (T::KSTEP / 4) * (lane >> 4) + e;
+        const int l = m + par + 2 * i;
+        double v = 0.0;
+        if (t < th_h && l <= L) v = (w < 2 ? P : dP)[(pos + (l - m)) * th_h + t];
+        typename T::elem *dst = tab + (u0 + u) * unit_elems<T>() + lane * T::EPL + e;
+        T::store(dst, v, 64 * T::EPL);
+    }
+}
+
+__global__ void k_rsin(int th_h, const double *colat, double *rsin)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < th_h) rsin[t] = 1.0 / sin(colat[t]);
+}
+
+// Staged column of one input component (both directions).  GEMM1 column of
+// v_r / scalar q: (kk*2+sg)*(nv+ns) + q.  GEMM2 column (kk, sg, q, ax):
+// CX or CY + ((kk*2+sg)*nv + q)*2 + ax.  A theta/phi component x (1, 2) of
+// sign sg is the dP operand X of its own column (kk, sg, q, x-1) and the P
+// operand Y of the partner column (kk, 1-sg, q, 2-x).
+struct StageCols {
+    int ca, cb;   // cb < 0: no second column
+};
+__device__ inline StageCols stage_cols(int comp, int kk, int sg, int nv, int ns, int CX, int CY)
+{
+    StageCols s;
+    const int gb = kk * 2 + sg;
+    s.cb = -1;
+    if (comp >= 3 * nv) {
+        s.ca = gb * (nv + ns) + nv + (comp - 3 * nv);
+    } else {
+        const int q3 = comp / 3, x = comp - 3 * q3;
+        if (x == 0) {
+            s.ca = gb * (nv + ns) + q3;
+        } else {
+            s.ca = CX + (gb * nv + q3) * 2 + (x - 1);
+            s.cb = CY + ((kk * 2 + (1 - sg)) * nv + q3) * 2 + (2 - x);
+        }
+    }
+    return s;
+}
+
+// Forward, third form.  Work item = (m, radial tile, row group); wave w takes
+// rows of parity w & 1, row tiles (rgroup*2 + w/2)*RPW + r.
+//   c1 = P  * B1          (S_pol, scalars)
+//   c2 = dP * X + P * Y   (dS, T), Y staged with m/sin(theta)
+template <class T, int RPW, int NC1, int NC2, int PD>
+__global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
+{
+    using elem = typename T::elem;
+    using frag = typename T::frag;
+    using acc = typename T::acc;
+    constexpr int KS_ = T::KSTEP, NS = T::NS, EPL = T::EPL;
+    constexpr int NB = 16 * (NC1 + 2 * NC2) + 1;
+    constexpr int CX = 16 * NC1, CY = 16 * (NC1 + NC2), CZ = NB - 1;
+    constexpr int UE = unit_elems<T>();
+    const int TC = a.TC, TCP = TC + lds_pad<T>();
+
+    const WorkItem it = a.items[blockIdx.x];
+    if (it.m < 0) return;   // padding of the XCD interleave
+    const int m = it.m;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int K = a.L - m + 1;
+    const int par = wave & 1;
+    const int Kp = par ? K / 2 : (K + 1) / 2;
+    const int RTp = (Kp + 15) / 16;
+    const int kt0 = it.ktile * a.KT;
+    const int nkk = min(a.KT, a.nr - kt0);
+    const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
+    const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 4 * nv;
+    const int nks = a.KSt;
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    elem *V = reinterpret_cast<elem *>(smem_raw);   // [NS][2 fold][NB][TCP]
+    const int slice = 2 * NB * TCP;
+    // per-row factors (w, w m/sin(theta)) after the tile, zero past th_h; the
+    // equator row of an odd grid is its own mirror and gets half the weight
+    double2 *WF = reinterpret_cast<double2 *>(
+        smem_raw + ((NS * slice * sizeof(elem) + 15) & ~(size_t)15));
+    for (int x = threadIdx.x; x < 2 * TCP; x += kThreads)
+        T::store(V + ((x / TCP) * NB + CZ) * TCP + x % TCP, 0.0, slice);
+    for (int t = threadIdx.x; t < nks * KS_; t += kThreads) {
+        double w = 0.0, y = 0.0;
+        if (t < a.th_h) {
+            w = a.weight[t];
+            if (a.nth - 1 - t == t) w *= 0.5;
+            y = w * ((double)m * a.rsin[t]);
+        }
+        WF[t] = make_double2(w, y);
+    }
+
+    int rtile[RPW];
+    bool rok[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int rt = (it.rgroup * 2 + (wave >> 1)) * RPW + r;
+        rok[r] = rt < RTp;
+        rtile[r] = rok[r] ? rt : 0;
+    }
+    // an order with no odd rows (m = L) lets its odd waves read even tiles
+    const int pt = RTp > 0 ? par : 0;
+    const elem *tab = reinterpret_cast<const elem *>(a.tab);
+    const elem *ap[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r)
+        ap[r] = tab + (a.off[2 * m + pt] + (long long)rtile[r] * nks * 2) * UE + lane * EPL;
+
+    // B fragment offsets of this lane (fold 0: V(t)+V(pi-t), 1: V(t)-V(pi-t));
+    // P terms of parity par read fold par, dP terms fold 1-par
+    const int kq = (KS_ / 4) * (lane >> 4);
+    int o1[NC1], ox[NC2], oy[NC2];
+#pragma unroll
+    for (int ct = 0; ct < NC1; ++ct) {
+        const int c = ct * 16 + (lane & 15);
+        o1[ct] = (par * NB + (c < c1n ? c : CZ)) * TCP + kq;
+    }
+#pragma unroll
+    for (int ct = 0; ct < NC2; ++ct) {
+        const int c = ct * 16 + (lane & 15);
+        ox[ct] = ((1 - par) * NB + (c < c2n ? CX + c : CZ)) * TCP + kq;
+        oy[ct] = (par * NB + (c < c2n ? CY + c : CZ)) * TCP + kq;
+    }
+
+    acc c1[RPW][NC1], c2[RPW][NC2];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+        for (int ct = 0; ct < NC1; ++ct) c1[r][ct] = acc{};
+#pragma unroll
+        for (int ct = 0; ct < NC2; ++ct) c2[r][ct] = acc{};
+    }
+
+    // staging: (component, row, radius/sign) triples over the threads,
+    // component fastest so a wave reads contiguous runs of vr_rtm.  Each
+    // slot's source offset (at chunk 0), LDS destinations and signs are
+    // formed once; per chunk a slot costs two loads, one LDS read of the row
+    // factors, the fold and its stores.
+    const int nst = TC * a.KT * 2 * ncomp;
+    const int nthnr = a.nth * a.nr;
+    constexpr int kNone = 0xFFFF;
+    int s_src[kSlotF], s_dst[kSlotF], s_meta[kSlotF];
+#pragma unroll
+    for (int i = 0; i < kSlotF; ++i) {
+        const int q = threadIdx.x + i * kThreads;
+        s_src[i] = -2;   // no slot
+        s_dst[i] = 0;
+        s_meta[i] = 0;
+        if (q < nst) {
+            const int comp = q % ncomp, rest = q / ncomp;
+            const int tl = rest % TC, g = rest / TC;
+            const int kk = g >> 1, sg = g & 1;
+            const StageCols sc = stage_cols(comp, kk, sg, nv, ns, CX, CY);
+            const bool live = kk < nkk && (sg == 0 || m > 0);
+            s_src[i] = live ? comp + ncomp * (a.nth * (kt0 + kk) + nthnr * (a.L + (sg ? -m : m)) + tl)
+                            : -1;
+            s_dst[i] = (sc.ca * TCP + tl) | ((sc.cb >= 0 ? sc.cb * TCP + tl : kNone) << 16);
+            // X = +v_theta | -v_phi; Y = -+ (m/sin) v_(phi|theta)(-sg)
+            s_meta[i] = tl | ((sc.cb >= 0 && comp % 3 == 2) ? 0x10000 : 0) | (sg == 1 ? 0x20000 : 0);
+        }
+    }
+    double g1[kSlotF], g2[kSlotF];
+    auto fetch = [&](int t0) {
+#pragma unroll
+        for (int i = 0; i < kSlotF; ++i) {
+            g1[i] = 0.0;
+            g2[i] = 0.0;
+            if (s_src[i] >= 0) {
+                const int t = t0 + (s_meta[i] & 0xFFFF);
+                if (t < a.th_h) {
+                    const int o = s_src[i] + ncomp * t0;
+                    g1[i] = a.in[o];
+                    g2[i] = a.in[o + ncomp * (a.nth - 1 - 2 * t)];   // row nth-1-t
+                }
+            }
+        }
+    };
+    auto put = [&](int t0) {
+#pragma unroll
+        for (int i = 0; i < kSlotF; ++i) {
+            if (s_src[i] == -2) continue;
+            const double2 wf = WF[t0 + (s_meta[i] & 0xFFFF)];
+            const double vs = g1[i] + g2[i], va = g1[i] - g2[i];
+            const double wx = (s_meta[i] & 0x10000) ? -wf.x : wf.x;
+            const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
+            T::store(V + da, wx * vs, slice);
+            T::store(V + NB * TCP + da, wx * va, slice);
+            if (db != kNone) {
+                const double fy = (s_meta[i] & 0x20000) ? -wf.y : wf.y;
+                T::store(V + db, fy * vs, slice);
+                T::store(V + NB * TCP + db, fy * va, slice);
+            }
+        }
+    };
+
+    auto load_a = [&](frag (&A)[RPW][2][NS], int ks) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int w = 0; w < 2; ++w)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    A[r][w][s] = T::load(ap[r] + (ks * 2 + w) * UE + s * 64 * EPL);
+    };
+    auto mstep = [&](const frag (&A)[RPW][2][NS], int kl) {
+        frag b1[NC1][NS], bx[NC2][NS], by[NC2][NS];
+#pragma unroll
+        for (int ct = 0; ct < NC1; ++ct)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) b1[ct][s] = T::load(V + s * slice + o1[ct] + kl);
+#pragma unroll
+        for (int ct = 0; ct < NC2; ++ct)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                bx[ct][s] = T::load(V + s * slice + ox[ct] + kl);
+                by[ct][s] = T::load(V + s * slice + oy[ct] + kl);
+            }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+            for (int ct = 0; ct < NC1; ++ct) T::mma(c1[r][ct], A[r][0], b1[ct]);
+#pragma unroll
+            for (int ct = 0; ct < NC2; ++ct) {
+                T::mma(c2[r][ct], A[r][1], bx[ct]);
+                T::mma(c2[r][ct], A[r][0], by[ct]);
+            }
+        }
+    };
+
+    frag Aq[PD][RPW][2][NS];
+    fetch(0);
+#pragma unroll
+    for (int d = 0; d < PD; ++d) load_a(Aq[d], min(d, nks - 1));
+    for (int t0 = 0; t0 < nks * KS_; t0 += TC) {
+        __syncthreads();
+        put(t0);
+        __syncthreads();
+        if (t0 + TC < nks * KS_) fetch(t0 + TC);
+        const int kse = min(nks, (t0 + TC) / KS_);
+        for (int kb = t0 / KS_; kb < kse; kb += PD) {
+#pragma unroll
+            for (int d = 0; d < PD; ++d) {
+                const int ks = kb + d;
+                mstep(Aq[d], ks * KS_ - t0);
+                load_a(Aq[d], min(ks + PD, nks - 1));
+            }
+        }
+    }
+
+    // epilogue: scale by r^2 g7, r g7 or g6 (sph_normalizations,
+    // spherical_harmonics.f90:121-139) and store in rlm order
+    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        if (!rok[r]) continue;
+        const int rt = rtile[r];
+#pragma unroll
+        for (int ct = 0; ct < NC1 + NC2; ++ct) {
+            const bool g1c = ct < NC1;
+            const int cc = g1c ? ct : ct - NC1;
+            const int c = cc * 16 + (lane & 15);
+            int kk, sg, comp, kind;  // kind 0: r^2 g7, 1: r g7, 2: g6
+            if (g1c) {
+                if (c >= c1n) continue;
+                const int per = nv + ns;
+                kk = c / (2 * per);
+                const int rem = c % (2 * per);
+                sg = rem / per;
+                const int q = rem % per;
+                comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+                kind = q < nv ? 0 : 2;
+            } else {
+                if (c >= c2n) continue;
+                kk = c / (4 * nv);
+                const int rem = c % (4 * nv);
+                sg = rem / (2 * nv);
+                const int r2 = rem % (2 * nv);
+                comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
+                kind = 1;
+            }
+            if (kk >= nkk || (sg == 1 && m == 0)) continue;
+            const int k = kt0 + kk;
+            const double rad = a.radius[k];
+            const int j0 = sg ? j0n : j0p;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rt * 16 + T::row(lane, v);
+                if (i >= Kp) continue;
+                const int l = m + par + 2 * i;
+                double fac;
+                if (kind == 2) {
+                    fac = (l == 0) ? 0.5 : (m == 0 ? (double)(2 * l + 1) / 2.0
+                                                   : (double)(2 * l + 1) / 4.0);
+                } else {
+                    const double g7 = (l == 0) ? 1.0
+                                    : (m == 0 ? (double)(2 * l + 1) / (double)(2 * l * (l + 1))
+                                              : (double)(2 * l + 1) / (double)(4 * l * (l + 1)));
+                    fac = (kind == 0) ? rad * rad * g7 : rad * g7;
+                }
+                const double val = (g1c ? T::get(c1[r][cc], v) : T::get(c2[r][cc], v)) * fac;
+                a.out[comp + (size_t)ncomp * ((j0 + (l - m)) + (size_t)a.J * k)] = val;
+            }
+        }
+    }
+}
+
+// Backward, third form.  Work item = (m, radial tile, row group); rows are
+// the hemisphere colatitudes, wave w takes row tiles (rgroup*4 + w)*RPW + r;
+// K runs over the degrees of both parities (folded output S +- A).
+//   a1 = P * B1                         (v_r, scalars)
+//   ax = dP * X,  ay = P * Y;  v_theta, v_phi = ax + (m/sin(theta)) ay
+template <class T, int RPW, int NC1, int NC2, int PD>
+__global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
+{
+    using elem = typename T::elem;
+    using frag = typename T::frag;
+    using acc = typename T::acc;
+    constexpr int KS_ = T::KSTEP, NS = T::NS, EPL = T::EPL;
+    constexpr int NB = 16 * (NC1 + 2 * NC2) + 1;
+    constexpr int CX = 16 * NC1, CY = 16 * (NC1 + NC2), CZ = NB - 1;
+    constexpr int UE = unit_elems<T>();
+    const int KC = a.TC, KCP = KC + lds_pad<T>();
+
+    const WorkItem it = a.items[blockIdx.x];
+    if (it.m < 0) return;   // padding of the XCD interleave
+    const int m = it.m, kt0 = it.ktile * a.KT;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int K0 = (a.L - m + 2) / 2;
+    const int KS = ((K0 + KS_ - 1) / KS_ + kpad3b<T>() - 1) / kpad3b<T>() * kpad3b<T>();
+    const int nkk = min(a.KT, a.nr - kt0);
+    const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
+    const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 4 * nv;
+    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    elem *X = reinterpret_cast<elem *>(smem_raw);   // [NS][2 par][NB][KCP]
+    const int slice = 2 * NB * KCP;
+    __shared__ double rinv[8];                      // 1/r of the item's radii
+    if (threadIdx.x < 8)
+        rinv[threadIdx.x] = ((int)threadIdx.x < nkk) ? 1.0 / a.radius[kt0 + threadIdx.x] : 0.0;
+    for (int x = threadIdx.x; x < 2 * KCP; x += kThreads)
+        T::store(X + ((x / KCP) * NB + CZ) * KCP + x % KCP, 0.0, slice);
+
+    int rtile[RPW];
+    bool rok[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const int rt = (it.rgroup * 4 + wave) * RPW + r;
+        rok[r] = rt < a.RT;
+        rtile[r] = rok[r] ? rt : 0;
+    }
+    const elem *tab = reinterpret_cast<const elem *>(a.tab);
+    const elem *ap[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r)
+        ap[r] = tab + (a.off[m] + (long long)rtile[r] * KS * 4) * UE + lane * EPL;
+
+    const int kq = (KS_ / 4) * (lane >> 4);
+    int o1[NC1], ox[NC2], oy[NC2];   // parity-even offsets; odd = + NB*KCP
+#pragma unroll
+    for (int ct = 0; ct < NC1; ++ct) {
+        const int c = ct * 16 + (lane & 15);
+        o1[ct] = (c < c1n ? c : CZ) * KCP + kq;
+    }
+#pragma unroll
+    for (int ct = 0; ct < NC2; ++ct) {
+        const int c = ct * 16 + (lane & 15);
+        ox[ct] = (c < c2n ? CX + c : CZ) * KCP + kq;
+        oy[ct] = (c < c2n ? CY + c : CZ) * KCP + kq;
+    }
+    const int po = NB * KCP;
+
+    acc a1[RPW][NC1][2], ax[RPW][NC2][2], ay[RPW][NC2][2];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+        for (int ct = 0; ct < NC1; ++ct) a1[r][ct][0] = a1[r][ct][1] = acc{};
+#pragma unroll
+        for (int ct = 0; ct < NC2; ++ct) {
+            ax[r][ct][0] = ax[r][ct][1] = acc{};
+            ay[r][ct][0] = ay[r][ct][1] = acc{};
+        }
+    }
+
+    // staging: (component, degree, radius/sign) triples, component fastest
+    // (contiguous runs of sp_rlm); degrees [m + 2 c0, m + 2 (c0 + KC)).  Slot
+    // descriptors (source offset at chunk 0, LDS destinations, kind and
+    // signs) are formed once.
+    const int nst = 2 * KC * ncomp * a.KT * 2;
+    constexpr int kNone = 0xFFFF;
+    int s_src[kSlotB], s_dst[kSlotB], s_meta[kSlotB];
+#pragma unroll
+    for (int i = 0; i < kSlotB; ++i) {
+        const int q = threadIdx.x + i * kThreads;
+        s_src[i] = -2;   // no slot
+        s_dst[i] = 0;
+        s_meta[i] = 0;
+        if (q < nst) {
+            const int comp = q % ncomp, rest = q / ncomp;
+            const int li = rest % (2 * KC), gg = rest / (2 * KC);
+            const int kk = gg >> 1, sg = gg & 1;
+            const int p = li & 1, il = li >> 1;
+            const StageCols sc = stage_cols(comp, kk, sg, nv, ns, CX, CY);
+            const bool live = kk < nkk && (sg == 0 || m > 0);
+            s_src[i] = live ? comp + ncomp * ((sg ? j0n : j0p) + li + a.J * (kt0 + kk)) : -1;
+            s_dst[i] = (p * NB * KCP + sc.ca * KCP + il)
+                     | ((sc.cb >= 0 ? p * NB * KCP + sc.cb * KCP + il : kNone) << 16);
+            const int kind = comp < 3 * nv ? (comp % 3 == 0 ? 0 : 1) : 2;
+            // X = +dS | -T; Y = +- T | dS of the partner sign
+            s_meta[i] = li | (kk << 8) | (kind << 12) | ((sc.cb >= 0 && comp % 3 == 2) ? 1 << 14 : 0)
+                      | (sg == 0 ? 1 << 15 : 0);
+        }
+    }
+    double g[kSlotB];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int i = 0; i < kSlotB; ++i) {
+            g[i] = 0.0;
+            if (s_src[i] >= 0 && (s_meta[i] & 0xFF) + 2 * c0 <= a.L - m)
+                g[i] = a.in[s_src[i] + 2 * ncomp * c0];
+        }
+    };
+    auto put = [&](int c0) {
+#pragma unroll
+        for (int i = 0; i < kSlotB; ++i) {
+            if (s_src[i] == -2) continue;
+            const int meta = s_meta[i];
+            const int kind = (meta >> 12) & 3;
+            double v = g[i];
+            if (kind != 2) {
+                // S_pol l(l+1)/r^2 (g_sph_rlm(j,3) = 1/2 at l = 0), dS/r, T/r
+                const double ri = rinv[(meta >> 8) & 7];
+                if (kind == 0) {
+                    const int l = m + 2 * c0 + (meta & 0xFF);
+                    v = v * (ri * ri) * ((l == 0) ? 0.5 : (double)(l * (l + 1)));
+                } else {
+                    v = v * ri;
+                }
+            }
+            const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
+            T::store(X + da, (meta & (1 << 14)) ? -v : v, slice);
+            if (db != kNone) T::store(X + db, (meta & (1 << 15)) ? -v : v, slice);
+        }
+    };
+
+    auto load_a = [&](frag (&A)[RPW][4][NS], int ks) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    A[r][w][s] = T::load(ap[r] + (ks * 4 + w) * UE + s * 64 * EPL);
+    };
+    auto mstep = [&](const frag (&A)[RPW][4][NS], int kl) {
+        frag b1[NC1][2][NS], bx[NC2][2][NS], by[NC2][2][NS];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+#pragma unroll
+            for (int ct = 0; ct < NC1; ++ct)
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    b1[ct][p][s] = T::load(X + s * slice + p * po + o1[ct] + kl);
+#pragma unroll
+            for (int ct = 0; ct < NC2; ++ct)
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    bx[ct][p][s] = T::load(X + s * slice + p * po + ox[ct] + kl);
+                    by[ct][p][s] = T::load(X + s * slice + p * po + oy[ct] + kl);
+                }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+            for (int ct = 0; ct < NC1; ++ct) {
+                T::mma(a1[r][ct][0], A[r][0], b1[ct][0]);
+                T::mma(a1[r][ct][1], A[r][1], b1[ct][1]);
+            }
+#pragma unroll
+            for (int ct = 0; ct < NC2; ++ct) {
+                // symmetric part: dP odd, P even; antisymmetric: dP even, P odd
+                T::mma(ax[r][ct][0], A[r][3], bx[ct][1]);
+                T::mma(ax[r][ct][1], A[r][2], bx[ct][0]);
+                T::mma(ay[r][ct][0], A[r][0], by[ct][0]);
+                T::mma(ay[r][ct][1], A[r][1], by[ct][1]);
+            }
+        }
+    };
+
+    frag Aq[PD][RPW][4][NS];
+    fetch(0);
+#pragma unroll
+    for (int d = 0; d < PD; ++d) load_a(Aq[d], min(d, KS - 1));
+    for (int c0 = 0; c0 < KS * KS_; c0 += KC) {
+        __syncthreads();
+        put(c0);
+        __syncthreads();
+        if (c0 + KC < KS * KS_) fetch(c0 + KC);
+        const int kse = min(KS, (c0 + KC) / KS_);
+        for (int kb = c0 / KS_; kb < kse; kb += PD) {
+#pragma unroll
+            for (int d = 0; d < PD; ++d) {
+                const int ks = kb + d;
+                mstep(Aq[d], ks * KS_ - c0);
+                load_a(Aq[d], min(ks + PD, KS - 1));
+            }
+        }
+    }
+
+    // epilogue: hemisphere row t -> out(t) = S + A, mirror nth-1-t -> S - A
+    const size_t sl = (size_t)a.nth * a.nr;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        if (!rok[r]) continue;
+        const int rt = rtile[r];
+#pragma unroll
+        for (int ct = 0; ct < NC1; ++ct) {
+            const int c = ct * 16 + (lane & 15);
+            if (c >= c1n) continue;
+            const int per = nv + ns, kk = c / (2 * per), rem = c % (2 * per);
+            const int sg = rem / per, q = rem % per;
+            if (kk >= nkk || (sg == 1 && m == 0)) continue;
+            const int comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+            const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
+            double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int t = rt * 16 + T::row(lane, v);
+                if (t >= a.th_h) continue;
+                const double S = T::get(a1[r][ct][0], v), An = T::get(a1[r][ct][1], v);
+                const int tm = a.nth - 1 - t;
+                o[(size_t)ncomp * t] = S + An;
+                if (tm != t) o[(size_t)ncomp * tm] = S - An;
+            }
+        }
+#pragma unroll
+        for (int ct = 0; ct < NC2; ++ct) {
+            const int c = ct * 16 + (lane & 15);
+            if (c >= c2n) continue;
+            const int kk = c / (4 * nv), rem = c % (4 * nv);
+            const int sg = rem / (2 * nv), r2 = rem % (2 * nv);
+            if (kk >= nkk || (sg == 1 && m == 0)) continue;
+            const int comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
+            const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
+            double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int t = rt * 16 + T::row(lane, v);
+                if (t >= a.th_h) continue;
+                const double f = (double)m * a.rsin[t];
+                const double S = T::get(ax[r][ct][0], v) + f * T::get(ay[r][ct][0], v);
+                const double An = T::get(ax[r][ct][1], v) + f * T::get(ay[r][ct][1], v);
+                const int tm = a.nth - 1 - t;
+                o[(size_t)ncomp * t] = S + An;
+                if (tm != t) o[(size_t)ncomp * tm] = S - An;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------- host ----
 
 // Gauss-Legendre points on [-1,1] by Newton iteration on [0,1] with the
@@ -1282,6 +1923,40 @@ static int build_tables(LegContext *c, const double *dP_raw_P, const double *dP_
                        dP_raw_P, dP_raw_dP, dP_raw_PM, c->d_off_fwd,
                        reinterpret_cast<typename T::elem *>(c->d_tab_fwd));
     HIP_TRY(hipGetLastError());
+
+    // third form: P and dP only, K steps padded with zero tiles
+    constexpr int pb = kpad3b<T>(), pf = kpad3f<T>();
+    c->ks_fwd3 = ((th_h + T::KSTEP - 1) / T::KSTEP + pf - 1) / pf * pf;
+    std::vector<long long> ob3(L + 2), of3(2 * (L + 1) + 1);
+    ob3[0] = 0;
+    for (int m = 0; m <= L; ++m) {
+        const int K0 = (L - m + 2) / 2;
+        const int KS = ((K0 + T::KSTEP - 1) / T::KSTEP + pb - 1) / pb * pb;
+        ob3[m + 1] = ob3[m] + (long long)c->rt_bwd * KS * 4;
+    }
+    of3[0] = 0;
+    for (int m = 0; m <= L; ++m) {
+        const int K = L - m + 1, K0 = (K + 1) / 2, K1 = K / 2;
+        of3[2 * m + 1] = of3[2 * m] + (long long)((K0 + 15) / 16) * c->ks_fwd3 * 2;
+        of3[2 * m + 2] = of3[2 * m + 1] + (long long)((K1 + 15) / 16) * c->ks_fwd3 * 2;
+    }
+    const size_t bb3 = (size_t)ob3[L + 1] * ue * sizeof(typename T::elem);
+    const size_t bf3 = (size_t)of3[2 * (L + 1)] * ue * sizeof(typename T::elem);
+    HIP_TRY(hipMalloc(&c->d_off_bwd3, sizeof(long long) * ob3.size()));
+    HIP_TRY(hipMalloc(&c->d_off_fwd3, sizeof(long long) * of3.size()));
+    HIP_TRY(hipMemcpy(c->d_off_bwd3, ob3.data(), sizeof(long long) * ob3.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_off_fwd3, of3.data(), sizeof(long long) * of3.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&c->d_tab_bwd3, bb3));
+    HIP_TRY(hipMalloc(&c->d_tab_fwd3, bf3));
+    c->tab_bytes += bb3 + bf3;
+    hipLaunchKernelGGL(k_pack_bwd3<T>, gb, dim3(256), 0, c->stream, L, th_h, c->rt_bwd,
+                       dP_raw_P, dP_raw_dP, c->d_off_bwd3,
+                       reinterpret_cast<typename T::elem *>(c->d_tab_bwd3));
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_pack_fwd3<T>, gf, dim3(256), 0, c->stream, L, th_h, c->ks_fwd3,
+                       dP_raw_P, dP_raw_dP, c->d_off_fwd3,
+                       reinterpret_cast<typename T::elem *>(c->d_tab_fwd3));
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 
@@ -1289,7 +1964,8 @@ static int build_tables(LegContext *c, const double *dP_raw_P, const double *dP_
 // (column tiles of the P GEMM, of the dP|PM GEMM): (1,1), (1,2), (2,4); the
 // radial tile KT is the largest that fits the instance; rows per wave 2, 3 or
 // 5; the forward kernel optionally carries orders m and L-m together.
-static size_t plan_lds(const LegPlan &p, bool fwd, int prec)
+// rows3: padded colatitude rows of the third-form forward tables
+static size_t plan_lds(const LegPlan &p, bool fwd, int prec, int rows3 = 0)
 {
     const int ns = (prec == kBF16x3) ? 3 : 1;
     const size_t es = (prec == kF64) ? 8 : (prec == kF32 ? 4 : 2);
@@ -1298,9 +1974,77 @@ static size_t plan_lds(const LegPlan &p, bool fwd, int prec)
         const int ncol = 2 * p.KT * 2 * p.ncomp;
         return (size_t)2 * ns * 2 * ncol * (p.tc + pad) * es;
     }
+    if (p.ver == 3) {
+        const int nb = 16 * (p.mct1 + 2 * p.mct2) + 1;
+        return (((size_t)ns * 2 * nb * (p.tc + pad) * es + 15) & ~(size_t)15)
+               + (fwd ? (size_t)16 * rows3 : 0);   // + row factors (fwd)
+    }
     const int ncol = p.KT * 2 * p.ncomp * ((fwd && p.pair) ? 2 : 1);
     const int kc = fwd ? p.tc : 32;
     return (size_t)ns * 2 * ncol * (kc + pad) * es;
+}
+
+// Third-form candidates: column-tile instances (GEMM1 tiles, GEMM2 tiles)
+// with the largest radial tile KT each holds (an instance is offered only if
+// no smaller one holds the same KT); rows per wave; prefetch depth; LDS chunk
+// (colatitudes fwd, degrees per parity bwd) limited by the staging slots and
+// an LDS budget that leaves room for two work-groups per CU.
+static const int kInst3[4][2] = {{1, 1}, {1, 2}, {2, 3}, {2, 4}};
+
+static void plan_candidates3(int ncomp, int nv, int ns, bool fwd, int prec,
+                             std::vector<LegPlan> *out)
+{
+    const int per1 = 2 * (nv + ns), per2 = 4 * nv;
+    const int kstep = (prec == kBF16x3) ? 32 : 4;
+    int kt_prev = 0;
+    for (int i = 0; i < 4; ++i) {
+        int kt = 0;
+        for (int k = 8; k >= 1; --k)
+            if ((per1 * k + 15) / 16 <= kInst3[i][0] && (per2 * k + 15) / 16 <= kInst3[i][1]) {
+                kt = k;
+                break;
+            }
+        if (!kt || kt <= kt_prev) continue;
+        kt_prev = kt;
+        // three row tiles per wave in the backward kernel only with one
+        // theta/phi column tile (registers)
+        const std::vector<int> rpws = fwd ? std::vector<int>{1, 2}
+                                    : kInst3[i][1] == 1 ? std::vector<int>{1, 3}
+                                                        : std::vector<int>{1};
+        std::vector<int> pds;
+        if (prec == kBF16x3) pds = {1};
+        else if (fwd) pds = {2, 4};
+        else pds = {2};
+        const std::vector<int> tcs = (prec == kBF16x3) ? std::vector<int>{64, 32}
+                                     : fwd ? std::vector<int>{64, 32, 16}
+                                           : std::vector<int>{32, 16, 8};
+        for (int rpw : rpws)
+            for (int pd : pds) {
+                int taken = 0;
+                for (int tc : tcs) {
+                    if (taken == 2) break;
+                    if (tc % (pd * kstep)) continue;
+                    const long long nst = fwd ? (long long)tc * kt * 2 * ncomp
+                                              : (long long)2 * tc * ncomp * kt * 2;
+                    if (nst > (long long)(fwd ? kSlotF : kSlotB) * kThreads) continue;
+                    LegPlan p;
+                    p.ncomp = ncomp;
+                    p.nvec = nv;
+                    p.nscl = ns;
+                    p.fwd = fwd;
+                    p.KT = kt;
+                    p.mct1 = kInst3[i][0];
+                    p.mct2 = kInst3[i][1];
+                    p.rpw = rpw;
+                    p.pd = pd;
+                    p.tc = tc;
+                    p.ver = 3;
+                    if (plan_lds(p, fwd, prec) > 80 * 1024) continue;
+                    out->push_back(p);
+                    ++taken;
+                }
+            }
+    }
 }
 
 static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd, int prec)
@@ -1308,6 +2052,7 @@ static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd,
     const int per1 = 2 * (nv + ns), per2 = 4 * nv;
     static const int inst[3][2] = {{1, 2}, {2, 4}, {1, 1}};
     std::vector<LegPlan> out;
+    plan_candidates3(ncomp, nv, ns, fwd, prec, &out);
     for (int i = 0; i < 3; ++i) {
         int kt = 0;
         for (int k = 8; k >= 1; --k) {
@@ -1346,7 +2091,7 @@ static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd,
         const size_t n1 = out.size();
         for (size_t i = 0; i < n1; ++i) {
             LegPlan p = out[i];
-            if (p.mct1 != 1 || (p.rpw != 1 && p.rpw != 3)) continue;
+            if (p.ver != 1 || p.mct1 != 1 || (p.rpw != 1 && p.rpw != 3)) continue;
             const int steps = fwd ? p.tc / 4 : 32 / 4;
             if (steps % 4) continue;
             p.pd = 4;
@@ -1437,6 +2182,35 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
             p->n_items = 0;
             p->d_items = nullptr;
             return 1;   // this plan cannot hold every row of an order pair
+        }
+        xcd_interleave(g.L, &items);
+        p->n_items = (int)items.size();
+        HIP_TRY(hipMalloc(&p->d_items, sizeof(WorkItem) * items.size()));
+        HIP_TRY(hipMemcpy(p->d_items, items.data(), sizeof(WorkItem) * items.size(),
+                          hipMemcpyHostToDevice));
+        return 0;
+    }
+    if (p->ver == 3) {
+        if (p->KT > 8) { *err = "radial tile above 8"; return -1; }
+        // the third form addresses the rtm / rlm arrays with 32-bit offsets
+        if ((double)p->ncomp * g.nth * g.nr * g.M >= 2147483647.0 ||
+            (double)p->ncomp * g.J * g.nr >= 2147483647.0) {
+            p->n_items = 0;
+            p->d_items = nullptr;
+            return 1;
+        }
+        for (int m = 0; m <= g.L; ++m) {
+            int ngroup;
+            if (fwd) {
+                const int K = g.L - m + 1, RTe = ((K + 1) / 2 + 15) / 16, RTo = (K / 2 + 15) / 16;
+                ngroup = (std::max(RTe, RTo) + 2 * p->rpw - 1) / (2 * p->rpw);
+            } else {
+                ngroup = (c->rt_bwd + 4 * p->rpw - 1) / (4 * p->rpw);
+            }
+            // the row groups of one radial tile are adjacent: they stage the
+            // same input, which they then share through L2
+            for (int kt = 0; kt < nkt; ++kt)
+                for (int rg = 0; rg < ngroup; ++rg) items.push_back({m, kt, rg, 0});
         }
         xcd_interleave(g.L, &items);
         p->n_items = (int)items.size();
@@ -1621,6 +2395,10 @@ int leg_context_build(LegContext *c, int L, int nth, int nr, const double *radiu
     hipLaunchKernelGGL(k_schmidt_raw, grid, dim3(128), 0, c->stream, L, c->g.th_h, c->d_colat,
                        rP, rdP, rPM);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMalloc(&c->d_rsin, sizeof(double) * c->g.th_h));
+    hipLaunchKernelGGL(k_rsin, dim3((c->g.th_h + 127) / 128), dim3(128), 0, c->stream, c->g.th_h,
+                       c->d_colat, c->d_rsin);
+    HIP_TRY(hipGetLastError());
     int rc;
     if (prec == kF64) rc = build_tables<TF64>(c, rP, rdP, rPM, err);
     else if (prec == kF32) rc = build_tables<TF32>(c, rP, rdP, rPM, err);
@@ -1648,6 +2426,11 @@ void leg_context_free(LegContext *c)
     (void)hipFree(c->d_tab_fwd);
     (void)hipFree(c->d_off_bwd);
     (void)hipFree(c->d_off_fwd);
+    (void)hipFree(c->d_tab_bwd3);
+    (void)hipFree(c->d_tab_fwd3);
+    (void)hipFree(c->d_off_bwd3);
+    (void)hipFree(c->d_off_fwd3);
+    (void)hipFree(c->d_rsin);
     for (LegPlan &p : c->plans) (void)hipFree(p.d_items);
     c->plans.clear();
     (void)hipFree(c->d_rlm);
@@ -1681,6 +2464,12 @@ static LegArgs make_args(LegContext *c, const LegPlan &p, bool fwd, const double
     a.KSt = c->ks_fwd;
     a.TC = p.tc;
     a.pair = p.pair;
+    a.rsin = c->d_rsin;
+    if (p.ver == 3) {
+        a.tab = fwd ? c->d_tab_fwd3 : c->d_tab_bwd3;
+        a.off = fwd ? c->d_off_fwd3 : c->d_off_bwd3;
+        a.KSt = c->ks_fwd3;
+    }
     return a;
 }
 
@@ -1744,10 +2533,81 @@ static int launch_f2m(LegContext *c, const LegPlan &p, const double *in, double 
     return -1;
 }
 
+template <class T, int RPW, int NC1, int NC2, int PD>
+static int launch3_f(LegContext *c, const LegPlan &p, const double *in, double *out,
+                     hipStream_t s, std::string *err)
+{
+    LegArgs a = make_args(c, p, true, in, out);
+    const size_t lds = plan_lds(p, true, c->prec, c->ks_fwd3 * T::KSTEP);
+    if (lds > 160 * 1024) { *err = "LDS tile too large for this grid"; return -1; }
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_leg_fwd3<T, RPW, NC1, NC2, PD>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_leg_fwd3<T, RPW, NC1, NC2, PD>), dim3(p.n_items), dim3(kThreads), lds, s, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <class T, int RPW, int NC1, int NC2, int PD>
+static int launch3_b(LegContext *c, const LegPlan &p, const double *in, double *out,
+                     hipStream_t s, std::string *err)
+{
+    LegArgs a = make_args(c, p, false, in, out);
+    const size_t lds = plan_lds(p, false, c->prec);
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_leg_bwd3<T, RPW, NC1, NC2, PD>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_leg_bwd3<T, RPW, NC1, NC2, PD>), dim3(p.n_items), dim3(kThreads), lds, s, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <class T, int NC1, int NC2>
+static int launch3_n(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
+                     hipStream_t s, std::string *err)
+{
+    if constexpr (T::NS == 1) {
+        if (fwd) {
+            if (p.rpw == 1 && p.pd == 2) return launch3_f<T, 1, NC1, NC2, 2>(c, p, in, out, s, err);
+            if (p.rpw == 1 && p.pd == 4) return launch3_f<T, 1, NC1, NC2, 4>(c, p, in, out, s, err);
+            if (p.rpw == 2 && p.pd == 2) return launch3_f<T, 2, NC1, NC2, 2>(c, p, in, out, s, err);
+            if (p.rpw == 2 && p.pd == 4) return launch3_f<T, 2, NC1, NC2, 4>(c, p, in, out, s, err);
+        } else {
+            if (p.rpw == 1 && p.pd == 2) return launch3_b<T, 1, NC1, NC2, 2>(c, p, in, out, s, err);
+            if constexpr (NC2 == 1)
+                if (p.rpw == 3 && p.pd == 2) return launch3_b<T, 3, NC1, NC2, 2>(c, p, in, out, s, err);
+        }
+    } else {
+        if (p.pd == 1) {
+            if (fwd && p.rpw == 1) return launch3_f<T, 1, NC1, NC2, 1>(c, p, in, out, s, err);
+            if (fwd && p.rpw == 2) return launch3_f<T, 2, NC1, NC2, 1>(c, p, in, out, s, err);
+            if (!fwd && p.rpw == 1) return launch3_b<T, 1, NC1, NC2, 1>(c, p, in, out, s, err);
+            if constexpr (NC2 == 1)
+                if (!fwd && p.rpw == 3) return launch3_b<T, 3, NC1, NC2, 1>(c, p, in, out, s, err);
+        }
+    }
+    *err = "no third-form kernel instance for this plan";
+    return -1;
+}
+
+template <class T>
+static int launch3(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
+                   hipStream_t s, std::string *err)
+{
+    if (p.KT > 8 || p.KT < 1) { *err = "radial tile outside 1..8"; return -1; }
+    if (p.mct1 == 1 && p.mct2 == 1) return launch3_n<T, 1, 1>(c, p, fwd, in, out, s, err);
+    if (p.mct1 == 1 && p.mct2 == 2) return launch3_n<T, 1, 2>(c, p, fwd, in, out, s, err);
+    if (p.mct1 == 2 && p.mct2 == 3) return launch3_n<T, 2, 3>(c, p, fwd, in, out, s, err);
+    if (p.mct1 == 2 && p.mct2 == 4) return launch3_n<T, 2, 4>(c, p, fwd, in, out, s, err);
+    *err = "no third-form kernel instance for this column tiling";
+    return -1;
+}
+
 template <class T>
 static int launch_p(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
                     hipStream_t s, std::string *err)
 {
+    if (p.ver == 3) return launch3<T>(c, p, fwd, in, out, s, err);
     if (fwd && p.ver == 2) {
         if (p.mct1 == 1) return launch_f2m<T, 1, 2>(c, p, in, out, s, err);
         return launch_f2m<T, 2, 4>(c, p, in, out, s, err);
