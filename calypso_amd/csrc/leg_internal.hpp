@@ -14,9 +14,10 @@ namespace calypso {
 
 enum Precision : int { kF64 = 1, kF32 = 2, kBF16x3 = 3 };
 
-// One (order m, radial tile, row group) unit of Legendre work.
+// One (order m, radial tile, row group) unit of Legendre work; the third
+// kernel form runs nkt consecutive radial tiles from ktile on.
 struct WorkItem {
-    int m, ktile, rgroup, pad;
+    int m, ktile, rgroup, nkt;
 };
 
 struct LegGeometry {
@@ -36,9 +37,8 @@ struct LegPlan {
     int mct1 = 1, mct2 = 1; // column-tile capacity of the kernel instance
     int pair = 0;           // fwd: orders m and L-m in one work item
     int tc = 64;            // fwd: colatitudes per LDS chunk
-    int ver = 1;            // kernel form (2: one work-group per order pair,
-                            // double-buffered staging; 3: two P tables,
-                            // GEMM-ready LDS columns)
+    int ver = 1;            // kernel form (3: two P tables, GEMM-ready LDS
+                            // columns, radial tiles looped per work-group)
     int nw = 4;             // waves per work-group
     int pd = 2;             // K steps the P fragments are fetched ahead
     float tuned_ms = -1.f;  // autotuner timing of the chosen plan
@@ -65,6 +65,7 @@ struct LegContext {
     long long *d_off_bwd3 = nullptr, *d_off_fwd3 = nullptr;
     int ks_fwd3 = 0;
     double *d_rsin = nullptr;         // 1/sin(theta) at the hemisphere rows
+    double *d_gfac = nullptr;         // g7, g6 normalisations of order m > 0
     size_t tab_bytes = 0;
 
     std::deque<LegPlan> plans;    // per (ncomp, nvec, nscl, direction)

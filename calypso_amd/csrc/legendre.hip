@@ -303,6 +303,7 @@ struct LegArgs {
     const double *weight;
     const WorkItem *items;
     const double *rsin;     // 1/sin(theta) at the hemisphere rows
+    const double *gfac;     // [L+1] g7 and [L+1] g6 of order m > 0
     int L, nth, nr, th_h, J;
     int ncomp, nvec, nscl;
     int KT, RT, KSt;
@@ -897,303 +898,6 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
     }
 }
 
-// Forward kernel, second form: one work-group holds every degree row of the
-// order pair (m, L-m) for KT radial points, so the folded input is staged
-// once per (pair, radial tile); the staging of colatitude chunk c+1 is
-// loaded into registers while chunk c feeds the MFMAs and written to the
-// other half of a double LDS buffer (one barrier per chunk).
-constexpr int kStageMax = 8;   // staged elements per thread per chunk
-
-template <class T, int RPW, int MCT1, int MCT2, int NW>
-__global__ __launch_bounds__(NW * 64) void k_leg_fwd2(LegArgs a)
-{
-    using elem = typename T::elem;
-    using frag = typename T::frag;
-    using acc = typename T::acc;
-    constexpr int KS_ = T::KSTEP, NS = T::NS, EPL = T::EPL, NT = NW * 64;
-    const int TC = a.TC;
-    const int TCP = TC + lds_pad<T>();
-
-    const WorkItem it = a.items[blockIdx.x];
-    if (it.m < 0) return;   // padding of the XCD interleave
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nm = (a.L - it.m != it.m) ? 2 : 1;
-    int mm[2], K0v[2], K1v[2], RTe[2], RTo[2], j0p[2], j0n[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int m = (i == 0) ? it.m : a.L - it.m;
-        mm[i] = m;
-        const int K = a.L - m + 1;
-        K0v[i] = (K + 1) / 2;
-        K1v[i] = K / 2;
-        RTe[i] = (K0v[i] + 15) / 16;
-        RTo[i] = (K1v[i] + 15) / 16;
-        j0p[i] = a.lstack[a.L + m];
-        j0n[i] = a.lstack[a.L - m];
-    }
-    const int rows0 = RTe[0] + RTo[0];
-    const int nrows = rows0 + (nm == 2 ? RTe[1] + RTo[1] : 0);
-    const int kt0 = it.ktile * a.KT;
-    const int nkk = min(a.KT, a.nr - kt0);
-    const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
-    const int ncolm = a.KT * 2 * ncomp;
-    const int ncol = 2 * ncolm;                 // room for both orders
-    const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 2 * 2 * nv;
-    const int nct1 = (c1n + 15) / 16, nct2 = (c2n + 15) / 16;
-
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    elem *Vb = reinterpret_cast<elem *>(smem_raw);  // [2 buf][NS][2 fold][ncol][TCP]
-    const int slice = 2 * ncol * TCP;
-    const int bufsz = NS * slice;
-
-    int b1col[MCT1];
-    int b2x[MCT2], b2y[MCT2];
-    bool b2xn[MCT2], b2yn[MCT2];
-#pragma unroll
-    for (int ct = 0; ct < MCT1; ++ct) {
-        int c = ct * 16 + (lane & 15);
-        b1col[ct] = 0;
-        if (c < c1n) {
-            int per = nv + ns, kk = c / (2 * per), rem = c % (2 * per);
-            int sg = rem / per, q = rem % per;
-            int comp = q < nv ? 3 * q : 3 * nv + (q - nv);
-            b1col[ct] = (kk * 2 + sg) * ncomp + comp;
-        }
-    }
-#pragma unroll
-    for (int ct = 0; ct < MCT2; ++ct) {
-        int c = ct * 16 + (lane & 15);
-        b2x[ct] = b2y[ct] = 0;
-        b2xn[ct] = b2yn[ct] = false;
-        if (c < c2n) {
-            int kk = c / (4 * nv), rem = c % (4 * nv);
-            int sg = rem / (2 * nv), r2 = rem % (2 * nv);
-            int q = r2 >> 1, ax = r2 & 1;
-            int xs = (kk * 2 + sg) * ncomp, ys = (kk * 2 + (1 - sg)) * ncomp;
-            if (ax == 0) {
-                b2x[ct] = xs + 3 * q + 1; b2xn[ct] = false;
-                b2y[ct] = ys + 3 * q + 2; b2yn[ct] = (sg == 0);
-            } else {
-                b2x[ct] = xs + 3 * q + 2; b2xn[ct] = true;
-                b2y[ct] = ys + 3 * q + 1; b2yn[ct] = (sg == 0);
-            }
-        }
-    }
-
-    int ri[RPW], rpar[RPW], rtile[RPW];
-    bool rok[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        const int rr = wave * RPW + r;
-        rok[r] = rr < nrows;
-        int i = (rr < rows0) ? 0 : 1;
-        int x = (i == 0) ? rr : rr - rows0;
-        int p = (x < RTe[i]) ? 0 : 1;
-        ri[r] = i;
-        rpar[r] = p;
-        rtile[r] = p ? x - RTe[i] : x;
-        if (!rok[r]) { ri[r] = 0; rpar[r] = 0; rtile[r] = 0; }
-    }
-
-    // staging descriptors: element e = comp + ncomp*(tl + TC*g), g = column
-    // group (order i, radial kk, sign sg); consecutive threads read
-    // consecutive addresses of one (k, mp) line
-    const size_t sl = (size_t)a.nth * a.nr;
-    const double *ssrc[kStageMax];
-    int sdst[kStageMax], stl[kStageMax];
-    bool sok[kStageMax];
-    const int nstage = TC * ncomp * 2 * a.KT * nm;
-#pragma unroll
-    for (int s2 = 0; s2 < kStageMax; ++s2) {
-        const int e = threadIdx.x + s2 * NT;
-        ssrc[s2] = a.in;
-        sdst[s2] = 0;
-        stl[s2] = 0;
-        sok[s2] = false;
-        if (e < nstage) {
-            const int comp = e % ncomp, q = e / ncomp;
-            const int tl = q % TC, g = q / TC;
-            const int sg = g & 1, kk = (g >> 1) % a.KT, i = (g >> 1) / a.KT;
-            const int m = mm[i];
-            stl[s2] = tl;
-            sdst[s2] = (i * ncolm + (kk * 2 + sg) * ncomp + comp) * TCP + tl;
-            if (kk < nkk && (sg == 0 || m > 0)) {
-                const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
-                ssrc[s2] = a.in + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
-                sok[s2] = true;
-            }
-        }
-    }
-    double sv1[kStageMax], sv2[kStageMax];
-    auto stage_load = [&](int t0) {
-#pragma unroll
-        for (int s2 = 0; s2 < kStageMax; ++s2) {
-            sv1[s2] = 0.0;
-            sv2[s2] = 0.0;
-            const int t = t0 + stl[s2];
-            if (sok[s2] && t < a.th_h) {
-                const int tm = a.nth - 1 - t;
-                sv1[s2] = ssrc[s2][(size_t)ncomp * t];
-                sv2[s2] = (tm != t) ? ssrc[s2][(size_t)ncomp * tm] : 0.0;
-            }
-        }
-    };
-    auto stage_store = [&](int t0, int buf) {
-        elem *V = Vb + buf * bufsz;
-#pragma unroll
-        for (int s2 = 0; s2 < kStageMax; ++s2) {
-            if (threadIdx.x + s2 * NT >= nstage) continue;
-            const int t = t0 + stl[s2];
-            const double w = (t < a.th_h) ? a.weight[t] : 0.0;
-            T::store(V + sdst[s2], w * (sv1[s2] + sv2[s2]), slice);
-            T::store(V + ncol * TCP + sdst[s2], w * (sv1[s2] - sv2[s2]), slice);
-        }
-    };
-
-    acc a1[RPW][MCT1], a2[RPW][MCT2];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-#pragma unroll
-        for (int ct = 0; ct < MCT1; ++ct) a1[r][ct] = acc{};
-#pragma unroll
-        for (int ct = 0; ct < MCT2; ++ct) a2[r][ct] = acc{};
-    }
-
-    const elem *tab = reinterpret_cast<const elem *>(a.tab);
-    constexpr int UE = unit_elems<T>();
-    auto aptr = [&](int r, int ks) -> const elem * {
-        const int m = mm[ri[r]];
-        return tab + (a.off[2 * m + rpar[r]] + ((long long)rtile[r] * a.KSt + ks) * 3) * UE
-               + lane * EPL;
-    };
-    frag Acur[RPW][3][NS];
-    auto load_a = [&](frag (&A)[RPW][3][NS], int ks) {
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-            if (!rok[r]) continue;
-            const elem *ap = aptr(r, ks);
-#pragma unroll
-            for (int w = 0; w < 3; ++w)
-#pragma unroll
-                for (int s2 = 0; s2 < NS; ++s2) A[r][w][s2] = T::load(ap + w * UE + s2 * 64 * EPL);
-        }
-    };
-
-    const int nchunk = (a.KSt * KS_ + TC - 1) / TC;
-    stage_load(0);
-    stage_store(0, 0);
-    load_a(Acur, 0);
-    __syncthreads();
-    for (int c = 0; c < nchunk; ++c) {
-        const int t0 = c * TC;
-        const elem *V = Vb + (c & 1) * bufsz;
-        if (c + 1 < nchunk) stage_load(t0 + TC);
-        const int ks_end = min(a.KSt, (t0 + TC) / KS_);
-        for (int ks = t0 / KS_; ks < ks_end; ++ks) {
-            const int kl = (ks * KS_ - t0) + (KS_ / 4) * (lane >> 4);
-            frag Anx[RPW][3][NS];
-            const bool more = ks + 1 < a.KSt;
-            if (more) load_a(Anx, ks + 1);
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-                if (!rok[r]) continue;
-                const int par = rpar[r];
-                const int cofs = ri[r] * ncolm;
-                const int fP = par, fD = 1 - par;
-#pragma unroll
-                for (int ct = 0; ct < MCT1; ++ct) {
-                    if (ct >= nct1) break;
-                    frag b[NS];
-#pragma unroll
-                    for (int s2 = 0; s2 < NS; ++s2)
-                        b[s2] = T::load(V + s2 * slice + (fP * ncol + cofs + b1col[ct]) * TCP + kl);
-                    T::mma(a1[r][ct], Acur[r][0], b);
-                }
-#pragma unroll
-                for (int ct = 0; ct < MCT2; ++ct) {
-                    if (ct >= nct2) break;
-                    frag x[NS], y[NS];
-#pragma unroll
-                    for (int s2 = 0; s2 < NS; ++s2) {
-                        x[s2] = T::load(V + s2 * slice + (fD * ncol + cofs + b2x[ct]) * TCP + kl);
-                        y[s2] = T::load(V + s2 * slice + (fP * ncol + cofs + b2y[ct]) * TCP + kl);
-                        if (b2xn[ct]) x[s2] = T::neg(x[s2]);
-                        if (b2yn[ct]) y[s2] = T::neg(y[s2]);
-                    }
-                    T::mma(a2[r][ct], Acur[r][1], x);
-                    T::mma(a2[r][ct], Acur[r][2], y);
-                }
-            }
-            if (more) {
-#pragma unroll
-                for (int r = 0; r < RPW; ++r)
-#pragma unroll
-                    for (int w = 0; w < 3; ++w)
-#pragma unroll
-                        for (int s2 = 0; s2 < NS; ++s2) Acur[r][w][s2] = Anx[r][w][s2];
-            }
-        }
-        if (c + 1 < nchunk) stage_store(t0 + TC, (c + 1) & 1);
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        if (!rok[r]) continue;
-        const int i_m = ri[r], par = rpar[r], rt = rtile[r];
-        const int m = mm[i_m];
-        const int Kp = par ? K1v[i_m] : K0v[i_m];
-#pragma unroll
-        for (int ct = 0; ct < MCT1 + MCT2; ++ct) {
-            const bool g1 = ct < MCT1;
-            const int cc = g1 ? ct : ct - MCT1;
-            if (g1 ? (cc >= nct1) : (cc >= nct2)) continue;
-            int c = cc * 16 + (lane & 15);
-            int kk, sg, comp, kind;
-            if (g1) {
-                if (c >= c1n) continue;
-                int per = nv + ns;
-                kk = c / (2 * per);
-                int rem = c % (2 * per);
-                sg = rem / per;
-                int q = rem % per;
-                comp = q < nv ? 3 * q : 3 * nv + (q - nv);
-                kind = q < nv ? 0 : 2;
-            } else {
-                if (c >= c2n) continue;
-                kk = c / (4 * nv);
-                int rem = c % (4 * nv);
-                sg = rem / (2 * nv);
-                int r2 = rem % (2 * nv);
-                comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
-                kind = 1;
-            }
-            if (kk >= nkk || (sg == 1 && m == 0)) continue;
-            const int k = kt0 + kk;
-            const double rad = a.radius[k];
-            const int j0 = sg ? j0n[i_m] : j0p[i_m];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = rt * 16 + T::row(lane, v);
-                if (i >= Kp) continue;
-                const int l = m + par + 2 * i;
-                double fac;
-                if (kind == 2) {
-                    fac = (l == 0) ? 0.5 : (m == 0 ? (double)(2 * l + 1) / 2.0
-                                                   : (double)(2 * l + 1) / 4.0);
-                } else {
-                    double g7 = (l == 0) ? 1.0
-                              : (m == 0 ? (double)(2 * l + 1) / (double)(2 * l * (l + 1))
-                                        : (double)(2 * l + 1) / (double)(4 * l * (l + 1)));
-                    fac = (kind == 0) ? rad * rad * g7 : rad * g7;
-                }
-                double val = (g1 ? T::get(a1[r][cc], v) : T::get(a2[r][cc], v)) * fac;
-                a.out[comp + (size_t)ncomp * ((j0 + (l - m)) + (size_t)a.J * k)] = val;
-            }
-        }
-    }
-}
-
 // ====================================================== third form ====
 //
 // Both directions, restructured so that the K loop is nothing but operand
@@ -1215,7 +919,16 @@ __global__ __launch_bounds__(NW * 64) void k_leg_fwd2(LegArgs a)
 //    the current one and written to LDS after them.
 
 constexpr int kSlotF = 6;    // (row, mirror row) pairs staged per thread per chunk (fwd)
-constexpr int kSlotB = 8;    // spectral values staged per thread per chunk (bwd)
+constexpr int kSlotB = 6;    // spectral values staged per thread per chunk (bwd)
+
+// Marks per-lane values as redefined here, so that quantities derived from
+// them are recomputed after this point instead of being hoisted out of the
+// enclosing loops and held in registers across the MFMA loop.
+template <int N> __device__ inline void opaque(int (&x)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) asm volatile("" : "+v"(x[i]));
+}
 
 // K-step padding of the third-form tables (backward, forward): the
 // prefetch depth of a plan must divide it
@@ -1317,8 +1030,10 @@ __device__ inline StageCols stage_cols(int comp, int kk, int sg, int nv, int ns,
     return s;
 }
 
-// Forward, third form.  Work item = (m, radial tile, row group); wave w takes
-// rows of parity w & 1, row tiles (rgroup*2 + w/2)*RPW + r.
+// Forward, third form.  Work item = (m, first radial tile, row group, number
+// of radial tiles); wave w takes rows of parity w & 1, row tiles
+// (rgroup*2 + w/2)*RPW + r, and the work-group runs its radial tiles one
+// after the other (everything that depends only on m is formed once).
 //   c1 = P  * B1          (S_pol, scalars)
 //   c2 = dP * X + P * Y   (dS, T), Y staged with m/sin(theta)
 template <class T, int RPW, int NC1, int NC2, int PD>
@@ -1331,6 +1046,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
     constexpr int NB = 16 * (NC1 + 2 * NC2) + 1;
     constexpr int CX = 16 * NC1, CY = 16 * (NC1 + NC2), CZ = NB - 1;
     constexpr int UE = unit_elems<T>();
+    constexpr int kNone = 0xFFFF;
     const int TC = a.TC, TCP = TC + lds_pad<T>();
 
     const WorkItem it = a.items[blockIdx.x];
@@ -1341,11 +1057,10 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
     const int par = wave & 1;
     const int Kp = par ? K / 2 : (K + 1) / 2;
     const int RTp = (Kp + 15) / 16;
-    const int kt0 = it.ktile * a.KT;
-    const int nkk = min(a.KT, a.nr - kt0);
     const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
     const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 4 * nv;
     const int nks = a.KSt;
+    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
 
     extern __shared__ __align__(16) unsigned char smem_raw[];
     elem *V = reinterpret_cast<elem *>(smem_raw);   // [NS][2 fold][NB][TCP]
@@ -1397,24 +1112,41 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
         ox[ct] = ((1 - par) * NB + (c < c2n ? CX + c : CZ)) * TCP + kq;
         oy[ct] = (par * NB + (c < c2n ? CY + c : CZ)) * TCP + kq;
     }
-
-    acc c1[RPW][NC1], c2[RPW][NC2];
+    // output column of this lane per column tile: comp | kk << 8 | sg << 11 |
+    // kind << 12 (kind 0: r^2 g7, 1: r g7, 2: g6), or -1
+    int ocol[NC1 + NC2];
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-#pragma unroll
-        for (int ct = 0; ct < NC1; ++ct) c1[r][ct] = acc{};
-#pragma unroll
-        for (int ct = 0; ct < NC2; ++ct) c2[r][ct] = acc{};
+    for (int ct = 0; ct < NC1 + NC2; ++ct) {
+        const bool g1c = ct < NC1;
+        const int c = (g1c ? ct : ct - NC1) * 16 + (lane & 15);
+        int kk, sg, comp, kind;
+        if (g1c) {
+            const int per = nv + ns;
+            kk = c / (2 * per);
+            const int rem = c % (2 * per);
+            sg = rem / per;
+            const int q = rem % per;
+            comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+            kind = q < nv ? 0 : 2;
+        } else {
+            kk = c / (4 * nv);
+            const int rem = c % (4 * nv);
+            sg = rem / (2 * nv);
+            const int r2 = rem % (2 * nv);
+            comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
+            kind = 1;
+        }
+        const bool ok = (g1c ? c < c1n : c < c2n) && !(sg == 1 && m == 0);
+        ocol[ct] = ok ? comp | (kk << 8) | (sg << 11) | (kind << 12) : -1;
     }
 
     // staging: (component, row, radius/sign) triples over the threads,
     // component fastest so a wave reads contiguous runs of vr_rtm.  Each
-    // slot's source offset (at chunk 0), LDS destinations and signs are
-    // formed once; per chunk a slot costs two loads, one LDS read of the row
-    // factors, the fold and its stores.
+    // slot's source offset (radial tile 0, chunk 0), LDS destinations and
+    // signs are formed once; per chunk a slot costs two loads, one LDS read of
+    // the row factors, the fold and its stores.
     const int nst = TC * a.KT * 2 * ncomp;
     const int nthnr = a.nth * a.nr;
-    constexpr int kNone = 0xFFFF;
     int s_src[kSlotF], s_dst[kSlotF], s_meta[kSlotF];
 #pragma unroll
     for (int i = 0; i < kSlotF; ++i) {
@@ -1427,47 +1159,14 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
             const int tl = rest % TC, g = rest / TC;
             const int kk = g >> 1, sg = g & 1;
             const StageCols sc = stage_cols(comp, kk, sg, nv, ns, CX, CY);
-            const bool live = kk < nkk && (sg == 0 || m > 0);
-            s_src[i] = live ? comp + ncomp * (a.nth * (kt0 + kk) + nthnr * (a.L + (sg ? -m : m)) + tl)
-                            : -1;
+            s_src[i] = (sg == 0 || m > 0)
+                     ? comp + ncomp * (a.nth * kk + nthnr * (a.L + (sg ? -m : m)) + tl) : -1;
             s_dst[i] = (sc.ca * TCP + tl) | ((sc.cb >= 0 ? sc.cb * TCP + tl : kNone) << 16);
             // X = +v_theta | -v_phi; Y = -+ (m/sin) v_(phi|theta)(-sg)
-            s_meta[i] = tl | ((sc.cb >= 0 && comp % 3 == 2) ? 0x10000 : 0) | (sg == 1 ? 0x20000 : 0);
+            s_meta[i] = tl | (kk << 8) | ((sc.cb >= 0 && comp % 3 == 2) ? 0x10000 : 0)
+                      | (sg == 1 ? 0x20000 : 0);
         }
     }
-    double g1[kSlotF], g2[kSlotF];
-    auto fetch = [&](int t0) {
-#pragma unroll
-        for (int i = 0; i < kSlotF; ++i) {
-            g1[i] = 0.0;
-            g2[i] = 0.0;
-            if (s_src[i] >= 0) {
-                const int t = t0 + (s_meta[i] & 0xFFFF);
-                if (t < a.th_h) {
-                    const int o = s_src[i] + ncomp * t0;
-                    g1[i] = a.in[o];
-                    g2[i] = a.in[o + ncomp * (a.nth - 1 - 2 * t)];   // row nth-1-t
-                }
-            }
-        }
-    };
-    auto put = [&](int t0) {
-#pragma unroll
-        for (int i = 0; i < kSlotF; ++i) {
-            if (s_src[i] == -2) continue;
-            const double2 wf = WF[t0 + (s_meta[i] & 0xFFFF)];
-            const double vs = g1[i] + g2[i], va = g1[i] - g2[i];
-            const double wx = (s_meta[i] & 0x10000) ? -wf.x : wf.x;
-            const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
-            T::store(V + da, wx * vs, slice);
-            T::store(V + NB * TCP + da, wx * va, slice);
-            if (db != kNone) {
-                const double fy = (s_meta[i] & 0x20000) ? -wf.y : wf.y;
-                T::store(V + db, fy * vs, slice);
-                T::store(V + NB * TCP + db, fy * va, slice);
-            }
-        }
-    };
 
     auto load_a = [&](frag (&A)[RPW][2][NS], int ks) {
 #pragma unroll
@@ -1478,111 +1177,146 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
                 for (int s = 0; s < NS; ++s)
                     A[r][w][s] = T::load(ap[r] + (ks * 2 + w) * UE + s * 64 * EPL);
     };
-    auto mstep = [&](const frag (&A)[RPW][2][NS], int kl) {
-        frag b1[NC1][NS], bx[NC2][NS], by[NC2][NS];
+
+    for (int kt = it.ktile; kt < it.ktile + it.nkt; ++kt) {
+        const int kt0 = kt * a.KT;
+        const int nkk = min(a.KT, a.nr - kt0);
+        const int koff = ncomp * a.nth * kt0;
+        double g1[kSlotF], g2[kSlotF];
+        auto fetch = [&](int t0) {
+            opaque(s_src);
+            opaque(s_meta);
 #pragma unroll
-        for (int ct = 0; ct < NC1; ++ct)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) b1[ct][s] = T::load(V + s * slice + o1[ct] + kl);
-#pragma unroll
-        for (int ct = 0; ct < NC2; ++ct)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                bx[ct][s] = T::load(V + s * slice + ox[ct] + kl);
-                by[ct][s] = T::load(V + s * slice + oy[ct] + kl);
+            for (int i = 0; i < kSlotF; ++i) {
+                g1[i] = 0.0;
+                g2[i] = 0.0;
+                if (s_src[i] >= 0 && ((s_meta[i] >> 8) & 7) < nkk) {
+                    const int t = t0 + (s_meta[i] & 0xFF);
+                    if (t < a.th_h) {
+                        const int o = s_src[i] + koff + ncomp * t0;
+                        g1[i] = a.in[o];
+                        g2[i] = a.in[o + ncomp * (a.nth - 1 - 2 * t)];   // row nth-1-t
+                    }
+                }
             }
+        };
+        auto put = [&](int t0) {
+            opaque(s_src);
+            opaque(s_dst);
+            opaque(s_meta);
+#pragma unroll
+            for (int i = 0; i < kSlotF; ++i) {
+                if (s_src[i] == -2) continue;
+                const double2 wf = WF[t0 + (s_meta[i] & 0xFF)];
+                const double vs = g1[i] + g2[i], va = g1[i] - g2[i];
+                const double wx = (s_meta[i] & 0x10000) ? -wf.x : wf.x;
+                const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
+                T::store(V + da, wx * vs, slice);
+                T::store(V + NB * TCP + da, wx * va, slice);
+                if (db != kNone) {
+                    const double fy = (s_meta[i] & 0x20000) ? -wf.y : wf.y;
+                    T::store(V + db, fy * vs, slice);
+                    T::store(V + NB * TCP + db, fy * va, slice);
+                }
+            }
+        };
+
+        acc c1[RPW][NC1], c2[RPW][NC2];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
 #pragma unroll
-            for (int ct = 0; ct < NC1; ++ct) T::mma(c1[r][ct], A[r][0], b1[ct]);
+            for (int ct = 0; ct < NC1; ++ct) c1[r][ct] = acc{};
 #pragma unroll
-            for (int ct = 0; ct < NC2; ++ct) {
-                T::mma(c2[r][ct], A[r][1], bx[ct]);
-                T::mma(c2[r][ct], A[r][0], by[ct]);
-            }
+            for (int ct = 0; ct < NC2; ++ct) c2[r][ct] = acc{};
         }
-    };
-
-    frag Aq[PD][RPW][2][NS];
-    fetch(0);
+        auto mstep = [&](const frag (&A)[RPW][2][NS], int kl) {
+            frag b1[NC1][NS], bx[NC2][NS], by[NC2][NS];
 #pragma unroll
-    for (int d = 0; d < PD; ++d) load_a(Aq[d], min(d, nks - 1));
-    for (int t0 = 0; t0 < nks * KS_; t0 += TC) {
-        __syncthreads();
-        put(t0);
-        __syncthreads();
-        if (t0 + TC < nks * KS_) fetch(t0 + TC);
-        const int kse = min(nks, (t0 + TC) / KS_);
-        for (int kb = t0 / KS_; kb < kse; kb += PD) {
+            for (int ct = 0; ct < NC1; ++ct)
 #pragma unroll
-            for (int d = 0; d < PD; ++d) {
-                const int ks = kb + d;
-                mstep(Aq[d], ks * KS_ - t0);
-                load_a(Aq[d], min(ks + PD, nks - 1));
-            }
-        }
-    }
-
-    // epilogue: scale by r^2 g7, r g7 or g6 (sph_normalizations,
-    // spherical_harmonics.f90:121-139) and store in rlm order
-    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
+                for (int s = 0; s < NS; ++s) b1[ct][s] = T::load(V + s * slice + o1[ct] + kl);
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        if (!rok[r]) continue;
-        const int rt = rtile[r];
+            for (int ct = 0; ct < NC2; ++ct)
 #pragma unroll
-        for (int ct = 0; ct < NC1 + NC2; ++ct) {
-            const bool g1c = ct < NC1;
-            const int cc = g1c ? ct : ct - NC1;
-            const int c = cc * 16 + (lane & 15);
-            int kk, sg, comp, kind;  // kind 0: r^2 g7, 1: r g7, 2: g6
-            if (g1c) {
-                if (c >= c1n) continue;
-                const int per = nv + ns;
-                kk = c / (2 * per);
-                const int rem = c % (2 * per);
-                sg = rem / per;
-                const int q = rem % per;
-                comp = q < nv ? 3 * q : 3 * nv + (q - nv);
-                kind = q < nv ? 0 : 2;
-            } else {
-                if (c >= c2n) continue;
-                kk = c / (4 * nv);
-                const int rem = c % (4 * nv);
-                sg = rem / (2 * nv);
-                const int r2 = rem % (2 * nv);
-                comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
-                kind = 1;
-            }
-            if (kk >= nkk || (sg == 1 && m == 0)) continue;
-            const int k = kt0 + kk;
-            const double rad = a.radius[k];
-            const int j0 = sg ? j0n : j0p;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = rt * 16 + T::row(lane, v);
-                if (i >= Kp) continue;
-                const int l = m + par + 2 * i;
-                double fac;
-                if (kind == 2) {
-                    fac = (l == 0) ? 0.5 : (m == 0 ? (double)(2 * l + 1) / 2.0
-                                                   : (double)(2 * l + 1) / 4.0);
-                } else {
-                    const double g7 = (l == 0) ? 1.0
-                                    : (m == 0 ? (double)(2 * l + 1) / (double)(2 * l * (l + 1))
-                                              : (double)(2 * l + 1) / (double)(4 * l * (l + 1)));
-                    fac = (kind == 0) ? rad * rad * g7 : rad * g7;
+                for (int s = 0; s < NS; ++s) {
+                    bx[ct][s] = T::load(V + s * slice + ox[ct] + kl);
+                    by[ct][s] = T::load(V + s * slice + oy[ct] + kl);
                 }
-                const double val = (g1c ? T::get(c1[r][cc], v) : T::get(c2[r][cc], v)) * fac;
-                a.out[comp + (size_t)ncomp * ((j0 + (l - m)) + (size_t)a.J * k)] = val;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+                for (int ct = 0; ct < NC1; ++ct) T::mma(c1[r][ct], A[r][0], b1[ct]);
+#pragma unroll
+                for (int ct = 0; ct < NC2; ++ct) {
+                    T::mma(c2[r][ct], A[r][1], bx[ct]);
+                    T::mma(c2[r][ct], A[r][0], by[ct]);
+                }
+            }
+        };
+
+        frag Aq[PD][RPW][2][NS];
+        fetch(0);
+#pragma unroll
+        for (int d = 0; d < PD; ++d) load_a(Aq[d], min(d, nks - 1));
+        for (int t0 = 0; t0 < nks * KS_; t0 += TC) {
+            __syncthreads();
+            put(t0);
+            __syncthreads();
+            if (t0 + TC < nks * KS_) fetch(t0 + TC);
+            const int kse = min(nks, (t0 + TC) / KS_);
+            for (int kb = t0 / KS_; kb < kse; kb += PD) {
+#pragma unroll
+                for (int d = 0; d < PD; ++d) {
+                    const int ks = kb + d;
+                    mstep(Aq[d], ks * KS_ - t0);
+                    load_a(Aq[d], min(ks + PD, nks - 1));
+                }
+            }
+        }
+
+        // epilogue: scale by r^2 g7, r g7 or g6 (sph_normalizations,
+        // spherical_harmonics.f90:121-139; gfac holds the m > 0 values, m = 0
+        // doubles them) and store in rlm order
+        opaque(ocol);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            if (!rok[r]) continue;
+            const int rt = rtile[r];
+#pragma unroll
+            for (int ct = 0; ct < NC1 + NC2; ++ct) {
+                const int oc = ocol[ct];
+                if (oc < 0) continue;
+                const int comp = oc & 0xFF, kk = (oc >> 8) & 7, sg = (oc >> 11) & 1;
+                const int kind = (oc >> 12) & 3;
+                if (kk >= nkk) continue;
+                const int k = kt0 + kk;
+                const double rad = a.radius[k];
+                const int j0 = sg ? j0n : j0p;
+                const double m2 = (m == 0) ? 2.0 : 1.0;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = rt * 16 + T::row(lane, v);
+                    if (i >= Kp) continue;
+                    const int l = m + par + 2 * i;
+                    double fac;
+                    if (kind == 2) {
+                        fac = (l == 0) ? 0.5 : m2 * a.gfac[a.L + 1 + l];
+                    } else {
+                        const double g7 = (l == 0) ? 1.0 : m2 * a.gfac[l];
+                        fac = (kind == 0) ? rad * rad * g7 : rad * g7;
+                    }
+                    const double val = (ct < NC1 ? T::get(c1[r][ct], v) : T::get(c2[r][ct - NC1], v)) * fac;
+                    a.out[comp + (size_t)ncomp * ((j0 + (l - m)) + (size_t)a.J * k)] = val;
+                }
             }
         }
     }
 }
 
-// Backward, third form.  Work item = (m, radial tile, row group); rows are
-// the hemisphere colatitudes, wave w takes row tiles (rgroup*4 + w)*RPW + r;
-// K runs over the degrees of both parities (folded output S +- A).
+// Backward, third form.  Work item = (m, first radial tile, row group, number
+// of radial tiles); rows are the hemisphere colatitudes, wave w takes row
+// tiles (rgroup*4 + w)*RPW + r; K runs over the degrees of both parities
+// (folded output S +- A).
 //   a1 = P * B1                         (v_r, scalars)
 //   ax = dP * X,  ay = P * Y;  v_theta, v_phi = ax + (m/sin(theta)) ay
 template <class T, int RPW, int NC1, int NC2, int PD>
@@ -1595,15 +1329,15 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
     constexpr int NB = 16 * (NC1 + 2 * NC2) + 1;
     constexpr int CX = 16 * NC1, CY = 16 * (NC1 + NC2), CZ = NB - 1;
     constexpr int UE = unit_elems<T>();
+    constexpr int kNone = 0xFFFF;
     const int KC = a.TC, KCP = KC + lds_pad<T>();
 
     const WorkItem it = a.items[blockIdx.x];
     if (it.m < 0) return;   // padding of the XCD interleave
-    const int m = it.m, kt0 = it.ktile * a.KT;
+    const int m = it.m;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int K0 = (a.L - m + 2) / 2;
     const int KS = ((K0 + KS_ - 1) / KS_ + kpad3b<T>() - 1) / kpad3b<T>() * kpad3b<T>();
-    const int nkk = min(a.KT, a.nr - kt0);
     const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
     const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 4 * nv;
     const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
@@ -1611,9 +1345,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     elem *X = reinterpret_cast<elem *>(smem_raw);   // [NS][2 par][NB][KCP]
     const int slice = 2 * NB * KCP;
-    __shared__ double rinv[8];                      // 1/r of the item's radii
-    if (threadIdx.x < 8)
-        rinv[threadIdx.x] = ((int)threadIdx.x < nkk) ? 1.0 / a.radius[kt0 + threadIdx.x] : 0.0;
+    __shared__ double rinv[8];                      // 1/r of the current radial tile
     for (int x = threadIdx.x; x < 2 * KCP; x += kThreads)
         T::store(X + ((x / KCP) * NB + CZ) * KCP + x % KCP, 0.0, slice);
 
@@ -1645,25 +1377,33 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
         oy[ct] = (c < c2n ? CY + c : CZ) * KCP + kq;
     }
     const int po = NB * KCP;
-
-    acc a1[RPW][NC1][2], ax[RPW][NC2][2], ay[RPW][NC2][2];
+    // output column of this lane per column tile: comp | kk << 8 | sg << 11, or -1
+    int ocol[NC1 + NC2];
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-#pragma unroll
-        for (int ct = 0; ct < NC1; ++ct) a1[r][ct][0] = a1[r][ct][1] = acc{};
-#pragma unroll
-        for (int ct = 0; ct < NC2; ++ct) {
-            ax[r][ct][0] = ax[r][ct][1] = acc{};
-            ay[r][ct][0] = ay[r][ct][1] = acc{};
+    for (int ct = 0; ct < NC1 + NC2; ++ct) {
+        const bool g1c = ct < NC1;
+        const int c = (g1c ? ct : ct - NC1) * 16 + (lane & 15);
+        int kk, sg, comp;
+        if (g1c) {
+            const int per = nv + ns, rem = c % (2 * per), q = rem % per;
+            kk = c / (2 * per);
+            sg = rem / per;
+            comp = q < nv ? 3 * q : 3 * nv + (q - nv);
+        } else {
+            const int rem = c % (4 * nv), r2 = rem % (2 * nv);
+            kk = c / (4 * nv);
+            sg = rem / (2 * nv);
+            comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
         }
+        const bool ok = (g1c ? c < c1n : c < c2n) && !(sg == 1 && m == 0);
+        ocol[ct] = ok ? comp | (kk << 8) | (sg << 11) : -1;
     }
 
     // staging: (component, degree, radius/sign) triples, component fastest
     // (contiguous runs of sp_rlm); degrees [m + 2 c0, m + 2 (c0 + KC)).  Slot
-    // descriptors (source offset at chunk 0, LDS destinations, kind and
-    // signs) are formed once.
+    // descriptors (source offset at radial tile 0 and chunk 0, LDS
+    // destinations, kind and signs) are formed once.
     const int nst = 2 * KC * ncomp * a.KT * 2;
-    constexpr int kNone = 0xFFFF;
     int s_src[kSlotB], s_dst[kSlotB], s_meta[kSlotB];
 #pragma unroll
     for (int i = 0; i < kSlotB; ++i) {
@@ -1677,8 +1417,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
             const int kk = gg >> 1, sg = gg & 1;
             const int p = li & 1, il = li >> 1;
             const StageCols sc = stage_cols(comp, kk, sg, nv, ns, CX, CY);
-            const bool live = kk < nkk && (sg == 0 || m > 0);
-            s_src[i] = live ? comp + ncomp * ((sg ? j0n : j0p) + li + a.J * (kt0 + kk)) : -1;
+            s_src[i] = (sg == 0 || m > 0) ? comp + ncomp * ((sg ? j0n : j0p) + li + a.J * kk) : -1;
             s_dst[i] = (p * NB * KCP + sc.ca * KCP + il)
                      | ((sc.cb >= 0 ? p * NB * KCP + sc.cb * KCP + il : kNone) << 16);
             const int kind = comp < 3 * nv ? (comp % 3 == 0 ? 0 : 1) : 2;
@@ -1687,37 +1426,6 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
                       | (sg == 0 ? 1 << 15 : 0);
         }
     }
-    double g[kSlotB];
-    auto fetch = [&](int c0) {
-#pragma unroll
-        for (int i = 0; i < kSlotB; ++i) {
-            g[i] = 0.0;
-            if (s_src[i] >= 0 && (s_meta[i] & 0xFF) + 2 * c0 <= a.L - m)
-                g[i] = a.in[s_src[i] + 2 * ncomp * c0];
-        }
-    };
-    auto put = [&](int c0) {
-#pragma unroll
-        for (int i = 0; i < kSlotB; ++i) {
-            if (s_src[i] == -2) continue;
-            const int meta = s_meta[i];
-            const int kind = (meta >> 12) & 3;
-            double v = g[i];
-            if (kind != 2) {
-                // S_pol l(l+1)/r^2 (g_sph_rlm(j,3) = 1/2 at l = 0), dS/r, T/r
-                const double ri = rinv[(meta >> 8) & 7];
-                if (kind == 0) {
-                    const int l = m + 2 * c0 + (meta & 0xFF);
-                    v = v * (ri * ri) * ((l == 0) ? 0.5 : (double)(l * (l + 1)));
-                } else {
-                    v = v * ri;
-                }
-            }
-            const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
-            T::store(X + da, (meta & (1 << 14)) ? -v : v, slice);
-            if (db != kNone) T::store(X + db, (meta & (1 << 15)) ? -v : v, slice);
-        }
-    };
 
     auto load_a = [&](frag (&A)[RPW][4][NS], int ks) {
 #pragma unroll
@@ -1728,107 +1436,150 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
                 for (int s = 0; s < NS; ++s)
                     A[r][w][s] = T::load(ap[r] + (ks * 4 + w) * UE + s * 64 * EPL);
     };
-    auto mstep = [&](const frag (&A)[RPW][4][NS], int kl) {
-        frag b1[NC1][2][NS], bx[NC2][2][NS], by[NC2][2][NS];
+
+    const size_t sl = (size_t)a.nth * a.nr;
+    for (int kt = it.ktile; kt < it.ktile + it.nkt; ++kt) {
+        const int kt0 = kt * a.KT;
+        const int nkk = min(a.KT, a.nr - kt0);
+        const int koff = ncomp * a.J * kt0;
+        // the previous tile's last reads of rinv were before its final barrier
+        if (threadIdx.x < 8)
+            rinv[threadIdx.x] = ((int)threadIdx.x < nkk) ? 1.0 / a.radius[kt0 + threadIdx.x] : 0.0;
+        double g[kSlotB];
+        auto fetch = [&](int c0) {
+            opaque(s_src);
+            opaque(s_meta);
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+            for (int i = 0; i < kSlotB; ++i) {
+                g[i] = 0.0;
+                if (s_src[i] >= 0 && ((s_meta[i] >> 8) & 7) < nkk &&
+                    (s_meta[i] & 0xFF) + 2 * c0 <= a.L - m)
+                    g[i] = a.in[s_src[i] + koff + 2 * ncomp * c0];
+            }
+        };
+        auto put = [&](int c0) {
+            opaque(s_src);
+            opaque(s_dst);
+            opaque(s_meta);
 #pragma unroll
-            for (int ct = 0; ct < NC1; ++ct)
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    b1[ct][p][s] = T::load(X + s * slice + p * po + o1[ct] + kl);
-#pragma unroll
-            for (int ct = 0; ct < NC2; ++ct)
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    bx[ct][p][s] = T::load(X + s * slice + p * po + ox[ct] + kl);
-                    by[ct][p][s] = T::load(X + s * slice + p * po + oy[ct] + kl);
+            for (int i = 0; i < kSlotB; ++i) {
+                if (s_src[i] == -2) continue;
+                const int meta = s_meta[i];
+                const int kind = (meta >> 12) & 3;
+                double v = g[i];
+                if (kind != 2) {
+                    // S_pol l(l+1)/r^2 (g_sph_rlm(j,3) = 1/2 at l = 0), dS/r, T/r
+                    const double ri = rinv[(meta >> 8) & 7];
+                    if (kind == 0) {
+                        const int l = m + 2 * c0 + (meta & 0xFF);
+                        v = v * (ri * ri) * ((l == 0) ? 0.5 : (double)(l * (l + 1)));
+                    } else {
+                        v = v * ri;
+                    }
                 }
-        }
+                const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
+                T::store(X + da, (meta & (1 << 14)) ? -v : v, slice);
+                if (db != kNone) T::store(X + db, (meta & (1 << 15)) ? -v : v, slice);
+            }
+        };
+
+        acc a1[RPW][NC1][2], ax[RPW][NC2][2], ay[RPW][NC2][2];
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
 #pragma unroll
-            for (int ct = 0; ct < NC1; ++ct) {
-                T::mma(a1[r][ct][0], A[r][0], b1[ct][0]);
-                T::mma(a1[r][ct][1], A[r][1], b1[ct][1]);
-            }
+            for (int ct = 0; ct < NC1; ++ct) a1[r][ct][0] = a1[r][ct][1] = acc{};
 #pragma unroll
             for (int ct = 0; ct < NC2; ++ct) {
-                // symmetric part: dP odd, P even; antisymmetric: dP even, P odd
-                T::mma(ax[r][ct][0], A[r][3], bx[ct][1]);
-                T::mma(ax[r][ct][1], A[r][2], bx[ct][0]);
-                T::mma(ay[r][ct][0], A[r][0], by[ct][0]);
-                T::mma(ay[r][ct][1], A[r][1], by[ct][1]);
+                ax[r][ct][0] = ax[r][ct][1] = acc{};
+                ay[r][ct][0] = ay[r][ct][1] = acc{};
             }
         }
-    };
+        auto mstep = [&](const frag (&A)[RPW][4][NS], int kl) {
+            frag b1[NC1][2][NS], bx[NC2][2][NS], by[NC2][2][NS];
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+#pragma unroll
+                for (int ct = 0; ct < NC1; ++ct)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s)
+                        b1[ct][p][s] = T::load(X + s * slice + p * po + o1[ct] + kl);
+#pragma unroll
+                for (int ct = 0; ct < NC2; ++ct)
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        bx[ct][p][s] = T::load(X + s * slice + p * po + ox[ct] + kl);
+                        by[ct][p][s] = T::load(X + s * slice + p * po + oy[ct] + kl);
+                    }
+            }
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+#pragma unroll
+                for (int ct = 0; ct < NC1; ++ct) {
+                    T::mma(a1[r][ct][0], A[r][0], b1[ct][0]);
+                    T::mma(a1[r][ct][1], A[r][1], b1[ct][1]);
+                }
+#pragma unroll
+                for (int ct = 0; ct < NC2; ++ct) {
+                    // symmetric part: dP odd, P even; antisymmetric: dP even, P odd
+                    T::mma(ax[r][ct][0], A[r][3], bx[ct][1]);
+                    T::mma(ax[r][ct][1], A[r][2], bx[ct][0]);
+                    T::mma(ay[r][ct][0], A[r][0], by[ct][0]);
+                    T::mma(ay[r][ct][1], A[r][1], by[ct][1]);
+                }
+            }
+        };
 
-    frag Aq[PD][RPW][4][NS];
-    fetch(0);
+        frag Aq[PD][RPW][4][NS];
+        fetch(0);
 #pragma unroll
-    for (int d = 0; d < PD; ++d) load_a(Aq[d], min(d, KS - 1));
-    for (int c0 = 0; c0 < KS * KS_; c0 += KC) {
-        __syncthreads();
-        put(c0);
-        __syncthreads();
-        if (c0 + KC < KS * KS_) fetch(c0 + KC);
-        const int kse = min(KS, (c0 + KC) / KS_);
-        for (int kb = c0 / KS_; kb < kse; kb += PD) {
+        for (int d = 0; d < PD; ++d) load_a(Aq[d], min(d, KS - 1));
+        for (int c0 = 0; c0 < KS * KS_; c0 += KC) {
+            __syncthreads();
+            put(c0);
+            __syncthreads();
+            if (c0 + KC < KS * KS_) fetch(c0 + KC);
+            const int kse = min(KS, (c0 + KC) / KS_);
+            for (int kb = c0 / KS_; kb < kse; kb += PD) {
 #pragma unroll
-            for (int d = 0; d < PD; ++d) {
-                const int ks = kb + d;
-                mstep(Aq[d], ks * KS_ - c0);
-                load_a(Aq[d], min(ks + PD, KS - 1));
+                for (int d = 0; d < PD; ++d) {
+                    const int ks = kb + d;
+                    mstep(Aq[d], ks * KS_ - c0);
+                    load_a(Aq[d], min(ks + PD, KS - 1));
+                }
             }
         }
-    }
 
-    // epilogue: hemisphere row t -> out(t) = S + A, mirror nth-1-t -> S - A
-    const size_t sl = (size_t)a.nth * a.nr;
+        // epilogue: hemisphere row t -> out(t) = S + A, mirror nth-1-t -> S - A
+        opaque(ocol);
 #pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        if (!rok[r]) continue;
-        const int rt = rtile[r];
+        for (int r = 0; r < RPW; ++r) {
+            if (!rok[r]) continue;
+            const int rt = rtile[r];
 #pragma unroll
-        for (int ct = 0; ct < NC1; ++ct) {
-            const int c = ct * 16 + (lane & 15);
-            if (c >= c1n) continue;
-            const int per = nv + ns, kk = c / (2 * per), rem = c % (2 * per);
-            const int sg = rem / per, q = rem % per;
-            if (kk >= nkk || (sg == 1 && m == 0)) continue;
-            const int comp = q < nv ? 3 * q : 3 * nv + (q - nv);
-            const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
-            double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
+            for (int ct = 0; ct < NC1 + NC2; ++ct) {
+                const int oc = ocol[ct];
+                if (oc < 0) continue;
+                const int comp = oc & 0xFF, kk = (oc >> 8) & 7, sg = (oc >> 11) & 1;
+                if (kk >= nkk) continue;
+                const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
+                double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int t = rt * 16 + T::row(lane, v);
-                if (t >= a.th_h) continue;
-                const double S = T::get(a1[r][ct][0], v), An = T::get(a1[r][ct][1], v);
-                const int tm = a.nth - 1 - t;
-                o[(size_t)ncomp * t] = S + An;
-                if (tm != t) o[(size_t)ncomp * tm] = S - An;
-            }
-        }
-#pragma unroll
-        for (int ct = 0; ct < NC2; ++ct) {
-            const int c = ct * 16 + (lane & 15);
-            if (c >= c2n) continue;
-            const int kk = c / (4 * nv), rem = c % (4 * nv);
-            const int sg = rem / (2 * nv), r2 = rem % (2 * nv);
-            if (kk >= nkk || (sg == 1 && m == 0)) continue;
-            const int comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
-            const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
-            double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int t = rt * 16 + T::row(lane, v);
-                if (t >= a.th_h) continue;
-                const double f = (double)m * a.rsin[t];
-                const double S = T::get(ax[r][ct][0], v) + f * T::get(ay[r][ct][0], v);
-                const double An = T::get(ax[r][ct][1], v) + f * T::get(ay[r][ct][1], v);
-                const int tm = a.nth - 1 - t;
-                o[(size_t)ncomp * t] = S + An;
-                if (tm != t) o[(size_t)ncomp * tm] = S - An;
+                for (int v = 0; v < 4; ++v) {
+                    const int t = rt * 16 + T::row(lane, v);
+                    if (t >= a.th_h) continue;
+                    double S, An;
+                    if (ct < NC1) {
+                        S = T::get(a1[r][ct][0], v);
+                        An = T::get(a1[r][ct][1], v);
+                    } else {
+                        const double f = (double)m * a.rsin[t];
+                        S = T::get(ax[r][ct - NC1][0], v) + f * T::get(ay[r][ct - NC1][0], v);
+                        An = T::get(ax[r][ct - NC1][1], v) + f * T::get(ay[r][ct - NC1][1], v);
+                    }
+                    const int tm = a.nth - 1 - t;
+                    o[(size_t)ncomp * t] = S + An;
+                    if (tm != t) o[(size_t)ncomp * tm] = S - An;
+                }
             }
         }
     }
@@ -1970,10 +1721,6 @@ static size_t plan_lds(const LegPlan &p, bool fwd, int prec, int rows3 = 0)
     const int ns = (prec == kBF16x3) ? 3 : 1;
     const size_t es = (prec == kF64) ? 8 : (prec == kF32 ? 4 : 2);
     const int pad = (prec == kBF16x3) ? 8 : (prec == kF64 ? 2 : 4);   // lds_pad<T>()
-    if (p.ver == 2) {
-        const int ncol = 2 * p.KT * 2 * p.ncomp;
-        return (size_t)2 * ns * 2 * ncol * (p.tc + pad) * es;
-    }
     if (p.ver == 3) {
         const int nb = 16 * (p.mct1 + 2 * p.mct2) + 1;
         return (((size_t)ns * 2 * nb * (p.tc + pad) * es + 15) & ~(size_t)15)
@@ -2098,45 +1845,6 @@ static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd,
             out.push_back(p);
         }
     }
-    if (fwd) {
-        // second form: (rows per wave, waves) pairs; the row capacity is
-        // checked against the order pairs when the items are built
-        static const int f2[6][2] = {{1, 4}, {3, 8}, {5, 4}, {2, 8}, {1, 8}, {3, 4}};
-        for (int i = 0; i < 2; ++i) {
-            const int mc1 = (i == 0) ? 1 : 2, mc2 = (i == 0) ? 2 : 4;
-            int kt = 0;
-            for (int k = 8; k >= 1; --k)
-                if ((per1 * k + 15) / 16 <= mc1 && (per2 * k + 15) / 16 <= mc2) { kt = k; break; }
-            if (!kt) continue;
-            for (const auto &f : f2) {
-                // instances that would spill registers are not offered
-                if (prec == kBF16x3 && (f[0] > 3 || (f[1] == 8 && f[0] > 1) ||
-                                        (mc1 == 2 && f[0] > 1))) continue;
-                if (prec == kF64 && mc1 == 2 && ((f[1] == 8 && f[0] > 2) || f[0] > 3)) continue;
-                LegPlan p;
-                p.ncomp = ncomp;
-                p.nvec = nv;
-                p.nscl = ns;
-                p.fwd = true;
-                p.KT = kt;
-                p.mct1 = mc1;
-                p.mct2 = mc2;
-                p.rpw = f[0];
-                p.nw = f[1];
-                p.pair = 1;
-                p.ver = 2;
-                const int kmin = (prec == kBF16x3) ? 32 : 4;
-                const int ncol = 2 * kt * 2 * ncomp;
-                p.tc = 64;
-                while (p.tc > kmin && (plan_lds(p, true, prec) > 160 * 1024 ||
-                                       p.tc * ncol > kStageMax * 64 * p.nw))
-                    p.tc /= 2;
-                if (plan_lds(p, true, prec) > 160 * 1024) continue;
-                if (p.tc * ncol > kStageMax * 64 * p.nw) continue;
-                out.push_back(p);
-            }
-        }
-    }
     return out;
 }
 
@@ -2171,54 +1879,6 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
         int K = g.L - mm + 1, K0 = (K + 1) / 2, K1 = K / 2;
         return (K0 + 15) / 16 + (K1 + 15) / 16;
     };
-    if (p->ver == 2) {
-        int maxrows = 0;
-        for (int m = 0; m <= g.L - m; ++m) {
-            const int nrow = rows(m) + (g.L - m != m ? rows(g.L - m) : 0);
-            maxrows = std::max(maxrows, nrow);
-            for (int kt = 0; kt < nkt; ++kt) items.push_back({m, kt, 0, 0});
-        }
-        if (maxrows > p->nw * p->rpw) {
-            p->n_items = 0;
-            p->d_items = nullptr;
-            return 1;   // this plan cannot hold every row of an order pair
-        }
-        xcd_interleave(g.L, &items);
-        p->n_items = (int)items.size();
-        HIP_TRY(hipMalloc(&p->d_items, sizeof(WorkItem) * items.size()));
-        HIP_TRY(hipMemcpy(p->d_items, items.data(), sizeof(WorkItem) * items.size(),
-                          hipMemcpyHostToDevice));
-        return 0;
-    }
-    if (p->ver == 3) {
-        if (p->KT > 8) { *err = "radial tile above 8"; return -1; }
-        // the third form addresses the rtm / rlm arrays with 32-bit offsets
-        if ((double)p->ncomp * g.nth * g.nr * g.M >= 2147483647.0 ||
-            (double)p->ncomp * g.J * g.nr >= 2147483647.0) {
-            p->n_items = 0;
-            p->d_items = nullptr;
-            return 1;
-        }
-        for (int m = 0; m <= g.L; ++m) {
-            int ngroup;
-            if (fwd) {
-                const int K = g.L - m + 1, RTe = ((K + 1) / 2 + 15) / 16, RTo = (K / 2 + 15) / 16;
-                ngroup = (std::max(RTe, RTo) + 2 * p->rpw - 1) / (2 * p->rpw);
-            } else {
-                ngroup = (c->rt_bwd + 4 * p->rpw - 1) / (4 * p->rpw);
-            }
-            // the row groups of one radial tile are adjacent: they stage the
-            // same input, which they then share through L2
-            for (int kt = 0; kt < nkt; ++kt)
-                for (int rg = 0; rg < ngroup; ++rg) items.push_back({m, kt, rg, 0});
-        }
-        xcd_interleave(g.L, &items);
-        p->n_items = (int)items.size();
-        HIP_TRY(hipMalloc(&p->d_items, sizeof(WorkItem) * items.size()));
-        HIP_TRY(hipMemcpy(p->d_items, items.data(), sizeof(WorkItem) * items.size(),
-                          hipMemcpyHostToDevice));
-        return 0;
-    }
     for (int m = 0; m <= g.L; ++m) {
         int nrow;
         if (fwd) {
@@ -2395,6 +2055,17 @@ int leg_context_build(LegContext *c, int L, int nth, int nr, const double *radiu
     hipLaunchKernelGGL(k_schmidt_raw, grid, dim3(128), 0, c->stream, L, c->g.th_h, c->d_colat,
                        rP, rdP, rPM);
     HIP_TRY(hipGetLastError());
+    {
+        // g7 = (2l+1)/(4l(l+1)) and g6 = (2l+1)/4 of order m > 0
+        // (spherical_harmonics.f90:121-139); order 0 doubles both exactly
+        std::vector<double> gf(2 * (L + 1));
+        for (int l = 0; l <= L; ++l) {
+            gf[l] = (l == 0) ? 1.0 : (double)(2 * l + 1) / (double)(4 * l * (l + 1));
+            gf[L + 1 + l] = (double)(2 * l + 1) / 4.0;
+        }
+        HIP_TRY(hipMalloc(&c->d_gfac, sizeof(double) * gf.size()));
+        HIP_TRY(hipMemcpy(c->d_gfac, gf.data(), sizeof(double) * gf.size(), hipMemcpyHostToDevice));
+    }
     HIP_TRY(hipMalloc(&c->d_rsin, sizeof(double) * c->g.th_h));
     hipLaunchKernelGGL(k_rsin, dim3((c->g.th_h + 127) / 128), dim3(128), 0, c->stream, c->g.th_h,
                        c->d_colat, c->d_rsin);
@@ -2431,6 +2102,7 @@ void leg_context_free(LegContext *c)
     (void)hipFree(c->d_off_bwd3);
     (void)hipFree(c->d_off_fwd3);
     (void)hipFree(c->d_rsin);
+    (void)hipFree(c->d_gfac);
     for (LegPlan &p : c->plans) (void)hipFree(p.d_items);
     c->plans.clear();
     (void)hipFree(c->d_rlm);
@@ -2465,6 +2137,7 @@ static LegArgs make_args(LegContext *c, const LegPlan &p, bool fwd, const double
     a.TC = p.tc;
     a.pair = p.pair;
     a.rsin = c->d_rsin;
+    a.gfac = c->d_gfac;
     if (p.ver == 3) {
         a.tab = fwd ? c->d_tab_fwd3 : c->d_tab_bwd3;
         a.off = fwd ? c->d_off_fwd3 : c->d_off_bwd3;
@@ -2497,40 +2170,6 @@ static int launch_t(LegContext *c, const LegPlan &p, bool fwd, const double *in,
         hipLaunchKernelGGL((k_leg_bwd<T, RPW, MCT1, MCT2, PD>), dim3(p.n_items), dim3(kThreads), lds, s, a);
     HIP_TRY(hipGetLastError());
     return 0;
-}
-
-template <class T, int RPW, int MCT1, int MCT2, int NW>
-static int launch_f2(LegContext *c, const LegPlan &p, const double *in, double *out,
-                     hipStream_t s, std::string *err)
-{
-    LegArgs a = make_args(c, p, true, in, out);
-    const size_t lds = plan_lds(p, true, c->prec);
-    if (lds > 160 * 1024) { *err = "LDS tile too large for this component count"; return -1; }
-    if (lds > 64 * 1024)
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_leg_fwd2<T, RPW, MCT1, MCT2, NW>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_leg_fwd2<T, RPW, MCT1, MCT2, NW>), dim3(p.n_items), dim3(NW * 64), lds,
-                       s, a);
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
-
-template <class T, int MCT1, int MCT2>
-static int launch_f2m(LegContext *c, const LegPlan &p, const double *in, double *out,
-                      hipStream_t s, std::string *err)
-{
-    if (p.nw == 4) {
-        if (p.rpw == 1) return launch_f2<T, 1, MCT1, MCT2, 4>(c, p, in, out, s, err);
-        if (p.rpw == 3) return launch_f2<T, 3, MCT1, MCT2, 4>(c, p, in, out, s, err);
-        if constexpr (T::NS == 1)
-            if (p.rpw == 5) return launch_f2<T, 5, MCT1, MCT2, 4>(c, p, in, out, s, err);
-    } else {
-        if (p.rpw == 1) return launch_f2<T, 1, MCT1, MCT2, 8>(c, p, in, out, s, err);
-        if (p.rpw == 2) return launch_f2<T, 2, MCT1, MCT2, 8>(c, p, in, out, s, err);
-        if (p.rpw == 3) return launch_f2<T, 3, MCT1, MCT2, 8>(c, p, in, out, s, err);
-    }
-    *err = "no forward kernel instance for this plan";
-    return -1;
 }
 
 template <class T, int RPW, int NC1, int NC2, int PD>
@@ -2608,10 +2247,6 @@ static int launch_p(LegContext *c, const LegPlan &p, bool fwd, const double *in,
                     hipStream_t s, std::string *err)
 {
     if (p.ver == 3) return launch3<T>(c, p, fwd, in, out, s, err);
-    if (fwd && p.ver == 2) {
-        if (p.mct1 == 1) return launch_f2m<T, 1, 2>(c, p, in, out, s, err);
-        return launch_f2m<T, 2, 4>(c, p, in, out, s, err);
-    }
     if constexpr (T::NS == 1) {
         if (p.pd == 4) {
             if (p.mct1 == 1 && p.mct2 == 1) {
