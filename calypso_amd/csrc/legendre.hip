@@ -1879,6 +1879,39 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
         int K = g.L - mm + 1, K0 = (K + 1) / 2, K1 = K / 2;
         return (K0 + 15) / 16 + (K1 + 15) / 16;
     };
+    if (p->ver == 3) {
+        if (p->KT > 8) { *err = "radial tile above 8"; return -1; }
+        // the third form addresses the rtm / rlm arrays with 32-bit offsets
+        if ((double)p->ncomp * g.nth * g.nr * g.M >= 2147483647.0 ||
+            (double)p->ncomp * g.J * g.nr >= 2147483647.0) {
+            p->n_items = 0;
+            p->d_items = nullptr;
+            return 1;
+        }
+        auto ngroup = [&](int m) {
+            const int K = g.L - m + 1, RTe = ((K + 1) / 2 + 15) / 16, RTo = (K / 2 + 15) / 16;
+            return fwd ? (std::max(RTe, RTo) + 2 * p->rpw - 1) / (2 * p->rpw)
+                       : (c->rt_bwd + 4 * p->rpw - 1) / (4 * p->rpw);
+        };
+        // radial tiles per work-group: as many as keep >= 2048 work-groups
+        long long n1 = 0;
+        for (int m = 0; m <= g.L; ++m) n1 += (long long)nkt * ngroup(m);
+        const int ntile = (int)std::max(1LL, std::min(8LL, n1 / 2048));
+        for (int m = 0; m <= g.L; ++m) {
+            // a work-group runs ntile consecutive radial tiles; the row groups
+            // of one radial tile are adjacent (they stage the same input and
+            // share it through L2)
+            for (int kt = 0; kt < nkt; kt += ntile)
+                for (int rg = 0; rg < ngroup(m); ++rg)
+                    items.push_back({m, kt, rg, std::min(ntile, nkt - kt)});
+        }
+        xcd_interleave(g.L, &items);
+        p->n_items = (int)items.size();
+        HIP_TRY(hipMalloc(&p->d_items, sizeof(WorkItem) * items.size()));
+        HIP_TRY(hipMemcpy(p->d_items, items.data(), sizeof(WorkItem) * items.size(),
+                          hipMemcpyHostToDevice));
+        return 0;
+    }
     for (int m = 0; m <= g.L; ++m) {
         int nrow;
         if (fwd) {

@@ -193,7 +193,9 @@ def test_full_size_roundtrip_fp64(torch_cuda):
     lt.close()
 
 
-PLAN_CASES = [(31, 48, 9, 4, 1), (31, 48, 5, 6, 1), (47, 72, 7, 4, 0), (64, 96, 6, 1, 1)]
+PLAN_CASES = [(31, 48, 9, 4, 1), (31, 48, 5, 6, 1), (47, 72, 7, 4, 0), (64, 96, 6, 1, 1),
+              # the golden-fixture shapes: tiny grid, odd number of colatitudes
+              (7, 12, 3, 3, 1), (10, 17, 2, 1, 2)]
 
 
 @pytest.mark.parametrize("mode", ["hip_fp64", "hip_fp32", "hip_bf16x3"])
