@@ -33,6 +33,31 @@ PEAK = {"hip_fp64": 78.6, "hip_fp32": 157.3, "hip_bf16x3": 2500.0}   # TFLOP/s d
 HBM_PEAK = 8000.0   # GB/s
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "legendre_L255_nth384_nr96_c12_traffic.md")
+
+
+def kernel_name(direction, mode, plan):
+    """Template instance that runs a plan (as rocprofv3 names it)."""
+    t = {"hip_fp64": "TF64", "hip_fp32": "TF32", "hip_bf16x3": "TBF3"}[mode]
+    form = "3" if plan["kernel_form"] == 3 else ""
+    return (f"k_leg_{direction}{form}<{t}, {plan['rows_per_wave']}, {plan['mct1']}, "
+            f"{plan['mct2']}, {plan['prefetch']}>")
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of `name` from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a 1 GiB copy), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic: ") and "=" in line:
+                k, v = line[len("traffic: "):].rsplit("=", 1)
+                if k.strip() == name:
+                    return float(v)
+    except OSError:
+        pass
+    return None
+
+
 def legendre_flops(L, nth, nr, nvec, nscl):
     """Symmetric-form work of one direction: 2 th_h Nr (L+1)^2 (5 nvec + nscl)."""
     th_h = (nth + 1) // 2
@@ -165,6 +190,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mhd", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device copy (reference for the byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the sharded "
@@ -207,6 +234,7 @@ def main():
     dom = "bwd" if h["bwd_ms"] >= h["fwd_ms"] else "fwd"
     dom_ms = h[f"{dom}_ms"]
     achieved = flops / (dom_ms * 1e-3) / 1e12
+    dom_kernel = kernel_name(dom, head, h[f"plan_{dom}"])
     line = {
         "metric": "steps/s + Legendre TFLOP/s, mixed-prec vs FP64 GPU, Lmax=255 Nr=96, err<=1e-11",
         "value": h["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
@@ -219,10 +247,16 @@ def main():
                    "err_bound": ERR_BOUND},
         "legendre_tflops": h["legendre_tflops"],
         "modes": out_modes,
-        "roofline": {"kernel": f"k_leg_{dom} ({head})", "bound": "mfma", "achieved": achieved,
+        "roofline": {"kernel": dom_kernel, "bound": "mfma", "achieved": achieved,
                      "peak": PEAK[head], "unit": "TFLOP/s", "frac": achieved / PEAK[head],
-                     "traffic": None},
+                     "traffic": measured_traffic(dom_kernel)},
     }
+    if args.copy_calibration:
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
     if not args.no_mhd:
         line["mhd_step"] = mhd_bench(torch)
     if not args.no_cpu:

@@ -24,6 +24,9 @@
 // 16-row tiles; the +m and -m orders share every P tile):
 //   backward: rows = colatitude, K = degree;  out(t) = S+A, out(pi-t) = S-A
 //   forward : rows = degree,     K = colatitude, input folded N+-S on load
+// Two forms: the first reads P, dP and mP/sin(theta) tiles; the third reads P
+// and dP only, stages the B operand GEMM-ready and keeps its K loop free of
+// branches (see its section).  The autotuner picks per direction.
 // Precisions: FP64 MFMA 16x16x4, FP32 MFMA 16x16x4, split-BF16x3 MFMA
 // 16x16x32 (three bf16 slices per operand, six slice products, FP32
 // accumulation).  P_l^m tables are computed on the GPU in FP64 and packed in
