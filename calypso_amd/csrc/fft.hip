@@ -11,15 +11,18 @@
 // spherical_SRs_N.f90:333-369) become the gather/scatter of these kernels.
 //
 // One work-group owns one (radius, colatitude) pencil and all of its
-// components: it stages the pencil's 2L+1 coefficients of every component in
-// LDS (reads of ncomp contiguous doubles per order), runs the FFTs of pairs of
-// real components as complex Stockham FFTs (radix 4/2/3, FP64, twiddles from
-// an FP64 table in LDS), and writes the other side.  The fused MHD kernel
-// keeps the grid values in LDS between the inverse FFT, the products
-// (cal_nonlinear_sph_MHD.f90:35-75) and the forward FFT, so the physical grid
-// never touches HBM.
+// components: it stages the pencil's 2L+1 coefficients of a group of
+// components in LDS (reads of contiguous doubles per order), runs the FFTs of
+// pairs of real components as complex Stockham FFTs in place (radix 4/2/3,
+// FP64, twiddles from an FP64 table in LDS, butterfly inputs held in
+// registers across each stage's barrier), and writes the other side.  The
+// fused MHD kernel keeps the grid values in LDS between the inverse FFT, the
+// products (cal_nonlinear_sph_MHD.f90:35-75) and the forward FFT, so the
+// physical grid never touches HBM; the spectra are staged in the same LDS
+// buffer, so a pencil needs (pairs + 1) * nphi complex words.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -58,88 +61,103 @@ struct FftArgs {
     double coef[4];     // coef_velo, coef_lor, coef_induct, coef_temp
 };
 
-// One Stockham stage of radix R over `nseq` complex sequences of length N:
-// data0 -> data1 (Govindaraju et al., SC'08, mixed-radix autosort form).
-template <int R>
-__device__ void stockham_stage(const cplx *d0, cplx *d1, const cplx *tw, int N, int Ns,
-                               int nseq, double sgn)
+constexpr int kFftMaxE = 12;   // complex values per thread per stage (any radix)
+constexpr int kFftMaxZ = 4;    // frequencies per thread per pair (inverse side)
+constexpr int kFftMaxU = 2;    // orders per thread per pair (forward side)
+
+// One in-place Stockham stage of radix R over `nseq` complex sequences of
+// length N held in buf (sequence s at buf + s N), mixed-radix autosort form
+// (Govindaraju et al., SC'08): every thread reads the R inputs of its
+// butterflies into registers, the work-group synchronises, and the outputs go
+// to their autosorted positions in the same buffer.
+template <int R, int NT>
+__device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq, double sgn)
 {
-    const int nb = N / R;
-    for (int idx = threadIdx.x; idx < nseq * nb; idx += blockDim.x) {
-        const int s = idx / nb, j = idx % nb;
-        const int kk = j % Ns;
-        const int tstep = N / (Ns * R);
-        const cplx *src = d0 + (size_t)s * N;
-        cplx v[R];
+    constexpr int MB = kFftMaxE / R;   // butterflies per thread
+    const int nb = N / R, total = nseq * nb;
+    const int tstep = N / (Ns * R);
+    cplx v[MB][R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            v[r] = src[j + r * nb];
-            if (r) {
+    for (int b = 0; b < MB; ++b) {
+        const int idx = threadIdx.x + b * NT;
+        if (idx < total) {
+            const int s = idx / nb, j = idx - s * nb;
+            const cplx *src = buf + (size_t)s * N;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = src[j + r * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int idx = threadIdx.x + b * NT;
+        if (idx < total) {
+            const int s = idx / nb, j = idx - s * nb;
+            const int kk = j % Ns;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
                 cplx w = tw[(r * kk * tstep) % N];
                 w.y *= -sgn;  // table holds exp(-i...); sgn=+1 (inverse) conjugates
-                v[r] = cmul(v[r], w);
+                v[b][r] = cmul(v[b][r], w);
             }
-        }
-        cplx o[R];
-        if constexpr (R == 2) {
-            o[0] = cadd(v[0], v[1]);
-            o[1] = csub(v[0], v[1]);
-        } else if constexpr (R == 4) {
-            cplx a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-            cplx b0 = cadd(v[1], v[3]), b1 = cmul_si(csub(v[1], v[3]), sgn);
-            o[0] = cadd(a0, b0);
-            o[1] = cadd(a1, b1);
-            o[2] = csub(a0, b0);
-            o[3] = csub(a1, b1);
-        } else {  // R == 3
-            const double c = -0.5, sn = sgn * 0.86602540378443864676;
-            cplx t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-            cplx m1 = {v[0].x + c * t1.x, v[0].y + c * t1.y};
-            cplx m2 = {-sn * t2.y, sn * t2.x};
-            o[0] = cadd(v[0], t1);
-            o[1] = cadd(m1, m2);
-            o[2] = csub(m1, m2);
-        }
-        cplx *dst = d1 + (size_t)s * N + (j / Ns) * Ns * R + kk;
+            cplx o[R];
+            if constexpr (R == 2) {
+                o[0] = cadd(v[b][0], v[b][1]);
+                o[1] = csub(v[b][0], v[b][1]);
+            } else if constexpr (R == 4) {
+                cplx a0 = cadd(v[b][0], v[b][2]), a1 = csub(v[b][0], v[b][2]);
+                cplx b0 = cadd(v[b][1], v[b][3]), b1 = cmul_si(csub(v[b][1], v[b][3]), sgn);
+                o[0] = cadd(a0, b0);
+                o[1] = cadd(a1, b1);
+                o[2] = csub(a0, b0);
+                o[3] = csub(a1, b1);
+            } else {  // R == 3
+                const double c = -0.5, sn = sgn * 0.86602540378443864676;
+                cplx t1 = cadd(v[b][1], v[b][2]), t2 = csub(v[b][1], v[b][2]);
+                cplx m1 = {v[b][0].x + c * t1.x, v[b][0].y + c * t1.y};
+                cplx m2 = {-sn * t2.y, sn * t2.x};
+                o[0] = cadd(v[b][0], t1);
+                o[1] = cadd(m1, m2);
+                o[2] = csub(m1, m2);
+            }
+            cplx *dst = buf + (size_t)s * N + (j / Ns) * Ns * R + kk;
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[r * Ns] = o[r];
+            for (int r = 0; r < R; ++r) dst[r * Ns] = o[r];
+        }
     }
+    __syncthreads();
 }
 
-// full FFT over nseq sequences; returns the buffer holding the result
-__device__ cplx *fft_lds(cplx *b0, cplx *b1, const cplx *tw, const FftArgs &a, int nseq,
-                         double sgn)
+// full in-place FFT of nseq sequences
+template <int NT>
+__device__ void fft_inplace(cplx *buf, const cplx *tw, const FftArgs &a, int nseq, double sgn)
 {
     int Ns = 1;
-    cplx *src = b0, *dst = b1;
     for (int st = 0; st < a.nstage; ++st) {
         const int R = a.radix[st];
-        if (R == 4) stockham_stage<4>(src, dst, tw, a.N, Ns, nseq, sgn);
-        else if (R == 2) stockham_stage<2>(src, dst, tw, a.N, Ns, nseq, sgn);
-        else stockham_stage<3>(src, dst, tw, a.N, Ns, nseq, sgn);
-        __syncthreads();
+        if (R == 4) stage_inplace<4, NT>(buf, tw, a.N, Ns, nseq, sgn);
+        else if (R == 2) stage_inplace<2, NT>(buf, tw, a.N, Ns, nseq, sgn);
+        else stage_inplace<3, NT>(buf, tw, a.N, Ns, nseq, sgn);
         Ns *= R;
-        cplx *t = src;
-        src = dst;
-        dst = t;
     }
-    return src;
 }
 
 enum PencilMode { kPencilBwd = 0, kPencilFwd = 1, kPencilMhd = 2 };
 
-// LDS: spec[ncs][M] doubles | tw[N] cplx | buf0[PB][N] cplx | buf1[PB][N] cplx
-template <int MODE>
-__global__ __launch_bounds__(256) void k_pencil(FftArgs a)
+// One work-group per (radius, colatitude) pencil.  LDS: tw[N] | buf[PB][N]
+// complex; the spectra of the pencil are staged in buf (as doubles) before
+// the inverse FFTs and after the forward ones, so a pencil needs (PB + 1) N
+// complex words.  Components go in pairs (real / imaginary part of one
+// complex FFT), PB pairs per group; the MHD mode holds all 7 input pairs.
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int M = 2 * a.L + 1, N = a.N, L = a.L;
-    const int ncs = a.ncomp_in > a.ncomp_out ? a.ncomp_in : a.ncomp_out;
-    double *spec = reinterpret_cast<double *>(smem_raw);
-    const size_t spec_bytes = ((size_t)ncs * M * sizeof(double) + 15) & ~(size_t)15;
-    cplx *tw = reinterpret_cast<cplx *>(smem_raw + spec_bytes);
-    cplx *buf0 = tw + N;
-    cplx *buf1 = buf0 + (size_t)a.pairs_per_batch * N;
+    cplx *tw = reinterpret_cast<cplx *>(smem_raw);
+    cplx *buf = tw + N;
+    double *bd = reinterpret_cast<double *>(buf);
+    const int PB = a.pairs_per_batch;
 
     const int pencil = blockIdx.x;            // t + nth*k (rtm pencil order)
     const int t = pencil % a.nth, k = pencil / a.nth;
@@ -147,81 +165,87 @@ __global__ __launch_bounds__(256) void k_pencil(FftArgs a)
     const size_t nnod = (size_t)a.nr * a.nth * N;
     const size_t gbase = (size_t)k + (size_t)a.nr * t;  // rtp node at phi = 0
 
-    for (int i = threadIdx.x; i < N; i += blockDim.x)
-        tw[i] = {a.tw[2 * i], a.tw[2 * i + 1]};
+    for (int i = threadIdx.x; i < N; i += NT) tw[i] = {a.tw[2 * i], a.tw[2 * i + 1]};
 
-    // ---- input side --------------------------------------------------
+    // ---- inverse side -----------------------------------------------
     const int nin = a.ncomp_in;
     const int npair_in = (nin + 1) / 2;
     if (MODE == kPencilBwd || MODE == kPencilMhd) {
-        for (int e = threadIdx.x; e < nin * M; e += blockDim.x) {
-            const int nd = e % nin, mp = e / nin;
-            spec[nd * M + mp] = a.in[nd + (size_t)nin * (pencil + sl * mp)];
-        }
-    }
-    __syncthreads();
-
-    // grid values live in the result buffer of the inverse FFT (pairs of
-    // components as real/imaginary parts); MHD mode needs all 7 pairs at once
-    cplx *grid = nullptr;
-    if (MODE == kPencilBwd || MODE == kPencilMhd) {
-        for (int p0 = 0; p0 < npair_in; p0 += a.pairs_per_batch) {
-            const int np = min(a.pairs_per_batch, npair_in - p0);
-            // Hermitian spectra of z = x_a + i x_b
-            for (int e = threadIdx.x; e < np * N; e += blockDim.x) {
-                const int p = e / N, kf = e % N;
-                const int ca = 2 * (p0 + p), cb = ca + 1;
-                const bool hb = cb < nin;
-                cplx z = {0.0, 0.0};
-                if (kf == 0) {
-                    z.x = spec[ca * M + L];
-                    z.y = hb ? spec[cb * M + L] : 0.0;
-                } else if (kf <= L) {
-                    const int m = kf;
-                    double aa = spec[ca * M + L + m], ba = spec[ca * M + L - m];
-                    double ab = hb ? spec[cb * M + L + m] : 0.0;
-                    double bb = hb ? spec[cb * M + L - m] : 0.0;
-                    z.x = 0.5 * (aa + bb);
-                    z.y = 0.5 * (ab - ba);
-                } else if (kf >= N - L) {
-                    const int m = N - kf;
-                    double aa = spec[ca * M + L + m], ba = spec[ca * M + L - m];
-                    double ab = hb ? spec[cb * M + L + m] : 0.0;
-                    double bb = hb ? spec[cb * M + L - m] : 0.0;
-                    z.x = 0.5 * (aa - bb);
-                    z.y = 0.5 * (ba + ab);
-                }
-                buf0[(size_t)p * N + kf] = z;
+        for (int p0 = 0; p0 < npair_in; p0 += PB) {
+            const int np = min(PB, npair_in - p0);
+            const int c0 = 2 * p0, nc = min(2 * np, nin - c0);
+            __syncthreads();   // buffer free (previous group written out)
+            // the group's spectra, nc contiguous doubles per order
+            for (int e = threadIdx.x; e < nc * M; e += NT) {
+                const int nd = e % nc, mp = e / nc;
+                bd[nd * M + mp] = a.in[c0 + nd + (size_t)nin * (pencil + sl * mp)];
             }
             __syncthreads();
-            grid = fft_lds(buf0, buf1, tw, a, np, +1.0);
-            if (MODE == kPencilBwd) {
-                for (int e = threadIdx.x; e < np * N; e += blockDim.x) {
-                    const int p = e / N, i = e % N;
-                    const int ca = 2 * (p0 + p), cb = ca + 1;
-                    const cplx z = grid[(size_t)p * N + i];
-                    a.out[gbase + (size_t)a.nr * a.nth * i + nnod * ca] = z.x;
-                    if (cb < nin) a.out[gbase + (size_t)a.nr * a.nth * i + nnod * cb] = z.y;
+            // Hermitian spectra of z = x_a + i x_b, pair by pair from the last:
+            // pair p's sequence covers doubles [2pN, 2pN+2N), which holds its
+            // own staged rows and those of later pairs only, so each pair
+            // passes through registers once, with one barrier
+            for (int p = np - 1; p >= 0; --p) {
+                const int ca = 2 * p, cb = ca + 1;
+                const bool hb = cb < nc;
+                cplx z[kFftMaxZ];
+#pragma unroll
+                for (int q = 0; q < kFftMaxZ; ++q) {
+                    const int kf = threadIdx.x + q * NT;
+                    z[q] = {0.0, 0.0};
+                    if (kf < N) {
+                        if (kf == 0) {
+                            z[q].x = bd[ca * M + L];
+                            z[q].y = hb ? bd[cb * M + L] : 0.0;
+                        } else if (kf <= L) {
+                            const int m = kf;
+                            const double aa = bd[ca * M + L + m], ba = bd[ca * M + L - m];
+                            const double ab = hb ? bd[cb * M + L + m] : 0.0;
+                            const double bb = hb ? bd[cb * M + L - m] : 0.0;
+                            z[q].x = 0.5 * (aa + bb);
+                            z[q].y = 0.5 * (ab - ba);
+                        } else if (kf >= N - L) {
+                            const int m = N - kf;
+                            const double aa = bd[ca * M + L + m], ba = bd[ca * M + L - m];
+                            const double ab = hb ? bd[cb * M + L + m] : 0.0;
+                            const double bb = hb ? bd[cb * M + L - m] : 0.0;
+                            z[q].x = 0.5 * (aa - bb);
+                            z[q].y = 0.5 * (ba + ab);
+                        }
+                    }
                 }
                 __syncthreads();
+#pragma unroll
+                for (int q = 0; q < kFftMaxZ; ++q) {
+                    const int kf = threadIdx.x + q * NT;
+                    if (kf < N) buf[(size_t)p * N + kf] = z[q];
+                }
+            }
+            __syncthreads();
+            fft_inplace<NT>(buf, tw, a, np, +1.0);
+            if (MODE == kPencilBwd) {
+                for (int e = threadIdx.x; e < np * N; e += NT) {
+                    const int p = e / N, i = e % N;
+                    const int ca = c0 + 2 * p, cb = ca + 1;
+                    const cplx zz = buf[e];
+                    a.out[gbase + (size_t)a.nr * a.nth * i + nnod * ca] = zz.x;
+                    if (cb < nin) a.out[gbase + (size_t)a.nr * a.nth * i + nnod * cb] = zz.y;
+                }
             }
         }
     }
     if (MODE == kPencilBwd) return;
 
-    // ---- products (MHD) or grid load (fwd) ---------------------------
-    // forward input pairs: fwd mode = component pairs of the grid array;
-    // MHD mode = the 12 product components in f_trns order 0-2, 9-17
-    cplx *fin = (grid == buf0) ? buf1 : buf0;
+    // ---- products (MHD, in place per longitude) ---------------------
     const int nout = a.ncomp_out;
     int npair_f;
     if (MODE == kPencilMhd) {
         npair_f = 6;
-        for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        for (int i = threadIdx.x; i < N; i += NT) {
             // b_trns: u 0-2, w 3-5, B 6-8, J 9-11, T 12 -> pairs (0,1)...(12,-)
-            const cplx c01 = grid[0 * N + i], c23 = grid[1 * N + i], c45 = grid[2 * N + i];
-            const cplx c67 = grid[3 * N + i], c89 = grid[4 * N + i], c1011 = grid[5 * N + i];
-            const cplx c12 = grid[6 * N + i];
+            const cplx c01 = buf[0 * N + i], c23 = buf[1 * N + i], c45 = buf[2 * N + i];
+            const cplx c67 = buf[3 * N + i], c89 = buf[4 * N + i], c1011 = buf[5 * N + i];
+            const cplx c12 = buf[6 * N + i];
             const double u1 = c01.x, u2 = c01.y, u3 = c23.x;
             const double w1 = c23.y, w2 = c45.x, w3 = c45.y;
             const double b1 = c67.x, b2 = c67.y, b3 = c89.x;
@@ -247,66 +271,78 @@ __global__ __launch_bounds__(256) void k_pencil(FftArgs a)
             f[11] = u3 * T * ct;
             }
 #pragma unroll
-            for (int p = 0; p < 6; ++p) fin[(size_t)p * N + i] = {f[2 * p], f[2 * p + 1]};
+            for (int p = 0; p < 6; ++p) buf[(size_t)p * N + i] = {f[2 * p], f[2 * p + 1]};
         }
         __syncthreads();
     } else {
         npair_f = (nin + 1) / 2;
     }
 
-    // ---- forward FFTs ---------------------------------------------------
-    for (int e = threadIdx.x; e < nout * M; e += blockDim.x) spec[e] = 0.0;
+    // ---- forward FFTs -----------------------------------------------------
     const double invN = 1.0 / (double)N;
-    for (int p0 = 0; p0 < npair_f; p0 += a.pairs_per_batch) {
-        const int np = min(a.pairs_per_batch, npair_f - p0);
-        cplx *src = fin;
+    for (int p0 = 0; p0 < npair_f; p0 += PB) {
+        const int np = min(PB, npair_f - p0);
         if (MODE == kPencilFwd) {
-            src = buf0;
-            for (int e = threadIdx.x; e < np * N; e += blockDim.x) {
+            __syncthreads();   // buffer free
+            for (int e = threadIdx.x; e < np * N; e += NT) {
                 const int p = e / N, i = e % N;
                 const int ca = 2 * (p0 + p), cb = ca + 1;
                 const size_t g = gbase + (size_t)a.nr * a.nth * i;
-                src[(size_t)p * N + i] = {a.in[g + nnod * ca], cb < nin ? a.in[g + nnod * cb] : 0.0};
+                buf[e] = {a.in[g + nnod * ca], cb < nin ? a.in[g + nnod * cb] : 0.0};
             }
             __syncthreads();
         }
-        cplx *other = (src == buf0) ? buf1 : buf0;
-        cplx *Z = fft_lds(src + (MODE == kPencilMhd ? (size_t)p0 * N : 0),
-                          other + (MODE == kPencilMhd ? (size_t)p0 * N : 0), tw, a, np, -1.0);
-        // unpack two real spectra; component ids in the output array
-        for (int e = threadIdx.x; e < np * (L + 1); e += blockDim.x) {
-            const int p = e / (L + 1), m = e % (L + 1);
-            const cplx zk = Z[(size_t)p * N + m];
-            const cplx zn = Z[(size_t)p * N + (m == 0 ? 0 : N - m)];
-            // X_a = (Z_k + conj Z_{N-k})/2, X_b = (Z_k - conj Z_{N-k})/(2i)
-            const double xar = 0.5 * (zk.x + zn.x), xai = 0.5 * (zk.y - zn.y);
-            const double xbr = 0.5 * (zk.y + zn.y), xbi = -0.5 * (zk.x - zn.x);
-            int ca, cb;
-            if (MODE == kPencilMhd) {
-                const int q = 2 * (p0 + p);              // product index 0..11
-                ca = q < 3 ? q : q + 6;                  // f_trns slot
-                cb = (q + 1) < 3 ? q + 1 : q + 7;
-            } else {
-                ca = 2 * (p0 + p);
-                cb = ca + 1;
+        fft_inplace<NT>(buf, tw, a, np, -1.0);
+        // unpack two real spectra per pair, pair by pair: the spectra of pair
+        // p (rows a, b of M doubles) go to the start of its own sequence,
+        // doubles [2pN, 2pN + 2M), which no later pair reads
+        for (int p = 0; p < np; ++p) {
+            double u[kFftMaxU][4];
+#pragma unroll
+            for (int q = 0; q < kFftMaxU; ++q) {
+                const int m = threadIdx.x + q * NT;
+                if (m <= L) {
+                    const cplx zk = buf[(size_t)p * N + m];
+                    const cplx zn = buf[(size_t)p * N + (m == 0 ? 0 : N - m)];
+                    // X_a = (Z_k + conj Z_{N-k})/2, X_b = (Z_k - conj Z_{N-k})/(2i)
+                    const double xar = 0.5 * (zk.x + zn.x), xai = 0.5 * (zk.y - zn.y);
+                    const double xbr = 0.5 * (zk.y + zn.y), xbi = -0.5 * (zk.x - zn.x);
+                    if (m == 0) {
+                        u[q][0] = xar * invN; u[q][1] = 0.0;
+                        u[q][2] = xbr * invN; u[q][3] = 0.0;
+                    } else {
+                        u[q][0] = 2.0 * xar * invN; u[q][1] = -2.0 * xai * invN;
+                        u[q][2] = 2.0 * xbr * invN; u[q][3] = -2.0 * xbi * invN;
+                    }
+                }
             }
-            if (m == 0) {
-                spec[ca * M + L] = xar * invN;
-                if (cb < nout) spec[cb * M + L] = xbr * invN;
-            } else {
-                spec[ca * M + L + m] = 2.0 * xar * invN;
-                spec[ca * M + L - m] = -2.0 * xai * invN;
-                if (cb < nout) {
-                    spec[cb * M + L + m] = 2.0 * xbr * invN;
-                    spec[cb * M + L - m] = -2.0 * xbi * invN;
+            __syncthreads();
+            double *rows = bd + (size_t)2 * p * N;
+#pragma unroll
+            for (int q = 0; q < kFftMaxU; ++q) {
+                const int m = threadIdx.x + q * NT;
+                if (m <= L) {
+                    rows[L + m] = u[q][0];
+                    rows[M + L + m] = u[q][2];
+                    if (m > 0) {
+                        rows[L - m] = u[q][1];
+                        rows[M + L - m] = u[q][3];
+                    }
                 }
             }
         }
         __syncthreads();
-    }
-    for (int e = threadIdx.x; e < nout * M; e += blockDim.x) {
-        const int nd = e % nout, mp = e / nout;
-        a.out[nd + (size_t)nout * (pencil + sl * mp)] = spec[nd * M + mp];
+        // write out, nout (MHD) or this group's components contiguous per order;
+        // MHD slots the products do not fill (3-8, 18) are zero
+        const int oc0 = (MODE == kPencilMhd) ? 0 : 2 * p0;
+        const int onc = (MODE == kPencilMhd) ? nout : min(2 * np, nout - oc0);
+        for (int e = threadIdx.x; e < onc * M; e += NT) {
+            const int nd = e % onc, mp = e / onc;
+            int q = nd;   // local component -> pair 2q, row q%2
+            if (MODE == kPencilMhd) q = nd < 3 ? nd : (nd >= 9 && nd < 18 ? nd - 6 : -1);
+            const double v = (q >= 0) ? bd[(size_t)2 * (q >> 1) * N + (q & 1) * M + mp] : 0.0;
+            a.out[oc0 + nd + (size_t)nout * (pencil + sl * mp)] = v;
+        }
     }
 }
 
@@ -376,11 +412,48 @@ void fft_context_free(FftContext *f)
     if (f) f->d_tw = nullptr;
 }
 
-static size_t pencil_lds(const FftContext *f, int ncs, int pb)
+// LDS of a pencil work-group with pb component pairs per group
+static size_t pencil_lds(const FftContext *f, int pb)
 {
-    const size_t M = 2 * (size_t)f->L + 1;
-    size_t s = ((size_t)ncs * M * sizeof(double) + 15) & ~(size_t)15;
-    return s + (size_t)f->N * 16 + 2 * (size_t)pb * f->N * 16;
+    return (size_t)(pb + 1) * f->N * 16;
+}
+
+// pairs per group: everything the register staging and the spectra staging
+// allow (the MHD mode needs its 7 input pairs in one group, and room for the
+// 19 output spectra)
+static int pencil_pairs(const FftContext *f, int mode, int nin, int nout, int nt)
+{
+    const int M = 2 * f->L + 1, N = f->N;
+    const int need = (mode == kPencilMhd) ? 7 : (std::max(nin, nout) + 1) / 2;
+    int pb = need;
+
+    if (N > kFftMaxZ * nt || f->L + 1 > kFftMaxU * nt) return 0;
+    auto fits = [&](int p) {
+        const int np = std::min(p, need);
+        for (int st = 0; st < f->nstage; ++st)
+            if (np * (N / f->radix[st]) > (kFftMaxE / f->radix[st]) * nt) return false;
+        return pencil_lds(f, p) <= 160 * 1024;
+    };
+    if (mode == kPencilMhd) return fits(pb) ? pb : 0;
+    while (pb > 1 && !fits(pb)) --pb;
+    return fits(pb) ? pb : 0;
+}
+
+template <int MODE>
+static int launch_nt(const FftContext *f, const FftArgs &a, size_t lds, int nt, hipStream_t s,
+                     std::string *err)
+{
+    const int grid = f->nr * f->nth;
+    const void *fn = nt == 512 ? reinterpret_cast<const void *>(&k_pencil<MODE, 512>)
+                               : reinterpret_cast<const void *>(&k_pencil<MODE, 256>);
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (nt == 512)
+        hipLaunchKernelGGL((k_pencil<MODE, 512>), dim3(grid), dim3(512), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_pencil<MODE, 256>), dim3(grid), dim3(256), lds, s, a);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 static int launch_pencil(const FftContext *f, int mode, int nin, int nout, const double *in,
@@ -399,34 +472,22 @@ static int launch_pencil(const FftContext *f, int mode, int nin, int nout, const
     for (int i = 0; i < FFT_MAX_STAGES; ++i) a.radix[i] = f->radix[i];
     a.tw = f->d_tw;
     for (int i = 0; i < 4; ++i) a.coef[i] = coef ? coef[i] : 0.0;
-    const int ncs = nin > nout ? nin : nout;
-    const size_t limit = 160 * 1024;
-    int pb;
-    if (mode == kPencilMhd) {
-        pb = 7;
-        if (pencil_lds(f, ncs, pb) > limit) { *err = "pencil too long for the fused MHD kernel"; return -1; }
-    } else {
-        const int npair = ((mode == kPencilBwd ? nin : nin) + 1) / 2;
-        pb = npair;
-        while (pb > 1 && pencil_lds(f, ncs, pb) > limit) --pb;
-        if (pencil_lds(f, ncs, pb) > limit) { *err = "pencil too long for LDS"; return -1; }
+    // 256 threads while two or more work-groups fit on a CU, else 512
+    int nt = 256, pb = pencil_pairs(f, mode, nin, nout, 256);
+    if (!pb || pencil_lds(f, pb) > 80 * 1024) {
+        const int pb2 = pencil_pairs(f, mode, nin, nout, 512);
+        if (pb2) { nt = 512; pb = pb2; }
+    }
+    if (!pb) {
+        *err = mode == kPencilMhd ? "pencil too long for the fused MHD kernel"
+                                  : "pencil too long for LDS";
+        return -1;
     }
     a.pairs_per_batch = pb;
-    const size_t lds = pencil_lds(f, ncs, pb);
-    const int grid = f->nr * f->nth;
-    const void *fn = mode == kPencilBwd ? reinterpret_cast<const void *>(&k_pencil<kPencilBwd>)
-                   : mode == kPencilFwd ? reinterpret_cast<const void *>(&k_pencil<kPencilFwd>)
-                                        : reinterpret_cast<const void *>(&k_pencil<kPencilMhd>);
-    if (lds > 64 * 1024)
-        HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (mode == kPencilBwd)
-        hipLaunchKernelGGL(k_pencil<kPencilBwd>, dim3(grid), dim3(256), lds, s, a);
-    else if (mode == kPencilFwd)
-        hipLaunchKernelGGL(k_pencil<kPencilFwd>, dim3(grid), dim3(256), lds, s, a);
-    else
-        hipLaunchKernelGGL(k_pencil<kPencilMhd>, dim3(grid), dim3(256), lds, s, a);
-    HIP_TRY(hipGetLastError());
-    return 0;
+    const size_t lds = pencil_lds(f, pb);
+    if (mode == kPencilBwd) return launch_nt<kPencilBwd>(f, a, lds, nt, s, err);
+    if (mode == kPencilFwd) return launch_nt<kPencilFwd>(f, a, lds, nt, s, err);
+    return launch_nt<kPencilMhd>(f, a, lds, nt, s, err);
 }
 
 int fft_backward_dev(const FftContext *f, int ncomp, const double *d_vr_rtm, double *d_fld_rtp,
@@ -443,7 +504,7 @@ int fft_forward_dev(const FftContext *f, int ncomp, const double *d_frc_rtp, dou
 
 bool fft_mhd_fused_fits(const FftContext *f)
 {
-    return pencil_lds(f, 19, 7) <= 160 * 1024;
+    return pencil_pairs(f, kPencilMhd, 13, 19, 256) > 0 || pencil_pairs(f, kPencilMhd, 13, 19, 512) > 0;
 }
 
 int fft_mhd_fused_dev(const FftContext *f, const double *d_bwd_rtm, double *d_fwd_rtm,
