@@ -314,14 +314,15 @@ __global__ void k_temp_bc(MhdDev p)
 // ------------------------------------------------------- rj <-> rlm -----
 __global__ void k_pack(MhdDev p)
 {
+    // one thread per rj node: the 13 field loads are contiguous over a
+    // wavefront, each node's 13 values land contiguous in rlm order
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
          i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J), jl = (int)(i % p.J);
-        const size_t src = (size_t)p.rj_of_rlm[jl] + (size_t)p.J * k;
-        double *o = p.sp_b + 13 * i;
+        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        double *o = p.sp_b + 13 * ((size_t)p.rlm_of_rj[j] + (size_t)p.J * k);
 #pragma unroll
-        for (int c = 0; c < 12; ++c) o[c] = FLD(kVelo + c)[src];
-        o[12] = FLD(kTemp)[src];
+        for (int c = 0; c < 12; ++c) o[c] = FLD(kVelo + c)[i];
+        o[12] = FLD(kTemp)[i];
     }
 }
 
@@ -869,6 +870,32 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
         HIP_TRY(hipMalloc(&c->d_piv[q], sizeof(int) * nnod));
         if (ludcmp_dev(nb, nr, J, d_a, c->d_lu[q], c->d_piv[q], d_ierr, s, err)) return -1;
         HIP_TRY(hipStreamSynchronize(s));
+        // the step solves with mode-fastest copies of the factors; its
+        // register window holds the (nb-1)/2 rows an interchange can reach
+        {
+            std::vector<int> piv((size_t)nnod);
+            HIP_TRY(hipMemcpy(piv.data(), c->d_piv[q], sizeof(int) * nnod, hipMemcpyDeviceToHost));
+            for (int j = 0; j < J; ++j)
+                for (int i = 1; i <= nr; ++i) {
+                    const int d = piv[(size_t)(i - 1) + (size_t)nr * j] - i;
+                    if (d < 0 || d > (nb - 1) / 2) {
+                        *err = "mhd_create: row interchange outside the band window";
+                        return -1;
+                    }
+                }
+        }
+        {
+            double *bt = nullptr;
+            int *pt = nullptr;
+            HIP_TRY(hipMalloc(&bt, sizeof(double) * (2 * nb - 1) * nnod));
+            HIP_TRY(hipMalloc(&pt, sizeof(int) * nnod));
+            if (lu_transpose_dev(nb, nr, J, c->d_lu[q], c->d_piv[q], bt, pt, s, err)) return -1;
+            HIP_TRY(hipStreamSynchronize(s));
+            (void)hipFree(c->d_lu[q]);
+            (void)hipFree(c->d_piv[q]);
+            c->d_lu[q] = bt;
+            c->d_piv[q] = pt;
+        }
         std::vector<int> ierr(J);
         HIP_TRY(hipMemcpy(ierr.data(), d_ierr, sizeof(int) * J, hipMemcpyDeviceToHost));
         for (int j = 0; j < J; ++j)
@@ -1076,11 +1103,11 @@ int mhd_steps(MhdContext *c, int nsteps, std::string *err)
         hipLaunchKernelGGL(k_cn_rhs, dim3(nb), dim3(256), 0, s, p);
         HIP_TRY(hipGetLastError());
         double *d = c->d_rj;
-        if (lubksb_dev(5, p.nr, (int)J, c->d_lu[0], c->d_piv[0], d + kVelo * p.nnod, s, err) ||
-            lubksb_dev(3, p.nr, (int)J, c->d_lu[1], c->d_piv[1], d + (kVelo + 2) * p.nnod, s, err) ||
-            lubksb_dev(3, p.nr, (int)J, c->d_lu[2], c->d_piv[2], d + kTemp * p.nnod, s, err) ||
-            lubksb_dev(3, p.nr, (int)J, c->d_lu[3], c->d_piv[3], d + kMagne * p.nnod, s, err) ||
-            lubksb_dev(3, p.nr, (int)J, c->d_lu[4], c->d_piv[4], d + (kMagne + 2) * p.nnod, s, err))
+        if (lubksb_t_dev(5, p.nr, (int)J, c->d_lu[0], c->d_piv[0], d + kVelo * p.nnod, s, err) ||
+            lubksb_t_dev(3, p.nr, (int)J, c->d_lu[1], c->d_piv[1], d + (kVelo + 2) * p.nnod, s, err) ||
+            lubksb_t_dev(3, p.nr, (int)J, c->d_lu[2], c->d_piv[2], d + kTemp * p.nnod, s, err) ||
+            lubksb_t_dev(3, p.nr, (int)J, c->d_lu[3], c->d_piv[3], d + kMagne * p.nnod, s, err) ||
+            lubksb_t_dev(3, p.nr, (int)J, c->d_lu[4], c->d_piv[4], d + (kMagne + 2) * p.nnod, s, err))
             return -1;
         if (derived_fields(c, err)) return -1;
         if (nonlinear(c, true, err)) return -1;

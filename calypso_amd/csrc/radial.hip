@@ -223,7 +223,167 @@ __global__ void k_lubksb_mul(int n, int mcomp, const double *band_lu, const int 
 #undef X
 }
 
+// The same substitution on factors stored mode-fastest: band_t[((i-1) W +
+// (k-1)) mcomp + m] = BL(k, i) of mode m and ipiv_t[(i-1) mcomp + m], so that
+// every load of a wavefront is one contiguous run (the reference layout
+// band_lu(W, n, mcomp) puts W n doubles between neighbouring modes).
+// The recurrences run in registers: the row interchanges (the reference
+// applies them all first) are applied in a window of the next H = (NB-1)/2
+// rows during the forward sweep, which is the same because interchange i
+// only touches rows i..i+H (checked on the host when the factors are made);
+// the loads of the next block of KB rows are issued before the current
+// block is computed.  Operation order per row is the reference's, so the
+// results are bitwise those of k_lubksb_mul.
+constexpr int kLuBlock = 8;
+
+template <int NB>
+__global__ __launch_bounds__(128) void k_lubksb_mul_t(int n, int mcomp, const double *__restrict__ band_t,
+                               const int *__restrict__ ipiv_t, double *__restrict__ x)
+{
+    constexpr int W = 2 * NB - 1, KL = NB - 1, H = (NB - 1) / 2, KB = kLuBlock;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mcomp) return;
+    const size_t mc = (size_t)mcomp;
+    auto BL = [&](int k, int i) { return band_t[((size_t)(i - 1) * W + (k - 1)) * mc + m]; };
+    auto XP = [&](int i) { return x + (size_t)m + mc * (i - 1); };
+
+    // ---- interchanges + forward substitution --------------------------
+    double win[H + 1];          // rows i..i+H after the interchanges before i
+#pragma unroll
+    for (int h = 0; h <= H; ++h) win[h] = (1 + h <= n) ? *XP(1 + h) : 0.0;
+    double yh[KL];              // y(i-1), y(i-2), ...
+#pragma unroll
+    for (int d = 0; d < KL; ++d) yh[d] = 0.0;
+    int pv[2][KB];
+    double lb[2][KB][KL], xr[2][KB];
+    auto load_f = [&](int buf, int i0) {
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            const int i = i0 + u;
+            pv[buf][u] = (i <= n) ? ipiv_t[(size_t)(i - 1) * mc + m] : i;
+            xr[buf][u] = (i + 1 + H <= n) ? *XP(i + 1 + H) : 0.0;
+#pragma unroll
+            for (int d = 1; d <= KL; ++d) lb[buf][u][d - 1] = (i <= n && i - d >= 1) ? BL(NB + d, i - d) : 0.0;
+        }
+    };
+    auto step_f = [&](int buf, int i0) {
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            const int i = i0 + u;
+            if (i > n) break;
+            const int dl = pv[buf][u] - i;   // 0..H
+            if (H >= 1 && dl == 1) { const double t = win[0]; win[0] = win[1]; win[1] = t; }
+            if (H >= 2 && dl == 2) { const double t = win[0]; win[0] = win[H]; win[H] = t; }
+            double y = win[0];
+#pragma unroll
+            for (int d = KL; d >= 1; --d)
+                if (d <= i - 1) y = y - lb[buf][u][d - 1] * yh[d - 1];
+            *XP(i) = y;
+#pragma unroll
+            for (int d = KL - 1; d >= 1; --d) yh[d] = yh[d - 1];
+            yh[0] = y;
+#pragma unroll
+            for (int h = 0; h < H; ++h) win[h] = win[h + 1];
+            win[H] = xr[buf][u];
+        }
+    };
+    // two blocks per iteration, so that both buffers are indexed statically
+    load_f(0, 1);
+    for (int i0 = 1; i0 <= n; i0 += 2 * KB) {
+        if (i0 + KB <= n) load_f(1, i0 + KB);
+        step_f(0, i0);
+        if (i0 + KB > n) break;
+        if (i0 + 2 * KB <= n) load_f(0, i0 + 2 * KB);
+        step_f(1, i0 + KB);
+    }
+
+    // ---- back substitution ----------------------------------------------
+    double zh[KL];              // z(i+1), z(i+2), ...
+#pragma unroll
+    for (int d = 0; d < KL; ++d) zh[d] = 0.0;
+    double ub[2][KB][KL], dg[2][KB], yb[2][KB];
+    auto load_b = [&](int buf, int i0) {   // rows i0, i0-1, ...
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            const int i = i0 - u;
+            yb[buf][u] = (i >= 1) ? *XP(i) : 0.0;
+            dg[buf][u] = (i >= 1) ? BL(NB, i) : 1.0;
+#pragma unroll
+            for (int d = 1; d <= KL; ++d) ub[buf][u][d - 1] = (i >= 1 && i + d <= n) ? BL(NB - d, i + d) : 0.0;
+        }
+    };
+    auto step_b = [&](int buf, int i0) {
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            const int i = i0 - u;
+            if (i < 1) break;
+            double z = yb[buf][u];
+#pragma unroll
+            for (int d = 1; d <= KL; ++d)
+                if (i + d <= n) z = z - ub[buf][u][d - 1] * zh[d - 1];
+            z = z / dg[buf][u];
+            *XP(i) = z;
+#pragma unroll
+            for (int d = KL - 1; d >= 1; --d) zh[d] = zh[d - 1];
+            zh[0] = z;
+        }
+    };
+    load_b(0, n);
+    for (int i0 = n; i0 >= 1; i0 -= 2 * KB) {
+        if (i0 - KB >= 1) load_b(1, i0 - KB);
+        step_b(0, i0);
+        if (i0 - KB < 1) break;
+        if (i0 - 2 * KB >= 1) load_b(0, i0 - 2 * KB);
+        step_b(1, i0 - KB);
+    }
+}
+
+// reference layout band_lu(W, n, mcomp), i_pivot(n, mcomp) -> mode-fastest
+__global__ void k_lu_transpose(int W, int n, int mcomp, const double *band_lu, const int *ipiv,
+                               double *band_t, int *ipiv_t)
+{
+    const size_t tot = (size_t)W * n * mcomp;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int m = (int)(e % mcomp);
+        const size_t r = e / mcomp;   // (i-1) W + (k-1)
+        band_t[e] = band_lu[r + (size_t)W * n * m];
+        if (r < (size_t)n) ipiv_t[e] = ipiv[r + (size_t)n * m];
+    }
+}
+
 // ----------------------------------------------------------------- host --
+
+int lu_transpose_dev(int nb, int n, int mcomp, const double *d_band_lu, const int *d_ipiv,
+                     double *d_band_t, int *d_ipiv_t, hipStream_t s, std::string *err)
+{
+    if (nb != 3 && nb != 5) { *err = "lu_transpose: nb must be 3 or 5"; return -1; }
+    const int W = 2 * nb - 1;
+    const size_t tot = (size_t)W * n * mcomp;
+    const int blocks = (int)std::min<size_t>((tot + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_lu_transpose, dim3(blocks), dim3(256), 0, s, W, n, mcomp, d_band_lu,
+                       d_ipiv, d_band_t, d_ipiv_t);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int lubksb_t_dev(int nb, int n, int mcomp, const double *d_band_t, const int *d_ipiv_t,
+                 double *d_x, hipStream_t s, std::string *err)
+{
+    if ((nb != 3 && nb != 5) || n < 5 || mcomp < 1) {
+        *err = "lubksb: nb must be 3 or 5, n >= 5";
+        return -1;
+    }
+    const int blocks = (mcomp + 127) / 128;
+    if (nb == 3)
+        hipLaunchKernelGGL(k_lubksb_mul_t<3>, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_t,
+                           d_ipiv_t, d_x);
+    else
+        hipLaunchKernelGGL(k_lubksb_mul_t<5>, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_t,
+                           d_ipiv_t, d_x);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 
 int ludcmp_dev(int nb, int n, int mcomp, const double *d_band_a, double *d_band_lu,
                int *d_ipiv, int *d_ierr, hipStream_t s, std::string *err)

@@ -10,6 +10,11 @@ namespace calypso {
 
 int ludcmp_dev(int nb, int n, int mcomp, const double *d_band_a, double *d_band_lu, int *d_ipiv,
                int *d_ierr, hipStream_t s, std::string *err);
+// mode-fastest copies of the factors for lubksb_t_dev (bitwise the same solve)
+int lu_transpose_dev(int nb, int n, int mcomp, const double *d_band_lu, const int *d_ipiv,
+                     double *d_band_t, int *d_ipiv_t, hipStream_t s, std::string *err);
+int lubksb_t_dev(int nb, int n, int mcomp, const double *d_band_t, const int *d_ipiv_t,
+                 double *d_x, hipStream_t s, std::string *err);
 int lubksb_dev(int nb, int n, int mcomp, const double *d_band_lu, const int *d_ipiv, double *d_x,
                hipStream_t s, std::string *err);
 
