@@ -64,6 +64,7 @@ struct FftArgs {
 constexpr int kFftMaxE = 12;   // complex values per thread per stage (any radix)
 constexpr int kFftMaxZ = 4;    // frequencies per thread per pair (inverse side)
 constexpr int kFftMaxU = 2;    // orders per thread per pair (forward side)
+constexpr int kFftPre = 16;    // next pencil's staged spectra per thread (prefetch)
 
 // One in-place Stockham stage of radix R over `nseq` complex sequences of
 // length N held in buf (sequence s at buf + s N), mixed-radix autosort form
@@ -159,26 +160,60 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
     double *bd = reinterpret_cast<double *>(buf);
     const int PB = a.pairs_per_batch;
 
-    const int pencil = blockIdx.x;            // t + nth*k (rtm pencil order)
-    const int t = pencil % a.nth, k = pencil / a.nth;
     const size_t sl = (size_t)a.nth * a.nr;   // rtm slot stride in pencils
     const size_t nnod = (size_t)a.nr * a.nth * N;
-    const size_t gbase = (size_t)k + (size_t)a.nr * t;  // rtp node at phi = 0
+    const int npencil = a.nth * a.nr;
+    const int nin = a.ncomp_in;
+    const int npair_in = (nin + 1) / 2;
 
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = {a.tw[2 * i], a.tw[2 * i + 1]};
 
+    // persistent work-groups: the first component group of the next pencil
+    // is loaded into registers while the current pencil is transformed
+    const int nc0 = min(2 * PB, nin);
+    double pre[kFftPre];
+    auto prefetch = [&](int pc) {
+        // the slot arithmetic is redone per call rather than held in
+        // registers across the pencil loop
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+#pragma unroll
+        for (int q = 0; q < kFftPre; ++q) {
+            const int e = tid + q * NT;
+            pre[q] = 0.0;
+            if (pc < npencil && e < nc0 * M) {
+                const int nd = e % nc0, mp = e / nc0;
+                pre[q] = a.in[nd + (size_t)nin * (pc + sl * mp)];
+            }
+        }
+    };
+    if (MODE != kPencilFwd) prefetch(blockIdx.x);
+
+    for (int pencil = blockIdx.x; pencil < npencil; pencil += gridDim.x) {
+    const int t = pencil % a.nth, k = pencil / a.nth;   // pencil = t + nth*k
+    const size_t gbase = (size_t)k + (size_t)a.nr * t;  // rtp node at phi = 0
+
     // ---- inverse side -----------------------------------------------
-    const int nin = a.ncomp_in;
-    const int npair_in = (nin + 1) / 2;
     if (MODE == kPencilBwd || MODE == kPencilMhd) {
         for (int p0 = 0; p0 < npair_in; p0 += PB) {
             const int np = min(PB, npair_in - p0);
             const int c0 = 2 * p0, nc = min(2 * np, nin - c0);
-            __syncthreads();   // buffer free (previous group written out)
+            __syncthreads();   // buffer free (previous group or pencil written out)
             // the group's spectra, nc contiguous doubles per order
-            for (int e = threadIdx.x; e < nc * M; e += NT) {
-                const int nd = e % nc, mp = e / nc;
-                bd[nd * M + mp] = a.in[c0 + nd + (size_t)nin * (pencil + sl * mp)];
+            if (p0 == 0) {
+                int tid = threadIdx.x;
+                asm volatile("" : "+v"(tid));
+#pragma unroll
+                for (int q = 0; q < kFftPre; ++q) {
+                    const int e = tid + q * NT;
+                    if (e < nc * M) bd[(e % nc) * M + e / nc] = pre[q];
+                }
+                prefetch(pencil + gridDim.x);
+            } else {
+                for (int e = threadIdx.x; e < nc * M; e += NT) {
+                    const int nd = e % nc, mp = e / nc;
+                    bd[nd * M + mp] = a.in[c0 + nd + (size_t)nin * (pencil + sl * mp)];
+                }
             }
             __syncthreads();
             // Hermitian spectra of z = x_a + i x_b, pair by pair from the last:
@@ -234,7 +269,7 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
             }
         }
     }
-    if (MODE == kPencilBwd) return;
+    if (MODE == kPencilBwd) continue;
 
     // ---- products (MHD, in place per longitude) ---------------------
     const int nout = a.ncomp_out;
@@ -344,6 +379,7 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
             a.out[oc0 + nd + (size_t)nout * (pencil + sl * mp)] = v;
         }
     }
+    }   // pencils
 }
 
 // pointwise products on the reference grid layout fld(nnod, 13) -> frc(nnod, 19)
@@ -394,6 +430,13 @@ int fft_context_build(FftContext *f, int L, int nth, int nr, int nphi, std::stri
     f->nr = nr;
     f->N = nphi;
     f->nstage = ns;
+    {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        f->n_cu = prop.multiProcessorCount;
+    }
     std::vector<double> tw(2 * (size_t)nphi);
     const double pi = 3.14159265358979323846;
     for (int i = 0; i < nphi; ++i) {
@@ -418,23 +461,23 @@ static size_t pencil_lds(const FftContext *f, int pb)
     return (size_t)(pb + 1) * f->N * 16;
 }
 
-// pairs per group: everything the register staging and the spectra staging
-// allow (the MHD mode needs its 7 input pairs in one group, and room for the
-// 19 output spectra)
+// pairs per group: everything the register staging, the prefetch of the
+// first group and the LDS allow (the MHD mode needs its 7 input pairs in one
+// group)
 static int pencil_pairs(const FftContext *f, int mode, int nin, int nout, int nt)
 {
     const int M = 2 * f->L + 1, N = f->N;
     const int need = (mode == kPencilMhd) ? 7 : (std::max(nin, nout) + 1) / 2;
-    int pb = need;
-
     if (N > kFftMaxZ * nt || f->L + 1 > kFftMaxU * nt) return 0;
     auto fits = [&](int p) {
         const int np = std::min(p, need);
         for (int st = 0; st < f->nstage; ++st)
             if (np * (N / f->radix[st]) > (kFftMaxE / f->radix[st]) * nt) return false;
+        if (mode != kPencilFwd && std::min(2 * p, nin) * M > kFftPre * nt) return false;
         return pencil_lds(f, p) <= 160 * 1024;
     };
-    if (mode == kPencilMhd) return fits(pb) ? pb : 0;
+    if (mode == kPencilMhd) return fits(need) ? need : 0;
+    int pb = need;
     while (pb > 1 && !fits(pb)) --pb;
     return fits(pb) ? pb : 0;
 }
@@ -443,7 +486,9 @@ template <int MODE>
 static int launch_nt(const FftContext *f, const FftArgs &a, size_t lds, int nt, hipStream_t s,
                      std::string *err)
 {
-    const int grid = f->nr * f->nth;
+    // persistent: one round of work-groups, as many per CU as LDS allows
+    const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / std::max<size_t>(lds, 1))));
+    const int grid = std::min(f->nr * f->nth, std::max(1, f->n_cu) * per_cu);
     const void *fn = nt == 512 ? reinterpret_cast<const void *>(&k_pencil<MODE, 512>)
                                : reinterpret_cast<const void *>(&k_pencil<MODE, 256>);
     if (lds > 64 * 1024)

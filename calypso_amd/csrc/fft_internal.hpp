@@ -15,6 +15,7 @@ struct FftContext {
     int nstage = 0;
     int radix[FFT_MAX_STAGES] = {0};
     double *d_tw = nullptr;   // N complex exp(-2 pi i t / N)
+    int n_cu = 256;           // compute units (persistent pencil grid)
 };
 
 int  fft_context_build(FftContext *f, int L, int nth, int nr, int nphi, std::string *err);
