@@ -16,7 +16,7 @@
 //                   const_sph_rotation.f90, cal_rot_buoyancies_sph_MHD.f90)
 //   k_pack          rj -> rlm, 13 components (u, w, B, J, T)
 //   Legendre bwd -> FFT, grid products, FFT (one pencil kernel) -> Legendre fwd
-//   k_unpack        rlm -> rj of the 19 forward components and the curl of the
+//   k_unpack        rlm -> rj of the forward components; k_coriolis the curl of the
 //                   Coriolis force in spectral space
 //                   (sum_coriolis_terms_rlm.f90:88-165)
 //   k_rot_forces    curl of the forces, curl of u x B, divergence of u T with
@@ -312,35 +312,93 @@ __global__ void k_temp_bc(MhdDev p)
 }
 
 // ------------------------------------------------------- rj <-> rlm -----
-__global__ void k_pack(MhdDev p)
+// rj <-> rlm transposes, tiled through LDS: a work-group takes 16 degrees x
+// 16 orders of one radial level.  In rj (j = l(l+1)+m) the orders of one
+// degree are contiguous; in rlm the degrees of one order are contiguous with
+// the node's components contiguous, so both sides move in contiguous runs.
+constexpr int kTile = 16;
+
+// tile origin (l0, m0) of block (blockIdx.x, blockIdx.y); false if the tile
+// holds no mode (l < |m| everywhere)
+__device__ inline bool tile_origin(int L, int *l0, int *m0)
 {
-    // one thread per rj node: the 13 field loads are contiguous over a
-    // wavefront, each node's 13 values land contiguous in rlm order
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J), j = (int)(i % p.J);
-        double *o = p.sp_b + 13 * ((size_t)p.rlm_of_rj[j] + (size_t)p.J * k);
+    *m0 = -L + (int)blockIdx.x * kTile;
+    *l0 = (int)blockIdx.y * kTile;
+    const int mlo = *m0, mhi = min(*m0 + kTile - 1, L);
+    const int minabs = (mlo <= 0 && mhi >= 0) ? 0 : min(abs(mlo), abs(mhi));
+    return minabs <= min(L, *l0 + kTile - 1);
+}
+
+__global__ __launch_bounds__(256) void k_pack(MhdDev p)
+{
+    __shared__ double tile[13][kTile][kTile + 1];
+    int l0, m0;
+    if (!tile_origin(p.L, &l0, &m0)) return;
+    const int k = blockIdx.z;
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    {
+        const int l = l0 + ty, m = m0 + tx;
+        const bool ok = l <= p.L && m <= p.L && abs(m) <= l;
+        const size_t i = (size_t)p.J * k + (ok ? l * (l + 1) + m : 0);
 #pragma unroll
-        for (int c = 0; c < 12; ++c) o[c] = FLD(kVelo + c)[i];
-        o[12] = FLD(kTemp)[i];
+        for (int c = 0; c < 12; ++c) tile[c][ty][tx] = ok ? FLD(kVelo + c)[i] : 0.0;
+        tile[12][ty][tx] = ok ? FLD(kTemp)[i] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kTile * kTile * 13; e += 256) {
+        const int col = e / (kTile * 13), r = e % (kTile * 13);
+        const int node = r / 13, c = r % 13;
+        const int l = l0 + node, m = m0 + col;
+        if (l > p.L || m > p.L || abs(m) > l) continue;
+        const size_t jl = (size_t)p.rlm_of_rj[l * (l + 1) + m];
+        p.sp_b[13 * (jl + (size_t)p.J * k) + c] = tile[c][node][col];
     }
 }
 
-// Unpack the forward transform into the rj fields and form the curl of the
-// Coriolis force from u and w (sum_rot_coriolis_rlm_10; omega = (r^2/2, r, 1)).
-__global__ void k_unpack(MhdDev p)
+// Unpack the forward transform into the rj fields: the 13 components the step
+// uses (advection 0-2, the poloidal Coriolis slot 7, Lorentz 9-11, induction
+// 12-14, heat flux 15-17) go through an LDS tile.
+__global__ __launch_bounds__(256) void k_unpack(MhdDev p)
+{
+    constexpr int NU = 13;
+    __shared__ double tile[NU][kTile][kTile + 1];
+    int l0, m0;
+    if (!tile_origin(p.L, &l0, &m0)) return;
+    const int k = blockIdx.z;
+    for (int e = threadIdx.x; e < kTile * kTile * NU; e += 256) {
+        const int col = e / (kTile * NU), r = e % (kTile * NU);
+        const int node = r / NU, u = r % NU;
+        const int c = u < 3 ? u : (u == 3 ? 7 : u + 5);   // 0-2, 7, 9-17
+        const int l = l0 + node, m = m0 + col;
+        double v = 0.0;
+        if (l <= p.L && m <= p.L && abs(m) <= l) {
+            const size_t jl = (size_t)p.rlm_of_rj[l * (l + 1) + m];
+            v = p.sp_f[19 * (jl + (size_t)p.J * k) + c];
+        }
+        tile[u][node][col] = v;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    const int l = l0 + ty, m = m0 + tx;
+    if (l > p.L || m > p.L || abs(m) > l) return;
+    const size_t i = (size_t)p.J * k + l * (l + 1) + m;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        FLD(kMAdvect + c)[i] = tile[c][ty][tx];
+        FLD(kLorentz + c)[i] = tile[4 + c][ty][tx];
+        FLD(kVpInduct + c)[i] = tile[7 + c][ty][tx];
+        FLD(kHFlux + c)[i] = tile[10 + c][ty][tx];
+    }
+    FLD(kRotCoriolis + 1)[i] = tile[3][ty][tx];
+}
+
+// Curl of the Coriolis force from u and w (sum_rot_coriolis_rlm_10;
+// omega = (r^2/2, r, 1)), one thread per rj node.
+__global__ void k_coriolis(MhdDev p)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
          i += (size_t)gridDim.x * blockDim.x) {
         const int k = (int)(i / p.J), j = (int)(i % p.J);
-        const double *s = p.sp_f + 19 * ((size_t)p.rlm_of_rj[j] + (size_t)p.J * k);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            FLD(kMAdvect + c)[i] = s[c];
-            FLD(kLorentz + c)[i] = s[9 + c];
-            FLD(kVpInduct + c)[i] = s[12 + c];
-            FLD(kHFlux + c)[i] = s[15 + c];
-        }
         const size_t base = (size_t)p.J * k;
         const size_t k1 = base + p.jk1[j], k2 = base + p.jk2[j], l1 = base + p.jl1[j];
         const double *us = FLD(kVelo), *ud = FLD(kVelo + 1);
@@ -363,7 +421,6 @@ __global__ void k_unpack(MhdDev p)
         pol = -p.coef_cor * pol * ar * ar;
         tor = -p.coef_cor * tor * ar * ar;
         FLD(kRotCoriolis)[i] = pol;
-        FLD(kRotCoriolis + 1)[i] = s[7];
         FLD(kRotCoriolis + 2)[i] = tor;
     }
 }
@@ -618,6 +675,7 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     p = MhdDev();
     p.nr = nr;
     p.J = J;
+    p.L = L;
     p.ki = ki;
     p.ko = ko;
     p.nnod = nnod;
@@ -1026,7 +1084,8 @@ int nonlinear(MhdContext *c, bool temp_roundtrip, std::string *err)
     MhdDev &p = c->p;
     hipStream_t s = c->stream;
     const int nb = blocks_for(p.nnod);
-    hipLaunchKernelGGL(k_pack, dim3(nb), dim3(256), 0, s, p);
+    const dim3 tiles((2 * c->L + 1 + kTile - 1) / kTile, (c->L + 1 + kTile - 1) / kTile, c->nr);
+    hipLaunchKernelGGL(k_pack, tiles, dim3(256), 0, s, p);
     HIP_TRY(hipGetLastError());
     if (leg_backward_dev(&c->leg, 13, 4, 1, c->d_sp_b, c->d_vr_b, s, err)) return -1;
     const double coef[4] = {c->co.coef_velo, c->co.coef_lor, c->co.coef_induct, c->co.coef_temp};
@@ -1040,7 +1099,9 @@ int nonlinear(MhdContext *c, bool temp_roundtrip, std::string *err)
             return -1;
     }
     if (leg_forward_dev(&c->leg, 19, 6, 1, c->d_vr_f, c->d_sp_f, s, err)) return -1;
-    hipLaunchKernelGGL(k_unpack, dim3(nb), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_unpack, tiles, dim3(256), 0, s, p);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_coriolis, dim3(nb), dim3(256), 0, s, p);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_rot_forces, dim3(nb), dim3(256), 0, s, p, temp_roundtrip ? 1 : 0);
     HIP_TRY(hipGetLastError());
