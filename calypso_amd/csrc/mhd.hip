@@ -980,6 +980,11 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
 
 void mhd_free(MhdContext *c)
 {
+    (void)hipStreamSynchronize(c->stream);   // replays of the step graph
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    if (c->gstream) (void)hipStreamDestroy(c->gstream);
+    c->gstream = nullptr;
     leg_context_free(&c->leg);
     fft_context_free(&c->fft);
     double *bufs[] = {c->d_rj, c->d_sp_b, c->d_sp_f, c->d_vr_b, c->d_vr_f, c->d_consts,
@@ -992,6 +997,8 @@ void mhd_free(MhdContext *c)
         if (c->d_piv[q]) (void)hipFree(c->d_piv[q]);
     }
 }
+
+static void drop_graph(MhdContext *c);
 
 int mhd_set_option(MhdContext *c, const std::string &name, long long value, std::string *err)
 {
@@ -1006,10 +1013,16 @@ int mhd_set_option(MhdContext *c, const std::string &name, long long value, std:
             HIP_TRY(hipMalloc(&c->d_grid_f, sizeof(double) * 19 * nrtp));
         }
         c->fused_fft = value != 0;
+        drop_graph(c);
         return 0;
     }
     if (name == "legendre_autotune") {
         c->leg.autotune = value != 0;
+        return 0;
+    }
+    if (name == "graph") {
+        c->use_graph = value != 0;
+        drop_graph(c);
         return 0;
     }
     *err = "unknown option " + name;
@@ -1147,31 +1160,84 @@ int mhd_initialize(MhdContext *c, std::string *err)
     return 0;
 }
 
+// One time step (SPH_analyze_MHD without output): explicit terms, CN solves,
+// derived fields, nonlinear terms; `first` selects the Euler start.
+static int step_body(MhdContext *c, int first, std::string *err)
+{
+    MhdDev &p = c->p;
+    hipStream_t s = c->stream;
+    const int nb = blocks_for(p.nnod);
+    const size_t J = p.J;
+    hipLaunchKernelGGL(k_explicit, dim3(nb), dim3(256), 0, s, p, first, first);
+    hipLaunchKernelGGL(k_cn_rhs, dim3(nb), dim3(256), 0, s, p);
+    HIP_TRY(hipGetLastError());
+    double *d = c->d_rj;
+    // the four 3-band systems (u_tor, T, B_pol, B_tor) are independent: one launch
+    const double *b3[4] = {c->d_lu[1], c->d_lu[2], c->d_lu[3], c->d_lu[4]};
+    const int *p3[4] = {c->d_piv[1], c->d_piv[2], c->d_piv[3], c->d_piv[4]};
+    double *x3[4] = {d + (kVelo + 2) * p.nnod, d + kTemp * p.nnod, d + kMagne * p.nnod,
+                     d + (kMagne + 2) * p.nnod};
+    if (lubksb_t_dev(5, p.nr, (int)J, c->d_lu[0], c->d_piv[0], d + kVelo * p.nnod, s, err) ||
+        lubksb_t_batch_dev(3, p.nr, (int)J, 4, b3, p3, x3, s, err))
+        return -1;
+    if (derived_fields(c, err)) return -1;
+    if (nonlinear(c, true, err)) return -1;
+    return 0;
+}
+
+// Capture one Adams-Bashforth step into a HIP graph on the library's own
+// stream (the caller's may be the legacy null stream, which cannot be
+// captured); later steps replay it on the caller's stream.  The step holds
+// kernel launches only: no memsets, copies or host synchronisation, and the
+// Legendre plans already exist (made during the initialisation).
+static int capture_step(MhdContext *c, std::string *err)
+{
+    if (!c->gstream) HIP_TRY(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+    hipStream_t user = c->stream;
+    c->stream = c->gstream;
+    HIP_TRY(hipStreamBeginCapture(c->gstream, hipStreamCaptureModeRelaxed));
+    std::string e2;
+    const int rc = step_body(c, 0, &e2);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(c->gstream, &g);
+    c->stream = user;
+    if (rc || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        *err = rc ? "capture of the step failed: " + e2
+                  : std::string("hipStreamEndCapture failed: ") + hipGetErrorString(ec);
+        return -1;
+    }
+    const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->gexec = nullptr;
+        *err = std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ei);
+        return -1;
+    }
+    return 0;
+}
+
+static void drop_graph(MhdContext *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+}
+
 int mhd_steps(MhdContext *c, int nsteps, std::string *err)
 {
     if (!c->initialized) {
         *err = "mhd_steps: call the initialisation first";
         return -1;
     }
-    MhdDev &p = c->p;
-    hipStream_t s = c->stream;
-    const int nb = blocks_for(p.nnod);
-    const size_t J = p.J;
     for (int it = 0; it < nsteps; ++it) {
         c->i_step += 1;
         const int first = c->i_step == 1;
-        hipLaunchKernelGGL(k_explicit, dim3(nb), dim3(256), 0, s, p, first, first);
-        hipLaunchKernelGGL(k_cn_rhs, dim3(nb), dim3(256), 0, s, p);
-        HIP_TRY(hipGetLastError());
-        double *d = c->d_rj;
-        if (lubksb_t_dev(5, p.nr, (int)J, c->d_lu[0], c->d_piv[0], d + kVelo * p.nnod, s, err) ||
-            lubksb_t_dev(3, p.nr, (int)J, c->d_lu[1], c->d_piv[1], d + (kVelo + 2) * p.nnod, s, err) ||
-            lubksb_t_dev(3, p.nr, (int)J, c->d_lu[2], c->d_piv[2], d + kTemp * p.nnod, s, err) ||
-            lubksb_t_dev(3, p.nr, (int)J, c->d_lu[3], c->d_piv[3], d + kMagne * p.nnod, s, err) ||
-            lubksb_t_dev(3, p.nr, (int)J, c->d_lu[4], c->d_piv[4], d + (kMagne + 2) * p.nnod, s, err))
-            return -1;
-        if (derived_fields(c, err)) return -1;
-        if (nonlinear(c, true, err)) return -1;
+        if (c->use_graph && !first) {
+            if (!c->gexec && capture_step(c, err)) return -1;
+            HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            if (step_body(c, first, err)) return -1;
+        }
     }
     return 0;
 }

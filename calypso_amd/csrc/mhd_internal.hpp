@@ -70,6 +70,9 @@ struct MhdContext {
     double *d_rj = nullptr, *d_sp_b = nullptr, *d_sp_f = nullptr;
     double *d_vr_b = nullptr, *d_vr_f = nullptr;
     bool fused_fft = true;        // one pencil kernel: FFT, products, FFT
+    bool use_graph = true;        // replay the captured Adams-Bashforth step
+    hipStream_t gstream = nullptr;   // capture stream (default priority)
+    hipGraphExec_t gexec = nullptr;
     double *d_grid_b = nullptr, *d_grid_f = nullptr;   // grids when not fused
     double *d_consts = nullptr;   // r, ar1, ar2, d1, d2, g1, g3, t_icb, t_cmb, reft0, reft1, cor
     int *d_iconsts = nullptr;     // jk1, jk2, jl1, rj_of_rlm, rlm_of_rj, l_of_j

@@ -235,14 +235,15 @@ __global__ void k_lubksb_mul(int n, int mcomp, const double *band_lu, const int 
 // block is computed.  Operation order per row is the reference's, so the
 // results are bitwise those of k_lubksb_mul.
 constexpr int kLuBlock = 8;
+constexpr int kLuBatchMax = 8;
 
 template <int NB>
-__global__ __launch_bounds__(128) void k_lubksb_mul_t(int n, int mcomp, const double *__restrict__ band_t,
-                               const int *__restrict__ ipiv_t, double *__restrict__ x)
+__device__ __forceinline__ void lubksb_t_mode(int n, int mcomp, int m,
+                                              const double *__restrict__ band_t,
+                                              const int *__restrict__ ipiv_t,
+                                              double *__restrict__ x)
 {
     constexpr int W = 2 * NB - 1, KL = NB - 1, H = (NB - 1) / 2, KB = kLuBlock;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= mcomp) return;
     const size_t mc = (size_t)mcomp;
     auto BL = [&](int k, int i) { return band_t[((size_t)(i - 1) * W + (k - 1)) * mc + m]; };
     auto XP = [&](int i) { return x + (size_t)m + mc * (i - 1); };
@@ -338,6 +339,32 @@ __global__ __launch_bounds__(128) void k_lubksb_mul_t(int n, int mcomp, const do
     }
 }
 
+template <int NB>
+__global__ __launch_bounds__(128) void k_lubksb_mul_t(int n, int mcomp, const double *band_t,
+                                                      const int *ipiv_t, double *x)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mcomp) return;
+    lubksb_t_mode<NB>(n, mcomp, m, band_t, ipiv_t, x);
+}
+
+// several independent systems of the same band width in one launch
+// (blockIdx.y selects the system)
+struct LuBatch {
+    const double *band[kLuBatchMax];
+    const int *ipiv[kLuBatchMax];
+    double *x[kLuBatchMax];
+};
+
+template <int NB>
+__global__ __launch_bounds__(128) void k_lubksb_mul_t_batch(int n, int mcomp, LuBatch b)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= mcomp) return;
+    const int q = blockIdx.y;
+    lubksb_t_mode<NB>(n, mcomp, m, b.band[q], b.ipiv[q], b.x[q]);
+}
+
 // reference layout band_lu(W, n, mcomp), i_pivot(n, mcomp) -> mode-fastest
 __global__ void k_lu_transpose(int W, int n, int mcomp, const double *band_lu, const int *ipiv,
                                double *band_t, int *ipiv_t)
@@ -381,6 +408,29 @@ int lubksb_t_dev(int nb, int n, int mcomp, const double *d_band_t, const int *d_
     else
         hipLaunchKernelGGL(k_lubksb_mul_t<5>, dim3(blocks), dim3(128), 0, s, n, mcomp, d_band_t,
                            d_ipiv_t, d_x);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int lubksb_t_batch_dev(int nb, int n, int mcomp, int count, const double *const *d_band_t,
+                       const int *const *d_ipiv_t, double *const *d_x, hipStream_t s,
+                       std::string *err)
+{
+    if ((nb != 3 && nb != 5) || n < 5 || mcomp < 1 || count < 1 || count > kLuBatchMax) {
+        *err = "lubksb batch: nb must be 3 or 5, n >= 5, 1..8 systems";
+        return -1;
+    }
+    LuBatch b = {};
+    for (int q = 0; q < count; ++q) {
+        b.band[q] = d_band_t[q];
+        b.ipiv[q] = d_ipiv_t[q];
+        b.x[q] = d_x[q];
+    }
+    const dim3 grid((mcomp + 127) / 128, count);
+    if (nb == 3)
+        hipLaunchKernelGGL(k_lubksb_mul_t_batch<3>, grid, dim3(128), 0, s, n, mcomp, b);
+    else
+        hipLaunchKernelGGL(k_lubksb_mul_t_batch<5>, grid, dim3(128), 0, s, n, mcomp, b);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -15,6 +15,10 @@ int lu_transpose_dev(int nb, int n, int mcomp, const double *d_band_lu, const in
                      double *d_band_t, int *d_ipiv_t, hipStream_t s, std::string *err);
 int lubksb_t_dev(int nb, int n, int mcomp, const double *d_band_t, const int *d_ipiv_t,
                  double *d_x, hipStream_t s, std::string *err);
+// up to 8 independent systems of one band width in one launch
+int lubksb_t_batch_dev(int nb, int n, int mcomp, int count, const double *const *d_band_t,
+                       const int *const *d_ipiv_t, double *const *d_x, hipStream_t s,
+                       std::string *err);
 int lubksb_dev(int nb, int n, int mcomp, const double *d_band_lu, const int *d_ipiv, double *d_x,
                hipStream_t s, std::string *err);
 

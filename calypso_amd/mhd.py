@@ -131,7 +131,8 @@ class MhdSimulation:
         _lib.check(_lib.load().calypso_mhd_step(self._h, nsteps), "calypso_mhd_step")
 
     def set_option(self, name: str, value: int):
-        """"fused_fft" (0/1) or "legendre_autotune" (0/1)."""
+        """"fused_fft" (0/1), "legendre_autotune" (0/1) or "graph" (0/1: replay
+        the captured Adams-Bashforth step as a HIP graph; on by default)."""
         _lib.check(_lib.load().calypso_mhd_set_option(self._h, name.encode(), int(value)),
                    "calypso_mhd_set_option")
 
