@@ -8,7 +8,10 @@ output is compared with the FP64 GPU path, and the FP64 path with the
 reference's own org-loop transforms (oracle/_ref, compiled from the reference
 sources) on a bounded sample of radial levels, timed on the host cores as the
 CPU baseline.  The full MHD step (Dynamobench case 1 physics) is timed as
-well at three sizes and reported under "mhd_step".
+well at three sizes ("mhd_step"), at L=127 with each precision of the
+Legendre transforms and the energy difference to the FP64 step
+("mhd_step_L127_precisions"), and the large shell Lmax=511, nth=768, Nr=128
+with 8 components is transformed in each precision ("legendre_L511").
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes ...]
 Prints one JSON line.
@@ -180,6 +183,90 @@ def mhd_bench(torch):
     return out
 
 
+def mhd_precision_bench(torch, steps=200):
+    """The L=127, Nr=65 step with each precision of the Legendre transforms
+    (FP64 radial solves, P_l^m recursion and diagnostics in every mode):
+    ms/step and the energies after `steps` steps against the FP64 step."""
+    from calypso_amd import mhd
+    out, ref = {}, None
+    for mode in ("hip_fp64", "hip_fp32", "hip_bf16x3"):
+        p, r = mhd.dynamobench_case1(l_truncation=127, nth=192, nphi=384, nri=64,
+                                     legendre_mode=mode)
+        sim = mhd.MhdSimulation(p, r, stream=torch.cuda.current_stream().cuda_stream)
+        sim.set_dynamobench_ic(400)
+        sim.initialize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        e = sim.energies()
+        sim.close()
+        if ref is None:
+            ref = e
+        out[mode] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms, "energies": [float(x) for x in e],
+                     "max_rel_energy_diff_vs_fp64": float(np.max(np.abs(e - ref) / np.abs(ref)))}
+        print(f"# mhd L127 {mode}: {ms:.3f} ms/step, energy diff "
+              f"{out[mode]['max_rel_energy_diff_vs_fp64']:.2e}", file=sys.stderr, flush=True)
+    return {"Lmax": 127, "nth": 192, "nphi": 384, "Nr": 65, "steps": steps, "dt": 5e-5,
+            "graph_replayed": True, "modes": out}
+
+
+def large_legendre_bench(torch, steps=3):
+    """Large shell: Lmax=511, nth=768, Nr=128, 8 components (2 vectors + 2
+    scalars), one bwd+fwd pair per step in each precision: time, per-field
+    error against the FP64 path and the device footprint."""
+    import calypso_amd
+    L5, nth5, nr5, nv5, ns5 = 511, 768, 128, 2, 2
+    nc5 = 3 * nv5 + ns5
+    radius = np.linspace(7.0 / 13.0, 20.0 / 13.0, nr5)
+    J5 = (L5 + 1) ** 2
+    l = torch.from_numpy(np.concatenate([np.arange(abs(m), L5 + 1) for m in range(-L5, L5 + 1)])
+                         .astype(np.float64)).cuda()
+    gen = torch.Generator(device="cuda").manual_seed(511)
+    sp = (torch.randn((nr5, J5, nc5), generator=gen, dtype=torch.float64, device="cuda")
+          * ((l + 1.0) ** -1.5)[None, :, None]).reshape(-1).contiguous()
+    out, ref = {}, None
+    for mode in ("hip_fp64", "hip_fp32", "hip_bf16x3"):
+        lt = calypso_amd.LegendreTransform(L5, nth5, radius, nc5, nv5, ns5, mode)
+        vr = torch.empty(lt.n_rtm(), dtype=torch.float64, device="cuda")
+        sp2 = torch.empty_like(sp)
+        lt.backward(sp, vr)
+        lt.forward(vr, sp2)
+        torch.cuda.synchronize()
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        bw = fw = 0.0
+        for _ in range(steps):
+            e0.record()
+            lt.backward(sp, vr)
+            e1.record()
+            lt.forward(vr, sp2)
+            e2.record()
+            torch.cuda.synchronize()
+            bw += e0.elapsed_time(e1)
+            fw += e1.elapsed_time(e2)
+        info = lt.info()
+        lt.close()
+        r = {"bwd_ms": bw / steps, "fwd_ms": fw / steps,
+             "legendre_tflops": 2 * legendre_flops(L5, nth5, nr5, nv5, ns5)
+             / ((bw + fw) / steps * 1e-3) / 1e12,
+             # HBM footprint: P_l^m tables of the mode + spectra in, grid
+             # values, spectra out (FP64)
+             "table_bytes": info["table_bytes"],
+             "field_bytes": int(8 * (2 * sp.numel() + vr.numel()))}
+        if ref is None:
+            ref = (vr.clone(), sp2.clone())
+        else:
+            r["err_vs_fp64_bwd"] = field_err(vr, ref[0], nc5)
+            r["err_vs_fp64_fwd"] = field_err(sp2, ref[1], nc5)
+        out[mode] = r
+        del vr, sp2
+        print(f"# L511 {mode}: bwd {r['bwd_ms']:.2f} ms fwd {r['fwd_ms']:.2f} ms", file=sys.stderr,
+              flush=True)
+    return {"Lmax": L5, "nth": nth5, "Nr": nr5, "components": nc5, "vectors": nv5,
+            "scalars": ns5, "modes": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +277,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mhd", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the Lmax=511 transforms")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (reference for the byte counters)")
     args = ap.parse_args()
@@ -259,6 +347,9 @@ def main():
         torch.cuda.synchronize()
     if not args.no_mhd:
         line["mhd_step"] = mhd_bench(torch)
+        line["mhd_step_L127_precisions"] = mhd_precision_bench(torch)
+    if not args.no_large:
+        line["legendre_L511"] = large_legendre_bench(torch)
     if not args.no_cpu:
         vr_host = ref_vr.cpu().numpy() if ref_vr is not None else res[modes[0]]["vr"].cpu().numpy()
         line["cpu_baseline"] = cpu_baseline(calypso_amd, radius, sp.cpu().numpy(), vr_host,
