@@ -10,8 +10,9 @@ sources) on a bounded sample of radial levels, timed on the host cores as the
 CPU baseline.  The full MHD step (Dynamobench case 1 physics) is timed as
 well at three sizes ("mhd_step"), at L=127 with each precision of the
 Legendre transforms and the energy difference to the FP64 step
-("mhd_step_L127_precisions"), and the large shell Lmax=511, nth=768, Nr=128
-with 8 components is transformed in each precision ("legendre_L511").
+("mhd_step_L127_precisions"), an ensemble of 64 uneven shells is stepped
+together ("ensemble"), and the large shell Lmax=511, nth=768, Nr=128 with 8
+components is transformed in each precision ("legendre_L511").
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--modes ...]
 Prints one JSON line.
@@ -267,6 +268,58 @@ def large_legendre_bench(torch, steps=3):
             "scalars": ns5, "modes": out}
 
 
+def ensemble_bench(torch, nshell=64, steps=20, nstream=4):
+    """Ensemble of independent shells, Lmax in {31,47,63,95}, Nr in
+    {32,48,64,96}, nth = 1.5(L+1), nphi = 2 nth, seeded Ra in [50, 500]: each
+    shell's step is a captured graph; the shells are dealt round-robin over
+    `nstream` streams (default priority) and stepped together.  Reported:
+    shell-steps per second of the ensemble and of the same shells stepped one
+    after another."""
+    from calypso_amd import mhd
+    rng = np.random.default_rng(64)
+    Ls, Nrs = (31, 47, 63, 95), (32, 48, 64, 96)
+    streams = [torch.cuda.Stream() for _ in range(nstream)]
+    sims, shells = [], []
+    for q in range(nshell):
+        L_, nr_ = Ls[q % 4], Nrs[(q // 4) % 4]
+        nth = 3 * (L_ + 1) // 2
+        ra = float(rng.uniform(50.0, 500.0))
+        p, r = mhd.dynamobench_case1(l_truncation=L_, nth=nth, nphi=2 * nth, nri=nr_ - 1)
+        p.coef_buo = (1.0 - 0.35) * ra / 1.0e-3   # (1 - r_i/r_o) Ra / E, as dynamobench_case1
+        sim = mhd.MhdSimulation(p, r, stream=streams[q % nstream].cuda_stream)
+        sim.set_dynamobench_ic(400)
+        sim.initialize()
+        sim.step(2)   # Euler start, then the captured Adams-Bashforth step
+        sims.append(sim)
+        shells.append({"Lmax": L_, "Nr": nr_, "nth": nth, "nphi": 2 * nth, "Ra": ra})
+    torch.cuda.synchronize()
+    # one after another: each shell alone for `steps` steps
+    t_seq = 0.0
+    for sim in sims:
+        t0 = time.perf_counter()
+        sim.step(steps)
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t0
+    # together
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sim in sims:
+            sim.step(1)
+    torch.cuda.synchronize()
+    t_ens = time.perf_counter() - t0
+    finite = all(bool(np.all(np.isfinite(sim.energies()))) for sim in sims)
+    for sim in sims:
+        sim.close()
+    res = {"shells": nshell, "streams": nstream, "steps": steps,
+           "shell_steps_per_s_ensemble": nshell * steps / t_ens,
+           "shell_steps_per_s_sequential": nshell * steps / t_seq,
+           "speedup": t_seq / t_ens, "energies_finite": finite,
+           "sizes": sorted({(d["Lmax"], d["Nr"]) for d in shells})}
+    print(f"# ensemble {nshell} shells: {res['shell_steps_per_s_ensemble']:.0f} shell-steps/s "
+          f"(sequential {res['shell_steps_per_s_sequential']:.0f})", file=sys.stderr, flush=True)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,6 +401,7 @@ def main():
     if not args.no_mhd:
         line["mhd_step"] = mhd_bench(torch)
         line["mhd_step_L127_precisions"] = mhd_precision_bench(torch)
+        line["ensemble"] = ensemble_bench(torch)
     if not args.no_large:
         line["legendre_L511"] = large_legendre_bench(torch)
     if not args.no_cpu:
