@@ -33,7 +33,11 @@ sys.path.insert(0, ROOT)
 L, NTH, NR, NV, NS = 255, 384, 96, 4, 0
 NCOMP = 3 * NV + NS
 ERR_BOUND = 1e-11
-PEAK = {"hip_fp64": 78.6, "hip_fp32": 157.3, "hip_bf16x3": 2500.0}   # TFLOP/s dense
+PEAK = {"hip_fp64": 78.6, "hip_fp32": 157.3, "hip_bf16x3": 2500.0,
+        "hip_int8x6": 5000.0}   # TFLOP/s (TOP/s for int8), dense
+# MFMA operations per algorithmic FP64 operation: six BF16 slice products;
+# 28 int8 digit-pair products (seven digits per operand, pairs i+j <= 6)
+OPS_PER_FLOP = {"hip_fp64": 1, "hip_fp32": 1, "hip_bf16x3": 6, "hip_int8x6": 28}
 HBM_PEAK = 8000.0   # GB/s
 
 
@@ -42,6 +46,9 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "legendre_L255_nth384_nr96_c12_tra
 
 def kernel_name(direction, mode, plan):
     """Template instance that runs a plan (as rocprofv3 names it)."""
+    if mode == "hip_int8x6":
+        return (f"k_leg_{direction}z<6, {plan['rows_per_wave']}, {plan['mct1']}, "
+                f"{plan['mct2']}>")
     t = {"hip_fp64": "TF64", "hip_fp32": "TF32", "hip_bf16x3": "TBF3"}[mode]
     form = "3" if plan["kernel_form"] == 3 else ""
     return (f"k_leg_{direction}{form}<{t}, {plan['rows_per_wave']}, {plan['mct1']}, "
@@ -190,7 +197,7 @@ def mhd_precision_bench(torch, steps=200):
     ms/step and the energies after `steps` steps against the FP64 step."""
     from calypso_amd import mhd
     out, ref = {}, None
-    for mode in ("hip_fp64", "hip_fp32", "hip_bf16x3"):
+    for mode in ("hip_fp64", "hip_fp32", "hip_bf16x3", "hip_int8x6"):
         p, r = mhd.dynamobench_case1(l_truncation=127, nth=192, nphi=384, nri=64,
                                      legendre_mode=mode)
         sim = mhd.MhdSimulation(p, r, stream=torch.cuda.current_stream().cuda_stream)
@@ -228,7 +235,7 @@ def large_legendre_bench(torch, steps=3):
     sp = (torch.randn((nr5, J5, nc5), generator=gen, dtype=torch.float64, device="cuda")
           * ((l + 1.0) ** -1.5)[None, :, None]).reshape(-1).contiguous()
     out, ref = {}, None
-    for mode in ("hip_fp64", "hip_fp32", "hip_bf16x3"):
+    for mode in ("hip_fp64", "hip_fp32", "hip_bf16x3", "hip_int8x6"):
         lt = calypso_amd.LegendreTransform(L5, nth5, radius, nc5, nv5, ns5, mode)
         vr = torch.empty(lt.n_rtm(), dtype=torch.float64, device="cuda")
         sp2 = torch.empty_like(sp)
@@ -325,7 +332,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--modes", default="hip_fp64,hip_fp32,hip_bf16x3")
+    ap.add_argument("--modes", default="hip_fp64,hip_fp32,hip_bf16x3,hip_int8x6")
     ap.add_argument("--cpu-sample", type=int, default=96)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -374,14 +381,15 @@ def main():
     h = out_modes[head]
     dom = "bwd" if h["bwd_ms"] >= h["fwd_ms"] else "fwd"
     dom_ms = h[f"{dom}_ms"]
-    achieved = flops / (dom_ms * 1e-3) / 1e12
+    achieved = flops * OPS_PER_FLOP[head] / (dom_ms * 1e-3) / 1e12
     dom_kernel = kernel_name(dom, head, h[f"plan_{dom}"])
     line = {
         "metric": "steps/s + Legendre TFLOP/s, mixed-prec vs FP64 GPU, Lmax=255 Nr=96, err<=1e-11",
         "value": h["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": {"hip_fp64": "f64", "hip_fp32": "f32", "hip_bf16x3": "bf16x3"}[head],
+        "dtype": {"hip_fp64": "f64", "hip_fp32": "f32", "hip_bf16x3": "bf16x3",
+                  "hip_int8x6": "int8x6 (FP64-exact sums)"}[head],
         "data": "synthetic (seeded Gaussian spectra, amplitude (l+1)^-1.5)",
         "config": {"workload": "legendre bwd+fwd pair", "Lmax": L, "nth": NTH, "Nr": NR,
                    "components": NCOMP, "vectors": NV, "scalars": NS, "headline_mode": head,
@@ -389,7 +397,8 @@ def main():
         "legendre_tflops": h["legendre_tflops"],
         "modes": out_modes,
         "roofline": {"kernel": dom_kernel, "bound": "mfma", "achieved": achieved,
-                     "peak": PEAK[head], "unit": "TFLOP/s", "frac": achieved / PEAK[head],
+                     "peak": PEAK[head], "unit": "TOP/s" if head == "hip_int8x6" else "TFLOP/s",
+                     "frac": achieved / PEAK[head],
                      "traffic": measured_traffic(dom_kernel)},
     }
     if args.copy_calibration:

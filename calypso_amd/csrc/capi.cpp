@@ -94,6 +94,7 @@ int calypso_legendre_mode_id(const char *name)
     if (s == "hip_fp64") return CALYPSO_LEG_FP64;
     if (s == "hip_fp32") return CALYPSO_LEG_FP32;
     if (s == "hip_bf16x3") return CALYPSO_LEG_BF16X3;
+    if (s == "hip_int8x6") return CALYPSO_LEG_INT8X6;
     return -1;
 }
 
@@ -427,7 +428,7 @@ int calypso_mhd_create(const calypso_mhd_params *pp, const double *radius, void 
     co.high_temp = pp->high_temp;
     co.low_temp = pp->low_temp;
     if (co.dt <= 0.0) return fail("calypso_mhd_create: dt must be positive");
-    if (pp->legendre_mode < 1 || pp->legendre_mode > 3)
+    if (pp->legendre_mode < CALYPSO_LEG_FP64 || pp->legendre_mode > CALYPSO_LEG_INT8X6)
         return fail("calypso_mhd_create: unknown Legendre mode");
     calypso_mhd *h = new (std::nothrow) calypso_mhd;
     if (!h) return fail("calypso_mhd_create: out of host memory");

@@ -12,7 +12,7 @@
 
 namespace calypso {
 
-enum Precision : int { kF64 = 1, kF32 = 2, kBF16x3 = 3 };
+enum Precision : int { kF64 = 1, kF32 = 2, kBF16x3 = 3, kInt8x6 = 4 };
 
 // One (order m, radial tile, row group) unit of Legendre work; the third
 // kernel form runs nkt consecutive radial tiles from ktile on.

@@ -24,6 +24,7 @@
       integer(c_int), parameter :: iflag_leg_hip_fp64 =   101
       integer(c_int), parameter :: iflag_leg_hip_fp32 =   102
       integer(c_int), parameter :: iflag_leg_hip_bf16x3 = 103
+      integer(c_int), parameter :: iflag_leg_hip_int8x6 = 104
 !
       private :: check_status
 !
@@ -47,7 +48,7 @@
       logical function is_legendre_trans_hip(id_leg)
       integer(c_int), intent(in) :: id_leg
       is_legendre_trans_hip = (id_leg .ge. iflag_leg_hip_fp64           &
-     &                   .and. id_leg .le. iflag_leg_hip_bf16x3)
+     &                   .and. id_leg .le. iflag_leg_hip_int8x6)
       end function is_legendre_trans_hip
 !
 ! -----------------------------------------------------------------------

@@ -26,7 +26,7 @@ import numpy as np
 
 from . import _lib
 
-LEG_MODES = {"hip_fp64": 1, "hip_fp32": 2, "hip_bf16x3": 3}
+LEG_MODES = {"hip_fp64": 1, "hip_fp32": 2, "hip_bf16x3": 3, "hip_int8x6": 4}
 _P = ctypes.c_void_p
 
 

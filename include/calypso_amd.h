@@ -24,6 +24,9 @@ extern "C" {
 #define CALYPSO_LEG_FP64    1  /* FP64 MFMA 16x16x4                          */
 #define CALYPSO_LEG_FP32    2  /* FP32 MFMA 16x16x4                          */
 #define CALYPSO_LEG_BF16X3  3  /* split-BF16x3 MFMA 16x16x32, FP32 accumulate */
+#define CALYPSO_LEG_INT8X6  4  /* six int8 slices per operand (Ozaki splitting),
+                                  MFMA i32 16x16x64, exact int32 sums, FP64
+                                  accumulation: FP64-level (~1e-12) results   */
 
 typedef struct calypso_leg calypso_leg_t;
 
@@ -32,7 +35,7 @@ const char *calypso_last_error(void);
 
 /* Mode string -> id, the GPU counterpart of set_legendre_trans_mode_ctl
  * (legendre_transform_select.f90:181-224).  Accepts "hip_fp64", "hip_fp32",
- * "hip_bf16x3" (case-insensitive); returns -1 for anything else. */
+ * "hip_bf16x3", "hip_int8x6" (case-insensitive); returns -1 for anything else. */
 int calypso_legendre_mode_id(const char *name);
 
 /* Gauss-Legendre points/weights/colatitudes of construct_gauss_coefs and

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "legendre_ref.npz")
 
 FP64_TOL = 1e-12
-MIXED_TOL = {"hip_fp32": 2e-5, "hip_bf16x3": 2e-5}
+MIXED_TOL = {"hip_fp32": 2e-5, "hip_bf16x3": 2e-5, "hip_int8x6": 1e-11}
 
 
 def field_err(got, ref, ncomp):
@@ -85,7 +85,7 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("mode", ["hip_fp64", "hip_fp32", "hip_bf16x3"])
+@pytest.mark.parametrize("mode", ["hip_fp64", "hip_fp32", "hip_bf16x3", "hip_int8x6"])
 @pytest.mark.parametrize("L,nth,nr,nv,ns", CASES)
 def test_vs_oracle(torch_cuda, mode, L, nth, nr, nv, ns):
     nc = 3 * nv + ns
@@ -198,7 +198,7 @@ PLAN_CASES = [(31, 48, 9, 4, 1), (31, 48, 5, 6, 1), (47, 72, 7, 4, 0), (64, 96, 
               (7, 12, 3, 3, 1), (10, 17, 2, 1, 2)]
 
 
-@pytest.mark.parametrize("mode", ["hip_fp64", "hip_fp32", "hip_bf16x3"])
+@pytest.mark.parametrize("mode", ["hip_fp64", "hip_fp32", "hip_bf16x3", "hip_int8x6"])
 @pytest.mark.parametrize("L,nth,nr,nv,ns", PLAN_CASES)
 def test_every_tile_plan_vs_oracle(torch_cuda, mode, L, nth, nr, nv, ns):
     """Every candidate kernel plan the autotuner may pick, forced one by one."""
