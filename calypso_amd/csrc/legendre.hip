@@ -342,6 +342,12 @@ template <class T> __host__ __device__ constexpr int lds_pad()
 {
     return T::EPL == 8 ? 8 : (sizeof(typename T::elem) == 8 ? 2 : 4);
 }
+// third form: an odd FP64 column pitch puts the 16 columns of a B fragment
+// read on 16 distinct bank pairs
+template <class T> __host__ __device__ constexpr int lds_pad3()
+{
+    return sizeof(typename T::elem) == 8 ? 1 : lds_pad<T>();
+}
 
 // ------------------------------------------------------- backward ------
 
@@ -939,6 +945,8 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd(LegArgs a)
 //    the current one and written to LDS after them.
 
 constexpr int kSlotF = 6;    // (row, mirror row) pairs staged per thread per chunk (fwd)
+// sched_barrier mask: VALU, SALU and LDS may be moved across, MFMA and VMEM not
+constexpr int kSchedNoVmem = 0x2 | 0x4 | 0x80 | 0x100 | 0x200;
 constexpr int kSlotB = 6;    // spectral values staged per thread per chunk (bwd)
 
 // Marks per-lane values as redefined here, so that quantities derived from
@@ -1067,7 +1075,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
     constexpr int CX = 16 * NC1, CY = 16 * (NC1 + NC2), CZ = NB - 1;
     constexpr int UE = unit_elems<T>();
     constexpr int kNone = 0xFFFF;
-    const int TC = a.TC, TCP = TC + lds_pad<T>();
+    const int TC = a.TC, TCP = TC + lds_pad3<T>();
 
     const WorkItem it = a.items[blockIdx.x];
     if (it.m < 0) return;   // padding of the XCD interleave
@@ -1290,6 +1298,9 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
                     const int ks = kb + d;
                     mstep(Aq[d], ks * KS_ - t0);
                     load_a(Aq[d], min(ks + PD, nks - 1));
+                    // keep the table prefetch right after the MFMAs that free
+                    // its registers (only VMEM may not cross)
+                    __builtin_amdgcn_sched_barrier(kSchedNoVmem);
                 }
             }
         }
@@ -1350,7 +1361,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
     constexpr int CX = 16 * NC1, CY = 16 * (NC1 + NC2), CZ = NB - 1;
     constexpr int UE = unit_elems<T>();
     constexpr int kNone = 0xFFFF;
-    const int KC = a.TC, KCP = KC + lds_pad<T>();
+    const int KC = a.TC, KCP = KC + lds_pad3<T>();
 
     const WorkItem it = a.items[blockIdx.x];
     if (it.m < 0) return;   // padding of the XCD interleave
@@ -1565,6 +1576,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
                     const int ks = kb + d;
                     mstep(Aq[d], ks * KS_ - c0);
                     load_a(Aq[d], min(ks + PD, KS - 1));
+                    __builtin_amdgcn_sched_barrier(kSchedNoVmem);
                 }
             }
         }
@@ -2476,7 +2488,8 @@ static size_t plan_lds(const LegPlan &p, bool fwd, int prec, int rows3 = 0)
     }
     if (p.ver == 3) {
         const int nb = 16 * (p.mct1 + 2 * p.mct2) + 1;
-        return (((size_t)ns * 2 * nb * (p.tc + pad) * es + 15) & ~(size_t)15)
+        const int pad3 = (prec == kF64) ? 1 : pad;   // lds_pad3<T>()
+        return (((size_t)ns * 2 * nb * (p.tc + pad3) * es + 15) & ~(size_t)15)
                + (fwd ? (size_t)16 * rows3 : 0);   // + row factors (fwd)
     }
     const int ncol = p.KT * 2 * p.ncomp * ((fwd && p.pair) ? 2 : 1);
