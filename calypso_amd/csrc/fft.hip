@@ -62,9 +62,44 @@ struct FftArgs {
 };
 
 constexpr int kFftMaxE = 12;   // complex values per thread per stage (any radix)
+// butterflies per thread per stage
+__host__ __device__ constexpr int fft_mb(int R) { return kFftMaxE / R; }
 constexpr int kFftMaxZ = 4;    // frequencies per thread per pair (inverse side)
 constexpr int kFftMaxU = 2;    // orders per thread per pair (forward side)
 constexpr int kFftPre = 16;    // next pencil's staged spectra per thread (prefetch)
+
+// size-4 DFT X_k = sum_n x_n exp(sgn 2 pi i n k / 4)
+__device__ inline void dft4(cplx x0, cplx x1, cplx x2, cplx x3, double sgn, cplx &X0, cplx &X1,
+                            cplx &X2, cplx &X3)
+{
+    const cplx a0 = cadd(x0, x2), a1 = csub(x0, x2);
+    const cplx b0 = cadd(x1, x3), b1 = cmul_si(csub(x1, x3), sgn);
+    X0 = cadd(a0, b0);
+    X1 = cadd(a1, b1);
+    X2 = csub(a0, b0);
+    X3 = csub(a1, b1);
+}
+
+// size-R DFT of one butterfly's (already twiddled) inputs, R = 2, 3, 4,
+// X_k = sum_n x_n exp(sgn 2 pi i n k / R)
+template <int R>
+__device__ inline void dft(cplx (&v)[R], double sgn, cplx (&o)[R])
+{
+    if constexpr (R == 2) {
+        o[0] = cadd(v[0], v[1]);
+        o[1] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3], sgn, o[0], o[1], o[2], o[3]);
+    } else {   // R == 3
+        const double c = -0.5, sn = sgn * 0.86602540378443864676;
+        cplx t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        cplx m1 = {v[0].x + c * t1.x, v[0].y + c * t1.y};
+        cplx m2 = {-sn * t2.y, sn * t2.x};
+        o[0] = cadd(v[0], t1);
+        o[1] = cadd(m1, m2);
+        o[2] = csub(m1, m2);
+    }
+}
 
 // One in-place Stockham stage of radix R over `nseq` complex sequences of
 // length N held in buf (sequence s at buf + s N), mixed-radix autosort form
@@ -74,7 +109,7 @@ constexpr int kFftPre = 16;    // next pencil's staged spectra per thread (prefe
 template <int R, int NT>
 __device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq, double sgn)
 {
-    constexpr int MB = kFftMaxE / R;   // butterflies per thread
+    constexpr int MB = fft_mb(R);   // butterflies per thread
     const int nb = N / R, total = nseq * nb;
     const int tstep = N / (Ns * R);
     cplx v[MB][R];
@@ -102,25 +137,7 @@ __device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq
                 v[b][r] = cmul(v[b][r], w);
             }
             cplx o[R];
-            if constexpr (R == 2) {
-                o[0] = cadd(v[b][0], v[b][1]);
-                o[1] = csub(v[b][0], v[b][1]);
-            } else if constexpr (R == 4) {
-                cplx a0 = cadd(v[b][0], v[b][2]), a1 = csub(v[b][0], v[b][2]);
-                cplx b0 = cadd(v[b][1], v[b][3]), b1 = cmul_si(csub(v[b][1], v[b][3]), sgn);
-                o[0] = cadd(a0, b0);
-                o[1] = cadd(a1, b1);
-                o[2] = csub(a0, b0);
-                o[3] = csub(a1, b1);
-            } else {  // R == 3
-                const double c = -0.5, sn = sgn * 0.86602540378443864676;
-                cplx t1 = cadd(v[b][1], v[b][2]), t2 = csub(v[b][1], v[b][2]);
-                cplx m1 = {v[b][0].x + c * t1.x, v[b][0].y + c * t1.y};
-                cplx m2 = {-sn * t2.y, sn * t2.x};
-                o[0] = cadd(v[b][0], t1);
-                o[1] = cadd(m1, m2);
-                o[2] = csub(m1, m2);
-            }
+            dft<R>(v[b], sgn, o);
             cplx *dst = buf + (size_t)s * N + (j / Ns) * Ns * R + kk;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[r * Ns] = o[r];
@@ -144,6 +161,66 @@ __device__ void fft_inplace(cplx *buf, const cplx *tw, const FftArgs &a, int nse
 }
 
 enum PencilMode { kPencilBwd = 0, kPencilFwd = 1, kPencilMhd = 2 };
+
+// grid products at longitude i, in place: buf holds the 13 b_trns fields as
+// pairs (0,1) ... (12,-); the 12 products go to pairs 0..5
+// (cal_nonlinear_sph_MHD.f90:35-75, same operation order, no contraction:
+// bitwise equal to the reference formulas)
+__device__ inline void mhd_products(cplx *buf, int N, int i, const double *coef)
+{
+    // b_trns: u 0-2, w 3-5, B 6-8, J 9-11, T 12
+    const cplx c01 = buf[0 * N + i], c23 = buf[1 * N + i], c45 = buf[2 * N + i];
+    const cplx c67 = buf[3 * N + i], c89 = buf[4 * N + i], c1011 = buf[5 * N + i];
+    const cplx c12 = buf[6 * N + i];
+    const double u1 = c01.x, u2 = c01.y, u3 = c23.x;
+    const double w1 = c23.y, w2 = c45.x, w3 = c45.y;
+    const double b1 = c67.x, b2 = c67.y, b3 = c89.x;
+    const double j1 = c89.y, j2 = c1011.x, j3 = c1011.y;
+    const double T = c12.x;
+    const double cv = coef[0], cl = coef[1], ci = coef[2], ct = coef[3];
+    double f[12];
+    {
+#pragma clang fp contract(off)
+        f[0] = (w2 * u3 - w3 * u2) * cv;
+        f[1] = (w3 * u1 - w1 * u3) * cv;
+        f[2] = (w1 * u2 - w2 * u1) * cv;
+        f[3] = (j2 * b3 - j3 * b2) * cl;
+        f[4] = (j3 * b1 - j1 * b3) * cl;
+        f[5] = (j1 * b2 - j2 * b1) * cl;
+        f[6] = (u2 * b3 - u3 * b2) * ci;
+        f[7] = (u3 * b1 - u1 * b3) * ci;
+        f[8] = (u1 * b2 - u2 * b1) * ci;
+        f[9] = u1 * T * ct;
+        f[10] = u2 * T * ct;
+        f[11] = u3 * T * ct;
+    }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) buf[(size_t)p * N + i] = {f[2 * p], f[2 * p + 1]};
+}
+
+// Hermitian spectrum of z = x_a + i x_b at frequency kf from the real
+// spectra of components a, b (rows ra, rb of M doubles, rtm order: a_m at
+// L+m, b_m at L-m; rb null: x_b = 0)
+__device__ inline cplx herm_in(const double *ra, const double *rb, int kf, int N, int L)
+{
+    cplx z = {0.0, 0.0};
+    if (kf == 0) {
+        z.x = ra[L];
+        z.y = rb ? rb[L] : 0.0;
+    } else if (kf <= L) {
+        const double aa = ra[L + kf], ba = ra[L - kf];
+        const double ab = rb ? rb[L + kf] : 0.0, bb = rb ? rb[L - kf] : 0.0;
+        z.x = 0.5 * (aa + bb);
+        z.y = 0.5 * (ab - ba);
+    } else if (kf >= N - L) {
+        const int m = N - kf;
+        const double aa = ra[L + m], ba = ra[L - m];
+        const double ab = rb ? rb[L + m] : 0.0, bb = rb ? rb[L - m] : 0.0;
+        z.x = 0.5 * (aa - bb);
+        z.y = 0.5 * (ba + ab);
+    }
+    return z;
+}
 
 // One work-group per (radius, colatitude) pencil.  LDS: tw[N] | buf[PB][N]
 // complex; the spectra of the pencil are staged in buf (as doubles) before
@@ -192,6 +269,64 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
     for (int pencil = blockIdx.x; pencil < npencil; pencil += gridDim.x) {
     const int t = pencil % a.nth, k = pencil / a.nth;   // pencil = t + nth*k
     const size_t gbase = (size_t)k + (size_t)a.nr * t;  // rtp node at phi = 0
+
+    if constexpr (MODE == kPencilMhd) {
+        // spectra in their own LDS rows after the 7 FFT sequences: 13 rows
+        // of M doubles in, the 12 product rows out, so each unpack is one
+        // pass with no barrier inside
+        double *spec = reinterpret_cast<double *>(buf + (size_t)7 * N);
+        __syncthreads();   // spec and buf free (previous pencil written out)
+        {
+            int tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+#pragma unroll
+            for (int q = 0; q < kFftPre; ++q) {
+                const int e = tid + q * NT;
+                if (e < 13 * M) spec[(e % 13) * M + e / 13] = pre[q];
+            }
+        }
+        prefetch(pencil + gridDim.x);
+        __syncthreads();
+        for (int e = threadIdx.x; e < 7 * N; e += NT) {
+            const int p = e / N, kf = e - p * N;
+            buf[e] = herm_in(spec + (size_t)2 * p * M, p < 6 ? spec + (size_t)(2 * p + 1) * M : nullptr,
+                             kf, N, L);
+        }
+        __syncthreads();
+        fft_inplace<NT>(buf, tw, a, 7, +1.0);
+        for (int i = threadIdx.x; i < N; i += NT) mhd_products(buf, N, i, a.coef);
+        __syncthreads();
+        fft_inplace<NT>(buf, tw, a, 6, -1.0);
+        // two real spectra per pair: X_a = (Z_k + conj Z_{N-k})/2,
+        // X_b = (Z_k - conj Z_{N-k})/(2i), scaled as FFTPACK's forward
+        const double invN = 1.0 / (double)N;
+        for (int e = threadIdx.x; e < 6 * (L + 1); e += NT) {
+            const int p = e / (L + 1), m = e - p * (L + 1);
+            const cplx zk = buf[(size_t)p * N + m];
+            const cplx zn = buf[(size_t)p * N + (m == 0 ? 0 : N - m)];
+            const double xar = 0.5 * (zk.x + zn.x), xai = 0.5 * (zk.y - zn.y);
+            const double xbr = 0.5 * (zk.y + zn.y), xbi = -0.5 * (zk.x - zn.x);
+            double *rows = spec + (size_t)2 * p * M;
+            if (m == 0) {
+                rows[L] = xar * invN;
+                rows[M + L] = xbr * invN;
+            } else {
+                rows[L + m] = 2.0 * xar * invN;
+                rows[L - m] = -2.0 * xai * invN;
+                rows[M + L + m] = 2.0 * xbr * invN;
+                rows[M + L - m] = -2.0 * xbi * invN;
+            }
+        }
+        __syncthreads();
+        // 19 outputs per order; the slots the products do not fill (3-8, 18) are zero
+        const int nout = a.ncomp_out;
+        for (int e = threadIdx.x; e < nout * M; e += NT) {
+            const int nd = e % nout, mp = e / nout;
+            const int q = nd < 3 ? nd : (nd >= 9 && nd < 18 ? nd - 6 : -1);
+            a.out[nd + (size_t)nout * (pencil + sl * mp)] = (q >= 0) ? spec[(size_t)q * M + mp] : 0.0;
+        }
+        continue;
+    }
 
     // ---- inverse side -----------------------------------------------
     if (MODE == kPencilBwd || MODE == kPencilMhd) {
@@ -276,38 +411,7 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
     int npair_f;
     if (MODE == kPencilMhd) {
         npair_f = 6;
-        for (int i = threadIdx.x; i < N; i += NT) {
-            // b_trns: u 0-2, w 3-5, B 6-8, J 9-11, T 12 -> pairs (0,1)...(12,-)
-            const cplx c01 = buf[0 * N + i], c23 = buf[1 * N + i], c45 = buf[2 * N + i];
-            const cplx c67 = buf[3 * N + i], c89 = buf[4 * N + i], c1011 = buf[5 * N + i];
-            const cplx c12 = buf[6 * N + i];
-            const double u1 = c01.x, u2 = c01.y, u3 = c23.x;
-            const double w1 = c23.y, w2 = c45.x, w3 = c45.y;
-            const double b1 = c67.x, b2 = c67.y, b3 = c89.x;
-            const double j1 = c89.y, j2 = c1011.x, j3 = c1011.y;
-            const double T = c12.x;
-            const double cv = a.coef[0], cl = a.coef[1], ci = a.coef[2], ct = a.coef[3];
-            double f[12];
-            {
-            // same operation order and no contraction: bitwise equal to the
-            // reference formulas
-#pragma clang fp contract(off)
-            f[0] = (w2 * u3 - w3 * u2) * cv;
-            f[1] = (w3 * u1 - w1 * u3) * cv;
-            f[2] = (w1 * u2 - w2 * u1) * cv;
-            f[3] = (j2 * b3 - j3 * b2) * cl;
-            f[4] = (j3 * b1 - j1 * b3) * cl;
-            f[5] = (j1 * b2 - j2 * b1) * cl;
-            f[6] = (u2 * b3 - u3 * b2) * ci;
-            f[7] = (u3 * b1 - u1 * b3) * ci;
-            f[8] = (u1 * b2 - u2 * b1) * ci;
-            f[9] = u1 * T * ct;
-            f[10] = u2 * T * ct;
-            f[11] = u3 * T * ct;
-            }
-#pragma unroll
-            for (int p = 0; p < 6; ++p) buf[(size_t)p * N + i] = {f[2 * p], f[2 * p + 1]};
-        }
+        for (int i = threadIdx.x; i < N; i += NT) mhd_products(buf, N, i, a.coef);
         __syncthreads();
     } else {
         npair_f = (nin + 1) / 2;
@@ -418,9 +522,9 @@ int fft_context_build(FftContext *f, int L, int nth, int nr, int nphi, std::stri
     if (L < 1 || nth < 1 || nr < 1) { *err = "invalid grid size"; return -1; }
     if (nphi < 2 * L + 2) { *err = "nphi must be at least 2*(L+1)"; return -1; }
     int n = nphi, ns = 0;
-    while (n % 4 == 0) { f->radix[ns++] = 4; n /= 4; }
-    while (n % 2 == 0) { f->radix[ns++] = 2; n /= 2; }
-    while (n % 3 == 0) { f->radix[ns++] = 3; n /= 3; }
+    while (n % 4 == 0 && ns < FFT_MAX_STAGES) { f->radix[ns++] = 4; n /= 4; }
+    if (n % 2 == 0 && ns < FFT_MAX_STAGES) { f->radix[ns++] = 2; n /= 2; }
+    while (n % 3 == 0 && ns < FFT_MAX_STAGES) { f->radix[ns++] = 3; n /= 3; }
     if (n != 1 || ns > FFT_MAX_STAGES) {
         *err = "nphi must be of the form 2^a 3^b";
         return -1;
@@ -455,10 +559,13 @@ void fft_context_free(FftContext *f)
     if (f) f->d_tw = nullptr;
 }
 
-// LDS of a pencil work-group with pb component pairs per group
-static size_t pencil_lds(const FftContext *f, int pb)
+// LDS of a pencil work-group with pb component pairs per group: twiddles,
+// pb sequences of N complex values; the fused MHD pencil adds its 13 rows of
+// M doubles of spectra
+static size_t pencil_lds(const FftContext *f, int pb, int mode)
 {
-    return (size_t)(pb + 1) * f->N * 16;
+    const size_t b = (size_t)(pb + 1) * f->N * 16;
+    return mode == kPencilMhd ? b + (size_t)13 * (2 * f->L + 1) * 8 : b;
 }
 
 // pairs per group: everything the register staging, the prefetch of the
@@ -472,9 +579,9 @@ static int pencil_pairs(const FftContext *f, int mode, int nin, int nout, int nt
     auto fits = [&](int p) {
         const int np = std::min(p, need);
         for (int st = 0; st < f->nstage; ++st)
-            if (np * (N / f->radix[st]) > (kFftMaxE / f->radix[st]) * nt) return false;
+            if (np * (N / f->radix[st]) > fft_mb(f->radix[st]) * nt) return false;
         if (mode != kPencilFwd && std::min(2 * p, nin) * M > kFftPre * nt) return false;
-        return pencil_lds(f, p) <= 160 * 1024;
+        return pencil_lds(f, p, mode) <= 160 * 1024;
     };
     if (mode == kPencilMhd) return fits(need) ? need : 0;
     int pb = need;
@@ -519,7 +626,7 @@ static int launch_pencil(const FftContext *f, int mode, int nin, int nout, const
     for (int i = 0; i < 4; ++i) a.coef[i] = coef ? coef[i] : 0.0;
     // 256 threads while two or more work-groups fit on a CU, else 512
     int nt = 256, pb = pencil_pairs(f, mode, nin, nout, 256);
-    if (!pb || pencil_lds(f, pb) > 80 * 1024) {
+    if (!pb || pencil_lds(f, pb, mode) > 80 * 1024) {
         const int pb2 = pencil_pairs(f, mode, nin, nout, 512);
         if (pb2) { nt = 512; pb = pb2; }
     }
@@ -529,7 +636,7 @@ static int launch_pencil(const FftContext *f, int mode, int nin, int nout, const
         return -1;
     }
     a.pairs_per_batch = pb;
-    const size_t lds = pencil_lds(f, pb);
+    const size_t lds = pencil_lds(f, pb, mode);
     if (mode == kPencilBwd) return launch_nt<kPencilBwd>(f, a, lds, nt, s, err);
     if (mode == kPencilFwd) return launch_nt<kPencilFwd>(f, a, lds, nt, s, err);
     return launch_nt<kPencilMhd>(f, a, lds, nt, s, err);

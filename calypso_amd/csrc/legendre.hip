@@ -32,6 +32,9 @@
 // accumulation).  P_l^m tables are computed on the GPU in FP64 and packed in
 // the lane order the MFMA A operand is read in (one 64-lane unit per
 // 16 x KSTEP tile), so every A fetch is a contiguous 512 B - 1 KiB load.
+#include <array>
+#include <map>
+#include <mutex>
 #include "leg_internal.hpp"
 
 #include <hip/hip_bf16.h>
@@ -2748,6 +2751,21 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
 
 static int autotune_plan(LegContext *c, std::vector<LegPlan> &cand, bool fwd, std::string *err);
 
+// autotuner picks of this process: (L, nth, nr, ncomp, nvec, nscl, fwd,
+// precision) -> index into the candidate list (which depends on these only)
+// and its timing
+using PlanKey = std::array<int, 8>;
+static std::map<PlanKey, std::pair<int, float>> &plan_cache()
+{
+    static std::map<PlanKey, std::pair<int, float>> m;
+    return m;
+}
+static std::mutex &plan_cache_mutex()
+{
+    static std::mutex mu;
+    return mu;
+}
+
 const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
                         std::string *err)
 {
@@ -2777,8 +2795,21 @@ const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
     }
     int best = 0;
     if (c->autotune && cand.size() > 1) {
-        best = autotune_plan(c, cand, fwd, err);
-        if (best < 0) return nullptr;
+        // one timing per process and shape: later contexts of the same grid,
+        // field set, direction and precision take the same plan, so they
+        // compute bitwise the same (timing noise cannot pick another plan)
+        const PlanKey key{c->g.L, c->g.nth, c->g.nr, ncomp, nvec, nscl, fwd ? 1 : 0, c->prec};
+        std::lock_guard<std::mutex> lock(plan_cache_mutex());
+        auto &cache = plan_cache();
+        auto hit = cache.find(key);
+        if (hit != cache.end() && hit->second.first < (int)cand.size()) {
+            best = hit->second.first;
+            cand[best].tuned_ms = hit->second.second;
+        } else {
+            best = autotune_plan(c, cand, fwd, err);
+            if (best < 0) return nullptr;
+            cache[key] = {best, cand[best].tuned_ms};
+        }
     }
     for (size_t i = 0; i < cand.size(); ++i)
         if ((int)i != best) (void)hipFree(cand[i].d_items);
