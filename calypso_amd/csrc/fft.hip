@@ -68,6 +68,17 @@ constexpr int kFftMaxZ = 4;    // frequencies per thread per pair (inverse side)
 constexpr int kFftMaxU = 2;    // orders per thread per pair (forward side)
 constexpr int kFftPre = 16;    // next pencil's staged spectra per thread (prefetch)
 
+// n / d for 0 <= n < 2^23, 1 <= d, with rd = 1.0f / d: the float quotient is
+// within one of the true one; one integer step corrects it (a few VALU
+// instructions instead of an integer division sequence)
+__device__ inline int qdiv(int n, int d, float rd)
+{
+    int q = (int)((float)n * rd);
+    const int r = n - q * d;
+    q += (r >= d) - (r < 0);
+    return q;
+}
+
 // size-4 DFT X_k = sum_n x_n exp(sgn 2 pi i n k / 4)
 __device__ inline void dft4(cplx x0, cplx x1, cplx x2, cplx x3, double sgn, cplx &X0, cplx &X1,
                             cplx &X2, cplx &X3)
@@ -112,12 +123,16 @@ __device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq
     constexpr int MB = fft_mb(R);   // butterflies per thread
     const int nb = N / R, total = nseq * nb;
     const int tstep = N / (Ns * R);
+    const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
+    // Ns is a power of two whenever the threes come last (2^a 3 lengths)
+    const bool ns2 = (Ns & (Ns - 1)) == 0;
+    const int lns = __builtin_ctz(Ns);
     cplx v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
         const int idx = threadIdx.x + b * NT;
         if (idx < total) {
-            const int s = idx / nb, j = idx - s * nb;
+            const int s = qdiv(idx, nb, rnb), j = idx - s * nb;
             const cplx *src = buf + (size_t)s * N;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = src[j + r * nb];
@@ -128,17 +143,19 @@ __device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq
     for (int b = 0; b < MB; ++b) {
         const int idx = threadIdx.x + b * NT;
         if (idx < total) {
-            const int s = idx / nb, j = idx - s * nb;
-            const int kk = j % Ns;
+            const int s = qdiv(idx, nb, rnb), j = idx - s * nb;
+            const int jq = ns2 ? (j >> lns) : qdiv(j, Ns, rns);
+            const int kk = j - jq * Ns;
+            // r kk tstep < R Ns tstep = N: no wrap
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                cplx w = tw[(r * kk * tstep) % N];
+                cplx w = tw[r * kk * tstep];
                 w.y *= -sgn;  // table holds exp(-i...); sgn=+1 (inverse) conjugates
                 v[b][r] = cmul(v[b][r], w);
             }
             cplx o[R];
             dft<R>(v[b], sgn, o);
-            cplx *dst = buf + (size_t)s * N + (j / Ns) * Ns * R + kk;
+            cplx *dst = buf + (size_t)s * N + jq * Ns * R + kk;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[r * Ns] = o[r];
         }
@@ -227,8 +244,9 @@ __device__ inline cplx herm_in(const double *ra, const double *rb, int kf, int N
 // the inverse FFTs and after the forward ones, so a pencil needs (PB + 1) N
 // complex words.  Components go in pairs (real / imaginary part of one
 // complex FFT), PB pairs per group; the MHD mode holds all 7 input pairs.
+// (256 threads: two work-groups per CU, i.e. at most 256 registers a lane)
 template <int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
+__global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int M = 2 * a.L + 1, N = a.N, L = a.L;
@@ -259,7 +277,7 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
             const int e = tid + q * NT;
             pre[q] = 0.0;
             if (pc < npencil && e < nc0 * M) {
-                const int nd = e % nc0, mp = e / nc0;
+                const int mp = qdiv(e, nc0, 1.0f / (float)nc0), nd = e - mp * nc0;
                 pre[q] = a.in[nd + (size_t)nin * (pc + sl * mp)];
             }
         }
@@ -287,8 +305,9 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
         }
         prefetch(pencil + gridDim.x);
         __syncthreads();
+        const float rN = 1.0f / (float)N;
         for (int e = threadIdx.x; e < 7 * N; e += NT) {
-            const int p = e / N, kf = e - p * N;
+            const int p = qdiv(e, N, rN), kf = e - p * N;
             buf[e] = herm_in(spec + (size_t)2 * p * M, p < 6 ? spec + (size_t)(2 * p + 1) * M : nullptr,
                              kf, N, L);
         }
@@ -300,8 +319,9 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
         // two real spectra per pair: X_a = (Z_k + conj Z_{N-k})/2,
         // X_b = (Z_k - conj Z_{N-k})/(2i), scaled as FFTPACK's forward
         const double invN = 1.0 / (double)N;
+        const float rL1 = 1.0f / (float)(L + 1);
         for (int e = threadIdx.x; e < 6 * (L + 1); e += NT) {
-            const int p = e / (L + 1), m = e - p * (L + 1);
+            const int p = qdiv(e, L + 1, rL1), m = e - p * (L + 1);
             const cplx zk = buf[(size_t)p * N + m];
             const cplx zn = buf[(size_t)p * N + (m == 0 ? 0 : N - m)];
             const double xar = 0.5 * (zk.x + zn.x), xai = 0.5 * (zk.y - zn.y);
@@ -320,8 +340,9 @@ __global__ __launch_bounds__(NT) void k_pencil(FftArgs a)
         __syncthreads();
         // 19 outputs per order; the slots the products do not fill (3-8, 18) are zero
         const int nout = a.ncomp_out;
+        const float rno = 1.0f / (float)nout;
         for (int e = threadIdx.x; e < nout * M; e += NT) {
-            const int nd = e % nout, mp = e / nout;
+            const int mp = qdiv(e, nout, rno), nd = e - mp * nout;
             const int q = nd < 3 ? nd : (nd >= 9 && nd < 18 ? nd - 6 : -1);
             a.out[nd + (size_t)nout * (pencil + sl * mp)] = (q >= 0) ? spec[(size_t)q * M + mp] : 0.0;
         }
