@@ -56,7 +56,7 @@ struct FftArgs {
     int L, nth, nr, N, ncomp_in, ncomp_out;
     int nstage;
     int radix[FFT_MAX_STAGES];
-    const double *tw;   // N complex twiddles exp(-2 pi i t / N)
+    const double *tw;   // per-stage twiddle rows (N complex slots)
     int pairs_per_batch;
     double coef[4];     // coef_velo, coef_lor, coef_induct, coef_temp
 };
@@ -122,7 +122,6 @@ __device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq
 {
     constexpr int MB = fft_mb(R);   // butterflies per thread
     const int nb = N / R, total = nseq * nb;
-    const int tstep = N / (Ns * R);
     const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
     // Ns is a power of two whenever the threes come last (2^a 3 lengths)
     const bool ns2 = (Ns & (Ns - 1)) == 0;
@@ -146,10 +145,10 @@ __device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq
             const int s = qdiv(idx, nb, rnb), j = idx - s * nb;
             const int jq = ns2 ? (j >> lns) : qdiv(j, Ns, rns);
             const int kk = j - jq * Ns;
-            // r kk tstep < R Ns tstep = N: no wrap
+            // this stage's twiddle rows: entry (r-1) Ns + kk = W_N^(r kk tstep)
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                cplx w = tw[r * kk * tstep];
+                cplx w = tw[(r - 1) * Ns + kk];
                 w.y *= -sgn;  // table holds exp(-i...); sgn=+1 (inverse) conjugates
                 v[b][r] = cmul(v[b][r], w);
             }
@@ -163,16 +162,18 @@ __device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq
     __syncthreads();
 }
 
-// full in-place FFT of nseq sequences
+// full in-place FFT of nseq sequences; tw holds the stages' twiddle rows one
+// after the other ((R-1) Ns entries per stage, N-1 in all)
 template <int NT>
 __device__ void fft_inplace(cplx *buf, const cplx *tw, const FftArgs &a, int nseq, double sgn)
 {
-    int Ns = 1;
+    int Ns = 1, off = 0;
     for (int st = 0; st < a.nstage; ++st) {
         const int R = a.radix[st];
-        if (R == 4) stage_inplace<4, NT>(buf, tw, a.N, Ns, nseq, sgn);
-        else if (R == 2) stage_inplace<2, NT>(buf, tw, a.N, Ns, nseq, sgn);
-        else stage_inplace<3, NT>(buf, tw, a.N, Ns, nseq, sgn);
+        if (R == 4) stage_inplace<4, NT>(buf, tw + off, a.N, Ns, nseq, sgn);
+        else if (R == 2) stage_inplace<2, NT>(buf, tw + off, a.N, Ns, nseq, sgn);
+        else stage_inplace<3, NT>(buf, tw + off, a.N, Ns, nseq, sgn);
+        off += (R - 1) * Ns;
         Ns *= R;
     }
 }
@@ -282,9 +283,19 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
             }
         }
     };
-    if (MODE != kPencilFwd) prefetch(blockIdx.x);
+    // pencils of round i (offsets i G + o): chunks of 4 consecutive pencils
+    // go to one XCD (work-groups b = x mod 8 run on XCD x) and consecutive
+    // chunks to consecutive XCDs, so neighbouring colatitudes, which share
+    // cache lines of every order's record, are read through one L2 while a
+    // partial last round still spreads over all XCDs
+    const int G = gridDim.x;
+    const bool xm = (G & 31) == 0;
+    const int slot = blockIdx.x >> 3;
+    const int xbase = (((slot >> 2) << 3) + (blockIdx.x & 7)) * 4 + (slot & 3);
+    auto pencil_of = [&](int i) { return xm ? i * G + xbase : (int)blockIdx.x + i * G; };
+    if (MODE != kPencilFwd) prefetch(pencil_of(0));
 
-    for (int pencil = blockIdx.x; pencil < npencil; pencil += gridDim.x) {
+    for (int it = 0, pencil = pencil_of(0); pencil < npencil; pencil = pencil_of(++it)) {
     const int t = pencil % a.nth, k = pencil / a.nth;   // pencil = t + nth*k
     const size_t gbase = (size_t)k + (size_t)a.nr * t;  // rtp node at phi = 0
 
@@ -303,7 +314,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
                 if (e < 13 * M) spec[(e % 13) * M + e / 13] = pre[q];
             }
         }
-        prefetch(pencil + gridDim.x);
+        prefetch(pencil_of(it + 1));
         __syncthreads();
         const float rN = 1.0f / (float)N;
         for (int e = threadIdx.x; e < 7 * N; e += NT) {
@@ -364,7 +375,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
                     const int e = tid + q * NT;
                     if (e < nc * M) bd[(e % nc) * M + e / nc] = pre[q];
                 }
-                prefetch(pencil + gridDim.x);
+                prefetch(pencil_of(it + 1));
             } else {
                 for (int e = threadIdx.x; e < nc * M; e += NT) {
                     const int nd = e % nc, mp = e / nc;
@@ -562,12 +573,28 @@ int fft_context_build(FftContext *f, int L, int nth, int nr, int nphi, std::stri
         HIP_TRY(hipGetDeviceProperties(&prop, dev));
         f->n_cu = prop.multiProcessorCount;
     }
-    std::vector<double> tw(2 * (size_t)nphi);
+    // per-stage twiddle rows: stage st (radix R, Ns = product of the earlier
+    // radices) holds W_N^(r kk N/(Ns R)) = exp(-2 pi i r kk / (Ns R)) at
+    // (r-1) Ns + kk, so consecutive lanes (consecutive kk) read consecutive
+    // words; N-1 entries in all (the last slot stays zero)
+    std::vector<double> tw(2 * (size_t)nphi, 0.0);
     const double pi = 3.14159265358979323846;
-    for (int i = 0; i < nphi; ++i) {
-        long double ang = -2.0L * (long double)pi * (long double)i / (long double)nphi;
-        tw[2 * i] = (double)std::cos(ang);
-        tw[2 * i + 1] = (double)std::sin(ang);
+    {
+        int Ns = 1, off = 0;
+        for (int st = 0; st < ns; ++st) {
+            const int R = f->radix[st];
+            for (int r = 1; r < R; ++r)
+                for (int kk = 0; kk < Ns; ++kk) {
+                    // the angle of table entry i = r kk N/(Ns R) of exp(-2 pi i / N)
+                    const long long i = (long long)r * kk * (nphi / (Ns * R));
+                    long double ang = -2.0L * (long double)pi * (long double)i / (long double)nphi;
+                    const size_t e = (size_t)off + (size_t)(r - 1) * Ns + kk;
+                    tw[2 * e] = (double)std::cos(ang);
+                    tw[2 * e + 1] = (double)std::sin(ang);
+                }
+            off += (R - 1) * Ns;
+            Ns *= R;
+        }
     }
     HIP_TRY(hipMalloc(&f->d_tw, sizeof(double) * tw.size()));
     HIP_TRY(hipMemcpy(f->d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));

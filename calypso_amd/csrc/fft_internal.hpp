@@ -14,7 +14,7 @@ struct FftContext {
     int L = 0, nth = 0, nr = 0, N = 0;
     int nstage = 0;
     int radix[FFT_MAX_STAGES] = {0};
-    double *d_tw = nullptr;   // N complex exp(-2 pi i t / N)
+    double *d_tw = nullptr;   // per-stage twiddle rows, N complex slots
     int n_cu = 256;           // compute units (persistent pencil grid)
 };
 
