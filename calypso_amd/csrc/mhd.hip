@@ -329,67 +329,112 @@ __device__ inline bool tile_origin(int L, int *l0, int *m0)
     return minabs <= min(L, *l0 + kTile - 1);
 }
 
-__global__ __launch_bounds__(256) void k_pack(MhdDev p)
+// A work-group walks kz radial levels of its tile: the tile's rlm indices
+// are read once into LDS, and the next level's values are loaded into
+// registers while the current level is stored.
+__global__ __launch_bounds__(256) void k_pack(MhdDev p, int kz)
 {
     __shared__ double tile[13][kTile][kTile + 1];
+    __shared__ int jls[kTile][kTile];   // rlm index of (degree, order), -1 outside l >= |m|
     int l0, m0;
     if (!tile_origin(p.L, &l0, &m0)) return;
-    const int k = blockIdx.z;
+    const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, p.nr);
     const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-    {
-        const int l = l0 + ty, m = m0 + tx;
-        const bool ok = l <= p.L && m <= p.L && abs(m) <= l;
-        const size_t i = (size_t)p.J * k + (ok ? l * (l + 1) + m : 0);
+    const int l = l0 + ty, m = m0 + tx;
+    const bool ok = l <= p.L && m <= p.L && abs(m) <= l;
+    const int jrj = ok ? l * (l + 1) + m : 0;
+    jls[ty][tx] = ok ? p.rlm_of_rj[jrj] : -1;
+    double v[13];
+    auto load = [&](int k) {
+        const size_t i = (size_t)p.J * k + jrj;
 #pragma unroll
-        for (int c = 0; c < 12; ++c) tile[c][ty][tx] = ok ? FLD(kVelo + c)[i] : 0.0;
-        tile[12][ty][tx] = ok ? FLD(kTemp)[i] : 0.0;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < kTile * kTile * 13; e += 256) {
-        const int col = e / (kTile * 13), r = e % (kTile * 13);
-        const int node = r / 13, c = r % 13;
-        const int l = l0 + node, m = m0 + col;
-        if (l > p.L || m > p.L || abs(m) > l) continue;
-        const size_t jl = (size_t)p.rlm_of_rj[l * (l + 1) + m];
-        p.sp_b[13 * (jl + (size_t)p.J * k) + c] = tile[c][node][col];
+        for (int c = 0; c < 12; ++c) v[c] = ok ? FLD(kVelo + c)[i] : 0.0;
+        v[12] = ok ? FLD(kTemp)[i] : 0.0;
+    };
+    load(k0);
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+        for (int c = 0; c < 13; ++c) tile[c][ty][tx] = v[c];
+        __syncthreads();
+        if (k + 1 < k1) load(k + 1);
+        for (int e = threadIdx.x; e < kTile * kTile * 13; e += 256) {
+            const int col = e / (kTile * 13), r = e % (kTile * 13);
+            const int node = r / 13, c = r % 13;
+            const int jl = jls[node][col];
+            if (jl < 0) continue;
+            p.sp_b[13 * ((size_t)jl + (size_t)p.J * k) + c] = tile[c][node][col];
+        }
+        __syncthreads();
     }
 }
 
 // Unpack the forward transform into the rj fields: the 13 components the step
 // uses (advection 0-2, the poloidal Coriolis slot 7, Lorentz 9-11, induction
-// 12-14, heat flux 15-17) go through an LDS tile.
-__global__ __launch_bounds__(256) void k_unpack(MhdDev p)
+// 12-14, heat flux 15-17) go through an LDS tile; kz radial levels per
+// work-group as k_pack.
+__global__ __launch_bounds__(256) void k_unpack(MhdDev p, int kz)
 {
     constexpr int NU = 13;
+    constexpr int PER = (kTile * kTile * NU + 255) / 256;   // gathered values per thread
     __shared__ double tile[NU][kTile][kTile + 1];
+    __shared__ int jls[kTile][kTile];
     int l0, m0;
     if (!tile_origin(p.L, &l0, &m0)) return;
-    const int k = blockIdx.z;
-    for (int e = threadIdx.x; e < kTile * kTile * NU; e += 256) {
-        const int col = e / (kTile * NU), r = e % (kTile * NU);
-        const int node = r / NU, u = r % NU;
-        const int c = u < 3 ? u : (u == 3 ? 7 : u + 5);   // 0-2, 7, 9-17
-        const int l = l0 + node, m = m0 + col;
-        double v = 0.0;
-        if (l <= p.L && m <= p.L && abs(m) <= l) {
-            const size_t jl = (size_t)p.rlm_of_rj[l * (l + 1) + m];
-            v = p.sp_f[19 * (jl + (size_t)p.J * k) + c];
-        }
-        tile[u][node][col] = v;
+    const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, p.nr);
+    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+    {
+        const int l = l0 + ty, m = m0 + tx;
+        const bool ok = l <= p.L && m <= p.L && abs(m) <= l;
+        jls[ty][tx] = ok ? p.rlm_of_rj[l * (l + 1) + m] : -1;
     }
     __syncthreads();
-    const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
-    const int l = l0 + ty, m = m0 + tx;
-    if (l > p.L || m > p.L || abs(m) > l) return;
-    const size_t i = (size_t)p.J * k + l * (l + 1) + m;
+    // this thread's gathered values: rlm node offset (or -1), component
+    int off[PER];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        FLD(kMAdvect + c)[i] = tile[c][ty][tx];
-        FLD(kLorentz + c)[i] = tile[4 + c][ty][tx];
-        FLD(kVpInduct + c)[i] = tile[7 + c][ty][tx];
-        FLD(kHFlux + c)[i] = tile[10 + c][ty][tx];
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * 256;
+        off[q] = -1;
+        if (e < kTile * kTile * NU) {
+            const int col = e / (kTile * NU), r = e % (kTile * NU);
+            const int node = r / NU, u = r % NU;
+            const int c = u < 3 ? u : (u == 3 ? 7 : u + 5);   // 0-2, 7, 9-17
+            const int jl = jls[node][col];
+            if (jl >= 0) off[q] = 19 * jl + c;
+        }
     }
-    FLD(kRotCoriolis + 1)[i] = tile[3][ty][tx];
+    double g[PER];
+    auto load = [&](int k) {
+        const size_t base = (size_t)19 * p.J * k;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) g[q] = off[q] >= 0 ? p.sp_f[base + off[q]] : 0.0;
+    };
+    const int l = l0 + ty, m = m0 + tx;
+    const bool ok = l <= p.L && m <= p.L && abs(m) <= l;
+    load(k0);
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * 256;
+            if (e < kTile * kTile * NU) {
+                const int col = e / (kTile * NU), r = e % (kTile * NU);
+                tile[r % NU][r / NU][col] = g[q];
+            }
+        }
+        __syncthreads();
+        if (k + 1 < k1) load(k + 1);
+        if (ok) {
+            const size_t i = (size_t)p.J * k + l * (l + 1) + m;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                FLD(kMAdvect + c)[i] = tile[c][ty][tx];
+                FLD(kLorentz + c)[i] = tile[4 + c][ty][tx];
+                FLD(kVpInduct + c)[i] = tile[7 + c][ty][tx];
+                FLD(kHFlux + c)[i] = tile[10 + c][ty][tx];
+            }
+            FLD(kRotCoriolis + 1)[i] = tile[3][ty][tx];
+        }
+        __syncthreads();
+    }
 }
 
 // Curl of the Coriolis force from u and w (sum_rot_coriolis_rlm_10;
@@ -1097,8 +1142,12 @@ int nonlinear(MhdContext *c, bool temp_roundtrip, std::string *err)
     MhdDev &p = c->p;
     hipStream_t s = c->stream;
     const int nb = blocks_for(p.nnod);
-    const dim3 tiles((2 * c->L + 1 + kTile - 1) / kTile, (c->L + 1 + kTile - 1) / kTile, c->nr);
-    hipLaunchKernelGGL(k_pack, tiles, dim3(256), 0, s, p);
+    // radial levels per transpose work-group: as many as keep about 2048
+    // non-empty tiles (about half the tile grid lies outside l >= |m|)
+    const int tx = (2 * c->L + 1 + kTile - 1) / kTile, ty = (c->L + 1 + kTile - 1) / kTile;
+    const int kz = std::max(1, std::min(8, (tx * ty / 2) * c->nr / 2048));
+    const dim3 tiles(tx, ty, (c->nr + kz - 1) / kz);
+    hipLaunchKernelGGL(k_pack, tiles, dim3(256), 0, s, p, kz);
     HIP_TRY(hipGetLastError());
     if (leg_backward_dev(&c->leg, 13, 4, 1, c->d_sp_b, c->d_vr_b, s, err)) return -1;
     const double coef[4] = {c->co.coef_velo, c->co.coef_lor, c->co.coef_induct, c->co.coef_temp};
@@ -1112,7 +1161,7 @@ int nonlinear(MhdContext *c, bool temp_roundtrip, std::string *err)
             return -1;
     }
     if (leg_forward_dev(&c->leg, 19, 6, 1, c->d_vr_f, c->d_sp_f, s, err)) return -1;
-    hipLaunchKernelGGL(k_unpack, tiles, dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_unpack, tiles, dim3(256), 0, s, p, kz);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_coriolis, dim3(nb), dim3(256), 0, s, p);
     HIP_TRY(hipGetLastError());
