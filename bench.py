@@ -338,6 +338,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mhd", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the Lmax=511 transforms")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the L=127 precision comparison and the ensemble")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device copy (reference for the byte counters)")
     args = ap.parse_args()
@@ -409,8 +411,9 @@ def main():
         torch.cuda.synchronize()
     if not args.no_mhd:
         line["mhd_step"] = mhd_bench(torch)
-        line["mhd_step_L127_precisions"] = mhd_precision_bench(torch)
-        line["ensemble"] = ensemble_bench(torch)
+        if not args.no_extra:
+            line["mhd_step_L127_precisions"] = mhd_precision_bench(torch)
+            line["ensemble"] = ensemble_bench(torch)
     if not args.no_large:
         line["legendre_L511"] = large_legendre_bench(torch)
     if not args.no_cpu:
