@@ -79,6 +79,24 @@ __device__ inline int qdiv(int n, int d, float rd)
     return q;
 }
 
+// e = hi d + lo stepped by a constant: the quotient once, then carries
+struct DivStep {
+    int hi, lo, dhi, dlo, d;
+    __device__ DivStep(int e0, int step, int d_) : d(d_)
+    {
+        hi = qdiv(e0, d, 1.0f / (float)d);
+        lo = e0 - hi * d;
+        dhi = step / d;
+        dlo = step - dhi * d;
+    }
+    __device__ void next()
+    {
+        hi += dhi;
+        lo += dlo;
+        if (lo >= d) { lo -= d; ++hi; }
+    }
+};
+
 // size-4 DFT X_k = sum_n x_n exp(sgn 2 pi i n k / 4)
 __device__ inline void dft4(cplx x0, cplx x1, cplx x2, cplx x3, double sgn, cplx &X0, cplx &X1,
                             cplx &X2, cplx &X3)
@@ -117,46 +135,59 @@ __device__ inline void dft(cplx (&v)[R], double sgn, cplx (&o)[R])
 // (Govindaraju et al., SC'08): every thread reads the R inputs of its
 // butterflies into registers, the work-group synchronises, and the outputs go
 // to their autosorted positions in the same buffer.
-template <int R, int NT>
-__device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq, double sgn)
+template <int R, int NT, int SGN>
+__device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq)
 {
+    constexpr double sgn = SGN;
     constexpr int MB = fft_mb(R);   // butterflies per thread
     const int nb = N / R, total = nseq * nb;
-    const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
+    // butterfly b of this thread is idx = tid + b NT = s nb + j: (s, j) from
+    // one quotient, then stepped by NT = ds nb + dj
+    const int ds = NT / nb, dj = NT - ds * nb;
+    const int s0 = qdiv(threadIdx.x, nb, 1.0f / (float)nb), j0 = threadIdx.x - s0 * nb;
     // Ns is a power of two whenever the threes come last (2^a 3 lengths)
     const bool ns2 = (Ns & (Ns - 1)) == 0;
     const int lns = __builtin_ctz(Ns);
+    const float rns = 1.0f / (float)Ns;
     cplx v[MB][R];
+    {
+        int s = s0, j = j0;
 #pragma unroll
-    for (int b = 0; b < MB; ++b) {
-        const int idx = threadIdx.x + b * NT;
-        if (idx < total) {
-            const int s = qdiv(idx, nb, rnb), j = idx - s * nb;
-            const cplx *src = buf + (size_t)s * N;
+        for (int b = 0; b < MB; ++b) {
+            if (threadIdx.x + b * NT < total) {
+                const cplx *src = buf + (size_t)s * N;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = src[j + r * nb];
+                for (int r = 0; r < R; ++r) v[b][r] = src[j + r * nb];
+            }
+            s += ds;
+            j += dj;
+            if (j >= nb) { j -= nb; ++s; }
         }
     }
     __syncthreads();
+    {
+        int s = s0, j = j0;
 #pragma unroll
-    for (int b = 0; b < MB; ++b) {
-        const int idx = threadIdx.x + b * NT;
-        if (idx < total) {
-            const int s = qdiv(idx, nb, rnb), j = idx - s * nb;
-            const int jq = ns2 ? (j >> lns) : qdiv(j, Ns, rns);
-            const int kk = j - jq * Ns;
-            // this stage's twiddle rows: entry (r-1) Ns + kk = W_N^(r kk tstep)
+        for (int b = 0; b < MB; ++b) {
+            if (threadIdx.x + b * NT < total) {
+                const int jq = ns2 ? (j >> lns) : qdiv(j, Ns, rns);
+                const int kk = j - jq * Ns;
+                // this stage's twiddle rows: entry (r-1) Ns + kk = W_N^(r kk tstep);
+                // the table holds exp(-i...), the inverse (SGN = +1) conjugates
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                cplx w = tw[(r - 1) * Ns + kk];
-                w.y *= -sgn;  // table holds exp(-i...); sgn=+1 (inverse) conjugates
-                v[b][r] = cmul(v[b][r], w);
+                for (int r = 1; r < R; ++r) {
+                    const cplx w = tw[(r - 1) * Ns + kk];
+                    v[b][r] = cmul(v[b][r], {w.x, -sgn * w.y});
+                }
+                cplx o[R];
+                dft<R>(v[b], sgn, o);
+                cplx *dst = buf + (size_t)s * N + jq * Ns * R + kk;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[r * Ns] = o[r];
             }
-            cplx o[R];
-            dft<R>(v[b], sgn, o);
-            cplx *dst = buf + (size_t)s * N + jq * Ns * R + kk;
-#pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * Ns] = o[r];
+            s += ds;
+            j += dj;
+            if (j >= nb) { j -= nb; ++s; }
         }
     }
     __syncthreads();
@@ -164,15 +195,15 @@ __device__ void stage_inplace(cplx *buf, const cplx *tw, int N, int Ns, int nseq
 
 // full in-place FFT of nseq sequences; tw holds the stages' twiddle rows one
 // after the other ((R-1) Ns entries per stage, N-1 in all)
-template <int NT>
-__device__ void fft_inplace(cplx *buf, const cplx *tw, const FftArgs &a, int nseq, double sgn)
+template <int NT, int SGN>
+__device__ void fft_inplace(cplx *buf, const cplx *tw, const FftArgs &a, int nseq)
 {
     int Ns = 1, off = 0;
     for (int st = 0; st < a.nstage; ++st) {
         const int R = a.radix[st];
-        if (R == 4) stage_inplace<4, NT>(buf, tw + off, a.N, Ns, nseq, sgn);
-        else if (R == 2) stage_inplace<2, NT>(buf, tw + off, a.N, Ns, nseq, sgn);
-        else stage_inplace<3, NT>(buf, tw + off, a.N, Ns, nseq, sgn);
+        if (R == 4) stage_inplace<4, NT, SGN>(buf, tw + off, a.N, Ns, nseq);
+        else if (R == 2) stage_inplace<2, NT, SGN>(buf, tw + off, a.N, Ns, nseq);
+        else stage_inplace<3, NT, SGN>(buf, tw + off, a.N, Ns, nseq);
         off += (R - 1) * Ns;
         Ns *= R;
     }
@@ -273,14 +304,13 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
         // registers across the pencil loop
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
+        DivStep qp(tid, NT, nc0);   // e = mp nc0 + nd
 #pragma unroll
-        for (int q = 0; q < kFftPre; ++q) {
+        for (int q = 0; q < kFftPre; ++q, qp.next()) {
             const int e = tid + q * NT;
             pre[q] = 0.0;
-            if (pc < npencil && e < nc0 * M) {
-                const int mp = qdiv(e, nc0, 1.0f / (float)nc0), nd = e - mp * nc0;
-                pre[q] = a.in[nd + (size_t)nin * (pc + sl * mp)];
-            }
+            if (pc < npencil && e < nc0 * M)
+                pre[q] = a.in[qp.lo + (size_t)nin * (pc + sl * qp.hi)];
         }
     };
     // pencils of round i (offsets i G + o): chunks of 4 consecutive pencils
@@ -316,23 +346,25 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
         }
         prefetch(pencil_of(it + 1));
         __syncthreads();
-        const float rN = 1.0f / (float)N;
-        for (int e = threadIdx.x; e < 7 * N; e += NT) {
-            const int p = qdiv(e, N, rN), kf = e - p * N;
-            buf[e] = herm_in(spec + (size_t)2 * p * M, p < 6 ? spec + (size_t)(2 * p + 1) * M : nullptr,
-                             kf, N, L);
+        {
+            DivStep q(threadIdx.x, NT, N);   // e = p N + kf
+            for (int e = threadIdx.x; e < 7 * N; e += NT, q.next()) {
+                const int p = q.hi;
+                buf[e] = herm_in(spec + (size_t)2 * p * M,
+                                 p < 6 ? spec + (size_t)(2 * p + 1) * M : nullptr, q.lo, N, L);
+            }
         }
         __syncthreads();
-        fft_inplace<NT>(buf, tw, a, 7, +1.0);
+        fft_inplace<NT, +1>(buf, tw, a, 7);
         for (int i = threadIdx.x; i < N; i += NT) mhd_products(buf, N, i, a.coef);
         __syncthreads();
-        fft_inplace<NT>(buf, tw, a, 6, -1.0);
+        fft_inplace<NT, -1>(buf, tw, a, 6);
         // two real spectra per pair: X_a = (Z_k + conj Z_{N-k})/2,
         // X_b = (Z_k - conj Z_{N-k})/(2i), scaled as FFTPACK's forward
         const double invN = 1.0 / (double)N;
-        const float rL1 = 1.0f / (float)(L + 1);
-        for (int e = threadIdx.x; e < 6 * (L + 1); e += NT) {
-            const int p = qdiv(e, L + 1, rL1), m = e - p * (L + 1);
+        DivStep qf(threadIdx.x, NT, L + 1);   // e = p (L+1) + m
+        for (int e = threadIdx.x; e < 6 * (L + 1); e += NT, qf.next()) {
+            const int p = qf.hi, m = qf.lo;
             const cplx zk = buf[(size_t)p * N + m];
             const cplx zn = buf[(size_t)p * N + (m == 0 ? 0 : N - m)];
             const double xar = 0.5 * (zk.x + zn.x), xai = 0.5 * (zk.y - zn.y);
@@ -351,9 +383,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
         __syncthreads();
         // 19 outputs per order; the slots the products do not fill (3-8, 18) are zero
         const int nout = a.ncomp_out;
-        const float rno = 1.0f / (float)nout;
-        for (int e = threadIdx.x; e < nout * M; e += NT) {
-            const int mp = qdiv(e, nout, rno), nd = e - mp * nout;
+        DivStep qo(threadIdx.x, NT, nout);   // e = mp nout + nd
+        for (int e = threadIdx.x; e < nout * M; e += NT, qo.next()) {
+            const int mp = qo.hi, nd = qo.lo;
             const int q = nd < 3 ? nd : (nd >= 9 && nd < 18 ? nd - 6 : -1);
             a.out[nd + (size_t)nout * (pencil + sl * mp)] = (q >= 0) ? spec[(size_t)q * M + mp] : 0.0;
         }
@@ -424,7 +456,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
                 }
             }
             __syncthreads();
-            fft_inplace<NT>(buf, tw, a, np, +1.0);
+            fft_inplace<NT, +1>(buf, tw, a, np);
             if (MODE == kPencilBwd) {
                 for (int e = threadIdx.x; e < np * N; e += NT) {
                     const int p = e / N, i = e % N;
@@ -463,7 +495,7 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
             }
             __syncthreads();
         }
-        fft_inplace<NT>(buf, tw, a, np, -1.0);
+        fft_inplace<NT, -1>(buf, tw, a, np);
         // unpack two real spectra per pair, pair by pair: the spectra of pair
         // p (rows a, b of M doubles) go to the start of its own sequence,
         // doubles [2pN, 2pN + 2M), which no later pair reads
