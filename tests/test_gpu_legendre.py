@@ -225,3 +225,52 @@ def test_every_tile_plan_vs_oracle(torch_cuda, mode, L, nth, nr, nv, ns):
                 field_err(run_dev(torch_cuda, lt, sp=sp), ref_b, nc)
             assert e < tol, (fwd, idx, info, e)
     lt.close()
+
+
+def test_same_shape_contexts_bitwise(torch_cuda):
+    """Two contexts of one shape take the same (cached) autotuned plans, so
+    their transforms agree bitwise."""
+    L, nth, nr, nv, ns = 47, 72, 6, 4, 1
+    nc = 3 * nv + ns
+    radius = np.linspace(7.0 / 13.0, 20.0 / 13.0, nr)
+    rng = np.random.default_rng(5)
+    grid = oracle.Grid(L, nth, radius)
+    sp = rng.standard_normal(grid.n_rlm(nc))
+    outs = []
+    for _ in range(2):
+        lt = calypso_amd.LegendreTransform(L, nth, radius, nc, nv, ns, "hip_fp64")
+        vr = run_dev(torch_cuda, lt, sp=sp)
+        outs.append((vr, run_dev(torch_cuda, lt, vr=vr), lt.plan_info(False), lt.plan_info(True)))
+        lt.close()
+    assert outs[0][2] == outs[1][2] and outs[0][3] == outs[1][3]
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_int8x6_extreme_component_scales(torch_cuda):
+    """int8-slice mode on components that are zero, ~1e-200 and ~1e+100 (the
+    per-column exponents and the exponent floor): each field within 1e-11 of
+    the oracle, the zero field exactly zero."""
+    L, nth, nr, nv, ns = 31, 48, 3, 1, 3
+    nc = 3 * nv + ns
+    radius = np.linspace(7.0 / 13.0, 20.0 / 13.0, nr)
+    rng = np.random.default_rng(11)
+    grid = oracle.Grid(L, nth, radius)
+    l = grid.jlm[:, 1].astype(float)
+    scale = np.array([1.0, 1e-200, 1e100, 1.0, 0.0, 3e-7])
+    sp = (rng.standard_normal((nr, grid.J, nc)) * ((l + 1.0) ** -1.5)[None, :, None]
+          * scale[None, None, :]).ravel()
+    vr = (rng.standard_normal((grid.n_rtm(nc) // nc, nc)) * scale[None, :]).ravel()
+    lt = calypso_amd.LegendreTransform(L, nth, radius, nc, nv, ns, "hip_int8x6")
+    got_b = run_dev(torch_cuda, lt, sp=sp)
+    got_f = run_dev(torch_cuda, lt, vr=vr)
+    lt.close()
+    ref_b = oracle.leg_bwd(grid, nc, nv, ns, sp)
+    ref_f = oracle.leg_fwd(grid, nc, nv, ns, vr)
+    for got, ref in ((got_b, ref_b), (got_f, ref_f)):
+        g = got.reshape(-1, nc)
+        r = ref.reshape(-1, nc)
+        assert np.all(g[:, 4] == 0.0)
+        for c in (0, 1, 2, 3, 5):
+            e = np.max(np.abs(g[:, c] - r[:, c])) / np.max(np.abs(r[:, c]))
+            print(f"   comp {c} scale {scale[c]:.0e}: {e:.2e}", flush=True)
+            assert e < 1e-11, (c, e)
