@@ -58,6 +58,10 @@ SIGNATURES = [
     ("calypso_mhd_energies", _I, [_P, _P]),
     ("calypso_mhd_info", _I, [_P, _P]),
     ("calypso_mhd_set_option", _I, [_P, ctypes.c_char_p, ctypes.c_longlong]),
+    ("calypso_mhd_time", _I, [_P, _P]),
+    ("calypso_mhd_write_volume_ms", _I, [_P, ctypes.c_char_p]),
+    ("calypso_mhd_write_restart", _I, [_P, ctypes.c_char_p, _I, ctypes.c_char_p, _I]),
+    ("calypso_mhd_read_restart", _I, [_P, ctypes.c_char_p]),
     ("calypso_mhd_coriolis_table", _I, [_I, _P, _P]),
 ]
 

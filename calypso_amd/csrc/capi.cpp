@@ -509,6 +509,43 @@ int calypso_mhd_info(calypso_mhd_t *mhd, long long *info)
     return 0;
 }
 
+int calypso_mhd_time(calypso_mhd_t *mhd, double *time)
+{
+    if (!mhd || !time) return fail("calypso_mhd_time: null argument");
+    *time = calypso::mhd_time(&mhd->c);
+    return 0;
+}
+
+int calypso_mhd_write_volume_ms(calypso_mhd_t *mhd, const char *path)
+{
+    if (!mhd || !path) return fail("calypso_mhd_write_volume_ms: null argument");
+    std::string err;
+    if (calypso::mhd_write_volume_ms(&mhd->c, path, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_write_restart(calypso_mhd_t *mhd, const char *prefix, int istep_fld,
+                              char *name_out, int name_len)
+{
+    if (!mhd || !prefix) return fail("calypso_mhd_write_restart: null argument");
+    const std::string name = calypso::mhd_restart_name(prefix, istep_fld);
+    if (name_out && name_len > 0) {
+        std::strncpy(name_out, name.c_str(), (size_t)name_len - 1);
+        name_out[name_len - 1] = 0;
+    }
+    std::string err;
+    if (calypso::mhd_write_restart(&mhd->c, name, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_read_restart(calypso_mhd_t *mhd, const char *path)
+{
+    if (!mhd || !path) return fail("calypso_mhd_read_restart: null argument");
+    std::string err;
+    if (calypso::mhd_read_restart(&mhd->c, path, &err)) return fail(err);
+    return 0;
+}
+
 int calypso_mhd_set_option(calypso_mhd_t *mhd, const char *name, long long value)
 {
     if (!mhd || !name) return fail("calypso_mhd_set_option: null argument");

@@ -1204,7 +1204,12 @@ int mhd_initialize(MhdContext *c, std::string *err)
     if (derived_fields(c, err)) return -1;
     if (nonlinear(c, false, err)) return -1;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    c->i_step = 0;
+    // a restart resumes its step count and time (the next step is then an
+    // Adams-Bashforth step with the restart's previous explicit terms)
+    c->i_step = c->restart_step >= 0 ? c->restart_step : 0;
+    c->istep0 = c->i_step;
+    c->time0 = c->restart_step >= 0 ? c->restart_time : 0.0;
+    c->restart_step = -1;
     c->initialized = true;
     return 0;
 }

@@ -60,6 +60,10 @@ struct MhdContext {
     int leg_mode = kF64;
     int i_step = 0;
     bool initialized = false;
+    int istep0 = 0;               // step and time the run started from (restart)
+    double time0 = 0.0;
+    int restart_step = -1;        // set by a restart read, taken by the initialisation
+    double restart_time = 0.0;
     MhdCoefs co;
     std::vector<double> radius;
     LegContext leg;
@@ -93,6 +97,12 @@ int  mhd_steps(MhdContext *c, int nsteps, std::string *err);
 int  mhd_energies(MhdContext *c, double *out7, std::string *err);
 int  mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err);
 int  mhd_set_option(MhdContext *c, const std::string &name, long long value, std::string *err);
+// output files in the reference formats (sph_io.cpp)
+double mhd_time(const MhdContext *c);
+int  mhd_write_volume_ms(MhdContext *c, const std::string &path, std::string *err);
+std::string mhd_restart_name(const std::string &prefix, int istep_fld);
+int  mhd_write_restart(MhdContext *c, const std::string &path, std::string *err);
+int  mhd_read_restart(MhdContext *c, const std::string &path, std::string *err);
 
 // closed forms of the Coriolis couplings (exposed for the host tests)
 double gaunt_p1(int l, int m);   // K(1,0; l-1,m; l,m)

@@ -128,6 +128,37 @@
           real(c_double), intent(inout) :: ene(7)
         end function calypso_mhd_energies
 !
+        integer(c_int) function calypso_mhd_time(mhd, time)            &
+     &      bind(C, name="calypso_mhd_time")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: mhd
+          real(c_double), intent(inout) :: time
+        end function calypso_mhd_time
+!
+        integer(c_int) function calypso_mhd_write_volume_ms(mhd, path)  &
+     &      bind(C, name="calypso_mhd_write_volume_ms")
+          import :: c_int, c_char, c_ptr
+          type(c_ptr), value :: mhd
+          character(kind=c_char), intent(in) :: path(*)
+        end function calypso_mhd_write_volume_ms
+!
+        integer(c_int) function calypso_mhd_write_restart(mhd, prefix,  &
+     &      istep_fld, name_out, name_len)                              &
+     &      bind(C, name="calypso_mhd_write_restart")
+          import :: c_int, c_char, c_ptr
+          type(c_ptr), value :: mhd
+          character(kind=c_char), intent(in) :: prefix(*)
+          integer(c_int), value :: istep_fld, name_len
+          character(kind=c_char), intent(inout) :: name_out(*)
+        end function calypso_mhd_write_restart
+!
+        integer(c_int) function calypso_mhd_read_restart(mhd, path)     &
+     &      bind(C, name="calypso_mhd_read_restart")
+          import :: c_int, c_char, c_ptr
+          type(c_ptr), value :: mhd
+          character(kind=c_char), intent(in) :: path(*)
+        end function calypso_mhd_read_restart
+!
       end interface
 !
       end module calypso_amd_capi

@@ -143,6 +143,32 @@ class MhdSimulation:
         return out
 
     @property
+    def time(self) -> float:
+        t = ctypes.c_double()
+        _lib.check(_lib.load().calypso_mhd_time(self._h, ctypes.byref(t)), "calypso_mhd_time")
+        return t.value
+
+    def write_volume_ms(self, path):
+        """Append this step's energies to sph_pwr_volume_s.dat (created with
+        the reference's header if absent)."""
+        _lib.check(_lib.load().calypso_mhd_write_volume_ms(self._h, str(path).encode()),
+                   "calypso_mhd_write_volume_ms")
+
+    def write_restart(self, prefix, istep_fld) -> str:
+        """Write <prefix>.<istep_fld>.0.fst in the reference's ASCII restart
+        format; returns the file name."""
+        buf = ctypes.create_string_buffer(4096)
+        _lib.check(_lib.load().calypso_mhd_write_restart(self._h, str(prefix).encode(),
+                                                         int(istep_fld), buf, 4096),
+                   "calypso_mhd_write_restart")
+        return buf.value.decode()
+
+    def read_restart(self, path):
+        """Read a restart file; initialize() then resumes at its step and time."""
+        _lib.check(_lib.load().calypso_mhd_read_restart(self._h, str(path).encode()),
+                   "calypso_mhd_read_restart")
+
+    @property
     def i_step(self) -> int:
         a = np.zeros(4, dtype=np.int64)
         _lib.check(_lib.load().calypso_mhd_info(self._h, a.ctypes.data_as(_P)), "calypso_mhd_info")

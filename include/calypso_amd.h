@@ -251,6 +251,24 @@ int calypso_mhd_step(calypso_mhd_t *mhd, int nsteps);
 int calypso_mhd_energies(calypso_mhd_t *mhd, double *out);
 /* info[4]: i_step, nr, J, number of d_rj components. */
 int calypso_mhd_info(calypso_mhd_t *mhd, long long *info);
+/* Model time of the current step (time of the start or restart + steps * dt). */
+int calypso_mhd_time(calypso_mhd_t *mhd, double *time);
+/* Appends the current step's row to the volume mean square file
+ * (sph_pwr_volume_s.dat), creating it with its header first
+ * (write_sph_vol_ms_file, output_sph_m_square_file.f90:134-156;
+ * write_sph_mean_sq_header, m_rms_4_sph_spectr.f90:245-280). */
+int calypso_mhd_write_volume_ms(calypso_mhd_t *mhd, const char *path);
+/* Writes the ASCII restart file <prefix>.<istep_fld>.0.fst (velocity,
+ * temperature, magnetic_field, previous_momentum, previous_induction,
+ * previous_heat; sel_write_step_SPH_field_file, field_IO_select.F90:106-122,
+ * write_step_field_file, field_file_IO.f90:40-62); its name is copied to
+ * name_out[name_len] when given. */
+int calypso_mhd_write_restart(calypso_mhd_t *mhd, const char *prefix, int istep_fld,
+                              char *name_out, int name_len);
+/* Reads a restart file (domain 0) into the state; calypso_mhd_initialize then
+ * resumes at its step and time (sel_read_alloc_step_SPH_file and
+ * set_sph_restart_from_IO, sph_mhd_rst_IO_control.f90:125-140). */
+int calypso_mhd_read_restart(calypso_mhd_t *mhd, const char *path);
 /* Execution options: "fused_fft" (1: FFT, products and FFT in one pencil
  * kernel, the default where the pencil fits in LDS; 0: three kernels with
  * the grids in HBM), "legendre_autotune" (0/1). */
