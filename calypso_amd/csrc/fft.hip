@@ -381,12 +381,13 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
             }
         }
         __syncthreads();
-        // 19 outputs per order; the slots the products do not fill (3-8, 18) are zero
+        // 19 outputs per order (the reference's slots; 3-8 and 18, which the
+        // products do not fill, are zero) or the 12 products only
         const int nout = a.ncomp_out;
         DivStep qo(threadIdx.x, NT, nout);   // e = mp nout + nd
         for (int e = threadIdx.x; e < nout * M; e += NT, qo.next()) {
             const int mp = qo.hi, nd = qo.lo;
-            const int q = nd < 3 ? nd : (nd >= 9 && nd < 18 ? nd - 6 : -1);
+            const int q = nout == 12 ? nd : (nd < 3 ? nd : (nd >= 9 && nd < 18 ? nd - 6 : -1));
             a.out[nd + (size_t)nout * (pencil + sl * mp)] = (q >= 0) ? spec[(size_t)q * M + mp] : 0.0;
         }
         continue;
@@ -551,10 +552,13 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
 }
 
 // pointwise products on the reference grid layout fld(nnod, 13) -> frc(nnod, 19)
+// nout = 19: the reference's f_trns slots (0-2, 9-17; 3-8 and 18 zero);
+// nout = 12: the twelve products only (advection, Lorentz, induction, heat flux)
 __global__ void k_products_mhd(size_t nnod, const double *fld, double *frc, double cv,
-                               double cl, double ci, double ct)
+                               double cl, double ci, double ct, int nout)
 {
 #pragma clang fp contract(off)
+    const int o3 = nout == 12 ? 3 : 9;   // first slot after the advection
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nnod;
          i += (size_t)gridDim.x * blockDim.x) {
         const double u1 = fld[i], u2 = fld[nnod + i], u3 = fld[2 * nnod + i];
@@ -565,17 +569,18 @@ __global__ void k_products_mhd(size_t nnod, const double *fld, double *frc, doub
         frc[i] = (w2 * u3 - w3 * u2) * cv;
         frc[nnod + i] = (w3 * u1 - w1 * u3) * cv;
         frc[2 * nnod + i] = (w1 * u2 - w2 * u1) * cv;
-        for (int c = 3; c < 9; ++c) frc[c * nnod + i] = 0.0;
-        frc[9 * nnod + i] = (j2 * b3 - j3 * b2) * cl;
-        frc[10 * nnod + i] = (j3 * b1 - j1 * b3) * cl;
-        frc[11 * nnod + i] = (j1 * b2 - j2 * b1) * cl;
-        frc[12 * nnod + i] = (u2 * b3 - u3 * b2) * ci;
-        frc[13 * nnod + i] = (u3 * b1 - u1 * b3) * ci;
-        frc[14 * nnod + i] = (u1 * b2 - u2 * b1) * ci;
-        frc[15 * nnod + i] = u1 * T * ct;
-        frc[16 * nnod + i] = u2 * T * ct;
-        frc[17 * nnod + i] = u3 * T * ct;
-        frc[18 * nnod + i] = 0.0;
+        if (nout == 19)
+            for (int c = 3; c < 9; ++c) frc[c * nnod + i] = 0.0;
+        frc[(o3 + 0) * nnod + i] = (j2 * b3 - j3 * b2) * cl;
+        frc[(o3 + 1) * nnod + i] = (j3 * b1 - j1 * b3) * cl;
+        frc[(o3 + 2) * nnod + i] = (j1 * b2 - j2 * b1) * cl;
+        frc[(o3 + 3) * nnod + i] = (u2 * b3 - u3 * b2) * ci;
+        frc[(o3 + 4) * nnod + i] = (u3 * b1 - u1 * b3) * ci;
+        frc[(o3 + 5) * nnod + i] = (u1 * b2 - u2 * b1) * ci;
+        frc[(o3 + 6) * nnod + i] = u1 * T * ct;
+        frc[(o3 + 7) * nnod + i] = u2 * T * ct;
+        frc[(o3 + 8) * nnod + i] = u3 * T * ct;
+        if (nout == 19) frc[18 * nnod + i] = 0.0;
     }
 }
 
@@ -740,17 +745,19 @@ bool fft_mhd_fused_fits(const FftContext *f)
 }
 
 int fft_mhd_fused_dev(const FftContext *f, const double *d_bwd_rtm, double *d_fwd_rtm,
-                      const double *coef, hipStream_t s, std::string *err)
+                      const double *coef, hipStream_t s, std::string *err, int nout)
 {
-    return launch_pencil(f, kPencilMhd, 13, 19, d_bwd_rtm, d_fwd_rtm, coef, s, err);
+    if (nout != 19 && nout != 12) { *err = "fused pencil: 19 (reference slots) or 12 outputs"; return -1; }
+    return launch_pencil(f, kPencilMhd, 13, nout, d_bwd_rtm, d_fwd_rtm, coef, s, err);
 }
 
 int products_mhd_dev(size_t nnod, const double *d_fld, double *d_frc, const double *coef,
-                     hipStream_t s, std::string *err)
+                     hipStream_t s, std::string *err, int nout)
 {
+    if (nout != 19 && nout != 12) { *err = "products: 19 (reference slots) or 12 outputs"; return -1; }
     const int blocks = (int)std::min<size_t>((nnod + 255) / 256, 65536);
     hipLaunchKernelGGL(k_products_mhd, dim3(blocks), dim3(256), 0, s, nnod, d_fld, d_frc, coef[0],
-                       coef[1], coef[2], coef[3]);
+                       coef[1], coef[2], coef[3], nout);
     HIP_TRY(hipGetLastError());
     return 0;
 }

@@ -25,9 +25,11 @@ int  fft_backward_dev(const FftContext *f, int ncomp, const double *d_vr_rtm,
 int  fft_forward_dev(const FftContext *f, int ncomp, const double *d_frc_rtp,
                      double *d_vr_rtm, hipStream_t s, std::string *err);
 bool fft_mhd_fused_fits(const FftContext *f);   // 13 -> 19 pencil fits in LDS
+// nout: 19 = the reference's forward slots (zeros where the products leave
+// them), 12 = the products only (advection, Lorentz, induction, heat flux)
 int  fft_mhd_fused_dev(const FftContext *f, const double *d_bwd_rtm, double *d_fwd_rtm,
-                       const double *coef, hipStream_t s, std::string *err);
+                       const double *coef, hipStream_t s, std::string *err, int nout = 19);
 int  products_mhd_dev(size_t nnod, const double *d_fld, double *d_frc, const double *coef,
-                      hipStream_t s, std::string *err);
+                      hipStream_t s, std::string *err, int nout = 19);
 
 }  // namespace calypso

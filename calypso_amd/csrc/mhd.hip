@@ -317,6 +317,8 @@ __global__ void k_temp_bc(MhdDev p)
 // degree are contiguous; in rlm the degrees of one order are contiguous with
 // the node's components contiguous, so both sides move in contiguous runs.
 constexpr int kTile = 16;
+// components of the step's forward transform: the twelve grid products
+constexpr int kNFwd = 12;
 
 // tile origin (l0, m0) of block (blockIdx.x, blockIdx.y); false if the tile
 // holds no mode (l < |m| everywhere)
@@ -368,13 +370,14 @@ __global__ __launch_bounds__(256) void k_pack(MhdDev p, int kz)
     }
 }
 
-// Unpack the forward transform into the rj fields: the 13 components the step
-// uses (advection 0-2, the poloidal Coriolis slot 7, Lorentz 9-11, induction
-// 12-14, heat flux 15-17) go through an LDS tile; kz radial levels per
-// work-group as k_pack.
+// Unpack the forward transform into the rj fields: the step transforms the
+// twelve products only (advection 0-2, Lorentz 3-5, induction 6-8, heat
+// flux 9-11; the reference's other forward slots are zero, so the poloidal
+// Coriolis slot it unpacks is written as zero) through an LDS tile; kz radial
+// levels per work-group as k_pack.
 __global__ __launch_bounds__(256) void k_unpack(MhdDev p, int kz)
 {
-    constexpr int NU = 13;
+    constexpr int NU = kNFwd;
     constexpr int PER = (kTile * kTile * NU + 255) / 256;   // gathered values per thread
     __shared__ double tile[NU][kTile][kTile + 1];
     __shared__ int jls[kTile][kTile];
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(256) void k_unpack(MhdDev p, int kz)
         jls[ty][tx] = ok ? p.rlm_of_rj[l * (l + 1) + m] : -1;
     }
     __syncthreads();
-    // this thread's gathered values: rlm node offset (or -1), component
+    // this thread's gathered values: rlm offset of (node, component), or -1
     int off[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -397,14 +400,13 @@ __global__ __launch_bounds__(256) void k_unpack(MhdDev p, int kz)
         if (e < kTile * kTile * NU) {
             const int col = e / (kTile * NU), r = e % (kTile * NU);
             const int node = r / NU, u = r % NU;
-            const int c = u < 3 ? u : (u == 3 ? 7 : u + 5);   // 0-2, 7, 9-17
             const int jl = jls[node][col];
-            if (jl >= 0) off[q] = 19 * jl + c;
+            if (jl >= 0) off[q] = NU * jl + u;
         }
     }
     double g[PER];
     auto load = [&](int k) {
-        const size_t base = (size_t)19 * p.J * k;
+        const size_t base = (size_t)NU * p.J * k;
 #pragma unroll
         for (int q = 0; q < PER; ++q) g[q] = off[q] >= 0 ? p.sp_f[base + off[q]] : 0.0;
     };
@@ -427,11 +429,11 @@ __global__ __launch_bounds__(256) void k_unpack(MhdDev p, int kz)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 FLD(kMAdvect + c)[i] = tile[c][ty][tx];
-                FLD(kLorentz + c)[i] = tile[4 + c][ty][tx];
-                FLD(kVpInduct + c)[i] = tile[7 + c][ty][tx];
-                FLD(kHFlux + c)[i] = tile[10 + c][ty][tx];
+                FLD(kLorentz + c)[i] = tile[3 + c][ty][tx];
+                FLD(kVpInduct + c)[i] = tile[6 + c][ty][tx];
+                FLD(kHFlux + c)[i] = tile[9 + c][ty][tx];
             }
-            FLD(kRotCoriolis + 1)[i] = tile[3][ty][tx];
+            FLD(kRotCoriolis + 1)[i] = 0.0;
         }
         __syncthreads();
     }
@@ -835,9 +837,9 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     HIP_TRY(hipMalloc(&c->d_rj, sizeof(double) * kMhdNumComp * nnod));
     HIP_TRY(hipMemset(c->d_rj, 0, sizeof(double) * kMhdNumComp * nnod));
     HIP_TRY(hipMalloc(&c->d_sp_b, sizeof(double) * 13 * nnod));
-    HIP_TRY(hipMalloc(&c->d_sp_f, sizeof(double) * 19 * nnod));
+    HIP_TRY(hipMalloc(&c->d_sp_f, sizeof(double) * kNFwd * nnod));
     HIP_TRY(hipMalloc(&c->d_vr_b, sizeof(double) * 13 * nrtm));
-    HIP_TRY(hipMalloc(&c->d_vr_f, sizeof(double) * 19 * nrtm));
+    HIP_TRY(hipMalloc(&c->d_vr_f, sizeof(double) * kNFwd * nrtm));
     HIP_TRY(hipMalloc(&c->d_scratch, sizeof(double) * (nnod + 3 * (size_t)J)));
     HIP_TRY(hipMalloc(&c->d_ms_out, sizeof(double) * 4));
     p.d = c->d_rj;
@@ -1010,14 +1012,14 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     (void)hipFree(d_a);
     (void)hipFree(d_ierr);
     // transforms
-    if (leg_context_build(&c->leg, L, nth, nr, radius, nullptr, nullptr, 19, 6, 1, leg_mode, err))
+    if (leg_context_build(&c->leg, L, nth, nr, radius, nullptr, nullptr, kNFwd, 4, 0, leg_mode, err))
         return -1;
     if (fft_context_build(&c->fft, L, nth, nr, nphi, err)) return -1;
     c->fused_fft = fft_mhd_fused_fits(&c->fft);
     if (!c->fused_fft) {   // long pencils: grids in HBM between the three stages
         const size_t nrtp = (size_t)nr * nth * nphi;
         HIP_TRY(hipMalloc(&c->d_grid_b, sizeof(double) * 13 * nrtp));
-        HIP_TRY(hipMalloc(&c->d_grid_f, sizeof(double) * 19 * nrtp));
+        HIP_TRY(hipMalloc(&c->d_grid_f, sizeof(double) * kNFwd * nrtp));
     }
     HIP_TRY(hipDeviceSynchronize());
     return 0;
@@ -1055,7 +1057,7 @@ int mhd_set_option(MhdContext *c, const std::string &name, long long value, std:
         if (!value && !c->d_grid_b) {
             const size_t nrtp = (size_t)c->nr * c->nth * c->nphi;
             HIP_TRY(hipMalloc(&c->d_grid_b, sizeof(double) * 13 * nrtp));
-            HIP_TRY(hipMalloc(&c->d_grid_f, sizeof(double) * 19 * nrtp));
+            HIP_TRY(hipMalloc(&c->d_grid_f, sizeof(double) * kNFwd * nrtp));
         }
         c->fused_fft = value != 0;
         drop_graph(c);
@@ -1152,15 +1154,15 @@ int nonlinear(MhdContext *c, bool temp_roundtrip, std::string *err)
     if (leg_backward_dev(&c->leg, 13, 4, 1, c->d_sp_b, c->d_vr_b, s, err)) return -1;
     const double coef[4] = {c->co.coef_velo, c->co.coef_lor, c->co.coef_induct, c->co.coef_temp};
     if (c->fused_fft) {
-        if (fft_mhd_fused_dev(&c->fft, c->d_vr_b, c->d_vr_f, coef, s, err)) return -1;
+        if (fft_mhd_fused_dev(&c->fft, c->d_vr_b, c->d_vr_f, coef, s, err, kNFwd)) return -1;
     } else {
         const size_t nrtp = (size_t)c->nr * c->nth * c->nphi;
         if (fft_backward_dev(&c->fft, 13, c->d_vr_b, c->d_grid_b, s, err) ||
-            products_mhd_dev(nrtp, c->d_grid_b, c->d_grid_f, coef, s, err) ||
-            fft_forward_dev(&c->fft, 19, c->d_grid_f, c->d_vr_f, s, err))
+            products_mhd_dev(nrtp, c->d_grid_b, c->d_grid_f, coef, s, err, kNFwd) ||
+            fft_forward_dev(&c->fft, kNFwd, c->d_grid_f, c->d_vr_f, s, err))
             return -1;
     }
-    if (leg_forward_dev(&c->leg, 19, 6, 1, c->d_vr_f, c->d_sp_f, s, err)) return -1;
+    if (leg_forward_dev(&c->leg, kNFwd, 4, 0, c->d_vr_f, c->d_sp_f, s, err)) return -1;
     hipLaunchKernelGGL(k_unpack, tiles, dim3(256), 0, s, p, kz);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_coriolis, dim3(nb), dim3(256), 0, s, p);
