@@ -292,6 +292,9 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
     const int npencil = a.nth * a.nr;
     const int nin = a.ncomp_in;
     const int npair_in = (nin + 1) / 2;
+    // rtm words of one order across every pencil: the host keeps whole rtm
+    // arrays below 2^32 words, so offsets within them are 32-bit
+    const unsigned nsl = (unsigned)nin * (unsigned)sl;
 
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = {a.tw[2 * i], a.tw[2 * i + 1]};
 
@@ -304,13 +307,21 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
         // registers across the pencil loop
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        DivStep qp(tid, NT, nc0);   // e = mp nc0 + nd
+        // e = mp nc0 + nd sits at nd + nin (pc + sl mp): a 32-bit offset from
+        // the pencil's first word, stepped by the quotient's carries (no
+        // multiplies per load)
+        DivStep qp(tid, NT, nc0);
+        unsigned off = (unsigned)qp.lo + nsl * (unsigned)qp.hi;
+        const unsigned doff = (unsigned)qp.dlo + nsl * (unsigned)qp.dhi;
+        const double *src = a.in + (size_t)nin * (pc < npencil ? pc : 0);
 #pragma unroll
-        for (int q = 0; q < kFftPre; ++q, qp.next()) {
+        for (int q = 0; q < kFftPre; ++q) {
             const int e = tid + q * NT;
             pre[q] = 0.0;
-            if (pc < npencil && e < nc0 * M)
-                pre[q] = a.in[qp.lo + (size_t)nin * (pc + sl * qp.hi)];
+            if (pc < npencil && e < nc0 * M) pre[q] = src[off];
+            off += doff;
+            qp.lo += qp.dlo;
+            if (qp.lo >= nc0) { qp.lo -= nc0; off += nsl - (unsigned)nc0; }
         }
     };
     // pencils of round i (offsets i G + o): chunks of 4 consecutive pencils
@@ -338,10 +349,17 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
         {
             int tid = threadIdx.x;
             asm volatile("" : "+v"(tid));
+            // e = mp 13 + nd goes to row nd, column mp: index stepped by carries
+            DivStep qs(tid, NT, 13);
+            int dst = qs.lo * M + qs.hi;
+            const int ddst = qs.dlo * M + qs.dhi;
 #pragma unroll
             for (int q = 0; q < kFftPre; ++q) {
                 const int e = tid + q * NT;
-                if (e < 13 * M) spec[(e % 13) * M + e / 13] = pre[q];
+                if (e < 13 * M) spec[dst] = pre[q];
+                dst += ddst;
+                qs.lo += qs.dlo;
+                if (qs.lo >= 13) { qs.lo -= 13; dst += 1 - 13 * M; }
             }
         }
         prefetch(pencil_of(it + 1));
@@ -383,12 +401,26 @@ __global__ __launch_bounds__(NT, NT == 256 ? 2 : 1) void k_pencil(FftArgs a)
         __syncthreads();
         // 19 outputs per order (the reference's slots; 3-8 and 18, which the
         // products do not fill, are zero) or the 12 products only
+        // e = mp nout + nd: output offset nd + nout sl mp from the pencil's
+        // first word and spectrum index nd M + mp, both stepped by carries
         const int nout = a.ncomp_out;
-        DivStep qo(threadIdx.x, NT, nout);   // e = mp nout + nd
-        for (int e = threadIdx.x; e < nout * M; e += NT, qo.next()) {
-            const int mp = qo.hi, nd = qo.lo;
-            const int q = nout == 12 ? nd : (nd < 3 ? nd : (nd >= 9 && nd < 18 ? nd - 6 : -1));
-            a.out[nd + (size_t)nout * (pencil + sl * mp)] = (q >= 0) ? spec[(size_t)q * M + mp] : 0.0;
+        const unsigned nslo = (unsigned)nout * (unsigned)sl;
+        double *dstp = a.out + (size_t)nout * pencil;
+        DivStep qo(threadIdx.x, NT, nout);
+        unsigned oo = (unsigned)qo.lo + nslo * (unsigned)qo.hi;
+        const unsigned doo = (unsigned)qo.dlo + nslo * (unsigned)qo.dhi;
+        int si = qo.lo * M + qo.hi;
+        const int dsi = qo.dlo * M + qo.dhi;
+        for (int e = threadIdx.x; e < nout * M; e += NT) {
+            const int nd = qo.lo;
+            double v;
+            if (nout == 12) v = spec[si];
+            else v = nd < 3 ? spec[si] : (nd >= 9 && nd < 18 ? spec[si - 6 * M] : 0.0);
+            dstp[oo] = v;
+            oo += doo;
+            si += dsi;
+            qo.lo += qo.dlo;
+            if (qo.lo >= nout) { qo.lo -= nout; oo += nslo - (unsigned)nout; si += 1 - nout * M; }
         }
         continue;
     }
@@ -718,6 +750,13 @@ static int launch_pencil(const FftContext *f, int mode, int nin, int nout, const
     if (!pb) {
         *err = mode == kPencilMhd ? "pencil too long for the fused MHD kernel"
                                   : "pencil too long for LDS";
+        return -1;
+    }
+    // the kernels index an rtm array with 32-bit offsets
+    const size_t rtm_words = (size_t)f->nth * f->nr * (2 * f->L + 1);
+    if ((mode != kPencilFwd && (size_t)nin * rtm_words > 0xFFFFFFFFull) ||
+        (mode == kPencilMhd && (size_t)nout * rtm_words > 0xFFFFFFFFull)) {
+        *err = "rtm array of 2^32 words or more";
         return -1;
     }
     a.pairs_per_batch = pb;

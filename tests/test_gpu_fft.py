@@ -71,7 +71,8 @@ def test_products_bitwise(torch_cuda):
     assert np.array_equal(out.cpu().numpy(), ref)
 
 
-@pytest.mark.parametrize("L,nth,nr,nphi", [(31, 48, 3, 96), (15, 24, 2, 48)])
+# L=255 / nphi=768: the 512-thread work-group with 151 KB of LDS, one per CU
+@pytest.mark.parametrize("L,nth,nr,nphi", [(31, 48, 3, 96), (15, 24, 2, 48), (255, 6, 2, 768)])
 def test_fused_mhd_pencil_vs_oracle(torch_cuda, L, nth, nr, nphi):
     print(f"[fused] L={L} nth={nth} nr={nr} nphi={nphi}", flush=True)
     rng = np.random.default_rng(L)
