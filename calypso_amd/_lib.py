@@ -38,6 +38,12 @@ SIGNATURES = [
     ("calypso_sel_backward_legendre_trans", _I, [_I, _I, _I, _I, _I, _P, _P]),
     ("calypso_sel_forward_legendre_trans", _I, [_I, _I, _I, _I, _I, _P, _P]),
     ("calypso_sel_finalize_legendre_trans", _I, []),
+    ("calypso_init_MHD_FFT_select", _I, [_I, _I, _I, _I, _I, _I, _I]),
+    ("calypso_back_MHD_FFT_sel_from_recv", _I, [_I, _I, _P, _P]),
+    ("calypso_fwd_MHD_FFT_sel_from_recv", _I, [_I, _I, _P, _P]),
+    ("calypso_finalize_MHD_FFT_select", _I, []),
+    ("calypso_lubksb_3band_mul", _I, [_I, _P, _I, _I, _P, _P, _P]),
+    ("calypso_lubksb_5band_mul", _I, [_I, _P, _I, _I, _P, _P, _P]),
     ("calypso_fft_create", _I, [_I, _I, _I, _I, ctypes.POINTER(_P)]),
     ("calypso_fft_destroy", _I, [_P]),
     ("calypso_fft_backward_device", _I, [_P, _I, _P, _P, _P]),

@@ -1,10 +1,11 @@
 """Build the in-tree libraries for gfx950.
 
 * ``calypso_amd/lib/libcalypso_amd.so``   HIP kernels + C ABI (hipcc)
-* ``calypso_amd/lib/libcalypso_amd_f.so`` Fortran module shim (amdflang),
-  ``legendre_trans_hip`` and ``calypso_amd_capi`` (.mod files beside it)
-* ``calypso_amd/lib/leg_driver``           Fortran program used by the tests
-  to drive the shim end to end
+* ``calypso_amd/lib/libcalypso_amd_f.so`` Fortran module shim (amdflang):
+  ``legendre_trans_hip``, ``mhd_fft_select_hip``, ``lubksb_band_hip``,
+  ``sph_analyzer_mhd_hip`` and ``calypso_amd_capi`` (.mod files beside it)
+* ``calypso_amd/lib/leg_driver``, ``mhd_driver``, ``fft_lu_driver``  Fortran
+  programs used by the tests to drive the shim end to end
 
 Run ``python -m calypso_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -57,14 +58,15 @@ def build_fortran():
         return None
     obj = os.path.join(LIB, "fobj")
     os.makedirs(obj, exist_ok=True)
-    srcs = ["calypso_amd_capi.f90", "legendre_trans_hip.f90", "sph_analyzer_mhd_hip.f90"]
+    srcs = ["calypso_amd_capi.f90", "legendre_trans_hip.f90", "sph_analyzer_mhd_hip.f90",
+            "mhd_fft_select_hip.f90", "lubksb_band_hip.f90"]
     for s in srcs:
         _run([fc, "-O2", "-fPIC", "-c", os.path.join(FSRC, s), "-module-dir", LIB,
               "-o", os.path.join(obj, s.replace(".f90", ".o"))])
     so = os.path.join(LIB, "libcalypso_amd_f.so")
     _run([fc, "-shared", "-o", so, *[os.path.join(obj, s.replace(".f90", ".o")) for s in srcs],
           "-L", LIB, "-lcalypso_amd", "-Wl,-rpath,$ORIGIN"])
-    for prog in ("leg_driver", "mhd_driver"):
+    for prog in ("leg_driver", "mhd_driver", "fft_lu_driver"):
         _run([fc, "-O2", os.path.join(FSRC, prog + ".f90"), "-I", LIB,
               "-o", os.path.join(LIB, prog), "-L", LIB, "-lcalypso_amd_f", "-lcalypso_amd",
               "-Wl,-rpath,$ORIGIN"])

@@ -77,6 +77,72 @@ int host_transform(calypso_leg *h, bool fwd, int ncomp, int nvec, int nscl, int 
     return 0;
 }
 
+// Process-wide FFT selector (init_MHD_FFT_select analogue): context, its
+// stream and the device copies of one host call's buffers
+struct FftSel {
+    calypso_fft *f = nullptr;
+    hipStream_t s = nullptr;
+    double *d_rtm = nullptr, *d_rtp = nullptr;
+    size_t cap_rtm = 0, cap_rtp = 0;
+    size_t nnod_rtm = 0, nnod_rtp = 0;
+    int ncomp = 0;
+};
+FftSel g_fsel;
+
+// host band solves: device copies of one call's factors, pivots and modes
+struct LuSel {
+    double *d_lu = nullptr, *d_piv = nullptr, *d_x = nullptr;   // d_piv holds ints
+    size_t cap_lu = 0, cap_piv = 0, cap_x = 0;
+};
+LuSel g_lusel;
+
+void fsel_free()
+{
+    if (g_fsel.f) {
+        calypso::fft_context_free(&g_fsel.f->f);
+        delete g_fsel.f;
+    }
+    if (g_fsel.d_rtm) (void)hipFree(g_fsel.d_rtm);
+    if (g_fsel.d_rtp) (void)hipFree(g_fsel.d_rtp);
+    if (g_fsel.s) (void)hipStreamDestroy(g_fsel.s);
+    g_fsel = FftSel();
+}
+
+// lubksb_3band_mul / lubksb_5band_mul on host arrays: modes
+// msmp_stack[0]+1 .. msmp_stack[msmp] (1-based) of x(mcomp, n), the union of
+// the reference's per-thread ranges; the other modes are not touched
+int host_lubksb(int nb, int msmp, const int *msmp_stack, int mcomp, int n,
+                const double *band_lu, const int *i_pivot, double *x)
+{
+    if (msmp < 1 || !msmp_stack || mcomp < 1 || !band_lu || !i_pivot || !x)
+        return fail("lubksb_band_mul: bad argument");
+    for (int p = 0; p < msmp; ++p)
+        if (msmp_stack[p] > msmp_stack[p + 1]) return fail("lubksb_band_mul: Msmp_stack not ascending");
+    const int mst = msmp_stack[0], med = msmp_stack[msmp];
+    if (mst < 0 || med > mcomp) return fail("lubksb_band_mul: Msmp_stack outside 0..mcomp");
+    const int cnt = med - mst;
+    if (cnt == 0) return 0;
+    // a pivot outside 1..n would move a row from outside the matrix
+    for (size_t q = (size_t)mst * n; q < (size_t)med * n; ++q)
+        if (i_pivot[q] < 1 || i_pivot[q] > n) return fail("lubksb_band_mul: i_pivot outside 1..n");
+    const size_t w = 2 * (size_t)nb - 1, nlu = w * n * cnt, nx = (size_t)n * cnt;
+    LuSel &g = g_lusel;
+    if (ensure(&g.d_lu, &g.cap_lu, nlu) || ensure(&g.d_piv, &g.cap_piv, (nx + 1) / 2) ||
+        ensure(&g.d_x, &g.cap_x, nx))
+        return -1;
+    int *d_piv = reinterpret_cast<int *>(g.d_piv);
+    CHECK_HIP(hipMemcpy(g.d_lu, band_lu + w * n * mst, nlu * sizeof(double), hipMemcpyHostToDevice));
+    CHECK_HIP(hipMemcpy(d_piv, i_pivot + (size_t)n * mst, nx * sizeof(int), hipMemcpyHostToDevice));
+    // x(mst+1:med, 1:n): rows of cnt modes at host pitch mcomp
+    CHECK_HIP(hipMemcpy2D(g.d_x, cnt * sizeof(double), x + mst, mcomp * sizeof(double),
+                          cnt * sizeof(double), n, hipMemcpyHostToDevice));
+    std::string err;
+    if (calypso::lubksb_dev(nb, n, cnt, g.d_lu, d_piv, g.d_x, nullptr, &err)) return fail(err);
+    CHECK_HIP(hipMemcpy2D(x + mst, mcomp * sizeof(double), g.d_x, cnt * sizeof(double),
+                          cnt * sizeof(double), n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -560,6 +626,82 @@ int calypso_mhd_coriolis_table(int l_truncation, double *cor, int *idx)
     const int J = (l_truncation + 1) * (l_truncation + 1);
     calypso::coriolis_table(l_truncation, cor, idx, idx + J, idx + 2 * J);
     return 0;
+}
+
+int calypso_init_MHD_FFT_select(int l_truncation, int nth, int nr, int nphi, int ncomp,
+                                int ncomp_fwd, int ncomp_bwd)
+{
+    fsel_free();
+    if (ncomp < 1 || ncomp_fwd < 1 || ncomp_bwd < 1 || ncomp_fwd > ncomp || ncomp_bwd > ncomp)
+        return fail("init_MHD_FFT_select: need 1 <= ncomp_fwd, ncomp_bwd <= ncomp");
+    calypso_fft *h = nullptr;
+    if (calypso_fft_create(l_truncation, nth, nr, nphi, &h)) return -1;
+    g_fsel.f = h;
+    g_fsel.ncomp = ncomp;
+    g_fsel.nnod_rtm = (size_t)nth * nr * (2 * l_truncation + 1);
+    g_fsel.nnod_rtp = (size_t)nth * nr * nphi;
+    hipError_t e = hipStreamCreateWithFlags(&g_fsel.s, hipStreamNonBlocking);
+    if (e != hipSuccess || ensure(&g_fsel.d_rtm, &g_fsel.cap_rtm, g_fsel.nnod_rtm * ncomp) ||
+        ensure(&g_fsel.d_rtp, &g_fsel.cap_rtp, g_fsel.nnod_rtp * ncomp)) {
+        if (e != hipSuccess) g_err = std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e);
+        fsel_free();
+        return -1;
+    }
+    return 0;
+}
+
+int calypso_back_MHD_FFT_sel_from_recv(int ncomp_bwd, int n_WR, const double *WR,
+                                       double *fld_rtp)
+{
+    if (!g_fsel.f) return fail("back_MHD_FFT_sel_from_recv: FFT selector not initialised");
+    if (ncomp_bwd < 1 || ncomp_bwd > g_fsel.ncomp || !WR || !fld_rtp)
+        return fail("back_MHD_FFT_sel_from_recv: bad argument");
+    const size_t nin = (size_t)ncomp_bwd * g_fsel.nnod_rtm, nout = (size_t)ncomp_bwd * g_fsel.nnod_rtp;
+    if (n_WR < 0 || (size_t)n_WR < nin) return fail("back_MHD_FFT_sel_from_recv: n_WR smaller than ncomp_bwd * nnod_rtm");
+    hipStream_t s = g_fsel.s;
+    CHECK_HIP(hipMemcpyAsync(g_fsel.d_rtm, WR, nin * sizeof(double), hipMemcpyHostToDevice, s));
+    std::string err;
+    if (calypso::fft_backward_dev(&g_fsel.f->f, ncomp_bwd, g_fsel.d_rtm, g_fsel.d_rtp, s, &err))
+        return fail(err);
+    CHECK_HIP(hipMemcpyAsync(fld_rtp, g_fsel.d_rtp, nout * sizeof(double), hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int calypso_fwd_MHD_FFT_sel_from_recv(int ncomp_fwd, int n_WS, const double *frc_rtp,
+                                      double *WS)
+{
+    if (!g_fsel.f) return fail("fwd_MHD_FFT_sel_from_recv: FFT selector not initialised");
+    if (ncomp_fwd < 1 || ncomp_fwd > g_fsel.ncomp || !frc_rtp || !WS)
+        return fail("fwd_MHD_FFT_sel_from_recv: bad argument");
+    const size_t nin = (size_t)ncomp_fwd * g_fsel.nnod_rtp, nout = (size_t)ncomp_fwd * g_fsel.nnod_rtm;
+    if (n_WS < 0 || (size_t)n_WS < nout) return fail("fwd_MHD_FFT_sel_from_recv: n_WS smaller than ncomp_fwd * nnod_rtm");
+    hipStream_t s = g_fsel.s;
+    CHECK_HIP(hipMemcpyAsync(g_fsel.d_rtp, frc_rtp, nin * sizeof(double), hipMemcpyHostToDevice, s));
+    std::string err;
+    if (calypso::fft_forward_dev(&g_fsel.f->f, ncomp_fwd, g_fsel.d_rtp, g_fsel.d_rtm, s, &err))
+        return fail(err);
+    CHECK_HIP(hipMemcpyAsync(WS, g_fsel.d_rtm, nout * sizeof(double), hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int calypso_finalize_MHD_FFT_select(void)
+{
+    fsel_free();
+    return 0;
+}
+
+int calypso_lubksb_3band_mul(int msmp, const int *msmp_stack, int mcomp, int n,
+                             const double *band_lu, const int *i_pivot, double *x)
+{
+    return host_lubksb(3, msmp, msmp_stack, mcomp, n, band_lu, i_pivot, x);
+}
+
+int calypso_lubksb_5band_mul(int msmp, const int *msmp_stack, int mcomp, int n,
+                             const double *band_lu, const int *i_pivot, double *x)
+{
+    return host_lubksb(5, msmp, msmp_stack, mcomp, n, band_lu, i_pivot, x);
 }
 
 }  // extern "C"

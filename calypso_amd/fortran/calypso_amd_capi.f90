@@ -70,6 +70,57 @@
           import :: c_int
         end function calypso_sel_finalize_legendre_trans
 !
+        integer(c_int) function calypso_init_MHD_FFT_select(            &
+     &      l_truncation, nth, nr, nphi, ncomp, ncomp_fwd, ncomp_bwd)   &
+     &      bind(C, name="calypso_init_MHD_FFT_select")
+          import :: c_int
+          integer(c_int), value :: l_truncation, nth, nr, nphi
+          integer(c_int), value :: ncomp, ncomp_fwd, ncomp_bwd
+        end function calypso_init_MHD_FFT_select
+!
+        integer(c_int) function calypso_back_MHD_FFT_sel_from_recv(     &
+     &      ncomp_bwd, n_WR, WR, fld_rtp)                               &
+     &      bind(C, name="calypso_back_MHD_FFT_sel_from_recv")
+          import :: c_int, c_double
+          integer(c_int), value :: ncomp_bwd, n_WR
+          real(c_double), intent(in) :: WR(*)
+          real(c_double), intent(inout) :: fld_rtp(*)
+        end function calypso_back_MHD_FFT_sel_from_recv
+!
+        integer(c_int) function calypso_fwd_MHD_FFT_sel_from_recv(      &
+     &      ncomp_fwd, n_WS, frc_rtp, WS)                               &
+     &      bind(C, name="calypso_fwd_MHD_FFT_sel_from_recv")
+          import :: c_int, c_double
+          integer(c_int), value :: ncomp_fwd, n_WS
+          real(c_double), intent(in) :: frc_rtp(*)
+          real(c_double), intent(inout) :: WS(*)
+        end function calypso_fwd_MHD_FFT_sel_from_recv
+!
+        integer(c_int) function calypso_finalize_MHD_FFT_select()       &
+     &      bind(C, name="calypso_finalize_MHD_FFT_select")
+          import :: c_int
+        end function calypso_finalize_MHD_FFT_select
+!
+        integer(c_int) function calypso_lubksb_3band_mul(Msmp,          &
+     &      Msmp_stack, mcomp, n, band_lu, i_pivot, x)                  &
+     &      bind(C, name="calypso_lubksb_3band_mul")
+          import :: c_int, c_double
+          integer(c_int), value :: Msmp, mcomp, n
+          integer(c_int), intent(in) :: Msmp_stack(*), i_pivot(*)
+          real(c_double), intent(in) :: band_lu(*)
+          real(c_double), intent(inout) :: x(*)
+        end function calypso_lubksb_3band_mul
+!
+        integer(c_int) function calypso_lubksb_5band_mul(Msmp,          &
+     &      Msmp_stack, mcomp, n, band_lu, i_pivot, x)                  &
+     &      bind(C, name="calypso_lubksb_5band_mul")
+          import :: c_int, c_double
+          integer(c_int), value :: Msmp, mcomp, n
+          integer(c_int), intent(in) :: Msmp_stack(*), i_pivot(*)
+          real(c_double), intent(in) :: band_lu(*)
+          real(c_double), intent(inout) :: x(*)
+        end function calypso_lubksb_5band_mul
+!
         integer(c_int) function calypso_mhd_create(params, radius,      &
      &      stream, mhd) bind(C, name="calypso_mhd_create")
           import :: c_int, c_double, c_ptr, calypso_mhd_params

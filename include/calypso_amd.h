@@ -163,6 +163,28 @@ int calypso_fft_mhd_fused_device(calypso_fft_t *fft, const double *d_bwd_rtm,
                                  double *d_fwd_rtm, const double *coef,
                                  void *stream);
 
+/* Host-buffer FFT selector with one process-wide context, shaped after the
+ * reference module MHD_FFT_selector (MHD_FFT_selector.F90):
+ *   init_MHD_FFT_select (:86-114): the grid (l_truncation, nth, nr, nphi)
+ *     that the reference reads from module variables, and the component
+ *     counts; device buffers for ncomp components are allocated here
+ *   back_MHD_FFT_sel_from_recv (:219-249): WR(n_WR) = the rtm spectra,
+ *     component fastest with stride ncomp_bwd (what the single-domain
+ *     rtm -> rtp self-copy leaves in the receive buffer) -> fld_rtp(nnod_rtp,
+ *     ncomp_bwd), nnod_rtp = nr nth nphi, node k + nr t + nr nth i
+ *   fwd_MHD_FFT_sel_from_recv (:185-215): frc_rtp(nnod_rtp, ncomp_fwd) ->
+ *     WS(n_WS), stride ncomp_fwd
+ *   finalize_MHD_FFT_select (:118-145)
+ * Each call copies its buffers to the device, transforms and copies back
+ * before returning. */
+int calypso_init_MHD_FFT_select(int l_truncation, int nth, int nr, int nphi,
+                                int ncomp, int ncomp_fwd, int ncomp_bwd);
+int calypso_back_MHD_FFT_sel_from_recv(int ncomp_bwd, int n_WR,
+                                       const double *WR, double *fld_rtp);
+int calypso_fwd_MHD_FFT_sel_from_recv(int ncomp_fwd, int n_WS,
+                                      const double *frc_rtp, double *WS);
+int calypso_finalize_MHD_FFT_select(void);
+
 /* ---- banded LU of the Crank-Nicolson radial matrices (FP64) ------------ */
 /* Factor mcomp band matrices band_a(nb, n, mcomp) (nb = 3: ludcmp_3band,
  * m_ludcmp_3band.f90:76-173; nb = 5: ludcmp_band with nband 5,
@@ -178,6 +200,16 @@ int calypso_lubksb_band_mul_device(int nb, int n, int mcomp,
                                    const double *d_band_lu,
                                    const int *d_i_pivot, double *d_x,
                                    void *stream);
+/* The same solves on host arrays with the reference's argument list:
+ * lubksb_3band_mul / lubksb_5band_mul(Msmp, Msmp_stack, mcomp, n, band_lu,
+ * i_pivot, x) (lubksb_357band_mul.f90:73-195).  Modes Msmp_stack[0]+1 ..
+ * Msmp_stack[Msmp] of x(mcomp, n) are solved (the union of the reference's
+ * per-thread ranges), the others are left as they are; a pivot outside 1..n
+ * is an error.  The device copies are kept for the next call. */
+int calypso_lubksb_3band_mul(int msmp, const int *msmp_stack, int mcomp, int n,
+                             const double *band_lu, const int *i_pivot, double *x);
+int calypso_lubksb_5band_mul(int msmp, const int *msmp_stack, int mcomp, int n,
+                             const double *band_lu, const int *i_pivot, double *x);
 
 /* ---- explicit time integration and energy diagnostics (FP64) ----------- */
 /* Pointwise update of n d_rj values (one field component):

@@ -50,3 +50,29 @@ def test_singular_3band_reported(torch_cuda):
     torch.cuda.synchronize()
     err = derr.cpu().numpy()
     assert err[1] == 100 and err[0] == 0
+
+
+@pytest.mark.parametrize("nb", [3, 5])
+def test_lubksb_host_arrays_mode_range(torch_cuda, nb):
+    """lubksb_3band_mul / lubksb_5band_mul on host arrays with the reference's
+    (Msmp, Msmp_stack) ranges: the range solved bitwise as the oracle, the
+    other modes untouched, and a second call reusing the device copies."""
+    import ctypes
+    import calypso_amd
+    lib = calypso_amd.load()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    f = lib.calypso_lubksb_3band_mul if nb == 3 else lib.calypso_lubksb_5band_mul
+    for n, mcomp, stack in ((49, 700, [0, 100, 350, 700]), (9, 30, [4, 11, 19])):
+        print(f"[lu host] nb={nb} n={n} mcomp={mcomp} stack={stack}", flush=True)
+        rng = np.random.default_rng(n + mcomp + nb)
+        A = rng.standard_normal((mcomp, n, nb))
+        A[::2, :, (nb - 1) // 2] += 4.0
+        lu, piv = oracle.ludcmp(nb, A)
+        x = rng.standard_normal((n, mcomp))
+        xo = x.copy()
+        st = np.array(stack, dtype=np.int32)
+        assert f(st.size - 1, p(st), mcomp, n, p(lu), p(piv), p(xo)) == 0
+        m0, m1 = stack[0], stack[-1]
+        ref = oracle.lubksb(nb, lu[m0:m1], piv[m0:m1], x[:, m0:m1])
+        assert np.array_equal(xo[:, m0:m1], ref)
+        assert np.array_equal(xo[:, :m0], x[:, :m0]) and np.array_equal(xo[:, m1:], x[:, m1:])
