@@ -65,12 +65,32 @@ __device__ __forceinline__ double dr2(const MhdDev &p, const double *f, int k, s
 }
 
 // ---------------------------------------------------------- explicit ----
+// node i = k J + j -> (k, j) without a 64-bit division sequence: the host
+// keeps nnod below 2^31, so a double quotient and one correction are exact
+struct NodeKJ {
+    int k, j;
+};
+__device__ inline NodeKJ node_kj(const MhdDev &p, size_t i)
+{
+    const int ii = (int)i;
+    int k = (int)((double)ii * p.rJ);
+    int j = ii - k * p.J;
+    if (j < 0) {
+        --k;
+        j += p.J;
+    } else if (j >= p.J) {
+        ++k;
+        j -= p.J;
+    }
+    return {k, j};
+}
+
 __global__ void k_explicit(MhdDev p, int euler, int first)
 {
     const double a0 = 3.0 / 2.0, a1 = -1.0 / 2.0, dt = p.dt;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
          i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J);
+        const int k = node_kj(p, i).k;
         const bool in_shell = (k >= p.ki && k <= p.ko);
         for (int c = 0; c <= 2; c += 2) {
             double *w = FLD(kVort + c), *pm = FLD(kPreMom + c);
@@ -120,7 +140,8 @@ __global__ void k_cn_rhs(MhdDev p)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
          i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const NodeKJ kj = node_kj(p, i);
+        const int k = kj.k, j = kj.j;
         double up = FLD(kVort + 2)[i], ut = FLD(kVort)[i];
         if (j == 0) {
             up = 0.0;
@@ -222,7 +243,8 @@ __global__ void k_grad_interior(MhdDev p)
     const size_t ist = (size_t)p.J * (p.ki + 1), ied = (size_t)p.J * p.ko;
     for (size_t i = ist + blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ied;
          i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const NodeKJ kj = node_kj(p, i);
+        const int k = kj.k, j = kj.j;
         const double g3a = p.g3[j] * p.ar2[k];
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
@@ -247,7 +269,8 @@ __global__ void k_diffusion(MhdDev p)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
          i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const NodeKJ kj = node_kj(p, i);
+        const int k = kj.k, j = kj.j;
         const size_t J = p.J;
         const double g1 = p.g1[j], g3 = p.g3[j];
         const double *w0 = FLD(kVort), *w2 = FLD(kVort + 2);
@@ -445,7 +468,8 @@ __global__ void k_coriolis(MhdDev p)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
          i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const NodeKJ kj = node_kj(p, i);
+        const int k = kj.k, j = kj.j;
         const size_t base = (size_t)p.J * k;
         const size_t k1 = base + p.jk1[j], k2 = base + p.jk2[j], l1 = base + p.jl1[j];
         const double *us = FLD(kVelo), *ud = FLD(kVelo + 1);
@@ -495,7 +519,8 @@ __global__ void k_rot_forces(MhdDev p, int temp_roundtrip)
 {
     for (size_t i = (size_t)p.J * p.ki + blockIdx.x * (size_t)blockDim.x + threadIdx.x;
          i < (size_t)p.J * (p.ko + 1); i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const NodeKJ kj = node_kj(p, i);
+        const int k = kj.k, j = kj.j;
         const size_t J = p.J;
         const double g1 = p.g1[j], g3 = p.g3[j];
         double ri_p, ri_t, rl_p, rl_t;
@@ -555,7 +580,8 @@ __global__ void k_temp_full(MhdDev p, double *out)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
          i += (size_t)gridDim.x * blockDim.x) {
-        const int k = (int)(i / p.J), j = (int)(i % p.J);
+        const NodeKJ kj = node_kj(p, i);
+        const int k = kj.k, j = kj.j;
         double t = FLD(kTemp)[i];
         if (j == 0) t = t + p.reft0[k];
         out[i] = t;
@@ -684,6 +710,10 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
         *err = "mhd_create: need L >= 2, nth >= L+1, nphi >= 2L+2, nr >= 5";
         return -1;
     }
+    if ((long long)nr * (L + 1) * (L + 1) >= (1LL << 31)) {
+        *err = "mhd_create: nr (L+1)^2 must stay below 2^31 spectral nodes";
+        return -1;
+    }
     c->L = L;
     c->nth = nth;
     c->nphi = nphi;
@@ -726,6 +756,7 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     p.ki = ki;
     p.ko = ko;
     p.nnod = nnod;
+    p.rJ = 1.0 / (double)J;
     {   // one-sided wall formulas (cal_fdm_coefs_4_boundaries.f90)
         const double dp1 = r[ki + 1] - r[ki], dp2 = r[ki + 2] - r[ki];
         const double a1[3][3] = {{1, 0, 0}, {1, dp1, dp1 * dp1 / 2.0}, {1, dp2, dp2 * dp2 / 2.0}};

@@ -38,6 +38,7 @@ struct MhdCoefs {
 // Everything a kernel needs, passed by value.
 struct MhdDev {
     int nr = 0, J = 0, ki = 0, ko = 0, L = 0;
+    double rJ = 0.0;                           // 1 / J
     size_t nnod = 0;
     double *d = nullptr;                       // d_rj, kMhdNumComp * nnod
     const double *r = nullptr, *ar1 = nullptr, *ar2 = nullptr;
