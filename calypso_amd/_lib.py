@@ -42,6 +42,7 @@ SIGNATURES = [
     ("calypso_back_MHD_FFT_sel_from_recv", _I, [_I, _I, _P, _P]),
     ("calypso_fwd_MHD_FFT_sel_from_recv", _I, [_I, _I, _P, _P]),
     ("calypso_finalize_MHD_FFT_select", _I, []),
+    ("calypso_nonlinear_products", _I, [ctypes.c_longlong, _P, _P, _P]),
     ("calypso_lubksb_3band_mul", _I, [_I, _P, _I, _I, _P, _P, _P]),
     ("calypso_lubksb_5band_mul", _I, [_I, _P, _I, _I, _P, _P, _P]),
     ("calypso_fft_create", _I, [_I, _I, _I, _I, ctypes.POINTER(_P)]),

@@ -3,6 +3,7 @@
 * ``calypso_amd/lib/libcalypso_amd.so``   HIP kernels + C ABI (hipcc)
 * ``calypso_amd/lib/libcalypso_amd_f.so`` Fortran module shim (amdflang):
   ``legendre_trans_hip``, ``mhd_fft_select_hip``, ``lubksb_band_hip``,
+  ``cal_nonlinear_sph_mhd_hip``,
   ``sph_analyzer_mhd_hip`` and ``calypso_amd_capi`` (.mod files beside it)
 * ``calypso_amd/lib/leg_driver``, ``mhd_driver``, ``fft_lu_driver``  Fortran
   programs used by the tests to drive the shim end to end
@@ -59,7 +60,7 @@ def build_fortran():
     obj = os.path.join(LIB, "fobj")
     os.makedirs(obj, exist_ok=True)
     srcs = ["calypso_amd_capi.f90", "legendre_trans_hip.f90", "sph_analyzer_mhd_hip.f90",
-            "mhd_fft_select_hip.f90", "lubksb_band_hip.f90"]
+            "mhd_fft_select_hip.f90", "lubksb_band_hip.f90", "cal_nonlinear_sph_mhd_hip.f90"]
     for s in srcs:
         _run([fc, "-O2", "-fPIC", "-c", os.path.join(FSRC, s), "-module-dir", LIB,
               "-o", os.path.join(obj, s.replace(".f90", ".o"))])

@@ -96,6 +96,13 @@ struct LuSel {
 };
 LuSel g_lusel;
 
+// host grid products: device copies of one call's fields and forces
+struct ProdSel {
+    double *d_fld = nullptr, *d_frc = nullptr;
+    size_t cap_fld = 0, cap_frc = 0;
+};
+ProdSel g_prodsel;
+
 void fsel_free()
 {
     if (g_fsel.f) {
@@ -683,6 +690,20 @@ int calypso_fwd_MHD_FFT_sel_from_recv(int ncomp_fwd, int n_WS, const double *frc
         return fail(err);
     CHECK_HIP(hipMemcpyAsync(WS, g_fsel.d_rtm, nout * sizeof(double), hipMemcpyDeviceToHost, s));
     CHECK_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int calypso_nonlinear_products(long long nnod, const double *fld_rtp, double *frc_rtp,
+                               const double *coef)
+{
+    if (nnod < 1 || !fld_rtp || !frc_rtp || !coef) return fail("calypso_nonlinear_products: bad argument");
+    ProdSel &g = g_prodsel;
+    const size_t nin = (size_t)13 * nnod, nout = (size_t)19 * nnod;
+    if (ensure(&g.d_fld, &g.cap_fld, nin) || ensure(&g.d_frc, &g.cap_frc, nout)) return -1;
+    CHECK_HIP(hipMemcpy(g.d_fld, fld_rtp, nin * sizeof(double), hipMemcpyHostToDevice));
+    std::string err;
+    if (calypso::products_mhd_dev((size_t)nnod, g.d_fld, g.d_frc, coef, nullptr, &err)) return fail(err);
+    CHECK_HIP(hipMemcpy(frc_rtp, g.d_frc, nout * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 

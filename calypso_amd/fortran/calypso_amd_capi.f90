@@ -96,6 +96,15 @@
           real(c_double), intent(inout) :: WS(*)
         end function calypso_fwd_MHD_FFT_sel_from_recv
 !
+        integer(c_int) function calypso_nonlinear_products(nnod,        &
+     &      fld_rtp, frc_rtp, coef)                                     &
+     &      bind(C, name="calypso_nonlinear_products")
+          import :: c_int, c_double, c_long_long
+          integer(c_long_long), value :: nnod
+          real(c_double), intent(in) :: fld_rtp(*), coef(4)
+          real(c_double), intent(inout) :: frc_rtp(*)
+        end function calypso_nonlinear_products
+!
         integer(c_int) function calypso_finalize_MHD_FFT_select()       &
      &      bind(C, name="calypso_finalize_MHD_FFT_select")
           import :: c_int

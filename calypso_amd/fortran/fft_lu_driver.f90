@@ -1,6 +1,8 @@
-!> fft_lu_driver.f90 -- Fortran program that exercises the mhd_fft_select_hip
-!! and lubksb_band_hip modules end to end: a backward and a forward FFT
-!! through the selector, and one band solve over a range of modes.
+!> fft_lu_driver.f90 -- Fortran program that exercises the mhd_fft_select_hip,
+!! cal_nonlinear_sph_mhd_hip and lubksb_band_hip modules end to end: a
+!! backward and a forward FFT through the selector, the grid products of the
+!! backward result (when it has the 13 fields of the MHD step) and one band
+!! solve over a range of modes.
 !!
 !! usage: fft_lu_driver <in file> <out file>
 !! in file (stream, native endian):
@@ -10,12 +12,14 @@
 !!   real64 band_lu((2nb-1)*n*mcomp), int32 i_pivot(n*mcomp),
 !!   real64 x(mcomp*n)
 !! out file: real64 fld_rtp(nnod_rtp*ncomp_bwd), WS(ncomp_fwd*nnod_rtm),
-!!   x(mcomp*n)
+!!   x(mcomp*n), and if ncomp_bwd = 13 frc_rtp(nnod_rtp*19) = the products of
+!!   fld_rtp with coefficients (-1.0, 0.7, 1.3, -0.4)
       program fft_lu_driver
 !
       use iso_c_binding
       use mhd_fft_select_hip
       use lubksb_band_hip
+      use cal_nonlinear_sph_mhd_hip
       implicit none
 !
       character(len=256) :: fin, fout
@@ -23,7 +27,7 @@
       integer(c_int) :: nb, n, mcomp, msmp
       integer(c_int), allocatable :: stack(:), piv(:,:)
       real(c_double), allocatable :: WR(:), frc(:), fld(:), WS(:)
-      real(c_double), allocatable :: lu(:,:,:), x(:,:)
+      real(c_double), allocatable :: lu(:,:,:), x(:,:), frc2(:)
 !
       call get_command_argument(1, fin)
       call get_command_argument(2, fout)
@@ -48,6 +52,11 @@
       call back_MHD_FFT_sel_from_recv_hip(ncb, ncb*nrtm, WR, fld)
       call fwd_MHD_FFT_sel_from_recv_hip(ncf, ncf*nrtm, frc, WS)
       call finalize_MHD_FFT_select_hip
+      if(ncb .eq. 13) then
+        allocate(frc2(nrtp*19))
+        call s_cal_nonlinear_sph_MHD_hip(nrtp, fld, frc2,               &
+     &      -1.0d0, 0.7d0, 1.3d0, -0.4d0)
+      end if
 !
       if(nb .eq. 3) then
         call lubksb_3band_mul_hip(msmp, stack, mcomp, n, lu, piv, x)
@@ -58,6 +67,7 @@
       open(11, file=trim(fout), access='stream', form='unformatted',    &
      &     status='replace')
       write(11) fld, WS, x
+      if(ncb .eq. 13) write(11) frc2
       close(11)
       write(*,'(a,i0,a,i0)') 'fft_lu_driver done: nnod_rtp=', nrtp,     &
      &    ' mcomp=', mcomp

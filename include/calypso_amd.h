@@ -154,6 +154,11 @@ int calypso_fft_forward_device(calypso_fft_t *fft, int ncomp,
 int calypso_nonlinear_products_device(long long nnod, const double *d_fld,
                                       double *d_frc, const double *coef,
                                       void *stream);
+/* The same on host arrays fld_rtp(nnod, 13) -> frc_rtp(nnod, 19), copied to
+ * the device and back before returning (Fortran:
+ * s_cal_nonlinear_sph_MHD_hip). */
+int calypso_nonlinear_products(long long nnod, const double *fld_rtp,
+                               double *frc_rtp, const double *coef);
 
 /* Fused pencil step: rtm spectra of the 13 backward fields -> inverse FFT ->
  * products -> forward FFT -> rtm spectra of the 19 forward fields, the grid

@@ -126,8 +126,9 @@ def test_fft_selector_host_buffers(torch_cuda, L, nth, nr, nphi, ncb, ncf):
 
 @pytest.mark.parametrize("nb", [3, 5])
 def test_fortran_fft_lu_driver(torch_cuda, tmp_path, nb):
-    """The Fortran modules mhd_fft_select_hip and lubksb_band_hip, called from a
-    Fortran program with the reference's argument lists, against the oracle."""
+    """The Fortran modules mhd_fft_select_hip, cal_nonlinear_sph_mhd_hip and
+    lubksb_band_hip, called from a Fortran program with the reference's
+    argument lists, against the oracle."""
     import os
     import subprocess
     L, nth, nr, nphi, ncb, ncf = 15, 24, 2, 48, 13, 12
@@ -162,6 +163,9 @@ def test_fortran_fft_lu_driver(torch_cuda, tmp_path, nb):
     assert np.max(np.abs(fld - ref_b)) / np.max(np.abs(ref_b)) < 1e-13
     assert np.max(np.abs(ws - ref_f)) / np.max(np.abs(ref_f)) < 1e-13
     # modes stack[0]+1 .. stack[-1] solved bitwise as the reference; the rest untouched
+    xo, frc2 = xo[:n * mcomp], xo[n * mcomp:]
+    # grid products of the backward result, bitwise as the reference formulas
+    assert np.array_equal(frc2, oracle.products_mhd(nrtp, fld, -1.0, 0.7, 1.3, -0.4))
     xo = xo.reshape(n, mcomp)
     m0, m1 = stack[0], stack[-1]
     ref_x = oracle.lubksb(nb, lu[m0:m1], piv[m0:m1], x[:, m0:m1])

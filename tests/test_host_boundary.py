@@ -65,5 +65,13 @@ def test_fft_selector_needs_init():
 
 
 def test_fortran_modules_built():
-    for f in ("mhd_fft_select_hip.mod", "lubksb_band_hip.mod", "fft_lu_driver"):
+    for f in ("mhd_fft_select_hip.mod", "lubksb_band_hip.mod", "cal_nonlinear_sph_mhd_hip.mod",
+              "fft_lu_driver"):
         assert os.path.exists(os.path.join(_lib.LIB_DIR, f)), f
+
+
+def test_products_rejects_bad_arguments():
+    lib = calypso_amd.load()
+    fld = np.zeros(13)
+    assert lib.calypso_nonlinear_products(0, ptr(fld), ptr(fld), ptr(fld)) != 0
+    assert "calypso_nonlinear_products" in last_error()
