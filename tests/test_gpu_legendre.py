@@ -82,6 +82,11 @@ CASES = [
     (127, 192, 2, 1, 1),
     # more than one group of row tiles per order (fwd, m small)
     (191, 288, 1, 1, 0),
+    # tiny truncations (one partial row tile, one LDS chunk), and a grid with
+    # many more colatitudes than degrees
+    (1, 2, 1, 1, 1),
+    (2, 3, 2, 1, 0),
+    (5, 16, 1, 0, 2),
 ]
 
 
