@@ -24,6 +24,9 @@ FFT_CASES = [
     (10, 6, 2, 24, 3),
     (47, 8, 1, 144, 4),
     (127, 24, 1, 384, 19),
+    # powers of two only (radix 4 stages, then one radix 2)
+    (31, 8, 1, 64, 5),
+    (15, 4, 2, 32, 13),
 ]
 
 
@@ -72,7 +75,8 @@ def test_products_bitwise(torch_cuda):
 
 
 # L=255 / nphi=768: the 512-thread work-group with 151 KB of LDS, one per CU
-@pytest.mark.parametrize("L,nth,nr,nphi", [(31, 48, 3, 96), (15, 24, 2, 48), (255, 6, 2, 768)])
+@pytest.mark.parametrize("L,nth,nr,nphi", [(31, 48, 3, 96), (15, 24, 2, 48), (255, 6, 2, 768),
+                                            (31, 8, 2, 64)])
 def test_fused_mhd_pencil_vs_oracle(torch_cuda, L, nth, nr, nphi):
     print(f"[fused] L={L} nth={nth} nr={nr} nphi={nphi}", flush=True)
     rng = np.random.default_rng(L)
