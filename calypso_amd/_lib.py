@@ -58,6 +58,7 @@ SIGNATURES = [
     ("calypso_mhd_create", _I, [_P, _P, _P, ctypes.POINTER(_P)]),
     ("calypso_mhd_destroy", _I, [_P]),
     ("calypso_mhd_set_field", _I, [_P, _I, _P]),
+    ("calypso_mhd_snapshot_rtp", _I, [_P, _I, _P, _I, _P, _P]),
     ("calypso_mhd_get_field", _I, [_P, _I, _P]),
     ("calypso_mhd_set_dynamobench_ic", _I, [_P, _I]),
     ("calypso_mhd_initialize", _I, [_P]),

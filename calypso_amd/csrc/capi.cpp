@@ -532,6 +532,16 @@ int calypso_mhd_set_field(calypso_mhd_t *mhd, int comp, const double *host)
     return 0;
 }
 
+int calypso_mhd_snapshot_rtp(calypso_mhd_t *mhd, int nvec, const int *vec_comp, int nscl,
+                             const int *scl_comp, double *host_rtp)
+{
+    if (!mhd) return fail("calypso_mhd_snapshot_rtp: null argument");
+    std::string err;
+    if (calypso::mhd_snapshot_rtp(&mhd->c, nvec, vec_comp, nscl, scl_comp, host_rtp, &err))
+        return fail(err);
+    return 0;
+}
+
 int calypso_mhd_get_field(calypso_mhd_t *mhd, int comp, double *host)
 {
     if (!mhd || !host) return fail("calypso_mhd_get_field: null argument");

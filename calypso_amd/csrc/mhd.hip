@@ -1123,6 +1123,93 @@ int mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::strin
     return 0;
 }
 
+// Gather d_rj components into an rlm spectrum, component fastest (the layout
+// of the reference's send buffer after copy_snap_spectr_to_send)
+__global__ void k_gather_rlm(MhdDev p, int ncomp, const int *slot, double *sp)
+{
+    const size_t n = (size_t)ncomp * p.J * p.nr;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int comp = (int)(e % ncomp);
+        const size_t q = e / ncomp;
+        const int jr = (int)(q % p.J), k = (int)(q / p.J);
+        sp[e] = p.d[(size_t)slot[comp] * p.nnod + (size_t)k * p.J + p.rj_of_rlm[jr]];
+    }
+}
+
+// Fields on the physical grid at an output step: sph_back_trans_snapshot_MHD
+// (sph_transforms_4_MHD.f90:200-224) without the pole values.  The vectors
+// (first component of each: pol, dpol/dr, tor) and scalars named by d_rj
+// component go through the context's backward Legendre transform and FFT.
+int mhd_snapshot_rtp(MhdContext *c, int nvec, const int *vec, int nscl, const int *scl,
+                     double *host, std::string *err)
+{
+    if (nvec < 0 || nscl < 0 || nvec + nscl < 1 || 3 * nvec + nscl > 64 ||
+        (nvec && !vec) || (nscl && !scl) || !host) {
+        *err = "snapshot: need 1 <= 3 nvec + nscl <= 64 and the component lists";
+        return -1;
+    }
+    std::vector<int> slot;
+    for (int i = 0; i < nvec; ++i) {
+        if (vec[i] < 0 || vec[i] + 3 > kMhdNumComp) {
+            *err = "snapshot: vector component out of range";
+            return -1;
+        }
+        for (int q = 0; q < 3; ++q) slot.push_back(vec[i] + q);
+    }
+    for (int i = 0; i < nscl; ++i) {
+        if (scl[i] < 0 || scl[i] >= kMhdNumComp) {
+            *err = "snapshot: scalar component out of range";
+            return -1;
+        }
+        slot.push_back(scl[i]);
+    }
+    const int ncomp = (int)slot.size();
+    const size_t nrlm = (size_t)ncomp * c->J * c->nr;
+    const size_t nrtm = (size_t)ncomp * c->nth * c->nr * c->M;
+    const size_t nrtp = (size_t)ncomp * c->nth * c->nr * c->nphi;
+    hipStream_t s = c->stream;
+    double *d_sp = nullptr, *d_vr = nullptr, *d_grid = nullptr;
+    int *d_slot = nullptr;
+    auto release = [&]() {
+        if (d_sp) (void)hipFree(d_sp);
+        if (d_vr) (void)hipFree(d_vr);
+        if (d_grid) (void)hipFree(d_grid);
+        if (d_slot) (void)hipFree(d_slot);
+    };
+    int rc = -1;
+    do {
+        hipError_t e = hipMalloc(&d_sp, nrlm * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&d_vr, nrtm * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&d_grid, nrtp * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&d_slot, ncomp * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_slot, slot.data(), ncomp * sizeof(int), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) {
+            *err = std::string("snapshot: ") + hipGetErrorString(e);
+            break;
+        }
+        hipLaunchKernelGGL(k_gather_rlm, dim3(blocks_for(nrlm)), dim3(256), 0, s, c->p, ncomp,
+                           d_slot, d_sp);
+        e = hipGetLastError();
+        if (e != hipSuccess) {
+            *err = std::string("snapshot gather: ") + hipGetErrorString(e);
+            break;
+        }
+        if (leg_backward_dev(&c->leg, ncomp, nvec, nscl, d_sp, d_vr, s, err)) break;
+        if (fft_backward_dev(&c->fft, ncomp, d_vr, d_grid, s, err)) break;
+        e = hipMemcpyAsync(host, d_grid, nrtp * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            *err = std::string("snapshot copy: ") + hipGetErrorString(e);
+            break;
+        }
+        rc = 0;
+    } while (false);
+    release();
+    return rc;
+}
+
 // dynamo_benchmark_1 initial fields (set_initial_velo_sph,
 // set_ini_reference_temp_sph, set_initial_temp_sph,
 // initial_b_dynamobench_1): full temperature, as a restart holds it.

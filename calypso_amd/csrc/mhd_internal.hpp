@@ -97,6 +97,9 @@ int  mhd_initialize(MhdContext *c, std::string *err);
 int  mhd_steps(MhdContext *c, int nsteps, std::string *err);
 int  mhd_energies(MhdContext *c, double *out7, std::string *err);
 int  mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err);
+// backward transform of chosen d_rj fields to the rtp grid (host array)
+int  mhd_snapshot_rtp(MhdContext *c, int nvec, const int *vec, int nscl, const int *scl,
+                      double *host, std::string *err);
 int  mhd_set_option(MhdContext *c, const std::string &name, long long value, std::string *err);
 // output files in the reference formats (sph_io.cpp)
 double mhd_time(const MhdContext *c);

@@ -212,6 +212,16 @@
           character(kind=c_char), intent(inout) :: name_out(*)
         end function calypso_mhd_write_restart
 !
+        integer(c_int) function calypso_mhd_snapshot_rtp(mhd, nvec,     &
+     &      vec_comp, nscl, scl_comp, host_rtp)                         &
+     &      bind(C, name="calypso_mhd_snapshot_rtp")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: mhd
+          integer(c_int), value :: nvec, nscl
+          integer(c_int), intent(in) :: vec_comp(*), scl_comp(*)
+          real(c_double), intent(inout) :: host_rtp(*)
+        end function calypso_mhd_snapshot_rtp
+!
         integer(c_int) function calypso_mhd_read_restart(mhd, path)     &
      &      bind(C, name="calypso_mhd_read_restart")
           import :: c_int, c_char, c_ptr

@@ -98,6 +98,25 @@
 !
 ! -----------------------------------------------------------------------
 !
+!>    Fields on the physical grid at an output step
+!!    (sph_back_trans_snapshot_MHD, sph_transforms_4_MHD.f90:200-224, without
+!!    the pole values): nvec vectors (first d_rj component of each, e.g.
+!!    ihip_velo) and nscl scalar components to fld_rtp(nnod_rtp, 3 nvec + nscl)
+      subroutine sph_mhd_snapshot_rtp_hip(nvec, vec_comp, nscl,         &
+     &          scl_comp, nnod_rtp, fld_rtp)
+!
+      integer(c_int), intent(in) :: nvec, nscl, nnod_rtp
+      integer(c_int), intent(in) :: vec_comp(max(nvec,1))
+      integer(c_int), intent(in) :: scl_comp(max(nscl,1))
+      real(c_double), intent(inout) :: fld_rtp(nnod_rtp,3*nvec+nscl)
+!
+      call check_status(calypso_mhd_snapshot_rtp(mhd_ctx, nvec,         &
+     &    vec_comp, nscl, scl_comp, fld_rtp))
+!
+      end subroutine sph_mhd_snapshot_rtp_hip
+!
+! -----------------------------------------------------------------------
+!
       subroutine SPH_finalize_MHD_hip
 !
       call check_status(calypso_mhd_destroy(mhd_ctx))

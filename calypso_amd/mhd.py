@@ -120,6 +120,20 @@ class MhdSimulation:
                    "calypso_mhd_get_field")
         return out
 
+    def snapshot_rtp(self, vectors=("velocity", "magnetic_field"), scalars=("temperature",)):
+        """Fields on the physical grid (sph_back_trans_snapshot_MHD): returns
+        (3 len(vectors) + len(scalars), nphi, nth, nr) with the vectors as
+        (r, theta, phi) components, scalars after."""
+        vec = np.array([self._comp(v) for v in vectors], dtype=np.int32)
+        scl = np.array([self._comp(v) for v in scalars], dtype=np.int32)
+        nc = 3 * vec.size + scl.size
+        p = self.params
+        out = np.zeros((nc, p.nphi, p.nth, p.nr))
+        _lib.check(_lib.load().calypso_mhd_snapshot_rtp(
+            self._h, vec.size, vec.ctypes.data_as(_P), scl.size, scl.ctypes.data_as(_P),
+            out.ctypes.data_as(_P)), "calypso_mhd_snapshot_rtp")
+        return out
+
     def set_dynamobench_ic(self, isig=400):
         _lib.check(_lib.load().calypso_mhd_set_dynamobench_ic(self._h, isig),
                    "calypso_mhd_set_dynamobench_ic")

@@ -274,6 +274,17 @@ int calypso_mhd_destroy(calypso_mhd_t *mhd);
  * 26-28 induction, 32 heat advection, 34 buoyancy (m_sph_phys_address). */
 int calypso_mhd_set_field(calypso_mhd_t *mhd, int comp, const double *host);
 int calypso_mhd_get_field(calypso_mhd_t *mhd, int comp, double *host);
+/* Fields on the physical grid at an output step: sph_back_trans_snapshot_MHD
+ * (sph_transforms_4_MHD.f90:200-224) without the pole values.  nvec vector
+ * fields (vec_comp: the first d_rj component of each, pol / dpol/dr / tor)
+ * and nscl scalar components (scl_comp), 3 nvec + nscl <= 64, through the
+ * context's backward Legendre transform (its precision mode) and FFT.
+ * host_rtp(nnod_rtp, 3 nvec + nscl), nnod_rtp = nr nth nphi, node
+ * k + nr t + nr nth i; vectors as (r, theta, phi), scalars after.  The
+ * temperature component holds the perturbation from the reference profile
+ * during a run. */
+int calypso_mhd_snapshot_rtp(calypso_mhd_t *mhd, int nvec, const int *vec_comp,
+                             int nscl, const int *scl_comp, double *host_rtp);
 /* dynamo_benchmark_1 initial fields (set_initial_sph_dynamo.f90:
  * sph_initial_data_control, initial_magne_dynamobench.f90:
  * initial_b_dynamobench_1); isig selects the (m, m) temperature mode. */

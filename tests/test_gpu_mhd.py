@@ -237,3 +237,43 @@ def test_restart_resumes_the_run(torch_cuda, tmp_path):
     assert np.max(np.abs(ea - eb) / np.abs(ea)) < 1e-12
     a.close()
     b.close()
+
+
+def test_snapshot_fields_on_grid(torch_cuda):
+    """sph_back_trans_snapshot_MHD: u, B (vectors) and T on the grid after a few
+    steps, against the oracle's backward Legendre transform + FFT of the same
+    rj spectra, and transformed forward again back to those spectra (Gauss
+    quadrature is exact for these band-limited fields)."""
+    import oracle
+    p, r = mhd.dynamobench_case1()
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_dynamobench_ic(400)
+    sim.initialize()
+    sim.step(5)
+    L, nth, nphi, nr = p.l_truncation, p.nth, p.nphi, p.nr
+    print(f"[snapshot] L={L} nth={nth} nphi={nphi} nr={nr}", flush=True)
+    got = sim.snapshot_rtp(("velocity", "magnetic_field"), ("temperature",))
+    comps = [0, 1, 2, 6, 7, 8, 12]
+    spec = [sim.get_field(c) for c in comps]          # each (nr, J), mode fastest
+    sim.close()
+    J = (L + 1) ** 2
+    rj_of_rlm = np.array([l * (l + 1) + m for m in range(-L, L + 1) for l in range(abs(m), L + 1)])
+    nc = len(comps)
+    sp = np.stack([f[:, rj_of_rlm] for f in spec], axis=-1)   # (nr, J_rlm, nc)
+    grid = oracle.Grid(L, nth, np.asarray(r))
+    vr = oracle.leg_bwd(grid, nc, 2, 1, sp.ravel())
+    ref = oracle.fft_bwd(L, nth, nr, nphi, nc, vr).reshape(nc, nphi, nth, nr)
+    for q in range(nc):
+        err = np.max(np.abs(got[q] - ref[q])) / np.max(np.abs(ref[q]))
+        print(f"   comp {q}: err {err:.2e}", flush=True)
+        assert err < 1e-12
+    back = oracle.leg_fwd(grid, nc, 2, 1, oracle.fft_fwd(L, nth, nr, nphi, nc, got.ravel()))
+    back = back.reshape(nr, J, nc)
+    j00 = L * (L + 1) // 2                 # (l, m) = (0, 0) in rlm order
+    for q in range(nc):
+        d = back[..., q] - sp[..., q]
+        if q < 6:                          # a vector's l = 0 part has no field
+            d[:, j00] = 0.0
+        err = np.max(np.abs(d)) / np.max(np.abs(sp[..., q]))
+        print(f"   comp {q}: round trip {err:.2e}", flush=True)
+        assert err < 1e-10, (q, err)
