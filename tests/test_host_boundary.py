@@ -75,3 +75,29 @@ def test_products_rejects_bad_arguments():
     fld = np.zeros(13)
     assert lib.calypso_nonlinear_products(0, ptr(fld), ptr(fld), ptr(fld)) != 0
     assert "calypso_nonlinear_products" in last_error()
+
+
+def test_no_fallback_without_the_library(tmp_path):
+    """Every entry point fails loudly when the HIP library is absent: there is
+    no CPU path to fall back on."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {root!r})\n"
+        "import calypso_amd, calypso_amd._lib as L\n"
+        f"L.LIB_PATH = {str(tmp_path / 'absent.so')!r}\n"
+        "from calypso_amd import mhd\n"
+        "fails = 0\n"
+        "for make in (lambda: calypso_amd.LegendreTransform(7, 12, np.linspace(0.5, 1.5, 3), 3, 1, 0, 'hip_fp64'),\n"
+        "             lambda: mhd.MhdSimulation(*mhd.dynamobench_case1()),\n"
+        "             lambda: calypso_amd.load()):\n"
+        "    try:\n"
+        "        make()\n"
+        "    except L.CalypsoError as e:\n"
+        "        fails += 'not found' in str(e)\n"
+        "print(fails)\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "3"
