@@ -275,25 +275,34 @@ def large_legendre_bench(torch, steps=3):
             "scalars": ns5, "modes": out}
 
 
-def ensemble_bench(torch, nshell=64, steps=20, nstream=4):
+def ensemble_bench(torch, nshell=64, steps=20, nstream=8, balanced=True):
     """Ensemble of independent shells, Lmax in {31,47,63,95}, Nr in
     {32,48,64,96}, nth = 1.5(L+1), nphi = 2 nth, seeded Ra in [50, 500]: each
-    shell's step is a captured graph; the shells are dealt round-robin over
-    `nstream` streams (default priority) and stepped together.  Reported:
-    shell-steps per second of the ensemble and of the same shells stepped one
-    after another."""
+    shell's step is a captured graph; the shells are dealt over `nstream`
+    streams (default priority) longest first, each to the stream with the
+    least work so far (work of a shell ~ Nr nth (L+1)^2, the Legendre
+    transforms), and stepped together.  Reported: shell-steps per second of
+    the ensemble and of the same shells stepped one after another."""
     from calypso_amd import mhd
     rng = np.random.default_rng(64)
     Ls, Nrs = (31, 47, 63, 95), (32, 48, 64, 96)
     streams = [torch.cuda.Stream() for _ in range(nstream)]
-    sims, shells = [], []
+    specs = []
     for q in range(nshell):
         L_, nr_ = Ls[q % 4], Nrs[(q // 4) % 4]
-        nth = 3 * (L_ + 1) // 2
-        ra = float(rng.uniform(50.0, 500.0))
+        specs.append((L_, nr_, 3 * (L_ + 1) // 2, float(rng.uniform(50.0, 500.0))))
+    load = [0.0] * nstream
+    home = [0] * nshell
+    for q in sorted(range(nshell), key=lambda q: -specs[q][1] * specs[q][2] * (specs[q][0] + 1) ** 2):
+        L_, nr_, nth, _ = specs[q]
+        home[q] = min(range(nstream), key=lambda i: load[i]) if balanced else q % nstream
+        load[home[q]] += nr_ * nth * (L_ + 1) ** 2
+    sims, shells = [], []
+    for q in range(nshell):
+        L_, nr_, nth, ra = specs[q]
         p, r = mhd.dynamobench_case1(l_truncation=L_, nth=nth, nphi=2 * nth, nri=nr_ - 1)
         p.coef_buo = (1.0 - 0.35) * ra / 1.0e-3   # (1 - r_i/r_o) Ra / E, as dynamobench_case1
-        sim = mhd.MhdSimulation(p, r, stream=streams[q % nstream].cuda_stream)
+        sim = mhd.MhdSimulation(p, r, stream=streams[home[q]].cuda_stream)
         sim.set_dynamobench_ic(400)
         sim.initialize()
         sim.step(2)   # Euler start, then the captured Adams-Bashforth step
