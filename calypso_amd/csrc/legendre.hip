@@ -1120,6 +1120,11 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
         rok[r] = rt < RTp;
         rtile[r] = rok[r] ? rt : 0;
     }
+    // the last row group of an order can leave a wave without a degree tile:
+    // it still stages its share of the input but issues no MFMAs
+    bool anyrow = false;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) anyrow |= rok[r];
     // an order with no odd rows (m = L) lets its odd waves read even tiles
     const int pt = RTp > 0 ? par : 0;
     const elem *tab = reinterpret_cast<const elem *>(a.tab);
@@ -1294,7 +1299,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
             put(t0);
             __syncthreads();
             if (t0 + TC < nks * KS_) fetch(t0 + TC);
-            const int kse = min(nks, (t0 + TC) / KS_);
+            const int kse = anyrow ? min(nks, (t0 + TC) / KS_) : 0;
             for (int kb = t0 / KS_; kb < kse; kb += PD) {
 #pragma unroll
                 for (int d = 0; d < PD; ++d) {
@@ -1815,6 +1820,11 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwdz(LegArgs a)
         rok[r] = rt < RTp;
         rtile[r] = rok[r] ? rt : 0;
     }
+    // a wave without a degree tile (last row group of an order) stages and
+    // splits its share of the input but issues no MFMAs
+    bool anyrow = false;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) anyrow |= rok[r];
     const int pt = RTp > 0 ? par : 0;
     const signed char *tab = reinterpret_cast<const signed char *>(a.tab);
     const signed char *ap[RPW];
@@ -1981,6 +1991,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwdz(LegArgs a)
             __syncthreads();
             if (ks + 1 < nks) fetch(t0 + TC);
             // (3) the MFMAs of the step
+            if (!anyrow) continue;
 #pragma unroll
             for (int ct = 0; ct < NC1; ++ct) {
                 i4v b[kZD];
