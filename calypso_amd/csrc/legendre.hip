@@ -1396,6 +1396,10 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
         rok[r] = rt < a.RT;
         rtile[r] = rok[r] ? rt : 0;
     }
+    // a wave past the last colatitude tile stages its share but issues no MFMAs
+    bool anyrow = false;
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) anyrow |= rok[r];
     const elem *tab = reinterpret_cast<const elem *>(a.tab);
     const elem *ap[RPW];
 #pragma unroll
@@ -1577,7 +1581,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
             put(c0);
             __syncthreads();
             if (c0 + KC < KS * KS_) fetch(c0 + KC);
-            const int kse = min(KS, (c0 + KC) / KS_);
+            const int kse = anyrow ? min(KS, (c0 + KC) / KS_) : 0;
             for (int kb = c0 / KS_; kb < kse; kb += PD) {
 #pragma unroll
                 for (int d = 0; d < PD; ++d) {
