@@ -279,3 +279,36 @@ def test_snapshot_fields_on_grid(torch_cuda):
         err = np.max(np.abs(d)) / np.max(np.abs(sp[..., q]))
         print(f"   comp {q}: round trip {err:.2e}", flush=True)
         assert err < 1e-10, (q, err)
+
+
+def test_concurrent_shells_match_alone(torch_cuda):
+    """Shells of different sizes stepped together on several streams (the
+    ensemble bench) give bitwise the energies of each shell stepped alone:
+    contexts share no device state."""
+    torch = torch_cuda
+    specs = [(31, 48, 33), (47, 72, 25), (31, 48, 17), (63, 96, 33)]
+
+    def make(L, nth, nr, stream=None):
+        p, r = mhd.dynamobench_case1(l_truncation=L, nth=nth, nphi=2 * nth, nri=nr - 1)
+        sim = mhd.MhdSimulation(p, r, stream=stream)
+        sim.set_dynamobench_ic(400)
+        sim.initialize()
+        return sim
+
+    alone = []
+    for L, nth, nr in specs:
+        print(f"[shells] alone L={L} nth={nth} nr={nr}", flush=True)
+        sim = make(L, nth, nr)
+        sim.step(6)
+        alone.append(sim.energies())
+        sim.close()
+    streams = [torch.cuda.Stream() for _ in range(2)]
+    sims = [make(L, nth, nr, streams[q % 2].cuda_stream) for q, (L, nth, nr) in enumerate(specs)]
+    print("[shells] together on 2 streams", flush=True)
+    for _ in range(6):
+        for sim in sims:
+            sim.step(1)
+    torch.cuda.synchronize()
+    for q, sim in enumerate(sims):
+        assert np.array_equal(sim.energies(), alone[q]), q
+        sim.close()
