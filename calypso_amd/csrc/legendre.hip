@@ -1067,9 +1067,12 @@ __device__ inline StageCols stage_cols(int comp, int kk, int sg, int nv, int ns,
 // after the other (everything that depends only on m is formed once).
 //   c1 = P  * B1          (S_pol, scalars)
 //   c2 = dP * X + P * Y   (dS, T), Y staged with m/sin(theta)
-template <class T, int RPW, int NC1, int NC2, int PD>
-__global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
+template <class T, int RPW, int NC1, int NC2, int PD, int NW = 4>
+__global__ __launch_bounds__(NW * 64) void k_leg_fwd3(LegArgs a)
 {
+    // NW waves: NW/2 degree tiles of each parity per row group share one
+    // staged input
+    constexpr int NT = NW * 64;
     using elem = typename T::elem;
     using frag = typename T::frag;
     using acc = typename T::acc;
@@ -1100,9 +1103,9 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
     // equator row of an odd grid is its own mirror and gets half the weight
     double2 *WF = reinterpret_cast<double2 *>(
         smem_raw + ((NS * slice * sizeof(elem) + 15) & ~(size_t)15));
-    for (int x = threadIdx.x; x < 2 * TCP; x += kThreads)
+    for (int x = threadIdx.x; x < 2 * TCP; x += NT)
         T::store(V + ((x / TCP) * NB + CZ) * TCP + x % TCP, 0.0, slice);
-    for (int t = threadIdx.x; t < nks * KS_; t += kThreads) {
+    for (int t = threadIdx.x; t < nks * KS_; t += NT) {
         double w = 0.0, y = 0.0;
         if (t < a.th_h) {
             w = a.weight[t];
@@ -1116,7 +1119,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
     bool rok[RPW];
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-        const int rt = (it.rgroup * 2 + (wave >> 1)) * RPW + r;
+        const int rt = (it.rgroup * (NW / 2) + (wave >> 1)) * RPW + r;
         rok[r] = rt < RTp;
         rtile[r] = rok[r] ? rt : 0;
     }
@@ -1186,7 +1189,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_fwd3(LegArgs a)
     int s_src[kSlotF], s_dst[kSlotF], s_meta[kSlotF];
 #pragma unroll
     for (int i = 0; i < kSlotF; ++i) {
-        const int q = threadIdx.x + i * kThreads;
+        const int q = threadIdx.x + i * NT;
         s_src[i] = -2;   // no slot
         s_dst[i] = 0;
         s_meta[i] = 0;
@@ -2573,6 +2576,12 @@ static void plan_candidates3(int ncomp, int nv, int ns, bool fwd, int prec,
                     if (plan_lds(p, fwd, prec) > 80 * 1024) continue;
                     out->push_back(p);
                     ++taken;
+                    // the same plan with 8 waves: four degree tiles of a
+                    // parity per row group share one staged input
+                    if (fwd && rpw == 1 && prec != kBF16x3) {
+                        p.nw = 8;
+                        out->push_back(p);
+                    }
                 }
             }
     }
@@ -2717,7 +2726,8 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
         }
         auto ngroup = [&](int m) {
             const int K = g.L - m + 1, RTe = ((K + 1) / 2 + 15) / 16, RTo = (K / 2 + 15) / 16;
-            return fwd ? (std::max(RTe, RTo) + 2 * p->rpw - 1) / (2 * p->rpw)
+            const int per = (p->nw / 2) * p->rpw;   // degree tiles of a parity per group
+            return fwd ? (std::max(RTe, RTo) + per - 1) / per
                        : (c->rt_bwd + 4 * p->rpw - 1) / (4 * p->rpw);
         };
         // radial tiles per work-group: as many as keep >= 2048 work-groups
@@ -3064,7 +3074,7 @@ static int launch_t(LegContext *c, const LegPlan &p, bool fwd, const double *in,
     return 0;
 }
 
-template <class T, int RPW, int NC1, int NC2, int PD>
+template <class T, int RPW, int NC1, int NC2, int PD, int NW = 4>
 static int launch3_f(LegContext *c, const LegPlan &p, const double *in, double *out,
                      hipStream_t s, std::string *err)
 {
@@ -3072,9 +3082,9 @@ static int launch3_f(LegContext *c, const LegPlan &p, const double *in, double *
     const size_t lds = plan_lds(p, true, c->prec, c->ks_fwd3 * T::KSTEP);
     if (lds > 160 * 1024) { *err = "LDS tile too large for this grid"; return -1; }
     if (lds > 64 * 1024)
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_leg_fwd3<T, RPW, NC1, NC2, PD>),
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_leg_fwd3<T, RPW, NC1, NC2, PD, NW>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_leg_fwd3<T, RPW, NC1, NC2, PD>), dim3(p.n_items), dim3(kThreads), lds, s, a);
+    hipLaunchKernelGGL((k_leg_fwd3<T, RPW, NC1, NC2, PD, NW>), dim3(p.n_items), dim3(NW * 64), lds, s, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -3098,7 +3108,10 @@ static int launch3_n(LegContext *c, const LegPlan &p, bool fwd, const double *in
                      hipStream_t s, std::string *err)
 {
     if constexpr (T::NS == 1) {
-        if (fwd) {
+        if (fwd && p.nw == 8) {
+            if (p.rpw == 1 && p.pd == 2) return launch3_f<T, 1, NC1, NC2, 2, 8>(c, p, in, out, s, err);
+            if (p.rpw == 1 && p.pd == 4) return launch3_f<T, 1, NC1, NC2, 4, 8>(c, p, in, out, s, err);
+        } else if (fwd) {
             if (p.rpw == 1 && p.pd == 2) return launch3_f<T, 1, NC1, NC2, 2>(c, p, in, out, s, err);
             if (p.rpw == 1 && p.pd == 4) return launch3_f<T, 1, NC1, NC2, 4>(c, p, in, out, s, err);
             if (p.rpw == 2 && p.pd == 2) return launch3_f<T, 2, NC1, NC2, 2>(c, p, in, out, s, err);
