@@ -1361,9 +1361,11 @@ __global__ __launch_bounds__(NW * 64) void k_leg_fwd3(LegArgs a)
 // (folded output S +- A).
 //   a1 = P * B1                         (v_r, scalars)
 //   ax = dP * X,  ay = P * Y;  v_theta, v_phi = ax + (m/sin(theta)) ay
-template <class T, int RPW, int NC1, int NC2, int PD>
-__global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
+template <class T, int RPW, int NC1, int NC2, int PD, int NW = 4>
+__global__ __launch_bounds__(NW * 64) void k_leg_bwd3(LegArgs a)
 {
+    // NW waves: NW colatitude tiles per row group share one staged input
+    constexpr int NT = NW * 64;
     using elem = typename T::elem;
     using frag = typename T::frag;
     using acc = typename T::acc;
@@ -1388,14 +1390,14 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
     elem *X = reinterpret_cast<elem *>(smem_raw);   // [NS][2 par][NB][KCP]
     const int slice = 2 * NB * KCP;
     __shared__ double rinv[8];                      // 1/r of the current radial tile
-    for (int x = threadIdx.x; x < 2 * KCP; x += kThreads)
+    for (int x = threadIdx.x; x < 2 * KCP; x += NT)
         T::store(X + ((x / KCP) * NB + CZ) * KCP + x % KCP, 0.0, slice);
 
     int rtile[RPW];
     bool rok[RPW];
 #pragma unroll
     for (int r = 0; r < RPW; ++r) {
-        const int rt = (it.rgroup * 4 + wave) * RPW + r;
+        const int rt = (it.rgroup * NW + wave) * RPW + r;
         rok[r] = rt < a.RT;
         rtile[r] = rok[r] ? rt : 0;
     }
@@ -1453,7 +1455,7 @@ __global__ __launch_bounds__(kThreads) void k_leg_bwd3(LegArgs a)
     int s_src[kSlotB], s_dst[kSlotB], s_meta[kSlotB];
 #pragma unroll
     for (int i = 0; i < kSlotB; ++i) {
-        const int q = threadIdx.x + i * kThreads;
+        const int q = threadIdx.x + i * NT;
         s_src[i] = -2;   // no slot
         s_dst[i] = 0;
         s_meta[i] = 0;
@@ -2576,9 +2578,10 @@ static void plan_candidates3(int ncomp, int nv, int ns, bool fwd, int prec,
                     if (plan_lds(p, fwd, prec) > 80 * 1024) continue;
                     out->push_back(p);
                     ++taken;
-                    // the same plan with 8 waves: four degree tiles of a
-                    // parity per row group share one staged input
-                    if (fwd && rpw == 1 && prec != kBF16x3) {
+                    // the same plan with 8 waves: twice the row tiles (four
+                    // degree tiles of a parity, or eight colatitude tiles)
+                    // per row group share one staged input
+                    if (rpw == 1 && prec != kBF16x3) {
                         p.nw = 8;
                         out->push_back(p);
                     }
@@ -2728,7 +2731,7 @@ static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
             const int K = g.L - m + 1, RTe = ((K + 1) / 2 + 15) / 16, RTo = (K / 2 + 15) / 16;
             const int per = (p->nw / 2) * p->rpw;   // degree tiles of a parity per group
             return fwd ? (std::max(RTe, RTo) + per - 1) / per
-                       : (c->rt_bwd + 4 * p->rpw - 1) / (4 * p->rpw);
+                       : (c->rt_bwd + p->nw * p->rpw - 1) / (p->nw * p->rpw);
         };
         // radial tiles per work-group: as many as keep >= 2048 work-groups
         long long n1 = 0;
@@ -3089,16 +3092,16 @@ static int launch3_f(LegContext *c, const LegPlan &p, const double *in, double *
     return 0;
 }
 
-template <class T, int RPW, int NC1, int NC2, int PD>
+template <class T, int RPW, int NC1, int NC2, int PD, int NW = 4>
 static int launch3_b(LegContext *c, const LegPlan &p, const double *in, double *out,
                      hipStream_t s, std::string *err)
 {
     LegArgs a = make_args(c, p, false, in, out);
     const size_t lds = plan_lds(p, false, c->prec);
     if (lds > 64 * 1024)
-        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_leg_bwd3<T, RPW, NC1, NC2, PD>),
+        HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_leg_bwd3<T, RPW, NC1, NC2, PD, NW>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_leg_bwd3<T, RPW, NC1, NC2, PD>), dim3(p.n_items), dim3(kThreads), lds, s, a);
+    hipLaunchKernelGGL((k_leg_bwd3<T, RPW, NC1, NC2, PD, NW>), dim3(p.n_items), dim3(NW * 64), lds, s, a);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -3116,6 +3119,8 @@ static int launch3_n(LegContext *c, const LegPlan &p, bool fwd, const double *in
             if (p.rpw == 1 && p.pd == 4) return launch3_f<T, 1, NC1, NC2, 4>(c, p, in, out, s, err);
             if (p.rpw == 2 && p.pd == 2) return launch3_f<T, 2, NC1, NC2, 2>(c, p, in, out, s, err);
             if (p.rpw == 2 && p.pd == 4) return launch3_f<T, 2, NC1, NC2, 4>(c, p, in, out, s, err);
+        } else if (p.nw == 8) {
+            if (p.rpw == 1 && p.pd == 2) return launch3_b<T, 1, NC1, NC2, 2, 8>(c, p, in, out, s, err);
         } else {
             if (p.rpw == 1 && p.pd == 2) return launch3_b<T, 1, NC1, NC2, 2>(c, p, in, out, s, err);
             if constexpr (NC2 == 1)
