@@ -50,8 +50,10 @@ def kernel_name(direction, mode, plan):
         return (f"k_leg_{direction}z<6, {plan['rows_per_wave']}, {plan['mct1']}, "
                 f"{plan['mct2']}>")
     t = {"hip_fp64": "TF64", "hip_fp32": "TF32", "hip_bf16x3": "TBF3"}[mode]
-    form = "3" if plan["kernel_form"] == 3 else ""
-    return (f"k_leg_{direction}{form}<{t}, {plan['rows_per_wave']}, {plan['mct1']}, "
+    if plan["kernel_form"] == 3:   # third form: waves per work-group last
+        return (f"k_leg_{direction}3<{t}, {plan['rows_per_wave']}, {plan['mct1']}, "
+                f"{plan['mct2']}, {plan['prefetch']}, {plan.get('waves', 4)}>")
+    return (f"k_leg_{direction}<{t}, {plan['rows_per_wave']}, {plan['mct1']}, "
             f"{plan['mct2']}, {plan['prefetch']}>")
 
 
