@@ -41,13 +41,36 @@ def _run(cmd, cwd=None):
     return out
 
 
-def build_hip():
+def build_hip(incremental=False):
+    """Compile each source to its own object in parallel (every kernel is
+    launched from the file that defines it, so no relocatable device code is
+    needed), then link the shared library.  incremental=True skips objects
+    newer than their source and every header."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIB, exist_ok=True)
+    obj_dir = os.path.join(LIB, "hobj")
+    os.makedirs(obj_dir, exist_ok=True)
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    objs = [os.path.join(obj_dir, s.rsplit(".", 1)[0] + ".o") for s in HIP_SOURCES]
+
+    hdr_time = max(os.path.getmtime(os.path.join(d, f))
+                   for d in (CSRC, os.path.join(ROOT, "include"))
+                   for f in os.listdir(d) if f.endswith((".hpp", ".h")))
+
+    def compile_one(i):
+        src = os.path.join(CSRC, HIP_SOURCES[i])
+        if (incremental and os.path.exists(objs[i])
+                and os.path.getmtime(objs[i]) > max(os.path.getmtime(src), hdr_time)):
+            return
+        _run([hipcc, *cflags, "-I", os.path.join(ROOT, "include"), "-c",
+              os.path.join(CSRC, HIP_SOURCES[i]), "-o", objs[i]])
+
+    with ThreadPoolExecutor(max_workers=min(len(HIP_SOURCES), 6)) as ex:
+        list(ex.map(compile_one, range(len(HIP_SOURCES))))
     out = os.path.join(LIB, "libcalypso_amd.so")
     tmp = out + ".tmp"
-    _run([hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"),
-          *[os.path.join(CSRC, s) for s in HIP_SOURCES], "-o", tmp])
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 
