@@ -55,6 +55,7 @@ SIGNATURES = [
     ("calypso_lubksb_band_mul_device", _I, [_I, _I, _I, _P, _P, _P, _P]),
     ("calypso_explicit_update_device", _I, [_I, _I, ctypes.c_longlong, _D, _D, _P, _P, _P, _P, _P]),
     ("calypso_sph_mean_square", _I, [_I, _I, _I, _P, _P, _I, _I, _P, _P, _P]),
+    ("calypso_sph_mean_square_noalloc", _I, [_I, _I, _I, _P, _P, _I, _I, _P, _P, _P]),
     ("calypso_mhd_create", _I, [_P, _P, _P, ctypes.POINTER(_P)]),
     ("calypso_mhd_destroy", _I, [_P]),
     ("calypso_mhd_set_field", _I, [_P, _I, _P]),

@@ -477,6 +477,19 @@ int calypso_sph_mean_square(int kind, int nr, int J, const int *l_of_j, const do
     return 0;
 }
 
+int calypso_sph_mean_square_noalloc(int kind, int nr, int J, const int *l_of_j,
+                                    const double *radius, int kr_st, int kr_ed, const double *d_f,
+                                    double *out, void *stream)
+{
+    if (nr < 2 || J < 1 || !l_of_j || !radius || !d_f || !out)
+        return fail("calypso_sph_mean_square_noalloc: bad argument");
+    std::string err;
+    if (calypso::mean_square_noalloc_dev(kind, nr, J, l_of_j, radius, kr_st, kr_ed, d_f, out,
+                                         (hipStream_t)stream, &err))
+        return fail("calypso_sph_mean_square_noalloc: " + err);
+    return 0;
+}
+
 int calypso_mhd_create(const calypso_mhd_params *pp, const double *radius, void *stream,
                        calypso_mhd_t **out)
 {

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "radial_internal.hpp"
@@ -621,6 +622,91 @@ int mean_square_dev(int kind, int nr, int J, int lmax, const int *d_l_of_j,
     hipLaunchKernelGGL(k_ms_reduce, dim3(1), dim3(256), sizeof(double) * ncp * (lmax + 1), s, kind,
                        J, lmax, d_l_of_j, d_vol, h_radius[kr_st - 1], h_radius[kr_ed - 1], d_out);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// ---- allocation-free volume mean square -----------------------------------
+// The same two kernels on a file-scope device array instead of buffers
+// allocated per call: the per-mode volume integrals (3 x kMsMaxJ doubles), the
+// result (3, padded to 32), the radial grid (kMsMaxNr) and the degrees of the
+// modes (kMsMaxJ int32), 7.4 MB on every GPU the library runs on, held until
+// the process ends.  Every element the kernels read is written earlier in the
+// same call (the grid and degrees by the copies, the integrals by k_ms_modes,
+// the result by k_ms_reduce), so nothing needs clearing between calls; calls
+// that overlap on one GPU race on this array.
+constexpr int kMsMaxJ = 262144;   // (L+1)^2 at L = 511
+constexpr int kMsMaxNr = 4096;
+constexpr int kMsMaxL = 2047;     // k_ms_reduce's LDS: 3 x 2048 doubles
+constexpr size_t kMsVolOff = 0;
+constexpr size_t kMsOutOff = kMsVolOff + 3 * (size_t)kMsMaxJ;
+constexpr size_t kMsRadOff = kMsOutOff + 32;
+constexpr size_t kMsLOff = kMsRadOff + kMsMaxNr;
+constexpr size_t kMsScratchDoubles = kMsLOff + kMsMaxJ / 2;
+
+static __device__ __align__(256) double scratch[kMsScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array at its own address).
+int scratch_on_device(double **out, std::string *err)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        *err = "mean square (no-alloc): device index out of range";
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            *err = "mean square (no-alloc): no device address for the scratch array";
+            return -1;
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+int mean_square_noalloc_dev(int kind, int nr, int J, const int *h_l_of_j, const double *h_radius,
+                            int kr_st, int kr_ed, const double *d_f, double *h_out,
+                            hipStream_t s, std::string *err)
+{
+    if (J < 1 || J > kMsMaxJ || nr < 2 || nr > kMsMaxNr) {
+        *err = "mean square (no-alloc): needs 1 <= J <= " + std::to_string(kMsMaxJ) +
+               " and 2 <= nr <= " + std::to_string(kMsMaxNr) + " (got J=" + std::to_string(J) +
+               ", nr=" + std::to_string(nr) + ")";
+        return -1;
+    }
+    int lmax = 0;
+    for (int j = 0; j < J; ++j) {
+        if (h_l_of_j[j] < 0 || h_l_of_j[j] > kMsMaxL) {
+            *err = "mean square (no-alloc): degrees must lie in 0.." + std::to_string(kMsMaxL);
+            return -1;
+        }
+        lmax = std::max(lmax, h_l_of_j[j]);
+    }
+    double *base = nullptr;
+    if (scratch_on_device(&base, err)) return -1;
+    double *d_vol = base + kMsVolOff, *d_out = base + kMsOutOff, *d_r = base + kMsRadOff;
+    int *d_l = reinterpret_cast<int *>(base + kMsLOff);
+    HIP_TRY(hipMemcpyAsync(d_l, h_l_of_j, sizeof(int) * J, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_r, h_radius, sizeof(double) * nr, hipMemcpyHostToDevice, s));
+    if (mean_square_dev(kind, nr, J, lmax, d_l, d_r, h_radius, kr_st, kr_ed, d_f, d_vol, d_out, s,
+                        err))
+        return -1;
+    double h[3] = {0.0, 0.0, 0.0};
+    const int ncp = (kind == 2) ? 1 : 3;
+    HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(double) * ncp, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int c = 0; c < 3; ++c) h_out[c] = c < ncp ? h[c] : 0.0;
     return 0;
 }
 

@@ -29,5 +29,9 @@ int mean_square_dev(int kind, int nr, int J, int lmax, const int *d_l_of_j,
                     const double *d_radius, const double *h_radius, int kr_st, int kr_ed,
                     const double *d_f, double *d_vol, double *d_out, hipStream_t s,
                     std::string *err);
+// the same on a file-scope device scratch array: no device allocation
+int mean_square_noalloc_dev(int kind, int nr, int J, const int *h_l_of_j, const double *h_radius,
+                            int kr_st, int kr_ed, const double *d_f, double *h_out,
+                            hipStream_t s, std::string *err);
 
 }  // namespace calypso

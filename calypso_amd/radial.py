@@ -46,14 +46,15 @@ def explicit_update(kind, scheme, dt, coef_exp, x, diffuse, force, pre=None, str
         _stream(stream)), "calypso_explicit_update_device")
 
 
-def sph_mean_square(kind, radius, l_of_j, kr_st, kr_ed, f, stream=None):
-    """Volume mean square of one field; f is a (ncomp, nr, J) device tensor."""
+def sph_mean_square(kind, radius, l_of_j, kr_st, kr_ed, f, stream=None, noalloc=False):
+    """Volume mean square of one field; f is a (ncomp, nr, J) device tensor.
+    noalloc=True uses the entry point that allocates no device memory."""
     import numpy as np
     radius = np.ascontiguousarray(radius, dtype=np.float64)
     l_of_j = np.ascontiguousarray(l_of_j, dtype=np.int32)
     out = np.zeros(3)
-    _lib.check(_lib.load().calypso_sph_mean_square(
+    name = "calypso_sph_mean_square_noalloc" if noalloc else "calypso_sph_mean_square"
+    _lib.check(getattr(_lib.load(), name)(
         kind, radius.size, l_of_j.size, l_of_j.ctypes.data_as(_P), radius.ctypes.data_as(_P),
-        kr_st, kr_ed, _dev(f, f.numel(), "f"), out.ctypes.data_as(_P), _stream(stream)),
-        "calypso_sph_mean_square")
+        kr_st, kr_ed, _dev(f, f.numel(), "f"), out.ctypes.data_as(_P), _stream(stream)), name)
     return out

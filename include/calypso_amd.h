@@ -242,6 +242,18 @@ int calypso_sph_mean_square(int kind, int nr, int J, const int *l_of_j,
                             const double *radius, int kr_st, int kr_ed,
                             const double *d_f, double *out, void *stream);
 
+/* calypso_sph_mean_square without any device allocation: no counterpart in
+ * the reference; for callers that cannot call hipMalloc (diagnostics inside
+ * tight serving or time-stepping loops, harnesses that forbid allocation).
+ * Same arguments and bitwise the same results.  The per-mode partial sums,
+ * the radial grid and the degrees live in one library-owned device array of
+ * 7.4 MB per GPU (held until the process ends), so J <= 262144 (L <= 511),
+ * nr <= 4096 and degrees 0..2047; larger inputs are rejected with an error.
+ * Calls that overlap on one GPU share that array and race. */
+int calypso_sph_mean_square_noalloc(int kind, int nr, int J, const int *l_of_j,
+                                    const double *radius, int kr_st, int kr_ed,
+                                    const double *d_f, double *out, void *stream);
+
 /* ---- the full MHD time step, device resident (FP64 state) ---------------
  * Replaces the body of module SPH_analyzer_MHD: SPH_initialize_MHD and
  * SPH_analyze_MHD(i_step, iflag_finish) (programs/SPH_MHD/
