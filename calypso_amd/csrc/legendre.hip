@@ -3257,9 +3257,10 @@ static int autotune_plan(LegContext *c, std::vector<LegPlan> &cand, bool fwd, st
     const LegPlan &p0 = cand[0];
     const size_t nrlm = (size_t)p0.ncomp * c->g.J * c->g.nr;
     const size_t nrtm = (size_t)p0.ncomp * c->g.nth * c->g.nr * c->g.M;
-    double *din = nullptr, *dout = nullptr;
-    HIP_TRY(hipMalloc(&din, sizeof(double) * (fwd ? nrtm : nrlm)));
-    HIP_TRY(hipMalloc(&dout, sizeof(double) * (fwd ? nrlm : nrtm)));
+    DevBuf in_buf, out_buf;   // released on every exit path
+    HIP_TRY(hipMalloc(&in_buf.p, sizeof(double) * (fwd ? nrtm : nrlm)));
+    HIP_TRY(hipMalloc(&out_buf.p, sizeof(double) * (fwd ? nrlm : nrtm)));
+    double *din = static_cast<double *>(in_buf.p), *dout = static_cast<double *>(out_buf.p);
     HIP_TRY(hipMemsetAsync(din, 0, sizeof(double) * (fwd ? nrtm : nrlm), c->stream));
     int best = -1;
     float best_ms = 0.f;
@@ -3267,10 +3268,12 @@ static int autotune_plan(LegContext *c, std::vector<LegPlan> &cand, bool fwd, st
         std::string e2;
         if (launch_plan(c, cand[i], fwd, din, dout, c->stream, &e2)) continue;
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
-        for (int r = 0; r < 3; ++r)
-            if (launch_plan(c, cand[i], fwd, din, dout, c->stream, &e2)) break;
+        bool failed = false;
+        for (int r = 0; r < 3 && !failed; ++r)
+            failed = launch_plan(c, cand[i], fwd, din, dout, c->stream, &e2) != 0;
         HIP_TRY(hipEventRecord(c->ev1, c->stream));
         HIP_TRY(hipEventSynchronize(c->ev1));
+        if (failed) continue;   // a partial timing never wins
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         cand[i].tuned_ms = ms / 3.f;
@@ -3279,8 +3282,6 @@ static int autotune_plan(LegContext *c, std::vector<LegPlan> &cand, bool fwd, st
             best_ms = cand[i].tuned_ms;
         }
     }
-    (void)hipFree(din);
-    (void)hipFree(dout);
     if (best < 0) *err = "no Legendre tile plan could be launched";
     return best;
 }

@@ -995,10 +995,13 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     // LU on the GPU (ludcmp_band / ludcmp_3band)
     const std::vector<double> *mats[5] = {&vp, &vt, &tm, &bs, &bt};
     const int nbs[5] = {5, 3, 3, 3, 3};
-    double *d_a = nullptr;
-    int *d_ierr = nullptr;
-    HIP_TRY(hipMalloc(&d_a, sizeof(double) * 5 * nnod));
-    HIP_TRY(hipMalloc(&d_ierr, sizeof(int) * J));
+    // temporaries released on every exit path; the context's own buffers
+    // are released by mhd_free
+    DevBuf a_buf, ierr_buf;
+    HIP_TRY(hipMalloc(&a_buf.p, sizeof(double) * 5 * nnod));
+    HIP_TRY(hipMalloc(&ierr_buf.p, sizeof(int) * J));
+    double *d_a = static_cast<double *>(a_buf.p);
+    int *d_ierr = static_cast<int *>(ierr_buf.p);
     for (int q = 0; q < 5; ++q) {
         const int nb = nbs[q];
         HIP_TRY(hipMemcpy(d_a, mats[q]->data(), sizeof(double) * nb * nnod, hipMemcpyHostToDevice));
@@ -1021,16 +1024,17 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
                 }
         }
         {
-            double *bt = nullptr;
-            int *pt = nullptr;
-            HIP_TRY(hipMalloc(&bt, sizeof(double) * (2 * nb - 1) * nnod));
-            HIP_TRY(hipMalloc(&pt, sizeof(int) * nnod));
-            if (lu_transpose_dev(nb, nr, J, c->d_lu[q], c->d_piv[q], bt, pt, s, err)) return -1;
+            DevBuf bt, pt;
+            HIP_TRY(hipMalloc(&bt.p, sizeof(double) * (2 * nb - 1) * nnod));
+            HIP_TRY(hipMalloc(&pt.p, sizeof(int) * nnod));
+            if (lu_transpose_dev(nb, nr, J, c->d_lu[q], c->d_piv[q], static_cast<double *>(bt.p),
+                                 static_cast<int *>(pt.p), s, err))
+                return -1;
             HIP_TRY(hipStreamSynchronize(s));
             (void)hipFree(c->d_lu[q]);
             (void)hipFree(c->d_piv[q]);
-            c->d_lu[q] = bt;
-            c->d_piv[q] = pt;
+            c->d_lu[q] = static_cast<double *>(bt.release());
+            c->d_piv[q] = static_cast<int *>(pt.release());
         }
         std::vector<int> ierr(J);
         HIP_TRY(hipMemcpy(ierr.data(), d_ierr, sizeof(int) * J, hipMemcpyDeviceToHost));
@@ -1040,8 +1044,6 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
                 return -1;
             }
     }
-    (void)hipFree(d_a);
-    (void)hipFree(d_ierr);
     // transforms
     if (leg_context_build(&c->leg, L, nth, nr, radius, nullptr, nullptr, kNFwd, 4, 0, leg_mode, err))
         return -1;
@@ -1404,14 +1406,16 @@ int mhd_steps(MhdContext *c, int nsteps, std::string *err)
         return -1;
     }
     for (int it = 0; it < nsteps; ++it) {
-        c->i_step += 1;
-        const int first = c->i_step == 1;
+        // the step count (and so the model time) advances only once the
+        // step's work has been issued
+        const int first = c->i_step + 1 == 1;
         if (c->use_graph && !first) {
             if (!c->gexec && capture_step(c, err)) return -1;
             HIP_TRY(hipGraphLaunch(c->gexec, c->stream));
         } else {
             if (step_body(c, first, err)) return -1;
         }
+        c->i_step += 1;
     }
     return 0;
 }

@@ -312,3 +312,18 @@ def test_concurrent_shells_match_alone(torch_cuda):
     for q, sim in enumerate(sims):
         assert np.array_equal(sim.energies(), alone[q]), q
         sim.close()
+
+
+def test_failed_step_leaves_the_step_count(torch_cuda):
+    """A step that cannot run (no initialisation yet) reports an error and
+    leaves the step count and the model time where they were."""
+    p, r = mhd.dynamobench_case1()
+    sim = mhd.MhdSimulation(p, r)
+    with pytest.raises(Exception, match="initialisation"):
+        sim.step(3)
+    assert sim.i_step == 0 and sim.time == 0.0
+    sim.set_dynamobench_ic(400)
+    sim.initialize()
+    sim.step(2)
+    assert sim.i_step == 2 and abs(sim.time - 2 * p.dt) < 1e-18
+    sim.close()
