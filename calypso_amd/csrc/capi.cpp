@@ -12,6 +12,7 @@
 #include "leg_internal.hpp"
 #include "mhd_internal.hpp"
 #include "radial_internal.hpp"
+#include "seeded.hpp"
 
 using calypso::LegContext;
 
@@ -568,6 +569,26 @@ int calypso_mhd_set_dynamobench_ic(calypso_mhd_t *mhd, int isig)
     if (!mhd) return fail("calypso_mhd_set_dynamobench_ic: null handle");
     std::string err;
     if (calypso::mhd_set_dynamobench_ic(&mhd->c, isig, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_set_seeded_ic(calypso_mhd_t *mhd, unsigned long long seed, double amp_u,
+                              double amp_b, double amp_t)
+{
+    if (!mhd) return fail("calypso_mhd_set_seeded_ic: null handle");
+    std::string err;
+    if (calypso::mhd_set_seeded_ic(&mhd->c, seed, amp_u, amp_b, amp_t, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_seeded_spectrum(unsigned long long seed, int field, int l_truncation, int nr,
+                            const double *radius, double amplitude, double power, int envelope,
+                            int l_min, double *out)
+{
+    if (l_truncation < 0 || nr < 1 || !radius || !out || (envelope && nr < 2))
+        return fail("calypso_seeded_spectrum: bad argument");
+    calypso::seeded_spectrum(seed, field, l_truncation, nr, radius, amplitude, power,
+                             envelope != 0, l_min, out);
     return 0;
 }
 

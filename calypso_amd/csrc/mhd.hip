@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "mhd_internal.hpp"
+#include "seeded.hpp"
 #include "radial_internal.hpp"
 
 namespace calypso {
@@ -1252,6 +1253,42 @@ int mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err)
         {kVelo, &zero}, {kVelo + 1, &zero}, {kVelo + 2, &zero}, {kTemp, &temp},
         {kMagne, &bp}, {kMagne + 1, &bd}, {kMagne + 2, &bt},
         {kCurrent, &jp}, {kCurrent + 1, &jd}, {kCurrent + 2, &jt}};
+    for (const auto &sv : sets)
+        if (mhd_field_io(c, sv.first, const_cast<double *>(sv.second->data()), false, err)) return -1;
+    return 0;
+}
+
+// Seeded random initial fields: poloidal and toroidal velocity and magnetic
+// field (l >= 1) and the temperature perturbation (l >= 0), each a
+// counter-based Gaussian per (field, radius, mode) with amplitude (l+1)^-2
+// times the wall envelope (seeded.hpp), so the walls see zero velocity,
+// field and perturbation with zero radial derivatives.  The temperature is
+// stored as the full temperature (reference profile in the l = 0 mode plus
+// the perturbation), as for the benchmark start; the derived fields are
+// formed by mhd_initialize.
+int mhd_set_seeded_ic(MhdContext *c, unsigned long long seed, double amp_u, double amp_b,
+                      double amp_t, std::string *err)
+{
+    const int nr = c->nr, J = c->J, L = c->L;
+    const size_t n = (size_t)nr * J;
+    const double *r = c->radius.data();
+    std::vector<double> zero(n, 0.0), up(n), ut(n), bp(n), bt(n), temp(n);
+    seeded_spectrum(seed, kVelo, L, nr, r, amp_u, 2.0, true, 1, up.data());
+    seeded_spectrum(seed, kVelo + 2, L, nr, r, amp_u, 2.0, true, 1, ut.data());
+    seeded_spectrum(seed, kMagne, L, nr, r, amp_b, 2.0, true, 1, bp.data());
+    seeded_spectrum(seed, kMagne + 2, L, nr, r, amp_b, 2.0, true, 1, bt.data());
+    seeded_spectrum(seed, kTemp, L, nr, r, amp_t, 2.0, true, 0, temp.data());
+    const MhdCoefs &co = c->co;
+    const double dh = co.depth_high_t, dl = co.depth_low_t, hi = co.high_temp, lo = co.low_temp;
+    for (int k = 0; k < nr; ++k) {
+        const double ar1 = 1.0 / r[k];
+        const double ref = (dh * dl * ar1 * (hi - lo) - dh * hi + dl * lo) / (dl - dh);
+        temp[(size_t)J * k] = ref + temp[(size_t)J * k];
+    }
+    const std::pair<int, const std::vector<double> *> sets[] = {
+        {kVelo, &up}, {kVelo + 1, &zero}, {kVelo + 2, &ut}, {kTemp, &temp},
+        {kMagne, &bp}, {kMagne + 1, &zero}, {kMagne + 2, &bt},
+        {kCurrent, &zero}, {kCurrent + 1, &zero}, {kCurrent + 2, &zero}};
     for (const auto &sv : sets)
         if (mhd_field_io(c, sv.first, const_cast<double *>(sv.second->data()), false, err)) return -1;
     return 0;

@@ -93,6 +93,8 @@ int  mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *r
                 const MhdCoefs &co, int leg_mode, hipStream_t s, std::string *err);
 void mhd_free(MhdContext *c);
 int  mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err);
+int  mhd_set_seeded_ic(MhdContext *c, unsigned long long seed, double amp_u, double amp_b,
+                       double amp_t, std::string *err);
 int  mhd_initialize(MhdContext *c, std::string *err);
 int  mhd_steps(MhdContext *c, int nsteps, std::string *err);
 int  mhd_energies(MhdContext *c, double *out7, std::string *err);

@@ -54,11 +54,12 @@ def chebyshev_radius(nri: int, r_icb: float, r_cmb: float) -> np.ndarray:
 
 
 def dynamobench_case1(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6,
-                      legendre_mode="hip_fp64"):
+                      legendre_mode="hip_fp64", Ra=1.0e2):
     """Parameters of tests/Dynamobench_case1 (control_MHD, control_sph_shell):
     E = 1e-3, modified Ra = 100, Pr = 1, Pm = 5, r_i/r_o = 0.35 with
-    r_o - r_i = 1; returns (MhdParams, radius)."""
-    E, Ra, Pr, Pm = 1.0e-3, 1.0e2, 1.0, 5.0
+    r_o - r_i = 1; returns (MhdParams, radius).  The grid, time step,
+    Crank-Nicolson weight and Rayleigh number can be changed."""
+    E, Pr, Pm = 1.0e-3, 1.0, 5.0
     r_cmb = 1.0 / (1.0 - 0.35)
     r_icb = r_cmb * 0.35
     p = MhdParams()
@@ -138,6 +139,13 @@ class MhdSimulation:
         _lib.check(_lib.load().calypso_mhd_set_dynamobench_ic(self._h, isig),
                    "calypso_mhd_set_dynamobench_ic")
 
+    def set_seeded_ic(self, seed, amp_u=1.0, amp_b=1.0, amp_t=0.1):
+        """Seeded random u, B (l >= 1) and temperature perturbation with
+        amplitude (l+1)^-2 and the wall envelope (see include/calypso_amd.h)."""
+        _lib.check(_lib.load().calypso_mhd_set_seeded_ic(self._h, int(seed), float(amp_u),
+                                                         float(amp_b), float(amp_t)),
+                   "calypso_mhd_set_seeded_ic")
+
     def initialize(self):
         _lib.check(_lib.load().calypso_mhd_initialize(self._h), "calypso_mhd_initialize")
 
@@ -198,6 +206,19 @@ class MhdSimulation:
             self.close()
         except Exception:
             pass
+
+
+def seeded_spectrum(seed, field, l_truncation, radius, amplitude=1.0, power=2.0,
+                    envelope=True, l_min=0):
+    """Counter-based seeded spectrum (nr, J) of calypso_seeded_spectrum."""
+    radius = np.ascontiguousarray(radius, dtype=np.float64)
+    J = (l_truncation + 1) ** 2
+    out = np.zeros((radius.size, J))
+    _lib.check(_lib.load().calypso_seeded_spectrum(
+        int(seed), int(field), int(l_truncation), radius.size, radius.ctypes.data_as(_P),
+        float(amplitude), float(power), 1 if envelope else 0, int(l_min),
+        out.ctypes.data_as(_P)), "calypso_seeded_spectrum")
+    return out
 
 
 def coriolis_table(l_truncation: int):

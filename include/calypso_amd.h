@@ -24,9 +24,11 @@ extern "C" {
 #define CALYPSO_LEG_FP64    1  /* FP64 MFMA 16x16x4                          */
 #define CALYPSO_LEG_FP32    2  /* FP32 MFMA 16x16x4                          */
 #define CALYPSO_LEG_BF16X3  3  /* split-BF16x3 MFMA 16x16x32, FP32 accumulate */
-#define CALYPSO_LEG_INT8X6  4  /* six int8 slices per operand (Ozaki splitting),
-                                  MFMA i32 16x16x64, exact int32 sums, FP64
-                                  accumulation: FP64-level (~1e-12) results   */
+#define CALYPSO_LEG_INT8X6  4  /* each operand split into seven balanced base-128
+                                  int8 digits (Ozaki splitting), the digit pairs
+                                  i+j <= 6 on MFMA i32 16x16x64, exact int32
+                                  sums, FP64 accumulation: FP64-level (~1e-12)
+                                  results                                     */
 
 typedef struct calypso_leg calypso_leg_t;
 
@@ -254,6 +256,17 @@ int calypso_sph_mean_square_noalloc(int kind, int nr, int J, const int *l_of_j,
                                     const double *radius, int kr_st, int kr_ed,
                                     const double *d_f, double *out, void *stream);
 
+/* ---- seeded synthetic inputs (host only, no GPU) ---------------------- */
+/* out[j + J k] (J = (L+1)^2, mode j = l(l+1)+m, k < nr) = amplitude x
+ * (l+1)^-power x g(seed, field, k, j) x (envelope ? ((r_k - r_0)(r_{nr-1} -
+ * r_k))^2 / ((r_{nr-1} - r_0)/2)^4 : 1) for l >= l_min, 0 below: g is a
+ * standard Gaussian from a counter-based generator keyed by (seed, field,
+ * k, j) (SplitMix64 output function, Box-Muller), so a value does not depend
+ * on the array's size or on the order of generation. */
+int calypso_seeded_spectrum(unsigned long long seed, int field, int l_truncation, int nr,
+                            const double *radius, double amplitude, double power, int envelope,
+                            int l_min, double *out);
+
 /* ---- the full MHD time step, device resident (FP64 state) ---------------
  * Replaces the body of module SPH_analyzer_MHD: SPH_initialize_MHD and
  * SPH_analyze_MHD(i_step, iflag_finish) (programs/SPH_MHD/
@@ -301,6 +314,15 @@ int calypso_mhd_snapshot_rtp(calypso_mhd_t *mhd, int nvec, const int *vec_comp,
  * sph_initial_data_control, initial_magne_dynamobench.f90:
  * initial_b_dynamobench_1); isig selects the (m, m) temperature mode. */
 int calypso_mhd_set_dynamobench_ic(calypso_mhd_t *mhd, int isig);
+/* Seeded random initial fields (the synthetic inputs of the benchmark
+ * configurations; the reference reads such fields from a restart file):
+ * poloidal/toroidal velocity and magnetic field for l >= 1 and the
+ * temperature perturbation for l >= 0, each amplitude (amp_u, amp_b,
+ * amp_t) x (l+1)^-2 x a Gaussian of calypso_seeded_spectrum (field ids 0, 2,
+ * 6, 8, 12) x the wall envelope; the l = 0 temperature adds the reference
+ * profile, so the state holds the full temperature like the benchmark start. */
+int calypso_mhd_set_seeded_ic(calypso_mhd_t *mhd, unsigned long long seed, double amp_u,
+                              double amp_b, double amp_t);
 /* SPH_initialize_MHD after the initial fields: perturbation temperature,
  * set_sph_field_to_start and the first nonlinear terms. */
 int calypso_mhd_initialize(calypso_mhd_t *mhd);

@@ -274,3 +274,26 @@ def sph_mean_square(kind, radius, l_of_j, kr_st, kr_ed, f):
                               _I(int(l_of_j.max())), _ptr(radius), _I(kr_st), _I(kr_ed),
                               _ptr(f), _ptr(out))
     return out
+
+
+# ------------------------------------------------------- seeded inputs --
+
+def splitmix64(state: int) -> int:
+    lib().orc_splitmix64.restype = ctypes.c_uint64
+    lib().orc_splitmix64.argtypes = [ctypes.c_uint64]
+    return int(lib().orc_splitmix64(state))
+
+
+def box_muller(u1: float, u2: float) -> float:
+    lib().orc_box_muller.restype = _D
+    return float(lib().orc_box_muller(_D(u1), _D(u2)))
+
+
+def seeded_spectrum(seed, field, L, radius, amplitude=1.0, power=2.0, envelope=True, l_min=0):
+    """(nr, J) seeded spectrum, the C restatement orc_seeded_spectrum."""
+    radius = np.ascontiguousarray(radius, dtype=np.float64)
+    out = np.zeros((radius.size, (L + 1) ** 2))
+    lib().orc_seeded_spectrum(ctypes.c_uint64(int(seed)), _I(field), _I(L), _I(radius.size),
+                              _ptr(radius), _D(amplitude), _D(power), _I(1 if envelope else 0),
+                              _I(l_min), _ptr(out))
+    return out

@@ -19,6 +19,7 @@
 #include "calypso_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -220,21 +221,26 @@ void orc_g_sph_rlm(int L, double *g)
 void orc_legendre_tables(int L, int nth, const double *colat, double *P,
                          double *dP)
 {
-    double *p = (double *)malloc(sizeof(double) * (size_t)(L + 1));
-    double *dp = (double *)malloc(sizeof(double) * (size_t)(L + 1));
-    for (int t = 0; t < nth; ++t) {
-        int j = 0;
-        for (int mp = 0; mp < 2 * L + 1; ++mp) {
-            int mm = abs(mp - L);
-            orc_schmidt_legendres_m(L, mm, colat[t], p, dp);
-            for (int l = mm; l <= L; ++l, ++j) {
-                P[t + (size_t)nth * j] = p[l];
-                dP[t + (size_t)nth * j] = dp[l];
+    /* colatitudes are independent: one per thread */
+#pragma omp parallel
+    {
+        double *p = (double *)malloc(sizeof(double) * (size_t)(L + 1));
+        double *dp = (double *)malloc(sizeof(double) * (size_t)(L + 1));
+#pragma omp for schedule(dynamic)
+        for (int t = 0; t < nth; ++t) {
+            int j = 0;
+            for (int mp = 0; mp < 2 * L + 1; ++mp) {
+                int mm = abs(mp - L);
+                orc_schmidt_legendres_m(L, mm, colat[t], p, dp);
+                for (int l = mm; l <= L; ++l, ++j) {
+                    P[t + (size_t)nth * j] = p[l];
+                    dP[t + (size_t)nth * j] = dp[l];
+                }
             }
         }
+        free(p);
+        free(dp);
     }
-    free(p);
-    free(dp);
 }
 
 /* Backward transform, spectrum -> (r, theta, m) grid, following
@@ -269,9 +275,12 @@ void orc_leg_bwd(int L, int nth, int nr, int ncomp, int nvec, int nscl,
                 sp[i + 2] = sp[i + 2] * a1r;
             }
     }
+    /* radial levels are independent (every write is indexed by k): one per
+     * thread, the order over mp within a level as above */
+#pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < nr; ++k)
     for (int mp = 0; mp < M; ++mp) {
         int mn = M - 1 - mp;
-        for (int k = 0; k < nr; ++k)
             for (int t = 0; t < nth; ++t) {
                 for (int nd = 0; nd < nvec; ++nd) {
                     double v1 = 0.0, v2 = 0.0, v3 = 0.0, w2 = 0.0, w3 = 0.0;
@@ -333,9 +342,6 @@ void orc_leg_fwd(int L, int nth, int nr, int ncomp, int nvec, int nscl,
     int *mdx_n = (int *)malloc(sizeof(int) * (size_t)J);
     double *g = (double *)malloc(sizeof(double) * 17 * (size_t)J);
     double *asin_t = (double *)malloc(sizeof(double) * (size_t)nth);
-    double *pvw = (double *)malloc(sizeof(double) * (size_t)nth);
-    double *dpvw = (double *)malloc(sizeof(double) * (size_t)nth);
-    double *pgvw = (double *)malloc(sizeof(double) * (size_t)nth);
     orc_rlm_index(L, jlm, lstack);
     orc_g_sph_rlm(L, g);
     for (int t = 0; t < nth; ++t) asin_t[t] = 1.0 / sin(colat[t]);
@@ -346,7 +352,11 @@ void orc_leg_fwd(int L, int nth, int nr, int ncomp, int nvec, int nscl,
     }
     memset(sp_out, 0, sizeof(double) * (size_t)ncomp * J * nr);
 
+#pragma omp parallel for schedule(dynamic)
     for (int k = 0; k < nr; ++k) {
+        double *pvw = (double *)malloc(sizeof(double) * (size_t)nth);
+        double *dpvw = (double *)malloc(sizeof(double) * (size_t)nth);
+        double *pgvw = (double *)malloc(sizeof(double) * (size_t)nth);
         double r2 = radius[k] * radius[k];
         for (int j = 0; j < J; ++j) {
             const double g7 = g[17 * (size_t)j + 6], g6 = g[17 * (size_t)j + 5];
@@ -386,6 +396,9 @@ void orc_leg_fwd(int L, int nth, int nr, int ncomp, int nvec, int nscl,
                 sp_out[i] = sp_out[i] + s1;
             }
         }
+        free(pvw);
+        free(dpvw);
+        free(pgvw);
     }
     free(jlm);
     free(lstack);
@@ -393,9 +406,6 @@ void orc_leg_fwd(int L, int nth, int nr, int ncomp, int nvec, int nscl,
     free(mdx_n);
     free(g);
     free(asin_t);
-    free(pvw);
-    free(dpvw);
-    free(pgvw);
 }
 
 /* Longitude transforms with FFTPACK5 RFFTMF/RFFTMB semantics as used by
@@ -425,20 +435,26 @@ void orc_fft_bwd(int L, int nth, int nr, int nphi, int ncomp,
             st[(size_t)m * nphi + i] = sin((double)m * phi);
         }
     }
-    for (int nd = 0; nd < ncomp; ++nd)
-        for (int t = 0; t < nth; ++t)
-            for (int k = 0; k < nr; ++k) {
-                const size_t base = (size_t)t + (size_t)nth * k;
-                for (int i = 0; i < nphi; ++i) {
-                    double x = vr_rtm[nd + (size_t)ncomp * (base + sl * L)];
-                    for (int m = 1; m <= L; ++m) {
-                        double a = vr_rtm[nd + (size_t)ncomp * (base + sl * (L + m))];
-                        double b = vr_rtm[nd + (size_t)ncomp * (base + sl * (L - m))];
-                        x += a * ct[(size_t)m * nphi + i] + b * st[(size_t)m * nphi + i];
+    /* one pencil (nd, t, k) per iteration; each sum keeps its order */
+#pragma omp parallel
+    {
+        double *a = (double *)malloc(sizeof(double) * (size_t)(2 * L + 1));
+#pragma omp for collapse(3) schedule(static)
+        for (int nd = 0; nd < ncomp; ++nd)
+            for (int t = 0; t < nth; ++t)
+                for (int k = 0; k < nr; ++k) {
+                    const size_t base = (size_t)t + (size_t)nth * k;
+                    for (int mp = 0; mp < 2 * L + 1; ++mp)
+                        a[mp] = vr_rtm[nd + (size_t)ncomp * (base + sl * mp)];
+                    for (int i = 0; i < nphi; ++i) {
+                        double x = a[L];
+                        for (int m = 1; m <= L; ++m)
+                            x += a[L + m] * ct[(size_t)m * nphi + i] + a[L - m] * st[(size_t)m * nphi + i];
+                        fld_rtp[(size_t)k + (size_t)nr * t + (size_t)nr * nth * i + nnod * nd] = x;
                     }
-                    fld_rtp[(size_t)k + (size_t)nr * t + (size_t)nr * nth * i + nnod * nd] = x;
                 }
-            }
+        free(a);
+    }
     free(ct);
     free(st);
 }
@@ -458,26 +474,32 @@ void orc_fft_fwd(int L, int nth, int nr, int nphi, int ncomp,
             st[(size_t)m * nphi + i] = sin((double)m * phi);
         }
     }
-    for (int nd = 0; nd < ncomp; ++nd)
-        for (int t = 0; t < nth; ++t)
-            for (int k = 0; k < nr; ++k) {
-                const size_t base = (size_t)t + (size_t)nth * k;
-                const double *x = fld_rtp + (size_t)k + (size_t)nr * t + nnod * nd;
-                for (int m = 0; m <= L; ++m) {
-                    double sc = 0.0, ss = 0.0;
-                    for (int i = 0; i < nphi; ++i) {
-                        double xi = x[(size_t)nr * nth * i];
-                        sc += xi * ct[(size_t)m * nphi + i];
-                        ss += xi * st[(size_t)m * nphi + i];
-                    }
-                    if (m == 0) {
-                        vr_rtm[nd + (size_t)ncomp * (base + sl * L)] = sc / (double)nphi;
-                    } else {
-                        vr_rtm[nd + (size_t)ncomp * (base + sl * (L + m))] = 2.0 * sc / (double)nphi;
-                        vr_rtm[nd + (size_t)ncomp * (base + sl * (L - m))] = 2.0 * ss / (double)nphi;
+#pragma omp parallel
+    {
+        double *xs = (double *)malloc(sizeof(double) * (size_t)nphi);
+#pragma omp for collapse(3) schedule(static)
+        for (int nd = 0; nd < ncomp; ++nd)
+            for (int t = 0; t < nth; ++t)
+                for (int k = 0; k < nr; ++k) {
+                    const size_t base = (size_t)t + (size_t)nth * k;
+                    const double *x = fld_rtp + (size_t)k + (size_t)nr * t + nnod * nd;
+                    for (int i = 0; i < nphi; ++i) xs[i] = x[(size_t)nr * nth * i];
+                    for (int m = 0; m <= L; ++m) {
+                        double sc = 0.0, ss = 0.0;
+                        for (int i = 0; i < nphi; ++i) {
+                            sc += xs[i] * ct[(size_t)m * nphi + i];
+                            ss += xs[i] * st[(size_t)m * nphi + i];
+                        }
+                        if (m == 0) {
+                            vr_rtm[nd + (size_t)ncomp * (base + sl * L)] = sc / (double)nphi;
+                        } else {
+                            vr_rtm[nd + (size_t)ncomp * (base + sl * (L + m))] = 2.0 * sc / (double)nphi;
+                            vr_rtm[nd + (size_t)ncomp * (base + sl * (L - m))] = 2.0 * ss / (double)nphi;
+                        }
                     }
                 }
-            }
+        free(xs);
+    }
     free(ct);
     free(st);
 }
@@ -833,4 +855,66 @@ void orc_sph_mean_square(int kind, int nr, int J, const int *l_of_j, int lmax,
     }
     free(vol);
     free(lay);
+}
+
+/* Seeded synthetic spectra, restated from the published algorithms the
+ * product's generator names: the SplitMix64 output function (Steele, Lea &
+ * Flood, OOPSLA 2014) chained over (seed, field, k, j), two 53-bit uniforms
+ * and the Box-Muller transform (Box & Muller 1958); amplitude (l+1)^-power
+ * and the wall envelope ((r - r_0)(r_last - r))^2 / (d/2)^4.
+ * Pinned by tests/test_seeded.py: SplitMix64's published output sequence
+ * for seed 1234567, the Box-Muller closed form on chosen uniforms, Gaussian
+ * moments over 10^6 draws, and the envelope's zeros at the walls. */
+uint64_t orc_splitmix64(uint64_t state)
+{
+    uint64_t z = state + UINT64_C(0x9E3779B97F4A7C15);
+    z ^= z >> 30;
+    z *= UINT64_C(0xBF58476D1CE4E5B9);
+    z ^= z >> 27;
+    z *= UINT64_C(0x94D049BB133111EB);
+    z ^= z >> 31;
+    return z;
+}
+
+double orc_box_muller(double u1, double u2)
+{
+    return sqrt(-2.0 * log(u1)) * cos(8.0 * atan(1.0) * u2);
+}
+
+double orc_seeded_gauss(uint64_t seed, uint64_t field, uint64_t k, uint64_t j)
+{
+    uint64_t key = orc_splitmix64(seed);
+    key = orc_splitmix64(key ^ field);
+    key = orc_splitmix64(key ^ k);
+    key = orc_splitmix64(key ^ j);
+    const uint64_t a = orc_splitmix64(key) >> 11;
+    const uint64_t b = orc_splitmix64(key ^ UINT64_C(0xD1B54A32D192ED03)) >> 11;
+    const double u1 = ((double)a + 0.5) / 9007199254740992.0;   /* 2^53 */
+    const double u2 = (double)b / 9007199254740992.0;
+    return orc_box_muller(u1, u2);
+}
+
+void orc_seeded_spectrum(uint64_t seed, int field, int L, int nr, const double *radius,
+                         double amplitude, double power, int envelope, int l_min, double *out)
+{
+    const int J = (L + 1) * (L + 1);
+    const double half = 0.5 * (radius[nr - 1] - radius[0]);
+    for (int k = 0; k < nr; ++k) {
+        double e = 1.0;
+        if (envelope) {
+            const double q = (radius[k] - radius[0]) * (radius[nr - 1] - radius[k]);
+            e = q * q / (half * half * half * half);
+        }
+        for (int j = 0; j < J; ++j) {
+            int l = (int)sqrt((double)j);
+            while (l * l > j) --l;
+            while ((l + 1) * (l + 1) <= j) ++l;
+            double v = 0.0;
+            if (l >= l_min) {
+                const double amp = amplitude * pow((double)(l + 1), -power) * e;
+                v = amp * orc_seeded_gauss(seed, (uint64_t)field, (uint64_t)k, (uint64_t)j);
+            }
+            out[(size_t)j + (size_t)J * k] = v;
+        }
+    }
 }

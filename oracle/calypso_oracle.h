@@ -18,6 +18,7 @@
 #define CALYPSO_ORACLE_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -58,6 +59,11 @@ void orc_explicit_update(int kind, int scheme, size_t n, double dt, double coef_
 void orc_sph_mean_square(int kind, int nr, int J, const int *l_of_j, int lmax,
                          const double *radius, int kr_st, int kr_ed, const double *f,
                          double *out);
+uint64_t orc_splitmix64(uint64_t state);
+double orc_box_muller(double u1, double u2);
+double orc_seeded_gauss(uint64_t seed, uint64_t field, uint64_t k, uint64_t j);
+void orc_seeded_spectrum(uint64_t seed, int field, int L, int nr, const double *radius,
+                         double amplitude, double power, int envelope, int l_min, double *out);
 
 #ifdef __cplusplus
 }

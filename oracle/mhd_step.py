@@ -140,7 +140,10 @@ def gaunt_gs(m1, l1, m2, l2, m3, l3):
         tsum += Fraction((-1) ** it * _fact(l1 + m1 + it) * _fact(l2 + l3 - m1 - it),
                          _fact(l1 - m1 - it) * _fact(l2 - l3 + m1 + it)
                          * _fact(l3 - m3 - it) * _fact(it))
-    val = float((-1) ** (s - l2 - m3) * 2 * pre * tsum) * math.sqrt(float(sq))
+    # the square of the value is a moderate rational even where its factors
+    # overflow a double (high degrees): one rounding, then the square root
+    x = (-1) ** (s - l2 - m3) * 2 * pre * tsum
+    val = math.copysign(math.sqrt(float(x * x * sq)), float(x > 0) - 0.5) if x != 0 else 0.0
     for m in (m1, m2, m3):
         if m != 0:
             val *= math.sqrt(2.0)
@@ -320,6 +323,14 @@ class DynamoBenchmarkCase1:
     depth_high_t, depth_low_t = 0.5384615384615384, 1.5384615384615384
     high_temp, low_temp = 1.0, 0.0
     isig = 400
+
+
+def shell_case(L=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6, Ra_mod=1.0e2):
+    """The Dynamobench case 1 physics (rotating Boussinesq shell, rigid walls,
+    insulating magnetic boundaries, fixed temperature) on another grid, time
+    step, Crank-Nicolson weight or modified Rayleigh number."""
+    return type("ShellCase", (DynamoBenchmarkCase1,),
+                dict(L=L, nth=nth, nphi=nphi, nri=nri, dt=dt, coef_imp=coef_imp, Ra_mod=Ra_mod))
 
 
 class MhdStep:
@@ -791,11 +802,16 @@ class MhdStep:
                  + f["rot_buoyancy"])
 
     # ------------------------------------------------------ initialisation --
-    def initialize(self):
+    def initialize(self, seeded=None):
         """SPH_initialize_MHD: dynamo_benchmark_1 initial fields
         (set_initial_sph_dynamo.f90, initial_magne_dynamobench.f90), the
         perturbation temperature, the fields derived from them
-        (set_sph_field_to_start) and the first nonlinear terms."""
+        (set_sph_field_to_start) and the first nonlinear terms.
+        seeded=(seed, amp_u, amp_b, amp_t) starts instead from the seeded
+        random fields: poloidal / toroidal u and B (l >= 1) and the
+        temperature perturbation (l >= 0), amplitude (l+1)^-2 times the wall
+        envelope, field ids 0, 2, 6, 8, 12 (oracle.seeded_spectrum), with the
+        reference temperature added to the l = 0 mode."""
         nr, J = self.nr, self.J
         f = self.f
         for name in ("velo", "vort", "magne", "current", "w_diffuse", "b_diffuse",
@@ -804,6 +820,18 @@ class MhdStep:
             f[name] = np.zeros((3, nr, J))
         for name in ("temp", "t_diffuse", "rot_buoyancy", "h_advect", "pre_heat"):
             f[name] = np.zeros((nr, J))
+        if seeded is not None:
+            from oracle import seeded_spectrum
+            seed, amp_u, amp_b, amp_t = seeded
+            r, L = self.radius, self.L
+            f["velo"][0] = seeded_spectrum(seed, 0, L, r, amp_u, 2.0, True, 1)
+            f["velo"][2] = seeded_spectrum(seed, 2, L, r, amp_u, 2.0, True, 1)
+            f["magne"][0] = seeded_spectrum(seed, 6, L, r, amp_b, 2.0, True, 1)
+            f["magne"][2] = seeded_spectrum(seed, 8, L, r, amp_b, 2.0, True, 1)
+            f["temp"] = seeded_spectrum(seed, 12, L, r, amp_t, 2.0, True, 0)
+            f["temp"][:, 0] = self.reft0 + f["temp"][:, 0]
+            self._start()
+            return
         pi = 4.0 * math.atan(1.0)
         r = self.radius
         shell = self.r_cmb - self.r_icb
@@ -828,6 +856,9 @@ class MhdStep:
         f["magne"][2][:, jt] = (10.0 / 3.0) * r * sin_
         f["current"][0][:, jt] = f["magne"][2][:, jt]
         f["current"][1][:, jt] = (10.0 / 3.0) * (sin_ + pi * r * cos_)
+        self._start()
+
+    def _start(self):
         self.sync_temp_by_per_temp()
         # set_sph_field_to_start
         self.grad_vp_and_vorticity()

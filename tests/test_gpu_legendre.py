@@ -15,7 +15,10 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "legendre_ref.npz")
 
 FP64_TOL = 1e-12
-MIXED_TOL = {"hip_fp32": 2e-5, "hip_bf16x3": 2e-5, "hip_int8x6": 1e-11}
+# about 4x the largest error each mode reaches over CASES (FP32 4.4e-7,
+# split-BF16x3 3.8e-7); the int8-slice mode is held to the 1e-11 bound.
+# FP32 and split-BF16x3 do not meet 1e-11.
+MIXED_TOL = {"hip_fp32": 2e-6, "hip_bf16x3": 1.5e-6, "hip_int8x6": 1e-11}
 
 
 def field_err(got, ref, ncomp):
