@@ -62,6 +62,9 @@ SIGNATURES = [
     ("calypso_mhd_snapshot_rtp", _I, [_P, _I, _P, _I, _P, _P]),
     ("calypso_mhd_get_field", _I, [_P, _I, _P]),
     ("calypso_mhd_set_dynamobench_ic", _I, [_P, _I]),
+    ("calypso_sph_write_grid_files", _I, [ctypes.c_char_p, _I, _I, _I, _I, _P]),
+    ("calypso_sph_rj_mode_order", _I, [_I, _P]),
+    ("calypso_mhd_set_rj_order", _I, [_P, _I, _P]),
     ("calypso_mhd_set_seeded_ic", _I, [_P, ctypes.c_ulonglong, _D, _D, _D]),
     ("calypso_seeded_spectrum", _I, [ctypes.c_ulonglong, _I, _I, _I, _P, _D, _D, _I, _I, _P]),
     ("calypso_mhd_initialize", _I, [_P]),

@@ -542,7 +542,8 @@ int calypso_mhd_set_field(calypso_mhd_t *mhd, int comp, const double *host)
 {
     if (!mhd || !host) return fail("calypso_mhd_set_field: null argument");
     std::string err;
-    if (calypso::mhd_field_io(&mhd->c, comp, const_cast<double *>(host), false, &err)) return fail(err);
+    if (calypso::mhd_field_io_caller(&mhd->c, comp, const_cast<double *>(host), false, &err))
+        return fail(err);
     return 0;
 }
 
@@ -560,7 +561,7 @@ int calypso_mhd_get_field(calypso_mhd_t *mhd, int comp, double *host)
 {
     if (!mhd || !host) return fail("calypso_mhd_get_field: null argument");
     std::string err;
-    if (calypso::mhd_field_io(&mhd->c, comp, host, true, &err)) return fail(err);
+    if (calypso::mhd_field_io_caller(&mhd->c, comp, host, true, &err)) return fail(err);
     return 0;
 }
 
@@ -572,12 +573,37 @@ int calypso_mhd_set_dynamobench_ic(calypso_mhd_t *mhd, int isig)
     return 0;
 }
 
+int calypso_mhd_set_rj_order(calypso_mhd_t *mhd, int J, const int *j_global)
+{
+    if (!mhd || !j_global) return fail("calypso_mhd_set_rj_order: null argument");
+    std::string err;
+    if (calypso::mhd_set_rj_order(&mhd->c, J, j_global, &err)) return fail(err);
+    return 0;
+}
+
 int calypso_mhd_set_seeded_ic(calypso_mhd_t *mhd, unsigned long long seed, double amp_u,
                               double amp_b, double amp_t)
 {
     if (!mhd) return fail("calypso_mhd_set_seeded_ic: null handle");
     std::string err;
     if (calypso::mhd_set_seeded_ic(&mhd->c, seed, amp_u, amp_b, amp_t, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_sph_write_grid_files(const char *prefix, int l_truncation, int nth, int nphi,
+                                 int nr, const double *radius)
+{
+    if (!prefix || !radius) return fail("calypso_sph_write_grid_files: null argument");
+    std::string err;
+    if (calypso::sph_write_grid_files(prefix, l_truncation, nth, nphi, nr, radius, &err))
+        return fail("calypso_sph_write_grid_files: " + err);
+    return 0;
+}
+
+int calypso_sph_rj_mode_order(int l_truncation, int *j_global)
+{
+    if (l_truncation < 0 || !j_global) return fail("calypso_sph_rj_mode_order: bad argument");
+    calypso::sph_rj_mode_order(l_truncation, j_global);
     return 0;
 }
 

@@ -1126,6 +1126,49 @@ int mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::strin
     return 0;
 }
 
+// The caller's local mode order of its rj arrays: the reference's one-domain
+// rj modes come grouped by order m (const_global_rj_modes_by_rlm), while the
+// state here holds mode l(l+1)+m at j.  Host copies through
+// mhd_field_io_caller permute between the two.
+int mhd_set_rj_order(MhdContext *c, int J, const int *j_global, std::string *err)
+{
+    if (J != c->J) {
+        *err = "rj mode order: " + std::to_string(J) + " modes given, the grid has " +
+               std::to_string(c->J);
+        return -1;
+    }
+    std::vector<char> seen((size_t)J, 0);
+    for (int j = 0; j < J; ++j) {
+        const int g = j_global[j];
+        if (g < 0 || g >= J || seen[(size_t)g]) {
+            *err = "rj mode order: not a permutation of 0..J-1";
+            return -1;
+        }
+        seen[(size_t)g] = 1;
+    }
+    c->rj_order.assign(j_global, j_global + J);
+    return 0;
+}
+
+int mhd_field_io_caller(MhdContext *c, int comp, double *host, bool to_host, std::string *err)
+{
+    if (c->rj_order.empty()) return mhd_field_io(c, comp, host, to_host, err);
+    const int J = c->J, nr = c->nr;
+    std::vector<double> own((size_t)J * nr);
+    if (!to_host) {
+        for (int k = 0; k < nr; ++k)
+            for (int j = 0; j < J; ++j)
+                own[(size_t)c->rj_order[j] + (size_t)J * k] = host[(size_t)j + (size_t)J * k];
+    }
+    if (mhd_field_io(c, comp, own.data(), to_host, err)) return -1;
+    if (to_host) {
+        for (int k = 0; k < nr; ++k)
+            for (int j = 0; j < J; ++j)
+                host[(size_t)j + (size_t)J * k] = own[(size_t)c->rj_order[j] + (size_t)J * k];
+    }
+    return 0;
+}
+
 // Gather d_rj components into an rlm spectrum, component fastest (the layout
 // of the reference's send buffer after copy_snap_spectr_to_send)
 __global__ void k_gather_rlm(MhdDev p, int ncomp, const int *slot, double *sp)

@@ -85,6 +85,7 @@ struct MhdContext {
     double *d_radius = nullptr;
     double *d_lu[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // vp vt temp bs bt
     int *d_piv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::vector<int> rj_order;    // caller's local rj mode j -> l(l+1)+m; empty: the same
     double *d_scratch = nullptr;  // energies: full temperature + per-mode sums
     double *d_ms_out = nullptr;
 };
@@ -99,6 +100,9 @@ int  mhd_initialize(MhdContext *c, std::string *err);
 int  mhd_steps(MhdContext *c, int nsteps, std::string *err);
 int  mhd_energies(MhdContext *c, double *out7, std::string *err);
 int  mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err);
+// the same with the host array in the caller's rj mode order (mhd_set_rj_order)
+int  mhd_field_io_caller(MhdContext *c, int comp, double *host, bool to_host, std::string *err);
+int  mhd_set_rj_order(MhdContext *c, int J, const int *j_global, std::string *err);
 // backward transform of chosen d_rj fields to the rtp grid (host array)
 int  mhd_snapshot_rtp(MhdContext *c, int nvec, const int *vec, int nscl, const int *scl,
                       double *host, std::string *err);
@@ -109,6 +113,11 @@ int  mhd_write_volume_ms(MhdContext *c, const std::string &path, std::string *er
 std::string mhd_restart_name(const std::string &prefix, int istep_fld);
 int  mhd_write_restart(MhdContext *c, const std::string &path, std::string *err);
 int  mhd_read_restart(MhdContext *c, const std::string &path, std::string *err);
+
+// single-domain grid tables and the reference's grid files (sph_grids.cpp)
+void sph_rj_mode_order(int L, int *j_global);
+int  sph_write_grid_files(const std::string &prefix, int L, int nth, int nphi, int nr,
+                          const double *radius, std::string *err);
 
 // closed forms of the Coriolis couplings (exposed for the host tests)
 double gaunt_p1(int l, int m);   // K(1,0; l-1,m; l,m)

@@ -24,37 +24,12 @@
 #include <string>
 #include <vector>
 
+#include "fortran_fmt.hpp"
 #include "mhd_internal.hpp"
 
 namespace calypso {
 
 namespace {
-
-// Fortran 1PEw.dE3: one digit before the point, d after, a signed
-// three-digit exponent, right-justified in w characters
-std::string fort_e(double v, int w, int d)
-{
-    char buf[64];
-    std::snprintf(buf, sizeof buf, "%.*E", d, v);
-    char *e = std::strchr(buf, 'E');
-    int ex = 0;
-    if (e) {
-        ex = std::atoi(e + 1);
-        *e = 0;
-    }
-    char out[96];
-    std::snprintf(out, sizeof out, "%sE%c%03d", buf, ex < 0 ? '-' : '+', std::abs(ex));
-    std::string s(out);
-    if ((int)s.size() < w) s.insert(0, (size_t)(w - (int)s.size()), ' ');
-    return s;
-}
-
-std::string fort_i(long long v, int w)
-{
-    char buf[64];
-    std::snprintf(buf, sizeof buf, "%*lld", w, v);
-    return buf;
-}
 
 bool file_exists(const std::string &path)
 {
@@ -128,10 +103,14 @@ int mhd_write_restart(MhdContext *c, const std::string &path, std::string *err)
     for (const RstField &r : kRst)
         for (int q = 0; q < r.ncomp; ++q, ++n) {
             comp[n].resize(nnod);
-            if (mhd_field_io(c, r.comp0 + q, comp[n].data(), true, err)) return -1;
+            if (mhd_field_io_caller(c, r.comp0 + q, comp[n].data(), true, err)) return -1;
         }
-    if (c->initialized)   // T - T0 -> T at l = 0 (j = 0)
-        for (int k = 0; k < c->nr; ++k) comp[3][(size_t)c->J * k] += ref_temp(c, k);
+    // T - T0 -> T at l = 0 (the caller's position of that mode)
+    size_t j00 = 0;
+    for (size_t j = 0; j < c->rj_order.size(); ++j)
+        if (c->rj_order[j] == 0) j00 = j;
+    if (c->initialized)
+        for (int k = 0; k < c->nr; ++k) comp[3][j00 + (size_t)c->J * k] += ref_temp(c, k);
     std::ofstream f(path, std::ios::trunc);
     if (!f) {
         *err = "cannot open " + path;
@@ -244,7 +223,8 @@ int mhd_read_restart(MhdContext *c, const std::string &path, std::string *err)
         }
         if (!r) continue;   // a field this step does not evolve
         for (int q = 0; q < nc; ++q)
-            if (mhd_field_io(c, r->comp0 + q, buf.data() + (size_t)q * nnod, false, err)) return -1;
+            if (mhd_field_io_caller(c, r->comp0 + q, buf.data() + (size_t)q * nnod, false, err))
+                return -1;
         ++seen;
     }
     if (seen != 6) return fail("restart lacks one of velocity, temperature, magnetic_field, "

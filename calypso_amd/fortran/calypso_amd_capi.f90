@@ -161,6 +161,14 @@
           real(c_double), intent(inout) :: d(*)
         end function calypso_mhd_get_field
 !
+        integer(c_int) function calypso_mhd_set_rj_order(mhd, jmax, jgl)  &
+     &      bind(C, name="calypso_mhd_set_rj_order")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: mhd
+          integer(c_int), value :: jmax
+          integer(c_int), intent(in) :: jgl(*)
+        end function calypso_mhd_set_rj_order
+!
         integer(c_int) function calypso_mhd_set_dynamobench_ic(mhd,     &
      &      isig) bind(C, name="calypso_mhd_set_dynamobench_ic")
           import :: c_int, c_ptr

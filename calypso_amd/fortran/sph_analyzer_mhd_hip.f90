@@ -30,18 +30,26 @@
 !>    Create the GPU state; with isig > 0 the dynamo_benchmark_1 initial
 !!    fields are set on the device, otherwise the caller provides velocity
 !!    (nnod_rj,3), magnetic field (nnod_rj,3) and the full temperature.
+!!    idx_gl_rj_j, when given, is the caller's local rj mode list
+!!    (idx_gl_1d_rj_j(:,1) of m_spheric_parameter): its d_rj arrays, here
+!!    and in sph_mhd_get_field_hip, are then in that mode order.
       subroutine SPH_initialize_MHD_hip(params, radius, isig,           &
-     &          nnod_rj, velo, magne, temp)
+     &          nnod_rj, velo, magne, temp, idx_gl_rj_j)
 !
       type(calypso_mhd_params), intent(in) :: params
       real(c_double), intent(in) :: radius(*)
       integer(c_int), intent(in) :: isig, nnod_rj
       real(c_double), intent(in) :: velo(nnod_rj,3), magne(nnod_rj,3)
       real(c_double), intent(in) :: temp(nnod_rj)
+      integer(c_int), intent(in), optional :: idx_gl_rj_j(:)
       integer(c_int) :: nd
 !
       call check_status(calypso_mhd_create(params, radius, c_null_ptr,  &
      &    mhd_ctx))
+      if(present(idx_gl_rj_j)) then
+        call check_status(calypso_mhd_set_rj_order(mhd_ctx,             &
+     &      int(size(idx_gl_rj_j), c_int), idx_gl_rj_j))
+      end if
       if(isig .gt. 0) then
         call check_status(calypso_mhd_set_dynamobench_ic(mhd_ctx, isig))
       else

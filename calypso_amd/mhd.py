@@ -139,6 +139,13 @@ class MhdSimulation:
         _lib.check(_lib.load().calypso_mhd_set_dynamobench_ic(self._h, isig),
                    "calypso_mhd_set_dynamobench_ic")
 
+    def set_rj_order(self, j_global):
+        """Take and give d_rj (set_field, get_field, restart files) in the
+        caller's local mode order: j_global[j] = l(l+1)+m of local mode j."""
+        jg = np.ascontiguousarray(j_global, dtype=np.int32)
+        _lib.check(_lib.load().calypso_mhd_set_rj_order(self._h, jg.size, jg.ctypes.data_as(_P)),
+                   "calypso_mhd_set_rj_order")
+
     def set_seeded_ic(self, seed, amp_u=1.0, amp_b=1.0, amp_t=0.1):
         """Seeded random u, B (l >= 1) and temperature perturbation with
         amplitude (l+1)^-2 and the wall envelope (see include/calypso_amd.h)."""

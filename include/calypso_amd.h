@@ -267,6 +267,22 @@ int calypso_seeded_spectrum(unsigned long long seed, int field, int l_truncation
                             const double *radius, double amplitude, double power, int envelope,
                             int l_min, double *out);
 
+/* ---- grid tables of one domain (host only, no GPU) --------------------- */
+/* Writes <prefix>.0.rj, .rlm, .rtm and .rtp: the index tables, radial grid,
+ * communication tables and groups of a one-domain spherical shell, byte for
+ * byte as the reference's grid generator gen_sph_grids writes them for one
+ * subdomain (write_spectr_modes_rj_data, write_modes_rlm_data,
+ * write_geom_rtm_data, write_geom_rtp_data, sph_modes_grids_data_IO.f90:
+ * 128-245; programs/mesh_utilities/main_generate_sph_grids.f90).  radius[nr]
+ * from the inner to the outer wall (no inner-core points); nphi = 2 nth and
+ * L < nth, as the generator's zonal FFT table requires. */
+int calypso_sph_write_grid_files(const char *prefix, int l_truncation, int nth, int nphi,
+                                 int nr, const double *radius);
+/* The local mode order of the reference's one-domain rj (and rlm) arrays:
+ * j_global[j] = l(l+1)+m of local mode j, orders m = -L..L, degrees |m|..L
+ * inside (idx_gl_1d_rj_j(:,1) after gen_sph_grids); J = (L+1)^2 entries. */
+int calypso_sph_rj_mode_order(int l_truncation, int *j_global);
+
 /* ---- the full MHD time step, device resident (FP64 state) ---------------
  * Replaces the body of module SPH_analyzer_MHD: SPH_initialize_MHD and
  * SPH_analyze_MHD(i_step, iflag_finish) (programs/SPH_MHD/
@@ -292,7 +308,14 @@ typedef struct calypso_mhd_params {
 int calypso_mhd_create(const calypso_mhd_params *params, const double *radius,
                        void *stream, calypso_mhd_t **out);
 int calypso_mhd_destroy(calypso_mhd_t *mhd);
-/* One d_rj component (nr*J doubles, mode j = l(l+1)+m fastest), host copy.
+/* The caller's local rj mode order: j_global[j] = l(l+1)+m of its local mode
+ * j (the reference's idx_gl_1d_rj_j(:,1); for one domain from gen_sph_grids
+ * that is calypso_sph_rj_mode_order's list, orders grouped, not
+ * l(l+1)+m).  From then on calypso_mhd_set_field / _get_field and the
+ * restart files take and give d_rj in that order.  Default: j = l(l+1)+m. */
+int calypso_mhd_set_rj_order(calypso_mhd_t *mhd, int J, const int *j_global);
+/* One d_rj component (nr*J doubles, mode j = l(l+1)+m fastest unless
+ * calypso_mhd_set_rj_order gave another order), host copy.
  * Components: 0-2 velocity, 3-5 vorticity, 6-8 magnetic field, 9-11 current
  * density, 12 temperature (perturbation once initialised), 13-15 vorticity
  * diffusion, 16-18 magnetic diffusion, 19 thermal diffusion, 20-22 forces,
