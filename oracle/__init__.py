@@ -172,6 +172,27 @@ class RefLegendre:
         ref().ref_leg_fwd(_I(ncomp), _I(nvec), _I(nscl), _ptr(vr), _ptr(sp))
         return sp
 
+    # the SYMMETRIC_MATMUL variant (legendre_trans_sym_matmul.f90), the one
+    # the reference's own tests select
+    def _sym(self, nvec, nscl):
+        if getattr(self, "_sym_set", None) != (nvec, nscl):
+            ref().ref_leg_sym_init(_I(nvec), _I(nscl))
+            self._sym_set = (nvec, nscl)
+
+    def bwd_sym(self, ncomp, nvec, nscl, sp):
+        self._sym(nvec, nscl)
+        sp = np.ascontiguousarray(sp, dtype=np.float64).ravel()
+        vr = np.zeros(self.grid.n_rtm(ncomp))
+        ref().ref_leg_bwd_sym(_I(ncomp), _I(nvec), _I(nscl), _ptr(sp), _ptr(vr))
+        return vr
+
+    def fwd_sym(self, ncomp, nvec, nscl, vr):
+        self._sym(nvec, nscl)
+        vr = np.ascontiguousarray(vr, dtype=np.float64).ravel()
+        sp = np.zeros(self.grid.n_rlm(ncomp))
+        ref().ref_leg_fwd_sym(_I(ncomp), _I(nvec), _I(nscl), _ptr(vr), _ptr(sp))
+        return sp
+
 
 # ------------------------------------------------------- FFT, products --
 

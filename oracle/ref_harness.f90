@@ -139,13 +139,22 @@
       dPdt_rtm(1:nth,1:nidx_rlm(2)) = dP(1:nth,1:nidx_rlm(2))
       g_sph_rlm(1:nidx_rlm(2),1:17) = transpose(g(1:17,1:nidx_rlm(2)))
       call set_trans_legendre_rtm
+!     hemisphere tables of the symmetric variants (set_legendre_hemispher_rtm)
+      if(allocated(Ps_rtm)) call deallocate_hemi_schmidt_rtm
+      call allocate_hemi_schmidt_rtm
+      call set_legendre_hemispher_rtm
 !
       idx_rtm_smp_stack(0,1:3) = 0
       idx_rlm_smp_stack(0,1:2) = 0
+      maxidx_rlm_smp(1) = 0
       do ip = 1, np_smp
         idx_rtm_smp_stack(ip,1) = (nr * ip) / np_smp
         idx_rlm_smp_stack(ip,1) = (nr * ip) / np_smp
+        maxidx_rlm_smp(1) = max(maxidx_rlm_smp(1),                      &
+     &      idx_rlm_smp_stack(ip,1) - idx_rlm_smp_stack(ip-1,1))
       end do
+!     set_params_sph_trans.f90:47-55: the longest degree list of an order
+      maxdegree_rlm = ltr + 1
 !
       if(allocated(sp_work)) deallocate(sp_work, vr_work)
       allocate(sp_work(1), vr_work(1))
@@ -195,6 +204,94 @@
      &      vr_in, sp_out)
       end if
       end subroutine ref_leg_fwd
+!
+! -----------------------------------------------------------------------
+!> The reference's SYMMETRIC_MATMUL transforms (the variant its own tests
+!! select): leg_backward_trans_sym_matmul / leg_forward_trans_sym_matmul
+!! (legendre_trans_sym_matmul.f90:50-108) without the MPI waits, on
+!! one-domain buffers (identity receive / send tables).  The work arrays
+!! are sized for the field set as sel_init_legendre_trans does
+!! (legendre_transform_select.f90:228-265).
+      subroutine ref_leg_sym_init(nvector, nscalar)                     &
+     &          bind(C, name="ref_leg_sym_init")
+      use m_legendre_work_sym_matmul
+      integer(c_int), value :: nvector, nscalar
+!
+      if(allocated(pol_e)) call dealloc_leg_vec_sym_matmul
+      if(allocated(scl_e)) call dealloc_leg_scl_sym_matmul
+      call alloc_leg_vec_sym_matmul(max(nvector,1))
+      call alloc_leg_scl_sym_matmul(max(nscalar,1))
+      end subroutine ref_leg_sym_init
+!
+      subroutine ref_leg_bwd_sym(ncomp, nvector, nscalar, sp_in, vr_out) &
+     &          bind(C, name="ref_leg_bwd_sym")
+      use m_spheric_parameter
+      use legendre_bwd_sym_matmul
+      integer(c_int), value :: ncomp, nvector, nscalar
+      real(c_double), intent(in) :: sp_in(ncomp*nnod_rlm)
+      real(c_double), intent(inout) :: vr_out(ncomp*nnod_rtm)
+      integer(kind = kint), allocatable :: irev_rlm(:), irev_rtm(:)
+      integer(kind = kint) :: i, n_WR, n_WS
+!
+      allocate(irev_rlm(nnod_rlm), irev_rtm(nnod_rtm))
+      do i = 1, nnod_rlm
+        irev_rlm(i) = i
+      end do
+      do i = 1, nnod_rtm
+        irev_rtm(i) = i
+      end do
+      n_WR = ncomp*nnod_rlm
+      n_WS = ncomp*nnod_rtm
+      if(size(sp_work) .ne. n_WR) then
+        deallocate(sp_work)
+        allocate(sp_work(n_WR))
+      end if
+      sp_work(1:n_WR) = sp_in(1:n_WR)
+      vr_out(1:n_WS) = 0.0d0
+      if(nvector .gt. 0) then
+        call leg_b_trans_vec_sym_matmul(ncomp, nvector,                 &
+     &      irev_rlm, irev_rtm, n_WR, n_WS, sp_work, vr_out)
+      end if
+      if(nscalar .gt. 0) then
+        call leg_b_trans_scl_sym_matmul(ncomp, nvector, nscalar,        &
+     &      irev_rlm, irev_rtm, n_WR, n_WS, sp_work, vr_out)
+      end if
+      deallocate(irev_rlm, irev_rtm)
+      end subroutine ref_leg_bwd_sym
+!
+      subroutine ref_leg_fwd_sym(ncomp, nvector, nscalar, vr_in, sp_out) &
+     &          bind(C, name="ref_leg_fwd_sym")
+      use m_spheric_parameter
+      use legendre_fwd_sym_matmul
+      integer(c_int), value :: ncomp, nvector, nscalar
+      real(c_double), intent(in) :: vr_in(ncomp*nnod_rtm)
+      real(c_double), intent(inout) :: sp_out(ncomp*nnod_rlm)
+      integer(kind = kint), allocatable :: irev_rlm(:), irev_rtm(:)
+      real(kind = kreal), allocatable :: wr(:)
+      integer(kind = kint) :: i, n_WR, n_WS
+!
+      allocate(irev_rlm(nnod_rlm), irev_rtm(nnod_rtm))
+      do i = 1, nnod_rlm
+        irev_rlm(i) = i
+      end do
+      do i = 1, nnod_rtm
+        irev_rtm(i) = i
+      end do
+      n_WR = ncomp*nnod_rtm
+      n_WS = ncomp*nnod_rlm
+      allocate(wr(n_WR))
+      wr(1:n_WR) = vr_in(1:n_WR)
+      sp_out(1:n_WS) = 0.0d0
+      if(nvector .gt. 0) then
+        call leg_f_trans_vec_sym_matmul(ncomp, nvector,                 &
+     &      irev_rtm, irev_rlm, n_WR, n_WS, wr, sp_out)
+      end if
+      if(nscalar .gt. 0) then
+        call leg_f_trans_scl_sym_matmul(ncomp, nvector, nscalar,        &
+     &      irev_rtm, irev_rlm, n_WR, n_WS, wr, sp_out)
+      end if
+      deallocate(irev_rlm, irev_rtm, wr)
+      end subroutine ref_leg_fwd_sym
 !
 ! -----------------------------------------------------------------------
 !> ludcmp_3band (m_ludcmp_3band.f90:76-173) on each of mcomp matrices:

@@ -72,3 +72,20 @@ def test_backward_single_mode_closed_form():
     p32 = np.array([oracle.schmidt_legendres_m(L, 2, t)[0][3] for t in grid.colat])
     np.testing.assert_allclose(vr[L - 2, 0], p32, rtol=0, atol=0)
     assert np.count_nonzero(np.delete(vr, L - 2, axis=0)) == 0
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("L,nth,nr,nv,ns", [(7, 12, 3, 3, 1), (31, 48, 4, 4, 1), (20, 33, 2, 1, 2)])
+def test_reference_sym_matmul_agrees_with_org_loops(L, nth, nr, nv, ns):
+    """The reference's SYMMETRIC_MATMUL transforms (the CPU baseline, the
+    variant its own tests select) against its org loops and the oracle: the
+    same sums in another order, so agreement to rounding (1e-13 normwise)."""
+    rng = np.random.default_rng(L + nr)
+    grid = oracle.Grid(L, nth, np.linspace(0.5, 1.5, nr))
+    ref = oracle.RefLegendre(grid, nthreads=2)
+    nc = 3 * nv + ns
+    sp = rng.standard_normal(grid.n_rlm(nc))
+    vr = rng.standard_normal(grid.n_rtm(nc))
+    for a, b in ((ref.bwd_sym(nc, nv, ns, sp), oracle.leg_bwd(grid, nc, nv, ns, sp)),
+                 (ref.fwd_sym(nc, nv, ns, vr), oracle.leg_fwd(grid, nc, nv, ns, vr))):
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
