@@ -77,6 +77,15 @@ SIGNATURES = [
     ("calypso_mhd_write_restart", _I, [_P, ctypes.c_char_p, _I, ctypes.c_char_p, _I]),
     ("calypso_mhd_read_restart", _I, [_P, ctypes.c_char_p]),
     ("calypso_mhd_coriolis_table", _I, [_I, _P, _P]),
+    ("calypso_scalar_shell_create", _I, [_I, _I, _P, _D, _I, _P, _P, _P, _P, ctypes.POINTER(_P)]),
+    ("calypso_scalar_shell_destroy", _I, [_P]),
+    ("calypso_scalar_shell_set_noise", _I, [_P]),
+    ("calypso_scalar_shell_set_field", _I, [_P, _I, _P]),
+    ("calypso_scalar_shell_get_field", _I, [_P, _I, _P]),
+    ("calypso_scalar_shell_initialize", _I, [_P]),
+    ("calypso_scalar_shell_step", _I, [_P, _I]),
+    ("calypso_scalar_shell_mean_square", _I, [_P, _P]),
+    ("calypso_scalar_shell_write_volume_ms", _I, [_P, ctypes.c_char_p]),
 ]
 
 _lib = None

@@ -7,11 +7,13 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "fft_internal.hpp"
 #include "leg_internal.hpp"
 #include "mhd_internal.hpp"
 #include "radial_internal.hpp"
+#include "scalar_internal.hpp"
 #include "seeded.hpp"
 
 using calypso::LegContext;
@@ -24,6 +26,9 @@ struct calypso_fft {
     calypso::FftContext f;
 };
 
+struct calypso_scalar_shell {
+    calypso::ScalarShell c;
+};
 struct calypso_mhd {
     calypso::MhdContext c;
 };
@@ -793,6 +798,104 @@ int calypso_lubksb_5band_mul(int msmp, const int *msmp_stack, int mcomp, int n,
                              const double *band_lu, const int *i_pivot, double *x)
 {
     return host_lubksb(5, msmp, msmp_stack, mcomp, n, band_lu, i_pivot, x);
+}
+
+// ---- scalars diffusing in a shell at rest -----------------------------------
+int calypso_scalar_shell_create(int l_truncation, int nr, const double *radius, double dt,
+                                int nscalar, const calypso_scalar_bc *scalars,
+                                const double *val_icb, const double *val_cmb, void *stream,
+                                calypso_scalar_shell_t **out)
+{
+    if (!radius || !scalars || !val_icb || !val_cmb || !out)
+        return fail("calypso_scalar_shell_create: null argument");
+    if (nscalar < 1 || nscalar > calypso::kScalarMax)
+        return fail("calypso_scalar_shell_create: 1..8 scalars");
+    std::vector<calypso::ScalarSpec> spec((size_t)nscalar);
+    std::vector<std::string> names((size_t)nscalar);
+    for (int q = 0; q < nscalar; ++q) {
+        spec[q].coef_d = scalars[q].coef_d;
+        spec[q].coef_imp = scalars[q].coef_imp;
+        spec[q].bc_icb = scalars[q].bc_icb;
+        spec[q].bc_cmb = scalars[q].bc_cmb;
+        names[q] = scalars[q].name ? scalars[q].name : "scalar_" + std::to_string(q + 1);
+    }
+    calypso_scalar_shell *h = new (std::nothrow) calypso_scalar_shell;
+    if (!h) return fail("calypso_scalar_shell_create: out of host memory");
+    std::string err;
+    if (calypso::scalar_shell_create(&h->c, l_truncation, nr, radius, dt, spec, val_icb, val_cmb,
+                                     (hipStream_t)stream, &err)) {
+        calypso::scalar_shell_free(&h->c);
+        delete h;
+        return fail(err);
+    }
+    h->c.names = names;
+    *out = h;
+    return 0;
+}
+
+int calypso_scalar_shell_destroy(calypso_scalar_shell_t *sh)
+{
+    if (!sh) return 0;
+    calypso::scalar_shell_free(&sh->c);
+    delete sh;
+    return 0;
+}
+
+int calypso_scalar_shell_set_noise(calypso_scalar_shell_t *sh)
+{
+    if (!sh) return fail("calypso_scalar_shell_set_noise: null argument");
+    std::string err;
+    if (calypso::scalar_shell_set_noise(&sh->c, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_scalar_shell_set_field(calypso_scalar_shell_t *sh, int scalar, const double *host)
+{
+    if (!sh || !host) return fail("calypso_scalar_shell_set_field: null argument");
+    std::string err;
+    if (calypso::scalar_shell_set_field(&sh->c, scalar, host, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_scalar_shell_get_field(calypso_scalar_shell_t *sh, int field, double *host)
+{
+    if (!sh || !host) return fail("calypso_scalar_shell_get_field: null argument");
+    std::string err;
+    if (calypso::scalar_shell_get_field(&sh->c, field, host, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_scalar_shell_initialize(calypso_scalar_shell_t *sh)
+{
+    if (!sh) return fail("calypso_scalar_shell_initialize: null argument");
+    std::string err;
+    if (calypso::scalar_shell_initialize(&sh->c, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_scalar_shell_step(calypso_scalar_shell_t *sh, int nsteps)
+{
+    if (!sh) return fail("calypso_scalar_shell_step: null argument");
+    if (nsteps < 0) return fail("calypso_scalar_shell_step: nsteps must be >= 0");
+    std::string err;
+    if (calypso::scalar_shell_step(&sh->c, nsteps, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_scalar_shell_mean_square(calypso_scalar_shell_t *sh, double *out)
+{
+    if (!sh || !out) return fail("calypso_scalar_shell_mean_square: null argument");
+    std::string err;
+    if (calypso::scalar_shell_mean_square(&sh->c, out, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_scalar_shell_write_volume_ms(calypso_scalar_shell_t *sh, const char *path)
+{
+    if (!sh || !path) return fail("calypso_scalar_shell_write_volume_ms: null argument");
+    std::string err;
+    if (calypso::scalar_shell_write_volume_ms(&sh->c, path, &err)) return fail(err);
+    return 0;
 }
 
 }  // extern "C"

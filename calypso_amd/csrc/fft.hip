@@ -27,18 +27,11 @@
 #include <string>
 #include <vector>
 
+#include "hip_check.hpp"
 #include "fft_internal.hpp"
 
 namespace calypso {
 
-#define HIP_TRY(expr)                                                         \
-    do {                                                                      \
-        hipError_t e_ = (expr);                                               \
-        if (e_ != hipSuccess) {                                               \
-            if (err) *err = std::string(#expr " failed: ") + hipGetErrorString(e_); \
-            return -1;                                                        \
-        }                                                                     \
-    } while (0)
 
 struct cplx {
     double x, y;

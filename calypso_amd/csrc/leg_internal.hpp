@@ -10,19 +10,9 @@
 #include <string>
 #include <vector>
 
+#include "hip_check.hpp"
+
 namespace calypso {
-
-// A device allocation released when it goes out of scope (unless released
-// to an owner first).
-struct DevBuf {
-    void *p = nullptr;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    void *release() { void *q = p; p = nullptr; return q; }
-};
-
 
 enum Precision : int { kF64 = 1, kF32 = 2, kBF16x3 = 3, kInt8x6 = 4 };
 

@@ -35,6 +35,8 @@
 #include <array>
 #include <map>
 #include <mutex>
+
+#include "hip_check.hpp"
 #include "leg_internal.hpp"
 
 #include <hip/hip_bf16.h>
@@ -49,14 +51,6 @@
 
 namespace calypso {
 
-#define HIP_TRY(expr)                                                         \
-    do {                                                                      \
-        hipError_t e_ = (expr);                                               \
-        if (e_ != hipSuccess) {                                               \
-            if (err) *err = std::string(#expr " failed: ") + hipGetErrorString(e_); \
-            return -1;                                                        \
-        }                                                                     \
-    } while (0)
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));

@@ -36,20 +36,13 @@
 #include <string>
 #include <vector>
 
+#include "hip_check.hpp"
 #include "mhd_internal.hpp"
 #include "seeded.hpp"
 #include "radial_internal.hpp"
 
 namespace calypso {
 
-#define HIP_TRY(expr)                                                         \
-    do {                                                                      \
-        hipError_t e_ = (expr);                                               \
-        if (e_ != hipSuccess) {                                               \
-            if (err) *err = std::string(#expr " failed: ") + hipGetErrorString(e_); \
-            return -1;                                                        \
-        }                                                                     \
-    } while (0)
 
 #pragma clang fp contract(off)
 
@@ -704,6 +697,56 @@ void coriolis_table(int L, double *cor, int *jk1, int *jk2, int *jl1)
     }
 }
 
+// Radial finite differences of the grid r[0..nr-1]: the three-point
+// interior weights (set_dr_for_nonequi, nod_r_2nd_fdm_coefs_nonequi) and the
+// one-sided wall formulas (cal_fdm_coefs_4_boundaries.f90).
+void radial_fdm(const double *r, int nr, RadialFdm *out)
+{
+    RadialFdm &f = *out;
+    std::vector<double> dr(nr);
+    f.d1.assign(3 * nr, 0.0);
+    f.d2.assign(3 * nr, 0.0);
+    for (int k = 1; k < nr - 1; ++k) dr[k] = r[k + 1] - r[k];
+    dr[nr - 1] = r[nr - 1] - r[nr - 2];
+    dr[0] = r[1] - r[0];
+    for (int k = 0; k < nr; ++k) {
+        double dp, dn;
+        if (k == 0) { dp = dr[0]; dn = dr[0]; }
+        else if (k == nr - 1) { dp = dr[k]; dn = dr[k]; }
+        else { dp = dr[k]; dn = dr[k - 1]; }
+        const double a[3][3] = {{1.0, 0.0, 0.0}, {1.0, dp, 0.5 * dp * dp}, {1.0, -dn, 0.5 * dn * dn}};
+        double m[3][3];
+        inverse33(a, m);
+        f.d1[3 * k + 0] = m[1][2]; f.d1[3 * k + 1] = m[1][0]; f.d1[3 * k + 2] = m[1][1];
+        f.d2[3 * k + 0] = m[2][2]; f.d2[3 * k + 1] = m[2][0]; f.d2[3 * k + 2] = m[2][1];
+    }
+    const int ki = 0, ko = nr - 1;
+    const double dp1 = r[ki + 1] - r[ki], dp2 = r[ki + 2] - r[ki];
+    const double a1[3][3] = {{1, 0, 0}, {1, dp1, dp1 * dp1 / 2.0}, {1, dp2, dp2 * dp2 / 2.0}};
+    double m[3][3];
+    inverse33(a1, m);
+    for (int i = 0; i < 3; ++i)
+        for (int q = 0; q < 3; ++q) f.ffl_i[i][q] = m[q][i];
+    const double a2[3][3] = {{1, 0, 0}, {0, 1, 0}, {dp1, 1, dp1 * dp1 / 2.0}};
+    inverse33(a2, m);
+    for (int i = 0; i < 3; ++i)
+        for (int q = 0; q < 3; ++q) f.fdr_i[i][q] = m[q][i];
+    const double dn1 = r[ko] - r[ko - 1], dn2 = r[ko] - r[ko - 2];
+    const double a3[3][3] = {{1, 0, 0}, {1, -dn1, dn1 * dn1 / 2.0}, {1, -dn2, dn2 * dn2 / 2.0}};
+    inverse33(a3, m);
+    for (int i = 0; i < 3; ++i)
+        for (int q = 0; q < 3; ++q) f.ffl_c[i][q] = m[q][i];
+    const double a4[3][3] = {{1, 0, 0}, {0, 1, 0}, {1, -dn1, dn1 * dn1 / 2.0}};
+    inverse33(a4, m);
+    for (int q = 0; q < 3; ++q) {
+        f.fdr_c[0][q] = m[q][2];
+        f.fdr_c[1][q] = m[q][0];
+        f.fdr_c[2][q] = m[q][1];
+    }
+    f.ri[0] = r[ki]; f.ri[1] = 1.0 / r[ki]; f.ri[2] = f.ri[1] * f.ri[1];
+    f.rc[0] = r[ko]; f.rc[1] = 1.0 / r[ko]; f.rc[2] = f.rc[1] * f.rc[1];
+}
+
 int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *radius,
                const MhdCoefs &co, int leg_mode, hipStream_t s, std::string *err)
 {
@@ -733,22 +776,9 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
         ar1[k] = 1.0 / r[k];
         ar2[k] = ar1[k] * ar1[k];
     }
-    // finite differences (set_dr_for_nonequi, nod_r_2nd_fdm_coefs_nonequi)
-    std::vector<double> dr(nr), d1(3 * nr), d2(3 * nr);
-    for (int k = 1; k < nr - 1; ++k) dr[k] = r[k + 1] - r[k];
-    dr[nr - 1] = r[nr - 1] - r[nr - 2];
-    dr[0] = r[1] - r[0];
-    for (int k = 0; k < nr; ++k) {
-        double dp, dn;
-        if (k == 0) { dp = dr[0]; dn = dr[0]; }
-        else if (k == nr - 1) { dp = dr[k]; dn = dr[k]; }
-        else { dp = dr[k]; dn = dr[k - 1]; }
-        const double a[3][3] = {{1.0, 0.0, 0.0}, {1.0, dp, 0.5 * dp * dp}, {1.0, -dn, 0.5 * dn * dn}};
-        double m[3][3];
-        inverse33(a, m);
-        d1[3 * k + 0] = m[1][2]; d1[3 * k + 1] = m[1][0]; d1[3 * k + 2] = m[1][1];
-        d2[3 * k + 0] = m[2][2]; d2[3 * k + 1] = m[2][0]; d2[3 * k + 2] = m[2][1];
-    }
+    RadialFdm fd;
+    radial_fdm(r.data(), nr, &fd);
+    const std::vector<double> &d1 = fd.d1, &d2 = fd.d2;
     MhdDev &p = c->p;
     p = MhdDev();
     p.nr = nr;
@@ -758,32 +788,12 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     p.ko = ko;
     p.nnod = nnod;
     p.rJ = 1.0 / (double)J;
-    {   // one-sided wall formulas (cal_fdm_coefs_4_boundaries.f90)
-        const double dp1 = r[ki + 1] - r[ki], dp2 = r[ki + 2] - r[ki];
-        const double a1[3][3] = {{1, 0, 0}, {1, dp1, dp1 * dp1 / 2.0}, {1, dp2, dp2 * dp2 / 2.0}};
-        double m[3][3];
-        inverse33(a1, m);
-        for (int i = 0; i < 3; ++i)
-            for (int q = 0; q < 3; ++q) p.ffl_i[i][q] = m[q][i];
-        const double a2[3][3] = {{1, 0, 0}, {0, 1, 0}, {dp1, 1, dp1 * dp1 / 2.0}};
-        inverse33(a2, m);
-        for (int i = 0; i < 3; ++i)
-            for (int q = 0; q < 3; ++q) p.fdr_i[i][q] = m[q][i];
-        const double dn1 = r[ko] - r[ko - 1], dn2 = r[ko] - r[ko - 2];
-        const double a3[3][3] = {{1, 0, 0}, {1, -dn1, dn1 * dn1 / 2.0}, {1, -dn2, dn2 * dn2 / 2.0}};
-        inverse33(a3, m);
-        for (int i = 0; i < 3; ++i)
-            for (int q = 0; q < 3; ++q) p.ffl_c[i][q] = m[q][i];
-        const double a4[3][3] = {{1, 0, 0}, {0, 1, 0}, {1, -dn1, dn1 * dn1 / 2.0}};
-        inverse33(a4, m);
-        for (int q = 0; q < 3; ++q) {
-            p.fdr_c[0][q] = m[q][2];
-            p.fdr_c[1][q] = m[q][0];
-            p.fdr_c[2][q] = m[q][1];
-        }
-        p.ri[0] = r[ki]; p.ri[1] = 1.0 / r[ki]; p.ri[2] = p.ri[1] * p.ri[1];
-        p.rc[0] = r[ko]; p.rc[1] = 1.0 / r[ko]; p.rc[2] = p.rc[1] * p.rc[1];
-    }
+    std::memcpy(p.ffl_i, fd.ffl_i, sizeof p.ffl_i);
+    std::memcpy(p.fdr_i, fd.fdr_i, sizeof p.fdr_i);
+    std::memcpy(p.ffl_c, fd.ffl_c, sizeof p.ffl_c);
+    std::memcpy(p.fdr_c, fd.fdr_c, sizeof p.fdr_c);
+    std::memcpy(p.ri, fd.ri, sizeof p.ri);
+    std::memcpy(p.rc, fd.rc, sizeof p.rc);
     p.dt = co.dt;
     p.coef_exp_v = 1.0 - co.coef_imp_v;
     p.coef_exp_t = 1.0 - co.coef_imp_t;
@@ -996,55 +1006,12 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     // LU on the GPU (ludcmp_band / ludcmp_3band)
     const std::vector<double> *mats[5] = {&vp, &vt, &tm, &bs, &bt};
     const int nbs[5] = {5, 3, 3, 3, 3};
-    // temporaries released on every exit path; the context's own buffers
-    // are released by mhd_free
-    DevBuf a_buf, ierr_buf;
-    HIP_TRY(hipMalloc(&a_buf.p, sizeof(double) * 5 * nnod));
-    HIP_TRY(hipMalloc(&ierr_buf.p, sizeof(int) * J));
-    double *d_a = static_cast<double *>(a_buf.p);
-    int *d_ierr = static_cast<int *>(ierr_buf.p);
-    for (int q = 0; q < 5; ++q) {
-        const int nb = nbs[q];
-        HIP_TRY(hipMemcpy(d_a, mats[q]->data(), sizeof(double) * nb * nnod, hipMemcpyHostToDevice));
-        HIP_TRY(hipMalloc(&c->d_lu[q], sizeof(double) * (2 * nb - 1) * nnod));
-        HIP_TRY(hipMalloc(&c->d_piv[q], sizeof(int) * nnod));
-        if (ludcmp_dev(nb, nr, J, d_a, c->d_lu[q], c->d_piv[q], d_ierr, s, err)) return -1;
-        HIP_TRY(hipStreamSynchronize(s));
-        // the step solves with mode-fastest copies of the factors; its
-        // register window holds the (nb-1)/2 rows an interchange can reach
-        {
-            std::vector<int> piv((size_t)nnod);
-            HIP_TRY(hipMemcpy(piv.data(), c->d_piv[q], sizeof(int) * nnod, hipMemcpyDeviceToHost));
-            for (int j = 0; j < J; ++j)
-                for (int i = 1; i <= nr; ++i) {
-                    const int d = piv[(size_t)(i - 1) + (size_t)nr * j] - i;
-                    if (d < 0 || d > (nb - 1) / 2) {
-                        *err = "mhd_create: row interchange outside the band window";
-                        return -1;
-                    }
-                }
+    for (int q = 0; q < 5; ++q)
+        if (factor_band_for_solve(nbs[q], nr, J, mats[q]->data(), &c->d_lu[q], &c->d_piv[q], s,
+                                  err)) {
+            *err = "mhd_create: " + *err;
+            return -1;
         }
-        {
-            DevBuf bt, pt;
-            HIP_TRY(hipMalloc(&bt.p, sizeof(double) * (2 * nb - 1) * nnod));
-            HIP_TRY(hipMalloc(&pt.p, sizeof(int) * nnod));
-            if (lu_transpose_dev(nb, nr, J, c->d_lu[q], c->d_piv[q], static_cast<double *>(bt.p),
-                                 static_cast<int *>(pt.p), s, err))
-                return -1;
-            HIP_TRY(hipStreamSynchronize(s));
-            (void)hipFree(c->d_lu[q]);
-            (void)hipFree(c->d_piv[q]);
-            c->d_lu[q] = static_cast<double *>(bt.release());
-            c->d_piv[q] = static_cast<int *>(pt.release());
-        }
-        std::vector<int> ierr(J);
-        HIP_TRY(hipMemcpy(ierr.data(), d_ierr, sizeof(int) * J, hipMemcpyDeviceToHost));
-        for (int j = 0; j < J; ++j)
-            if (ierr[j]) {
-                *err = "mhd_create: singular radial matrix";
-                return -1;
-            }
-    }
     // transforms
     if (leg_context_build(&c->leg, L, nth, nr, radius, nullptr, nullptr, kNFwd, 4, 0, leg_mode, err))
         return -1;

@@ -26,6 +26,17 @@ enum MhdField : int {
     kMhdNumComp = 56
 };
 
+// Radial finite differences of a grid (nr >= 3): interior three-point
+// weights d1/d2 [nr][3] at offsets -1, 0, +1; wall formulas for a fixed value
+// (ffl_*) and a fixed gradient (fdr_*) at the inner (_i) and outer (_c)
+// walls; powers r, 1/r, 1/r^2 of the wall radii.
+struct RadialFdm {
+    std::vector<double> d1, d2;
+    double ffl_i[3][3], fdr_i[3][3], ffl_c[3][3], fdr_c[3][3];
+    double ri[3], rc[3];
+};
+void radial_fdm(const double *r, int nr, RadialFdm *out);
+
 struct MhdCoefs {
     double dt = 0;
     double coef_imp_v = 0, coef_imp_t = 0, coef_imp_b = 0;

@@ -16,19 +16,13 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <vector>
 
+#include "hip_check.hpp"
 #include "radial_internal.hpp"
 
 namespace calypso {
 
-#define HIP_TRY(expr)                                                         \
-    do {                                                                      \
-        hipError_t e_ = (expr);                                               \
-        if (e_ != hipSuccess) {                                               \
-            if (err) *err = std::string(#expr " failed: ") + hipGetErrorString(e_); \
-            return -1;                                                        \
-        }                                                                     \
-    } while (0)
 
 #pragma clang fp contract(off)
 
@@ -459,6 +453,53 @@ int ludcmp_dev(int nb, int n, int mcomp, const double *d_band_a, double *d_band_
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipFreeAsync(wk, s));
     HIP_TRY(hipFreeAsync(iwk, s));
+    return 0;
+}
+
+// LU factors of nb-band matrices h_band[(j n + k) nb + b] (Fortran
+// band(b+1, k+1, j+1)), ready for lubksb_t_dev / lubksb_t_batch_dev: factored
+// on the device with the reference's pivoting, checked (no singular mode, no
+// row interchange beyond the solver's register window of (nb-1)/2 rows) and
+// transposed to the mode-fastest layout.  *d_lu_t / *d_piv_t are allocated
+// here and owned by the caller; nothing is left allocated on failure.
+int factor_band_for_solve(int nb, int n, int mcomp, const double *h_band, double **d_lu_t,
+                          int **d_piv_t, hipStream_t s, std::string *err)
+{
+    const size_t nnod = (size_t)n * mcomp;
+    DevBuf a, lu, piv, ierr_d, lu_t, piv_t;
+    HIP_TRY(hipMalloc(&a.p, sizeof(double) * nb * nnod));
+    HIP_TRY(hipMalloc(&lu.p, sizeof(double) * (2 * nb - 1) * nnod));
+    HIP_TRY(hipMalloc(&piv.p, sizeof(int) * nnod));
+    HIP_TRY(hipMalloc(&ierr_d.p, sizeof(int) * mcomp));
+    HIP_TRY(hipMemcpy(a.p, h_band, sizeof(double) * nb * nnod, hipMemcpyHostToDevice));
+    if (ludcmp_dev(nb, n, mcomp, static_cast<double *>(a.p), static_cast<double *>(lu.p),
+                   static_cast<int *>(piv.p), static_cast<int *>(ierr_d.p), s, err))
+        return -1;
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int> ierr(mcomp), pv(nnod);
+    HIP_TRY(hipMemcpy(ierr.data(), ierr_d.p, sizeof(int) * mcomp, hipMemcpyDeviceToHost));
+    for (int j = 0; j < mcomp; ++j)
+        if (ierr[j]) {
+            *err = "singular radial matrix";
+            return -1;
+        }
+    HIP_TRY(hipMemcpy(pv.data(), piv.p, sizeof(int) * nnod, hipMemcpyDeviceToHost));
+    for (int j = 0; j < mcomp; ++j)
+        for (int i = 1; i <= n; ++i) {
+            const int d = pv[(size_t)(i - 1) + (size_t)n * j] - i;
+            if (d < 0 || d > (nb - 1) / 2) {
+                *err = "row interchange outside the band window";
+                return -1;
+            }
+        }
+    HIP_TRY(hipMalloc(&lu_t.p, sizeof(double) * (2 * nb - 1) * nnod));
+    HIP_TRY(hipMalloc(&piv_t.p, sizeof(int) * nnod));
+    if (lu_transpose_dev(nb, n, mcomp, static_cast<double *>(lu.p), static_cast<int *>(piv.p),
+                         static_cast<double *>(lu_t.p), static_cast<int *>(piv_t.p), s, err))
+        return -1;
+    HIP_TRY(hipStreamSynchronize(s));
+    *d_lu_t = static_cast<double *>(lu_t.release());
+    *d_piv_t = static_cast<int *>(piv_t.release());
     return 0;
 }
 

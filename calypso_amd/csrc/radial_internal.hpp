@@ -19,6 +19,10 @@ int lubksb_t_dev(int nb, int n, int mcomp, const double *d_band_t, const int *d_
 int lubksb_t_batch_dev(int nb, int n, int mcomp, int count, const double *const *d_band_t,
                        const int *const *d_ipiv_t, double *const *d_x, hipStream_t s,
                        std::string *err);
+// factor on the device, check, and transpose for lubksb_t_dev (allocates the
+// two outputs, which the caller owns)
+int factor_band_for_solve(int nb, int n, int mcomp, const double *h_band, double **d_lu_t,
+                          int **d_piv_t, hipStream_t s, std::string *err);
 int lubksb_dev(int nb, int n, int mcomp, const double *d_band_lu, const int *d_ipiv, double *d_x,
                hipStream_t s, std::string *err);
 

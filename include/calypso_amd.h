@@ -378,6 +378,48 @@ int calypso_mhd_read_restart(calypso_mhd_t *mhd, const char *path);
  * kernel, the default where the pencil fits in LDS; 0: three kernels with
  * the grids in HBM), "legendre_autotune" (0/1). */
 int calypso_mhd_set_option(calypso_mhd_t *mhd, const char *name, long long value);
+/* ---- scalars diffusing in a shell at rest ---------------------------------
+ * Replaces SPH_initialize_MHD / SPH_analyze_MHD (programs/SPH_MHD/
+ * SPH_analyzer_MHD.f90:31-202) for a model that evolves only scalars
+ * (time_evo_ctl temperature, composition) in a fluid at rest, each wall
+ * holding a value or a flux per spherical-harmonic mode from the spectral
+ * boundary file (set_bc_sph_scalars.f90, m_sph_boundary_input_data.f90:
+ * 147-300; the reference's tests/heterogineous_temp).  FP64, one stream. */
+#define CALYPSO_BC_FIXED_VALUE 0
+#define CALYPSO_BC_FIXED_FLUX 1
+typedef struct calypso_scalar_shell calypso_scalar_shell_t;
+typedef struct calypso_scalar_bc {
+    double coef_d;        /* diffusion coefficient (coef_d_temp, coef_d_light) */
+    double coef_imp;      /* Crank-Nicolson weight (coef_imp_t, coef_imp_c) */
+    int bc_icb, bc_cmb;   /* CALYPSO_BC_FIXED_VALUE or CALYPSO_BC_FIXED_FLUX */
+    const char *name;     /* column name in the energy file, or NULL */
+} calypso_scalar_bc;
+/* radius[nr] the radial grid (walls at its ends), 1..8 scalars;
+ * val_icb / val_cmb [nscalar][J], J = (L+1)^2, mode l(l+1)+m fastest: the
+ * wall value or the flux as the boundary file gives it (the inner flux is
+ * negated inside, as inner_sph_bc_scalar_sph does).  Errors: unknown
+ * boundary kind, grid not increasing, singular radial matrix. */
+int calypso_scalar_shell_create(int l_truncation, int nr, const double *radius, double dt,
+                                int nscalar, const calypso_scalar_bc *scalars,
+                                const double *val_icb, const double *val_cmb, void *stream,
+                                calypso_scalar_shell_t **out);
+int calypso_scalar_shell_destroy(calypso_scalar_shell_t *sh);
+/* set_noize_scalar_sph (set_initial_sph_dynamo.f90:390-441) on every scalar. */
+int calypso_scalar_shell_set_noise(calypso_scalar_shell_t *sh);
+/* host[nr*J] into scalar `scalar` (mode fastest). */
+int calypso_scalar_shell_set_field(calypso_scalar_shell_t *sh, int scalar, const double *host);
+/* field = which * nscalar + scalar, which 0 value, 1 diffusion term,
+ * 2 advection term, 3 Adams history; host[nr*J]. */
+int calypso_scalar_shell_get_field(calypso_scalar_shell_t *sh, int field, double *host);
+/* The diffusion terms of the initial fields (fixed walls take their values). */
+int calypso_scalar_shell_initialize(calypso_scalar_shell_t *sh);
+/* nsteps time steps (Euler on the first, then Adams-Bashforth). */
+int calypso_scalar_shell_step(calypso_scalar_shell_t *sh, int nsteps);
+/* out[nscalar]: volume mean square of each scalar over the shell. */
+int calypso_scalar_shell_mean_square(calypso_scalar_shell_t *sh, double *out);
+/* Appends the current step's row to sph_pwr_volume_s.dat, header first. */
+int calypso_scalar_shell_write_volume_ms(calypso_scalar_shell_t *sh, const char *path);
+
 /* Host-only: coupling table of the curl of the Coriolis force for
  * truncation L (init_sum_coriolis_rlm, const_coriolis_sph_rlm.f90:34-67).
  * cor[12*J]: sw11 sw21 sw12 sw22 sw13 tw13 tw23 tw14 tw24 tw11 tw12 g3 per rj
