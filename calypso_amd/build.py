@@ -23,7 +23,7 @@ LIB = os.path.join(HERE, "lib")
 CSRC = os.path.join(HERE, "csrc")
 FSRC = os.path.join(HERE, "fortran")
 
-HIP_SOURCES = ["legendre.hip", "fft.hip", "radial.hip", "mhd.hip", "scalar_shell.hip", "sph_io.cpp",
+HIP_SOURCES = ["legendre.hip", "leg_ozaki.hip", "fft.hip", "radial.hip", "mhd.hip", "scalar_shell.hip", "sph_io.cpp",
                "sph_grids.cpp", "capi.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -48,6 +48,8 @@ struct LegPlan {
     WorkItem *d_items = nullptr;
 };
 
+struct OzTables;   // int8 split-once form (leg_ozaki.hip)
+
 struct LegContext {
     LegGeometry g;
     int ncomp = 0, nvec = 0, nscl = 0;   // field set given at creation
@@ -69,6 +71,7 @@ struct LegContext {
     double *d_rsin = nullptr;         // 1/sin(theta) at the hemisphere rows
     double *d_gfac = nullptr;         // g7, g6 normalisations of order m > 0
     size_t tab_bytes = 0;
+    OzTables *oz = nullptr;           // int8 form: digit tables
 
     std::deque<LegPlan> plans;    // per (ncomp, nvec, nscl, direction)
     bool autotune = true;         // time candidate plans on first use
@@ -102,5 +105,15 @@ int  leg_force_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, int 
                     std::string *err);
 int  leg_tables_to_host(LegContext *c, double *P, double *dP,
                         std::string *err);
+
+// int8 split-once form (leg_ozaki.hip): digit tables from the FP64 P, dP and
+// P m/sin(theta) at the hemisphere rows; plans of kernel form 5
+int  oz_build_tables(LegContext *c, const double *rP, const double *rdP, const double *rPM,
+                     std::string *err);
+void oz_free(LegContext *c);
+void oz_plan_candidates(int ncomp, int nv, int ns, bool fwd, std::vector<LegPlan> *out);
+int  oz_build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err);
+int  oz_launch(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
+               hipStream_t s, std::string *err);
 
 }  // namespace calypso

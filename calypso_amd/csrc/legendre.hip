@@ -2629,7 +2629,7 @@ static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd,
     static const int inst[3][2] = {{1, 2}, {2, 4}, {1, 1}};
     std::vector<LegPlan> out;
     if (prec == kInt8x6) {
-        plan_candidates_z(ncomp, nv, ns, fwd, &out);
+        oz_plan_candidates(ncomp, nv, ns, fwd, &out);
         return out;
     }
     plan_candidates3(ncomp, nv, ns, fwd, prec, &out);
@@ -2704,6 +2704,7 @@ static void xcd_interleave(int L, std::vector<WorkItem> *items)
 
 static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
 {
+    if (p->ver == 5) return oz_build_items(c, p, fwd, err);
     const LegGeometry &g = c->g;
     const int nkt = (g.nr + p->KT - 1) / p->KT;
     std::vector<WorkItem> items;
@@ -2971,7 +2972,7 @@ int leg_context_build(LegContext *c, int L, int nth, int nr, const double *radiu
     int rc;
     if (prec == kF64) rc = build_tables<TF64>(c, rP, rdP, rPM, err);
     else if (prec == kF32) rc = build_tables<TF32>(c, rP, rdP, rPM, err);
-    else if (prec == kInt8x6) rc = build_tables_z(c, rP, rdP, err);
+    else if (prec == kInt8x6) rc = oz_build_tables(c, rP, rdP, rPM, err);
     else rc = build_tables<TBF3>(c, rP, rdP, rPM, err);
     HIP_TRY(hipStreamSynchronize(c->stream));
     (void)hipFree(rP);
@@ -3002,6 +3003,7 @@ void leg_context_free(LegContext *c)
     (void)hipFree(c->d_off_fwd3);
     (void)hipFree(c->d_rsin);
     (void)hipFree(c->d_gfac);
+    oz_free(c);
     for (LegPlan &p : c->plans) (void)hipFree(p.d_items);
     c->plans.clear();
     (void)hipFree(c->d_rlm);
@@ -3228,7 +3230,7 @@ static int launch_z(LegContext *c, const LegPlan &p, bool fwd, const double *in,
 static int launch_plan(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
                        hipStream_t s, std::string *err)
 {
-    if (c->prec == kInt8x6) return launch_z(c, p, fwd, in, out, s, err);
+    if (c->prec == kInt8x6) return oz_launch(c, p, fwd, in, out, s, err);
     if (c->prec == kF64) return launch_p<TF64>(c, p, fwd, in, out, s, err);
     if (c->prec == kF32) return launch_p<TF32>(c, p, fwd, in, out, s, err);
     return launch_p<TBF3>(c, p, fwd, in, out, s, err);
