@@ -29,7 +29,9 @@
 // branches (see its section).  The autotuner picks per direction.
 // Precisions: FP64 MFMA 16x16x4, FP32 MFMA 16x16x4, split-BF16x3 MFMA
 // 16x16x32 (three bf16 slices per operand, six slice products, FP32
-// accumulation).  P_l^m tables are computed on the GPU in FP64 and packed in
+// accumulation); the FP64-accurate int8 digit form lives in leg_ozaki.hip
+// (kernel form 5) and shares this file's tables, plans and autotuner.
+// P_l^m tables are computed on the GPU in FP64 and packed in
 // the lane order the MFMA A operand is read in (one 64-lane unit per
 // 16 x KSTEP tile), so every A fetch is a contiguous 512 B - 1 KiB load.
 #include <array>
@@ -64,7 +66,6 @@ struct TF64 {
     using frag = double;
     using acc = d4v;
     static constexpr int KSTEP = 4, NS = 1, NSA = 1, EPL = 1;
-    static constexpr bool OZ = false;
     __device__ static inline void store(elem *dst, double x, int stride) { dst[0] = x; }
     __device__ static inline frag load(const elem *p) { return *p; }
     __device__ static inline frag neg(frag a) { return -a; }
@@ -82,7 +83,6 @@ struct TF32 {
     using frag = float;
     using acc = f4v;
     static constexpr int KSTEP = 4, NS = 1, NSA = 1, EPL = 1;
-    static constexpr bool OZ = false;
     __device__ static inline void store(elem *dst, double x, int stride) { dst[0] = (float)x; }
     __device__ static inline frag load(const elem *p) { return *p; }
     __device__ static inline frag neg(frag a) { return -a; }
@@ -106,7 +106,6 @@ struct TBF3 {
     using frag = bf8v;
     using acc = f4v;
     static constexpr int KSTEP = 32, NS = 3, NSA = 3, EPL = 8;
-    static constexpr bool OZ = false;
     // three-slice split x = hi + mid + lo; each residual is exact in FP64
     __device__ static inline void store(elem *dst, double x, int stride) {
         unsigned short h = bf16_bits(x);
@@ -137,20 +136,6 @@ struct TBF3 {
     }
     __device__ static inline double get(const acc &c, int v) { return (double)c[v]; }
     __device__ static inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-
-typedef int i4v __attribute__((ext_vector_type(4)));
-
-// int8 slices with shared exponents (the int8 form below): a 16 x 64 table
-// tile is six digit slices plus one slice of row exponents; the staged
-// operand is split in LDS at kernel time.  Only the int8-form kernels use it.
-struct TI8 {
-    using elem = signed char;
-    using frag = i4v;
-    using acc = d4v;
-    static constexpr int KSTEP = 64, NS = 7, NSA = 8, EPL = 16;
-    static constexpr bool OZ = true;
-    static constexpr int GF = 6, GB = 6;   // digit-pair orders kept (fwd, bwd)
 };
 
 // unit = one 16 x KSTEP operand tile in lane order: NSA slices x 64 lanes x EPL
@@ -1628,700 +1613,6 @@ __global__ __launch_bounds__(NW * 64) void k_leg_bwd3(LegArgs a)
     }
 }
 
-// ======================================================= int8 form ====
-//
-// FP64-accurate contraction on the int8 MFMA by error-free splitting of both
-// operands into int8 slices (Ozaki, Ogita, Oishi & Rump, Numer. Algorithms 59
-// (2012) 95-118; integer-MMA variant: Ootomo, Ozaki & Yokota, IJHPCA 38 (2024)
-// 297-313).  Within one 64-long K step every table row and every staged
-// column has a power-of-two scale 2^e with |x| < 2^e for all its elements,
-// and each element is held as seven balanced base-128 digits
-//     x = 2^(e-6) (d0 + d1/128 + ... + d6/128^6),   |d_i| <= 64,
-// exact to 2^(e-55).  A row-column product over the K step is
-//     2^(ea+eb-12) sum_{i+j<=G} 128^-(i+j) (d_i . d_j),
-// each digit dot product an exact int32 MFMA sum (|.| <= 7*64*64*64); the G+1
-// digit groups are combined in FP64 (Horner) and added to FP64 accumulators.
-// G = 6: 28 MFMAs per tile and K step; the pairs left out (i+j = 7..12)
-// change a term by about 2^(ea+eb-56).  Six digits are not enough: their
-// truncation (2^(e-48)) gives ~1e-11 normwise in the backward direction,
-// where the m/sin(theta) terms near the poles amplify it.
-//
-// The kernels keep the third form's work items, column maps, staging slots
-// and epilogues; an LDS chunk is exactly one K step, staged in three phases:
-// column maxima of the exponents (LDS atomics), the split into digit slices
-// at the column scale, then the MFMAs of the step.
-
-constexpr int kZExpMin = -960;     // exponent floor: keeps 2^(6-e) finite
-constexpr int kZPad = 16;          // LDS bytes after each staged 64-byte column
-constexpr int kZD = 7;             // digits per operand element
-constexpr int kZUnit = (kZD + 1) * 1024;   // table unit bytes: digit slices + exponents
-constexpr int kSlotZF = 10;        // (row, mirror row) pairs per thread per step (fwd)
-constexpr int kSlotZB = 13;        // spectral values per thread per step (bwd)
-
-// e with |x| < 2^e for normal x != 0; zero and subnormals give -1022
-__device__ inline int z_exp(double x) { return ((__double2hiint(x) >> 20) & 0x7FF) - 1022; }
-
-// 2^e for e in [-1022, 1023]
-__device__ inline double z_pow2(int e) { return __hiloint2double((e + 1023) << 20, 0); }
-
-// the kZD digits of x at scale e (|x| < 2^e, e >= kZExpMin) to da[s*stride]
-// and, negated by nb, to db (if non-null); negated by na at da
-__device__ inline void z_split(signed char *da, signed char *db, double x, int e, bool na,
-                               bool nb, int stride)
-{
-    double y = x * z_pow2(6 - e);
-#pragma unroll
-    for (int s = 0; s < kZD; ++s) {
-        const double d = rint(y);
-        const int di = (int)d;
-        da[s * stride] = (signed char)(na ? -di : di);
-        if (db) db[s * stride] = (signed char)(nb ? -di : di);
-        y = (y - d) * 128.0;
-    }
-}
-
-// digit groups I_g = sum_{i+j=g} a_i . b_j of one K step
-template <int G>
-__device__ inline void z_mma(i4v (&I)[G + 1], const i4v *a, const i4v *b)
-{
-#pragma unroll
-    for (int g = 0; g <= G; ++g) I[g] = i4v{0, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < kZD && i <= G; ++i)
-#pragma unroll
-        for (int j = 0; j < kZD && i + j <= G; ++j)
-            I[i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], I[i + j], 0, 0, 0);
-}
-
-// c += 2^(ea+eb-12) sum_g 128^-g I_g; ea per accumulator row (the table's
-// exponent slice), eb of the lane's column
-template <int G>
-__device__ inline void z_flush(d4v &c, const i4v (&I)[G + 1], i4v ea, int eb)
-{
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        double t = (double)I[G][v];
-#pragma unroll
-        for (int g = G - 1; g >= 0; --g) t = fma(t, 0x1p-7, (double)I[g][v]);
-        const int E = min(max(ea[v] + eb + (1023 - 12), 0), 2046);   // 0: scale 0
-        c[v] = fma(t, __hiloint2double(E << 20, 0), c[v]);
-    }
-}
-
-// one table row of a unit: exponent over its 64 K values, digits, and the
-// exponent in the slice read by the lanes that hold that accumulator row
-// (lane/16 == row/4, int row%4)
-template <class F>
-__device__ inline void z_pack_row(signed char *unit, int r16, F val)
-{
-    int e = kZExpMin;
-    for (int k = 0; k < 64; ++k) e = max(e, z_exp(val(k)));
-    for (int k = 0; k < 64; ++k) {
-        const int lane = r16 + 16 * (k >> 4);
-        z_split(unit + lane * 16 + (k & 15), nullptr, val(k), e, false, false, 1024);
-    }
-    int *ex = reinterpret_cast<int *>(unit + kZD * 1024);
-    for (int q = 0; q < 16; ++q) ex[((r16 >> 2) * 16 + q) * 4 + (r16 & 3)] = e;
-}
-
-// forward int8 units: for m: for par: for rt < RT_par(m): for ks < KSt:
-// for w < 2 (P, dP); row = degree index rt*16 + r16, K = colatitude ks*64 + k
-__global__ void k_pack_fwdz(int L, int th_h, int KSt, const double *P, const double *dP,
-                            const long long *off, signed char *tab)
-{
-    const int mp = blockIdx.y;
-    const int m = mp >> 1, par = mp & 1;
-    const long long u0 = off[mp], nu = off[mp + 1] - off[mp];
-    const long long pos = (long long)m * (L + 1) - (long long)m * (m - 1) / 2;
-    for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < nu * 16;
-         x += (long long)gridDim.x * blockDim.x) {
-        const long long u = x >> 4;
-        const int r16 = (int)(x & 15);
-        const int w = (int)(u & 1);
-        const long long u2 = u >> 1;
-        const int ks = (int)(u2 % KSt), rt = (int)(u2 / KSt);
-        const int l = m + par + 2 * (rt * 16 + r16);
-        const double *src = (l <= L) ? (w ? dP : P) + (pos + (l - m)) * th_h : nullptr;
-        z_pack_row(tab + (u0 + u) * kZUnit, r16, [&](int k) {
-            const int t = ks * 64 + k;
-            return (src && t < th_h) ? src[t] : 0.0;
-        });
-    }
-}
-
-// backward int8 units: for m: for rt < RT: for ks < KS(m): for w < 4 (P
-// even, P odd, dP even, dP odd); row = colatitude rt*16 + r16, K = degree
-// index ks*64 + k of the parity
-__global__ void k_pack_bwdz(int L, int th_h, int RT, const double *P, const double *dP,
-                            const long long *off, signed char *tab)
-{
-    const int m = blockIdx.y;
-    const long long u0 = off[m], nu = off[m + 1] - off[m];
-    const int KS = ((L - m + 2) / 2 + 63) / 64;
-    const long long pos = (long long)m * (L + 1) - (long long)m * (m - 1) / 2;
-    for (long long x = blockIdx.x * (long long)blockDim.x + threadIdx.x; x < nu * 16;
-         x += (long long)gridDim.x * blockDim.x) {
-        const long long u = x >> 4;
-        const int r16 = (int)(x & 15);
-        const int w = (int)(u & 3);
-        const long long u2 = u >> 2;
-        const int ks = (int)(u2 % KS), rt = (int)(u2 / KS);
-        const int t = rt * 16 + r16, par = w & 1;
-        const double *src = (w < 2) ? P : dP;
-        z_pack_row(tab + (u0 + u) * kZUnit, r16, [&](int k) {
-            const int l = m + par + 2 * (ks * 64 + k);
-            return (t < th_h && l <= L) ? src[(pos + (l - m)) * th_h + t] : 0.0;
-        });
-    }
-}
-
-// Forward, int8 form (third-form forward with one K step per LDS chunk).
-template <int G, int RPW, int NC1, int NC2>
-__global__ __launch_bounds__(kThreads) void k_leg_fwdz(LegArgs a)
-{
-    constexpr int NB = 16 * (NC1 + 2 * NC2) + 1;
-    constexpr int CX = 16 * NC1, CY = 16 * (NC1 + NC2), CZ = NB - 1;
-    constexpr int TC = 64, TCP = TC + kZPad, SL = 2 * NB * TCP;
-    constexpr int kNone = 0xFFFF;
-
-    const WorkItem it = a.items[blockIdx.x];
-    if (it.m < 0) return;   // padding of the XCD interleave
-    const int m = it.m;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int K = a.L - m + 1;
-    const int par = wave & 1;
-    const int Kp = par ? K / 2 : (K + 1) / 2;
-    const int RTp = (Kp + 15) / 16;
-    const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
-    const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 4 * nv;
-    const int nks = a.KSt;
-    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
-
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    signed char *V = reinterpret_cast<signed char *>(smem_raw);   // [kZD][2 fold][NB][TCP]
-    constexpr int CEO = (kZD * SL + 15) & ~15;
-    int *CE = reinterpret_cast<int *>(smem_raw + CEO);             // [2 buf][2 fold][NB]
-    double2 *WF = reinterpret_cast<double2 *>(smem_raw + CEO + 16 * NB);
-    for (int x = threadIdx.x; x < kZD * 2 * TCP; x += kThreads)
-        V[(x / (2 * TCP)) * SL + (((x / TCP) & 1) * NB + CZ) * TCP + x % TCP] = 0;
-    for (int x = threadIdx.x; x < 2 * NB; x += kThreads) CE[x] = kZExpMin;
-    for (int t = threadIdx.x; t < nks * TC; t += kThreads) {
-        double w = 0.0, y = 0.0;
-        if (t < a.th_h) {
-            w = a.weight[t];
-            if (a.nth - 1 - t == t) w *= 0.5;
-            y = w * ((double)m * a.rsin[t]);
-        }
-        WF[t] = make_double2(w, y);
-    }
-
-    int rtile[RPW];
-    bool rok[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        const int rt = (it.rgroup * 2 + (wave >> 1)) * RPW + r;
-        rok[r] = rt < RTp;
-        rtile[r] = rok[r] ? rt : 0;
-    }
-    // a wave without a degree tile (last row group of an order) stages and
-    // splits its share of the input but issues no MFMAs
-    bool anyrow = false;
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) anyrow |= rok[r];
-    const int pt = RTp > 0 ? par : 0;
-    const signed char *tab = reinterpret_cast<const signed char *>(a.tab);
-    const signed char *ap[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r)
-        ap[r] = tab + (a.off[2 * m + pt] + (long long)rtile[r] * nks * 2) * kZUnit + lane * 16;
-
-    // B fragment byte offsets (slice 0) and exponent slots of this lane's
-    // columns; P terms of parity par read fold par, dP terms fold 1-par
-    const int kq = 16 * (lane >> 4);
-    int q1[NC1], qx[NC2], qy[NC2];
-#pragma unroll
-    for (int ct = 0; ct < NC1; ++ct) {
-        const int c = ct * 16 + (lane & 15);
-        q1[ct] = par * NB + (c < c1n ? c : CZ);
-    }
-#pragma unroll
-    for (int ct = 0; ct < NC2; ++ct) {
-        const int c = ct * 16 + (lane & 15);
-        qx[ct] = (1 - par) * NB + (c < c2n ? CX + c : CZ);
-        qy[ct] = par * NB + (c < c2n ? CY + c : CZ);
-    }
-    int ocol[NC1 + NC2];
-#pragma unroll
-    for (int ct = 0; ct < NC1 + NC2; ++ct) {
-        const bool g1c = ct < NC1;
-        const int c = (g1c ? ct : ct - NC1) * 16 + (lane & 15);
-        int kk, sg, comp, kind;
-        if (g1c) {
-            const int per = nv + ns;
-            kk = c / (2 * per);
-            const int rem = c % (2 * per);
-            sg = rem / per;
-            const int q = rem % per;
-            comp = q < nv ? 3 * q : 3 * nv + (q - nv);
-            kind = q < nv ? 0 : 2;
-        } else {
-            kk = c / (4 * nv);
-            const int rem = c % (4 * nv);
-            sg = rem / (2 * nv);
-            const int r2 = rem % (2 * nv);
-            comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
-            kind = 1;
-        }
-        const bool ok = (g1c ? c < c1n : c < c2n) && !(sg == 1 && m == 0);
-        ocol[ct] = ok ? comp | (kk << 8) | (sg << 11) | (kind << 12) : -1;
-    }
-
-    // staging slots as in the third form
-    const int nst = TC * a.KT * 2 * ncomp;
-    const int nthnr = a.nth * a.nr;
-    int s_src[kSlotZF], s_dst[kSlotZF], s_meta[kSlotZF];
-#pragma unroll
-    for (int i = 0; i < kSlotZF; ++i) {
-        const int q = threadIdx.x + i * kThreads;
-        s_src[i] = -2;
-        s_dst[i] = 0;
-        s_meta[i] = 0;
-        if (q < nst) {
-            const int comp = q % ncomp, rest = q / ncomp;
-            const int tl = rest % TC, g = rest / TC;
-            const int kk = g >> 1, sg = g & 1;
-            const StageCols sc = stage_cols(comp, kk, sg, nv, ns, CX, CY);
-            s_src[i] = (sg == 0 || m > 0)
-                     ? comp + ncomp * (a.nth * kk + nthnr * (a.L + (sg ? -m : m)) + tl) : -1;
-            s_dst[i] = (sc.ca * TCP + tl) | ((sc.cb >= 0 ? sc.cb * TCP + tl : kNone) << 16);
-            s_meta[i] = tl | (kk << 8) | ((sc.cb >= 0 && comp % 3 == 2) ? 0x10000 : 0)
-                      | (sg == 1 ? 0x20000 : 0);
-        }
-    }
-
-    i4v A[RPW][2][kZD + 1];
-    auto load_a = [&](int ks) {
-#pragma unroll
-        for (int r = 0; r < RPW; ++r)
-#pragma unroll
-            for (int w = 0; w < 2; ++w)
-#pragma unroll
-                for (int s = 0; s < kZD + 1; ++s)
-                    A[r][w][s] = *reinterpret_cast<const i4v *>(ap[r] + (ks * 2 + w) * kZUnit
-                                                                 + s * 1024);
-    };
-    load_a(0);
-    int cc = 0;   // K steps done by this work-group: exponent buffer cc & 1
-    for (int kt = it.ktile; kt < it.ktile + it.nkt; ++kt) {
-        const int kt0 = kt * a.KT;
-        const int nkk = min(a.KT, a.nr - kt0);
-        const int koff = ncomp * a.nth * kt0;
-        double g1[kSlotZF], g2[kSlotZF];
-        auto fetch = [&](int t0) {
-            opaque(s_src);
-            opaque(s_meta);
-#pragma unroll
-            for (int i = 0; i < kSlotZF; ++i) {
-                g1[i] = 0.0;
-                g2[i] = 0.0;
-                if (s_src[i] >= 0 && ((s_meta[i] >> 8) & 7) < nkk) {
-                    const int t = t0 + (s_meta[i] & 0xFF);
-                    if (t < a.th_h) {
-                        const int o = s_src[i] + koff + ncomp * t0;
-                        g1[i] = a.in[o];
-                        g2[i] = a.in[o + ncomp * (a.nth - 1 - 2 * t)];
-                    }
-                }
-            }
-        };
-        d4v c1[RPW][NC1], c2[RPW][NC2];
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-#pragma unroll
-            for (int ct = 0; ct < NC1; ++ct) c1[r][ct] = d4v{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int ct = 0; ct < NC2; ++ct) c2[r][ct] = d4v{0.0, 0.0, 0.0, 0.0};
-        }
-        fetch(0);
-        for (int ks = 0; ks < nks; ++ks, ++cc) {
-            const int t0 = ks * TC;
-            int *ce = CE + (cc & 1) * 2 * NB;
-            int *cn = CE + ((cc + 1) & 1) * 2 * NB;
-            __syncthreads();
-            // (1) column exponents: fold 0 at column a (b), fold 1 at NB + a (b)
-            opaque(s_src);
-            opaque(s_dst);
-            opaque(s_meta);
-#pragma unroll
-            for (int i = 0; i < kSlotZF; ++i) {
-                if (s_src[i] == -2) continue;
-                const double2 wf = WF[t0 + (s_meta[i] & 0xFF)];
-                const double vs = g1[i] + g2[i], va = g1[i] - g2[i];
-                const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
-                const int ca = da / TCP;
-                atomicMax(ce + ca, z_exp(wf.x * vs));
-                atomicMax(ce + NB + ca, z_exp(wf.x * va));
-                if (db != kNone) {
-                    const int cb = db / TCP;
-                    atomicMax(ce + cb, z_exp(wf.y * vs));
-                    atomicMax(ce + NB + cb, z_exp(wf.y * va));
-                }
-            }
-            __syncthreads();
-            // (2) digits at the column scales; clear the next step's maxima
-            for (int x = threadIdx.x; x < 2 * NB; x += kThreads) cn[x] = kZExpMin;
-            opaque(s_src);
-            opaque(s_dst);
-            opaque(s_meta);
-#pragma unroll
-            for (int i = 0; i < kSlotZF; ++i) {
-                if (s_src[i] == -2) continue;
-                const int meta = s_meta[i];
-                const double2 wf = WF[t0 + (meta & 0xFF)];
-                const double vs = g1[i] + g2[i], va = g1[i] - g2[i];
-                const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
-                const int ca = da / TCP;
-                const bool nx = (meta & 0x10000) != 0;   // X = +v_theta | -v_phi
-                z_split(V + da, nullptr, wf.x * vs, ce[ca], nx, false, SL);
-                z_split(V + NB * TCP + da, nullptr, wf.x * va, ce[NB + ca], nx, false, SL);
-                if (db != kNone) {
-                    const int cb = db / TCP;
-                    const bool ny = (meta & 0x20000) != 0;   // Y = -+ (m/sin) v
-                    z_split(V + db, nullptr, wf.y * vs, ce[cb], ny, false, SL);
-                    z_split(V + NB * TCP + db, nullptr, wf.y * va, ce[NB + cb], ny, false, SL);
-                }
-            }
-            __syncthreads();
-            if (ks + 1 < nks) fetch(t0 + TC);
-            // (3) the MFMAs of the step
-            if (!anyrow) continue;
-#pragma unroll
-            for (int ct = 0; ct < NC1; ++ct) {
-                i4v b[kZD];
-#pragma unroll
-                for (int s = 0; s < kZD; ++s)
-                    b[s] = *reinterpret_cast<const i4v *>(V + s * SL + q1[ct] * TCP + kq);
-                const int eb = ce[q1[ct]];
-#pragma unroll
-                for (int r = 0; r < RPW; ++r) {
-                    i4v I[G + 1];
-                    z_mma<G>(I, A[r][0], b);
-                    z_flush<G>(c1[r][ct], I, A[r][0][kZD], eb);
-                }
-            }
-#pragma unroll
-            for (int ct = 0; ct < NC2; ++ct) {
-                i4v b[kZD];
-#pragma unroll
-                for (int s = 0; s < kZD; ++s)
-                    b[s] = *reinterpret_cast<const i4v *>(V + s * SL + qx[ct] * TCP + kq);
-                int eb = ce[qx[ct]];
-#pragma unroll
-                for (int r = 0; r < RPW; ++r) {
-                    i4v I[G + 1];
-                    z_mma<G>(I, A[r][1], b);
-                    z_flush<G>(c2[r][ct], I, A[r][1][kZD], eb);
-                }
-#pragma unroll
-                for (int s = 0; s < kZD; ++s)
-                    b[s] = *reinterpret_cast<const i4v *>(V + s * SL + qy[ct] * TCP + kq);
-                eb = ce[qy[ct]];
-#pragma unroll
-                for (int r = 0; r < RPW; ++r) {
-                    i4v I[G + 1];
-                    z_mma<G>(I, A[r][0], b);
-                    z_flush<G>(c2[r][ct], I, A[r][0][kZD], eb);
-                }
-            }
-            load_a(ks + 1 < nks ? ks + 1 : 0);
-        }
-
-        // epilogue as the third form (rows 4*(lane/16) + v of the i32 MFMA)
-        opaque(ocol);
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-            if (!rok[r]) continue;
-            const int rt = rtile[r];
-#pragma unroll
-            for (int ct = 0; ct < NC1 + NC2; ++ct) {
-                const int oc = ocol[ct];
-                if (oc < 0) continue;
-                const int comp = oc & 0xFF, kk = (oc >> 8) & 7, sg = (oc >> 11) & 1;
-                const int kind = (oc >> 12) & 3;
-                if (kk >= nkk) continue;
-                const int k = kt0 + kk;
-                const double rad = a.radius[k];
-                const int j0 = sg ? j0n : j0p;
-                const double m2 = (m == 0) ? 2.0 : 1.0;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int i = rt * 16 + 4 * (lane >> 4) + v;
-                    if (i >= Kp) continue;
-                    const int l = m + par + 2 * i;
-                    double fac;
-                    if (kind == 2) {
-                        fac = (l == 0) ? 0.5 : m2 * a.gfac[a.L + 1 + l];
-                    } else {
-                        const double g7 = (l == 0) ? 1.0 : m2 * a.gfac[l];
-                        fac = (kind == 0) ? rad * rad * g7 : rad * g7;
-                    }
-                    const double val = (ct < NC1 ? c1[r][ct][v] : c2[r][ct - NC1][v]) * fac;
-                    a.out[comp + (size_t)ncomp * ((j0 + (l - m)) + (size_t)a.J * k)] = val;
-                }
-            }
-        }
-    }
-}
-
-// Backward, int8 form: the third-form backward with one LDS chunk per
-// (K step of 64 degree indices, parity); a chunk feeds the P and dP tables of
-// its parity only.  A spectral value goes to an X column and, for the
-// theta/phi parts, to the partner Y column, which holds the same values up to
-// sign: both get the same exponent and the value is split once.
-template <int G, int RPW, int NC1, int NC2>
-__global__ __launch_bounds__(kThreads) void k_leg_bwdz(LegArgs a)
-{
-    constexpr int NB = 16 * (NC1 + 2 * NC2) + 1;
-    constexpr int CX = 16 * NC1, CY = 16 * (NC1 + NC2), CZ = NB - 1;
-    constexpr int KC = 64, KCP = KC + kZPad, SL = NB * KCP;
-    constexpr int kNone = 0xFFFF;
-
-    const WorkItem it = a.items[blockIdx.x];
-    if (it.m < 0) return;   // padding of the XCD interleave
-    const int m = it.m;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int K0 = (a.L - m + 2) / 2;
-    const int KS = (K0 + KC - 1) / KC;
-    const int nv = a.nvec, ns = a.nscl, ncomp = a.ncomp;
-    const int c1n = a.KT * 2 * (nv + ns), c2n = a.KT * 4 * nv;
-    const int j0p = a.lstack[a.L + m], j0n = a.lstack[a.L - m];
-
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    signed char *X = reinterpret_cast<signed char *>(smem_raw);   // [kZD][NB][KCP]
-    int *CE = reinterpret_cast<int *>(smem_raw + ((kZD * SL + 15) & ~15));   // [2 buf][NB]
-    __shared__ double rinv[8];
-    for (int x = threadIdx.x; x < kZD * KCP; x += kThreads)
-        X[(x / KCP) * SL + CZ * KCP + x % KCP] = 0;
-    for (int x = threadIdx.x; x < NB; x += kThreads) CE[x] = kZExpMin;
-
-    int rtile[RPW];
-    bool rok[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r) {
-        const int rt = (it.rgroup * 4 + wave) * RPW + r;
-        rok[r] = rt < a.RT;
-        rtile[r] = rok[r] ? rt : 0;
-    }
-    const signed char *tab = reinterpret_cast<const signed char *>(a.tab);
-    const signed char *ap[RPW];
-#pragma unroll
-    for (int r = 0; r < RPW; ++r)
-        ap[r] = tab + (a.off[m] + (long long)rtile[r] * KS * 4) * kZUnit + lane * 16;
-
-    const int kq = 16 * (lane >> 4);
-    int q1[NC1], qx[NC2], qy[NC2];
-#pragma unroll
-    for (int ct = 0; ct < NC1; ++ct) {
-        const int c = ct * 16 + (lane & 15);
-        q1[ct] = c < c1n ? c : CZ;
-    }
-#pragma unroll
-    for (int ct = 0; ct < NC2; ++ct) {
-        const int c = ct * 16 + (lane & 15);
-        qx[ct] = c < c2n ? CX + c : CZ;
-        qy[ct] = c < c2n ? CY + c : CZ;
-    }
-    int ocol[NC1 + NC2];
-#pragma unroll
-    for (int ct = 0; ct < NC1 + NC2; ++ct) {
-        const bool g1c = ct < NC1;
-        const int c = (g1c ? ct : ct - NC1) * 16 + (lane & 15);
-        int kk, sg, comp;
-        if (g1c) {
-            const int per = nv + ns, rem = c % (2 * per), q = rem % per;
-            kk = c / (2 * per);
-            sg = rem / per;
-            comp = q < nv ? 3 * q : 3 * nv + (q - nv);
-        } else {
-            const int rem = c % (4 * nv), r2 = rem % (2 * nv);
-            kk = c / (4 * nv);
-            sg = rem / (2 * nv);
-            comp = 3 * (r2 >> 1) + 1 + (r2 & 1);
-        }
-        const bool ok = (g1c ? c < c1n : c < c2n) && !(sg == 1 && m == 0);
-        ocol[ct] = ok ? comp | (kk << 8) | (sg << 11) : -1;
-    }
-
-    // staging slots: (component, degree index of the parity, radius/sign),
-    // component fastest; degree m + p + 2 (c0 + il) at chunk (c0, p)
-    const int nst = KC * ncomp * a.KT * 2;
-    int s_src[kSlotZB], s_dst[kSlotZB], s_meta[kSlotZB];
-#pragma unroll
-    for (int i = 0; i < kSlotZB; ++i) {
-        const int q = threadIdx.x + i * kThreads;
-        s_src[i] = -2;   // no slot
-        s_dst[i] = 0;
-        s_meta[i] = 0;
-        if (q < nst) {
-            const int comp = q % ncomp, rest = q / ncomp;
-            const int il = rest % KC, gg = rest / KC;
-            const int kk = gg >> 1, sg = gg & 1;
-            const StageCols sc = stage_cols(comp, kk, sg, nv, ns, CX, CY);
-            s_src[i] = (sg == 0 || m > 0) ? comp + ncomp * ((sg ? j0n : j0p) + 2 * il + a.J * kk) : -1;
-            s_dst[i] = (sc.ca * KCP + il) | ((sc.cb >= 0 ? sc.cb * KCP + il : kNone) << 16);
-            const int kind = comp < 3 * nv ? (comp % 3 == 0 ? 0 : 1) : 2;
-            // X = +dS | -T; Y = +- T | dS of the partner sign
-            s_meta[i] = il | (kk << 8) | (kind << 12) | ((sc.cb >= 0 && comp % 3 == 2) ? 1 << 14 : 0)
-                      | (sg == 0 ? 1 << 15 : 0);
-        }
-    }
-
-    // tables of step (ks, p): A[r][0] = P of parity p, A[r][1] = dP of parity p
-    i4v A[RPW][2][kZD + 1];
-    auto load_a = [&](int ks, int p) {
-#pragma unroll
-        for (int r = 0; r < RPW; ++r)
-#pragma unroll
-            for (int w = 0; w < 2; ++w)
-#pragma unroll
-                for (int s = 0; s < kZD + 1; ++s)
-                    A[r][w][s] = *reinterpret_cast<const i4v *>(
-                        ap[r] + (ks * 4 + 2 * w + p) * kZUnit + s * 1024);
-    };
-    auto prod = [&](d4v &c, const i4v (&Aw)[kZD + 1], int q, const int *ce) {
-        i4v b[kZD];
-#pragma unroll
-        for (int s = 0; s < kZD; ++s)
-            b[s] = *reinterpret_cast<const i4v *>(X + s * SL + q * KCP + kq);
-        i4v I[G + 1];
-        z_mma<G>(I, Aw, b);
-        z_flush<G>(c, I, Aw[kZD], ce[q]);
-    };
-
-    const size_t sl = (size_t)a.nth * a.nr;
-    load_a(0, 0);
-    int cc = 0;   // chunks done by this work-group: exponent buffer cc & 1
-    for (int kt = it.ktile; kt < it.ktile + it.nkt; ++kt) {
-        const int kt0 = kt * a.KT;
-        const int nkk = min(a.KT, a.nr - kt0);
-        const int koff = ncomp * a.J * kt0;
-        // the previous tile's last reads of rinv were before its final barriers
-        if (threadIdx.x < 8)
-            rinv[threadIdx.x] = ((int)threadIdx.x < nkk) ? 1.0 / a.radius[kt0 + threadIdx.x] : 0.0;
-        d4v a1[RPW][NC1][2], ax[RPW][NC2][2], ay[RPW][NC2][2];
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-#pragma unroll
-            for (int ct = 0; ct < NC1; ++ct) a1[r][ct][0] = a1[r][ct][1] = d4v{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int ct = 0; ct < NC2; ++ct) {
-                ax[r][ct][0] = ax[r][ct][1] = d4v{0.0, 0.0, 0.0, 0.0};
-                ay[r][ct][0] = ay[r][ct][1] = d4v{0.0, 0.0, 0.0, 0.0};
-            }
-        }
-        for (int st = 0; st < 2 * KS; ++st, ++cc) {
-            const int ks = st >> 1, p = st & 1, c0 = ks * KC;
-            int *ce = CE + (cc & 1) * NB;
-            int *cn = CE + ((cc + 1) & 1) * NB;
-            __syncthreads();
-            // (1) values (scaled as the third form) and column exponents
-            double v[kSlotZB];
-            opaque(s_src);
-            opaque(s_dst);
-            opaque(s_meta);
-#pragma unroll
-            for (int i = 0; i < kSlotZB; ++i) {
-                v[i] = 0.0;
-                if (s_src[i] == -2) continue;
-                const int meta = s_meta[i];
-                const int li = 2 * (meta & 0xFF) + p;   // degree offset in the chunk
-                if (s_src[i] >= 0 && ((meta >> 8) & 7) < nkk && li + 2 * c0 <= a.L - m) {
-                    double g = a.in[s_src[i] + koff + ncomp * (2 * c0 + p)];
-                    const int kind = (meta >> 12) & 3;
-                    if (kind != 2) {
-                        // S_pol l(l+1)/r^2 (g_sph_rlm(j,3) = 1/2 at l = 0), dS/r, T/r
-                        const double ri = rinv[(meta >> 8) & 7];
-                        if (kind == 0) {
-                            const int l = m + 2 * c0 + li;
-                            g = g * (ri * ri) * ((l == 0) ? 0.5 : (double)(l * (l + 1)));
-                        } else {
-                            g = g * ri;
-                        }
-                    }
-                    v[i] = g;
-                }
-                const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
-                const int e = z_exp(v[i]);
-                atomicMax(ce + da / KCP, e);
-                if (db != kNone) atomicMax(ce + db / KCP, e);
-            }
-            __syncthreads();
-            // (2) digits; clear the next chunk's maxima
-            for (int x = threadIdx.x; x < NB; x += kThreads) cn[x] = kZExpMin;
-            opaque(s_src);
-            opaque(s_dst);
-            opaque(s_meta);
-#pragma unroll
-            for (int i = 0; i < kSlotZB; ++i) {
-                if (s_src[i] == -2) continue;
-                const int meta = s_meta[i];
-                const int da = s_dst[i] & 0xFFFF, db = (int)((unsigned)s_dst[i] >> 16);
-                z_split(X + da, db != kNone ? X + db : nullptr, v[i], ce[da / KCP],
-                        (meta & (1 << 14)) != 0, (meta & (1 << 15)) != 0, SL);
-            }
-            __syncthreads();
-            // (3) the MFMAs of the chunk: symmetric part S from P even and dP
-            // odd, antisymmetric A from P odd and dP even
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-#pragma unroll
-                for (int ct = 0; ct < NC1; ++ct) prod(a1[r][ct][p], A[r][0], q1[ct], ce);
-#pragma unroll
-                for (int ct = 0; ct < NC2; ++ct) {
-                    prod(ax[r][ct][1 - p], A[r][1], qx[ct], ce);
-                    prod(ay[r][ct][p], A[r][0], qy[ct], ce);
-                }
-            }
-            const int nx = st + 1 < 2 * KS ? st + 1 : 0;
-            load_a(nx >> 1, nx & 1);
-        }
-
-        // epilogue as the third form
-        opaque(ocol);
-#pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-            if (!rok[r]) continue;
-            const int rt = rtile[r];
-#pragma unroll
-            for (int ct = 0; ct < NC1 + NC2; ++ct) {
-                const int oc = ocol[ct];
-                if (oc < 0) continue;
-                const int comp = oc & 0xFF, kk = (oc >> 8) & 7, sg = (oc >> 11) & 1;
-                if (kk >= nkk) continue;
-                const int mp = a.L + (sg ? -m : m), k = kt0 + kk;
-                double *o = a.out + comp + (size_t)ncomp * ((size_t)a.nth * k + sl * mp);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int t = rt * 16 + 4 * (lane >> 4) + v;
-                    if (t >= a.th_h) continue;
-                    double S, An;
-                    if (ct < NC1) {
-                        S = a1[r][ct][0][v];
-                        An = a1[r][ct][1][v];
-                    } else {
-                        const double f = (double)m * a.rsin[t];
-                        S = ax[r][ct - NC1][0][v] + f * ay[r][ct - NC1][0][v];
-                        An = ax[r][ct - NC1][1][v] + f * ay[r][ct - NC1][1][v];
-                    }
-                    const int tm = a.nth - 1 - t;
-                    o[(size_t)ncomp * t] = S + An;
-                    if (tm != t) o[(size_t)ncomp * tm] = S - An;
-                }
-            }
-        }
-    }
-}
-
 // ------------------------------------------------------------- host ----
 
 // Gauss-Legendre points on [-1,1] by Newton iteration on [0,1] with the
@@ -2367,44 +1658,6 @@ void gauss_points_host(int n, double *point, double *weight, double *colat)
 
 template <class T>
 static size_t elem_size() { return sizeof(typename T::elem); }
-
-// int8 form: third-form unit order (K steps of 64, no padding), units of
-// six digit slices and one exponent slice; no first-form tables
-static int build_tables_z(LegContext *c, const double *rP, const double *rdP, std::string *err)
-{
-    const int L = c->g.L, th_h = c->g.th_h;
-    c->kstep = 64;
-    c->rt_bwd = (th_h + 15) / 16;
-    c->ks_fwd = c->ks_fwd3 = (th_h + 63) / 64;
-    std::vector<long long> ob3(L + 2), of3(2 * (L + 1) + 1);
-    ob3[0] = 0;
-    for (int m = 0; m <= L; ++m) {
-        const int KS = ((L - m + 2) / 2 + 63) / 64;
-        ob3[m + 1] = ob3[m] + (long long)c->rt_bwd * KS * 4;
-    }
-    of3[0] = 0;
-    for (int m = 0; m <= L; ++m) {
-        const int K = L - m + 1, K0 = (K + 1) / 2, K1 = K / 2;
-        of3[2 * m + 1] = of3[2 * m] + (long long)((K0 + 15) / 16) * c->ks_fwd3 * 2;
-        of3[2 * m + 2] = of3[2 * m + 1] + (long long)((K1 + 15) / 16) * c->ks_fwd3 * 2;
-    }
-    const size_t bb3 = (size_t)ob3[L + 1] * kZUnit, bf3 = (size_t)of3[2 * (L + 1)] * kZUnit;
-    HIP_TRY(hipMalloc(&c->d_off_bwd3, sizeof(long long) * ob3.size()));
-    HIP_TRY(hipMalloc(&c->d_off_fwd3, sizeof(long long) * of3.size()));
-    HIP_TRY(hipMemcpy(c->d_off_bwd3, ob3.data(), sizeof(long long) * ob3.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_off_fwd3, of3.data(), sizeof(long long) * of3.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&c->d_tab_bwd3, bb3));
-    HIP_TRY(hipMalloc(&c->d_tab_fwd3, bf3));
-    c->tab_bytes = bb3 + bf3;
-    hipLaunchKernelGGL(k_pack_bwdz, dim3(64, L + 1), dim3(256), 0, c->stream, L, th_h, c->rt_bwd,
-                       rP, rdP, c->d_off_bwd3, reinterpret_cast<signed char *>(c->d_tab_bwd3));
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_pack_fwdz, dim3(64, 2 * (L + 1)), dim3(256), 0, c->stream, L, th_h,
-                       c->ks_fwd3, rP, rdP, c->d_off_fwd3,
-                       reinterpret_cast<signed char *>(c->d_tab_fwd3));
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
 
 template <class T>
 static int build_tables(LegContext *c, const double *dP_raw_P, const double *dP_raw_dP,
@@ -2496,13 +1749,6 @@ static size_t plan_lds(const LegPlan &p, bool fwd, int prec, int rows3 = 0)
     const int ns = (prec == kBF16x3) ? 3 : 1;
     const size_t es = (prec == kF64) ? 8 : (prec == kF32 ? 4 : 2);
     const int pad = (prec == kBF16x3) ? 8 : (prec == kF64 ? 2 : 4);   // lds_pad<T>()
-    if (prec == kInt8x6) {
-        // digit slices [kZD][2 folds (fwd) | 1][NB][tc+pad], exponent slots
-        // [2 buf][2 | 1][NB], row factors (fwd)
-        const int nb = 16 * (p.mct1 + 2 * p.mct2) + 1, nf = fwd ? 2 : 1;
-        return (((size_t)kZD * nf * nb * (p.tc + kZPad) + 15) & ~(size_t)15) + (size_t)8 * nf * nb
-               + (fwd ? (size_t)16 * rows3 : 0);
-    }
     if (p.ver == 3) {
         const int nb = 16 * (p.mct1 + 2 * p.mct2) + 1;
         const int pad3 = (prec == kF64) ? 1 : pad;   // lds_pad3<T>()
@@ -2584,45 +1830,6 @@ static void plan_candidates3(int ncomp, int nv, int ns, bool fwd, int prec,
     }
 }
 
-// int8-form candidates: the third-form instances, one K step (64) per LDS
-// chunk, the A tables of one step in registers; up to 160 KB of LDS
-static void plan_candidates_z(int ncomp, int nv, int ns, bool fwd, std::vector<LegPlan> *out)
-{
-    const int per1 = 2 * (nv + ns), per2 = 4 * nv;
-    int kt_prev = 0;
-    for (int i = 0; i < 4; ++i) {
-        int kt = 0;
-        for (int k = 8; k >= 1; --k)
-            if ((per1 * k + 15) / 16 <= kInst3[i][0] && (per2 * k + 15) / 16 <= kInst3[i][1]) {
-                kt = k;
-                break;
-            }
-        if (!kt || kt <= kt_prev) continue;
-        // staging slots bound the radial tile
-        while (kt >= 1 && (fwd ? (long long)64 * kt * 2 * ncomp > (long long)kSlotZF * kThreads
-                               : (long long)64 * ncomp * kt * 2 > (long long)kSlotZB * kThreads))
-            --kt;
-        if (!kt || kt <= kt_prev) continue;
-        kt_prev = kt;
-        for (int rpw : fwd ? std::vector<int>{1, 2} : std::vector<int>{1}) {
-            LegPlan p;
-            p.ncomp = ncomp;
-            p.nvec = nv;
-            p.nscl = ns;
-            p.fwd = fwd;
-            p.KT = kt;
-            p.mct1 = kInst3[i][0];
-            p.mct2 = kInst3[i][1];
-            p.rpw = rpw;
-            p.pd = 1;
-            p.tc = 64;
-            p.ver = 3;
-            if (plan_lds(p, fwd, kInt8x6, 256) > 160 * 1024) continue;
-            out->push_back(p);
-        }
-    }
-}
-
 static std::vector<LegPlan> plan_candidates(int ncomp, int nv, int ns, bool fwd, int prec)
 {
     const int per1 = 2 * (nv + ns), per2 = 4 * nv;
@@ -2700,6 +1907,15 @@ static void xcd_interleave(int L, std::vector<WorkItem> *items)
     for (int x = 0; x < NX; ++x)
         for (size_t i = 0; i < per[x].size(); ++i) out[i * NX + x] = per[x][i];
     items->swap(out);
+}
+
+// device memory a plan owns: its work items and its workspace
+static void plan_release(LegPlan *p)
+{
+    (void)hipFree(p->d_items);
+    (void)hipFree(p->d_work);
+    p->d_items = nullptr;
+    p->d_work = nullptr;
 }
 
 static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
@@ -2835,7 +2051,7 @@ const LegPlan *leg_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd,
         }
     }
     for (size_t i = 0; i < cand.size(); ++i)
-        if ((int)i != best) (void)hipFree(cand[i].d_items);
+        if ((int)i != best) plan_release(&cand[i]);
     c->plans.push_back(cand[best]);
     return &c->plans.back();
 }
@@ -2862,7 +2078,7 @@ int leg_plan_count(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, std::
     }
     std::vector<LegPlan> v;
     if (valid_candidates(c, ncomp, nvec, nscl, fwd, &v, err)) return -1;
-    for (LegPlan &p : v) (void)hipFree(p.d_items);
+    for (LegPlan &p : v) plan_release(&p);
     return (int)v.size();
 }
 
@@ -2876,15 +2092,15 @@ int leg_force_plan(LegContext *c, int ncomp, int nvec, int nscl, bool fwd, int i
     std::vector<LegPlan> v;
     if (valid_candidates(c, ncomp, nvec, nscl, fwd, &v, err)) return -1;
     if (idx < 0 || idx >= (int)v.size()) {
-        for (LegPlan &p : v) (void)hipFree(p.d_items);
+        for (LegPlan &p : v) plan_release(&p);
         *err = "plan index out of range";
         return -1;
     }
     for (int i = 0; i < (int)v.size(); ++i)
-        if (i != idx) (void)hipFree(v[i].d_items);
+        if (i != idx) plan_release(&v[i]);
     for (LegPlan &p : c->plans)
         if (p.ncomp == ncomp && p.nvec == nvec && p.nscl == nscl && p.fwd == fwd) {
-            (void)hipFree(p.d_items);
+            plan_release(&p);
             p = v[idx];
             return 0;
         }
@@ -3004,7 +2220,7 @@ void leg_context_free(LegContext *c)
     (void)hipFree(c->d_rsin);
     (void)hipFree(c->d_gfac);
     oz_free(c);
-    for (LegPlan &p : c->plans) (void)hipFree(p.d_items);
+    for (LegPlan &p : c->plans) plan_release(&p);
     c->plans.clear();
     (void)hipFree(c->d_rlm);
     (void)hipFree(c->d_rtm);
@@ -3180,51 +2396,6 @@ static int launch_p(LegContext *c, const LegPlan &p, bool fwd, const double *in,
     if (p.rpw == 1) return launch_t<T, 1, 2, 4, 2>(c, p, fwd, in, out, s, err);
     if (p.rpw == 2) return launch_t<T, 2, 2, 4, 2>(c, p, fwd, in, out, s, err);
     return launch_t<T, 3, 2, 4, 2>(c, p, fwd, in, out, s, err);
-}
-
-template <bool FWD, int RPW, int NC1, int NC2>
-static int launch_z1(LegContext *c, const LegPlan &p, const double *in, double *out,
-                     hipStream_t s, std::string *err)
-{
-    LegArgs a = make_args(c, p, FWD, in, out);
-    const size_t lds = plan_lds(p, FWD, c->prec, c->ks_fwd3 * 64);
-    if (lds > 160 * 1024) { *err = "LDS tile too large for this grid"; return -1; }
-    const void *fn;
-    if constexpr (FWD) fn = reinterpret_cast<const void *>(&k_leg_fwdz<TI8::GF, RPW, NC1, NC2>);
-    else fn = reinterpret_cast<const void *>(&k_leg_bwdz<TI8::GB, RPW, NC1, NC2>);
-    if (lds > 64 * 1024)
-        HIP_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if constexpr (FWD)
-        hipLaunchKernelGGL((k_leg_fwdz<TI8::GF, RPW, NC1, NC2>), dim3(p.n_items), dim3(kThreads),
-                           lds, s, a);
-    else
-        hipLaunchKernelGGL((k_leg_bwdz<TI8::GB, RPW, NC1, NC2>), dim3(p.n_items), dim3(kThreads),
-                           lds, s, a);
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
-
-template <int NC1, int NC2>
-static int launch_zn(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
-                     hipStream_t s, std::string *err)
-{
-    if (fwd && p.rpw == 1) return launch_z1<true, 1, NC1, NC2>(c, p, in, out, s, err);
-    if (fwd && p.rpw == 2) return launch_z1<true, 2, NC1, NC2>(c, p, in, out, s, err);
-    if (!fwd && p.rpw == 1) return launch_z1<false, 1, NC1, NC2>(c, p, in, out, s, err);
-    *err = "no int8-form kernel instance for this plan";
-    return -1;
-}
-
-static int launch_z(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,
-                    hipStream_t s, std::string *err)
-{
-    if (p.ver != 3 || p.KT > 8 || p.KT < 1 || p.tc != 64) { *err = "invalid int8-form plan"; return -1; }
-    if (p.mct1 == 1 && p.mct2 == 1) return launch_zn<1, 1>(c, p, fwd, in, out, s, err);
-    if (p.mct1 == 1 && p.mct2 == 2) return launch_zn<1, 2>(c, p, fwd, in, out, s, err);
-    if (p.mct1 == 2 && p.mct2 == 3) return launch_zn<2, 3>(c, p, fwd, in, out, s, err);
-    if (p.mct1 == 2 && p.mct2 == 4) return launch_zn<2, 4>(c, p, fwd, in, out, s, err);
-    *err = "no int8-form kernel instance for this column tiling";
-    return -1;
 }
 
 static int launch_plan(LegContext *c, const LegPlan &p, bool fwd, const double *in, double *out,

@@ -25,10 +25,12 @@ extern "C" {
 #define CALYPSO_LEG_FP32    2  /* FP32 MFMA 16x16x4                          */
 #define CALYPSO_LEG_BF16X3  3  /* split-BF16x3 MFMA 16x16x32, FP32 accumulate */
 #define CALYPSO_LEG_INT8X6  4  /* each operand split into seven balanced base-128
-                                  int8 digits (Ozaki splitting), the digit pairs
-                                  i+j <= 6 on MFMA i32 16x16x64, exact int32
-                                  sums, FP64 accumulation: FP64-level (~1e-12)
-                                  results                                     */
+                                  int8 digits at one power-of-two scale per table
+                                  row and per data column (Ozaki splitting), the
+                                  digit pairs i+j <= 6 on MFMA i32 16x16x64,
+                                  exact int32 sums over the whole contraction,
+                                  one FP64 conversion per tile: ~1e-13 of the
+                                  FP64 results                                */
 
 typedef struct calypso_leg calypso_leg_t;
 
