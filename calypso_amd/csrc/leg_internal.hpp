@@ -46,8 +46,6 @@ struct LegPlan {
     float tuned_ms = -1.f;  // autotuner timing of the chosen plan
     int n_items = 0;
     WorkItem *d_items = nullptr;
-    void *d_work = nullptr; // kernel form 6: split columns and their offsets
-    long long work_units = 0, work_b0 = 0;   // form 6: split units, first of the GEMM items
 };
 
 struct OzTables;   // int8 split-once form (leg_ozaki.hip)

@@ -1909,13 +1909,11 @@ static void xcd_interleave(int L, std::vector<WorkItem> *items)
     items->swap(out);
 }
 
-// device memory a plan owns: its work items and its workspace
+// device memory a plan owns: its work items
 static void plan_release(LegPlan *p)
 {
     (void)hipFree(p->d_items);
-    (void)hipFree(p->d_work);
     p->d_items = nullptr;
-    p->d_work = nullptr;
 }
 
 static int build_items(LegContext *c, LegPlan *p, bool fwd, std::string *err)
