@@ -91,7 +91,7 @@ def build_fortran():
     so = os.path.join(LIB, "libcalypso_amd_f.so")
     _run([fc, "-shared", "-o", so, *[os.path.join(obj, s.replace(".f90", ".o")) for s in srcs],
           "-L", LIB, "-lcalypso_amd", "-Wl,-rpath,$ORIGIN"])
-    for prog in ("leg_driver", "mhd_driver", "fft_lu_driver"):
+    for prog in ("leg_driver", "mhd_driver", "fft_lu_driver", "scalar_shell_driver"):
         _run([fc, "-O2", os.path.join(FSRC, prog + ".f90"), "-I", LIB,
               "-o", os.path.join(LIB, prog), "-L", LIB, "-lcalypso_amd_f", "-lcalypso_amd",
               "-Wl,-rpath,$ORIGIN"])

@@ -24,6 +24,15 @@
         real(c_double) :: high_temp, low_temp
       end type calypso_mhd_params
 !
+!>    calypso_scalar_bc of include/calypso_amd.h (same field order)
+      integer(c_int), parameter :: CALYPSO_BC_FIXED_VALUE = 0
+      integer(c_int), parameter :: CALYPSO_BC_FIXED_FLUX =  1
+      type, bind(C) :: calypso_scalar_bc
+        real(c_double) :: coef_d, coef_imp
+        integer(c_int) :: bc_icb, bc_cmb
+        type(c_ptr) :: name
+      end type calypso_scalar_bc
+!
       interface
 !
         function calypso_last_error() bind(C, name="calypso_last_error")
@@ -202,6 +211,73 @@
           type(c_ptr), value :: mhd
           real(c_double), intent(inout) :: time
         end function calypso_mhd_time
+!
+        integer(c_int) function calypso_scalar_shell_create(ltr, nr,     &
+     &      radius, dt, nscalar, scalars, val_icb, val_cmb, stream, sh)   &
+     &      bind(C, name="calypso_scalar_shell_create")
+          import :: c_int, c_double, c_ptr, calypso_scalar_bc
+          integer(c_int), value :: ltr, nr, nscalar
+          real(c_double), intent(in) :: radius(*), val_icb(*), val_cmb(*)
+          real(c_double), value :: dt
+          type(calypso_scalar_bc), intent(in) :: scalars(*)
+          type(c_ptr), value :: stream
+          type(c_ptr), intent(out) :: sh
+        end function calypso_scalar_shell_create
+!
+        integer(c_int) function calypso_scalar_shell_destroy(sh)         &
+     &      bind(C, name="calypso_scalar_shell_destroy")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: sh
+        end function calypso_scalar_shell_destroy
+!
+        integer(c_int) function calypso_scalar_shell_set_noise(sh)       &
+     &      bind(C, name="calypso_scalar_shell_set_noise")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: sh
+        end function calypso_scalar_shell_set_noise
+!
+        integer(c_int) function calypso_scalar_shell_set_field(sh, iq,   &
+     &      host) bind(C, name="calypso_scalar_shell_set_field")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: sh
+          integer(c_int), value :: iq
+          real(c_double), intent(in) :: host(*)
+        end function calypso_scalar_shell_set_field
+!
+        integer(c_int) function calypso_scalar_shell_get_field(sh, iq,   &
+     &      host) bind(C, name="calypso_scalar_shell_get_field")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: sh
+          integer(c_int), value :: iq
+          real(c_double), intent(inout) :: host(*)
+        end function calypso_scalar_shell_get_field
+!
+        integer(c_int) function calypso_scalar_shell_initialize(sh)      &
+     &      bind(C, name="calypso_scalar_shell_initialize")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: sh
+        end function calypso_scalar_shell_initialize
+!
+        integer(c_int) function calypso_scalar_shell_step(sh, nsteps)   &
+     &      bind(C, name="calypso_scalar_shell_step")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: sh
+          integer(c_int), value :: nsteps
+        end function calypso_scalar_shell_step
+!
+        integer(c_int) function calypso_scalar_shell_mean_square(sh,    &
+     &      ms) bind(C, name="calypso_scalar_shell_mean_square")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: sh
+          real(c_double), intent(inout) :: ms(*)
+        end function calypso_scalar_shell_mean_square
+!
+        integer(c_int) function calypso_scalar_shell_write_volume_ms(sh, &
+     &      path) bind(C, name="calypso_scalar_shell_write_volume_ms")
+          import :: c_int, c_char, c_ptr
+          type(c_ptr), value :: sh
+          character(kind=c_char), intent(in) :: path(*)
+        end function calypso_scalar_shell_write_volume_ms
 !
         integer(c_int) function calypso_mhd_write_volume_ms(mhd, path)  &
      &      bind(C, name="calypso_mhd_write_volume_ms")

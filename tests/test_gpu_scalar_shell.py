@@ -140,3 +140,34 @@ def test_rejects_bad_arguments(torch_cuda):
     with pytest.raises(_lib.CalypsoError, match="out of range"):
         sim.set_field(2, np.zeros((radius.size, 25)))
     sim.close()
+
+
+def test_fortran_driver_heterogeneous_temp(torch_cuda, tmp_path):
+    """The Fortran face (bind(C) interfaces of calypso_amd_capi) runs the
+    same 100 steps and writes the same energy file as the reference's test."""
+    import subprocess
+    exe = os.path.join(_lib.LIB_DIR, "scalar_shell_driver")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built (no amdflang)")
+    print("[scalar_shell] Fortran driver, 100 steps", flush=True)
+    radius, dt, scalars = product.heterogeneous_temp_params(BC_FILE)
+    setup = tmp_path / "setup.txt"
+    with open(setup, "w") as f:
+        f.write(f"4 {radius.size - 1} {dt!r} {len(scalars)}\n")
+        for s in scalars:
+            f.write(f"{s['coef_d']!r} {s['coef_imp']!r} {s['icb'][0]} {s['cmb'][0]}\n")
+            f.write(" ".join(repr(float(v)) for v in s["icb"][1]) + "\n")
+            f.write(" ".join(repr(float(v)) for v in s["cmb"][1]) + "\n")
+    efile = tmp_path / "sph_pwr_volume_s.dat"
+    out = subprocess.run([exe, str(setup), str(efile), "100", "50"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    mine = efile.read_text().splitlines()
+    theirs = open(os.path.join(GOLD, "sph_pwr_volume_s.dat")).read().splitlines()
+    assert len(mine) == len(theirs) == 10
+    assert [l.rstrip() for l in mine[:8]] == [l.rstrip() for l in theirs[:8]]
+    for a, b in zip(mine[8:], theirs[8:]):
+        ta, tb = a.split(), b.split()
+        assert ta[0] == tb[0]
+        np.testing.assert_allclose([float(x) for x in ta[1:]], [float(x) for x in tb[1:]],
+                                   rtol=1e-12, atol=0)
