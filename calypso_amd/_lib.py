@@ -77,6 +77,8 @@ SIGNATURES = [
     ("calypso_mhd_write_restart", _I, [_P, ctypes.c_char_p, _I, ctypes.c_char_p, _I]),
     ("calypso_mhd_read_restart", _I, [_P, ctypes.c_char_p]),
     ("calypso_mhd_coriolis_table", _I, [_I, _P, _P]),
+    ("calypso_sph_write_field_file", _I, [ctypes.c_char_p, _I, _I, _D, _D, ctypes.c_longlong, _I,
+                                          _P, _P, _P]),
     ("calypso_scalar_shell_create", _I, [_I, _I, _P, _D, _I, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     ("calypso_scalar_shell_destroy", _I, [_P]),
     ("calypso_scalar_shell_set_noise", _I, [_P]),

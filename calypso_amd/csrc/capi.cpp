@@ -800,6 +800,27 @@ int calypso_lubksb_5band_mul(int msmp, const int *msmp_stack, int mcomp, int n,
     return host_lubksb(5, msmp, msmp_stack, mcomp, n, band_lu, i_pivot, x);
 }
 
+int calypso_sph_write_field_file(const char *path, int rank, int istep, double time,
+                                 double dt, long long nnod, int nfld, const int *ncomp,
+                                 const char *const *names, const double *data)
+{
+    if (!path || (nfld > 0 && (!ncomp || !names || !data)) || nnod < 0 || nfld < 0)
+        return fail("calypso_sph_write_field_file: null argument or negative size");
+    std::vector<const double *> comp;
+    long long ntot = 0;
+    for (int i = 0; i < nfld; ++i) {
+        if (ncomp[i] < 1 || ncomp[i] > 9 || !names[i])
+            return fail("calypso_sph_write_field_file: 1..9 components and a name per field");
+        ntot += ncomp[i];
+    }
+    for (long long c = 0; c < ntot; ++c) comp.push_back(data + c * nnod);
+    std::string err;
+    if (calypso::sph_write_field_file(path, rank, istep, time, dt, nnod, nfld, ncomp, names,
+                                      comp.data(), &err))
+        return fail(err);
+    return 0;
+}
+
 // ---- scalars diffusing in a shell at rest -----------------------------------
 int calypso_scalar_shell_create(int l_truncation, int nr, const double *radius, double dt,
                                 int nscalar, const calypso_scalar_bc *scalars,

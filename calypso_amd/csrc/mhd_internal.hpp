@@ -127,6 +127,10 @@ int  mhd_read_restart(MhdContext *c, const std::string &path, std::string *err);
 
 // single-domain grid tables and the reference's grid files (sph_grids.cpp)
 void sph_rj_mode_order(int L, int *j_global);
+// host: the reference's ASCII field file of one domain (restart / snapshot)
+int sph_write_field_file(const std::string &path, int rank, long long istep, double time,
+                         double dt, long long nnod, int nfld, const int *ncomp,
+                         const char *const *names, const double *const *comp, std::string *err);
 int  sph_write_grid_files(const std::string &prefix, int L, int nth, int nphi, int nr,
                           const double *radius, std::string *err);
 

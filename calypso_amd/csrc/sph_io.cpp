@@ -111,29 +111,53 @@ int mhd_write_restart(MhdContext *c, const std::string &path, std::string *err)
         if (c->rj_order[j] == 0) j00 = j;
     if (c->initialized)
         for (int k = 0; k < c->nr; ++k) comp[3][j00 + (size_t)c->J * k] += ref_temp(c, k);
+    const char *names[6];
+    int ncomp[6];
+    const double *data[16];
+    for (int i = 0; i < 6; ++i) {
+        names[i] = kRst[i].name;
+        ncomp[i] = kRst[i].ncomp;
+    }
+    for (int i = 0; i < 16; ++i) data[i] = comp[i].data();
+    return sph_write_field_file(path, 0, c->i_step, mhd_time(c), c->co.dt, (long long)nnod, 6,
+                                ncomp, names, data, err);
+}
+
+// The ASCII field file of one domain: write_step_data (m_time_data_IO.f90:
+// 107-119) then write_field_data (field_data_IO.f90:267-298); the component
+// counts wrap after ten per record, as the '(10i5)' format does.
+int sph_write_field_file(const std::string &path, int rank, long long istep, double time,
+                         double dt, long long nnod, int nfld, const int *ncomp,
+                         const char *const *names, const double *const *comp, std::string *err)
+{
     std::ofstream f(path, std::ios::trunc);
     if (!f) {
         *err = "cannot open " + path;
         return -1;
     }
-    f << "!  domain ID\n" << fort_i(0, 16) << "\n";
-    f << "!  time step number\n" << fort_i(c->i_step, 16) << "\n";
-    f << "!  time, Delta t\n" << fort_e(mhd_time(c), 25, 15) << fort_e(c->co.dt, 25, 15) << "\n";
+    f << "!  domain ID\n" << fort_i(rank, 16) << "\n";
+    f << "!  time step number\n" << fort_i(istep, 16) << "\n";
+    f << "!  time, Delta t\n" << fort_e(time, 25, 15) << fort_e(dt, 25, 15) << "\n";
     f << "! Number of field and component\n";
-    f << fort_i((long long)nnod, 16) << fort_i(6, 16) << "\n";
-    for (const RstField &r : kRst) f << fort_i(r.ncomp, 5);
-    f << "\n";
-    n = 0;
+    f << fort_i(nnod, 16) << fort_i(nfld, 16) << "\n";
+    for (int i = 0; i < nfld; ++i) {
+        f << fort_i(ncomp[i], 5);
+        if (i % 10 == 9 || i == nfld - 1) f << "\n";
+    }
+    if (nfld == 0) f << "\n";
+    int n = 0;
     std::string line;
-    for (const RstField &r : kRst) {
-        f << r.name << "\n";
-        for (size_t i = 0; i < nnod; ++i) {
+    for (int i = 0; i < nfld; ++i) {
+        std::string name(names[i]);
+        const size_t e = name.find_last_not_of(' ');
+        f << (e == std::string::npos ? std::string() : name.substr(0, e + 1)) << "\n";
+        for (long long k = 0; k < nnod; ++k) {
             line.clear();
-            for (int q = 0; q < r.ncomp; ++q) line += fort_e(comp[n + q][i], 25, 15);
+            for (int q = 0; q < ncomp[i]; ++q) line += fort_e(comp[n + q][k], 25, 15);
             line += '\n';
             f << line;
         }
-        n += r.ncomp;
+        n += ncomp[i];
     }
     if (!f) {
         *err = "write error on " + path;

@@ -280,6 +280,14 @@ int calypso_seeded_spectrum(unsigned long long seed, int field, int l_truncation
  * L < nth, as the generator's zonal FFT table requires. */
 int calypso_sph_write_grid_files(const char *prefix, int l_truncation, int nth, int nphi,
                                  int nr, const double *radius);
+/* Host-only: the reference's ASCII field file of one domain (the restart and
+ * spectral-field format): write_step_data (m_time_data_IO.f90:107-119) with
+ * (rank, istep, time, dt), then write_field_data (field_data_IO.f90:267-298)
+ * of nfld fields with ncomp[i] components named names[i], data(nnod, ntot)
+ * column-major as the reference's fld_IO%d_IO (component c at data + c nnod). */
+int calypso_sph_write_field_file(const char *path, int rank, int istep, double time,
+                                 double dt, long long nnod, int nfld, const int *ncomp,
+                                 const char *const *names, const double *data);
 /* The local mode order of the reference's one-domain rj (and rlm) arrays:
  * j_global[j] = l(l+1)+m of local mode j, orders m = -L..L, degrees |m|..L
  * inside (idx_gl_1d_rj_j(:,1) after gen_sph_grids); J = (L+1)^2 entries. */

@@ -12,6 +12,8 @@
 !!   legendre_f_trans_vector_org / _scalar_org    (legendre_fwd_trans_org.f90:45-160)
 !!   ludcmp_3band, ludcmp_band, lubksb_*band_mul  (direct_solver/)
 !!   rfftmi, rfftmf, rfftmb                       (fftpack5.1d.f90)
+!!   write_step_data, write_field_data            (m_time_data_IO.f90,
+!!                                                 field_data_IO.f90)
       module calypso_ref_harness
 !
       use iso_c_binding
@@ -353,5 +355,50 @@
       end subroutine ref_lubksb_mul
 !
 ! -----------------------------------------------------------------------
+!
+! -----------------------------------------------------------------------
+!
+!> The reference's ASCII field file of one domain, as write_step_field_file
+!! (field_file_IO.f90:40-62) writes it: write_step_data, then
+!! write_field_data.  names: nfld blank-padded names of 255 characters.
+      subroutine ref_write_field_file(path, rank, istep, time, dt, nnod, &
+     &          nfld, ncomp, names, fdata)                                &
+     &          bind(C, name="ref_write_field_file")
+      use m_precision
+      use m_time_data_IO
+      use field_data_IO
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: rank, istep, nnod, nfld
+      real(c_double), value :: time, dt
+      integer(c_int), intent(in) :: ncomp(nfld)
+      character(kind=c_char), intent(in) :: names(kchara, nfld)
+      real(c_double), intent(in) :: fdata(nnod, *)
+!
+      character(len=kchara) :: fname
+      character(len=kchara) :: fnames(nfld)
+      integer(kind = kint) :: ncf(nfld)
+      integer(kind = kint) :: i, j, ntot
+!
+      fname = ' '
+      do i = 1, kchara
+        if(path(i) .eq. c_null_char) exit
+        fname(i:i) = path(i)
+      end do
+      do j = 1, nfld
+        do i = 1, kchara
+          fnames(j)(i:i) = names(i,j)
+        end do
+        ncf(j) = ncomp(j)
+      end do
+      ntot = sum(ncf)
+      i_time_step_IO = istep
+      time_IO = time
+      delta_t_IO = dt
+      open(91, file = fname, form = 'formatted')
+      call write_step_data(91, rank)
+      call write_field_data(91, nnod, nfld, ntot, ncf, fnames,          &
+     &    fdata(1:nnod,1:ntot))
+      close(91)
+      end subroutine ref_write_field_file
 !
       end module calypso_ref_harness
