@@ -2,13 +2,14 @@
 (SPH_analyze_MHD, programs/SPH_MHD/SPH_analyzer_MHD.f90:115-202) for a
 rotating Boussinesq shell with rigid walls, insulating magnetic boundaries,
 fixed-temperature boundaries and the spherical-shell reference temperature --
-the model of the reference's Dynamobench_case1 test.
+the model of the reference's Dynamobench_case1 test -- and the same shell with
+free-slip velocity walls at either boundary (bc_velocity free_slip_sph).
 
 It is the checker for the GPU step, never the product.  Each routine cites the
 reference routine it restates; numpy arrays are indexed [k, j] with k the
 radial point (0-based) and j the rj mode, j = l(l+1) + m.
 
-Composition, free-slip / rotating-inner-core walls, the centre point, and the
+Composition, rotating-inner-core walls, the centre point, and the
 diagnostic-only fields (pressure, grad T, viscous diffusion of u) are not part
 of this model and are not restated.
 """
@@ -116,6 +117,120 @@ class BoundaryFdm:
                            [m[c][1] for c in range(3)]]
         self.r_icb = (r[i0], 1.0 / r[i0], (1.0 / r[i0]) ** 2)
         self.r_cmb = (r[o], 1.0 / r[o], (1.0 / r[o]) ** 2)
+        # free-slip walls (cal_fdm2_ICB_free_vp / _vt, m_coef_fdm_free_ICB.f90:
+        # 85-173; cal_fdm2_CMB_free_vp / _vt, m_coef_fdm_free_CMB.f90): the
+        # second Taylor row is the wall condition, f - r f' + r^2/2 f'' = 0 for
+        # the poloidal scalar (with f = 0) and 2 f - r f' = 0 for the toroidal
+        # one.  Stored as [pt + 1][c] like fix_dr_*: pt -1 / 0 / +1 the points
+        # below / at / above the wall, c = 0 value, 1 d/dr, 2 d2/dr2.
+        r0 = r[i0]
+        mvp = inverse33([[1.0, 0.0, 0.0], [1.0, -r0, 0.5 * r0 * r0],
+                         [1.0, dp1, 0.5 * dp1 * dp1]])
+        mvt = inverse33([[1.0, 0.0, 0.0], [2.0, -r0, 0.0], [1.0, dp1, 0.5 * dp1 * dp1]])
+        self.free_vp_icb = [[0.0, 0.0, 0.0], [1.0, mvp[1][0], mvp[2][0]],
+                            [0.0, mvp[1][2], mvp[2][2]]]
+        self.free_vt_icb = [[0.0, 0.0, 0.0], [1.0, mvt[1][0], mvt[2][0]],
+                            [0.0, 0.0, mvt[2][2]]]
+        r0 = r[o]
+        mvp = inverse33([[1.0, 0.0, 0.0], [1.0, -r0, 0.5 * r0 * r0],
+                         [1.0, -dn1, 0.5 * dn1 * dn1]])
+        mvt = inverse33([[1.0, 0.0, 0.0], [2.0, -r0, 0.0], [1.0, -dn1, 0.5 * dn1 * dn1]])
+        self.free_vp_cmb = [[0.0, mvp[1][2], mvp[2][2]], [1.0, mvp[1][0], mvp[2][0]],
+                            [0.0, 0.0, 0.0]]
+        self.free_vt_cmb = [[0.0, 0.0, mvt[2][2]], [1.0, mvt[1][0], mvt[2][0]],
+                            [1.0, 1.0, 1.0]]
+
+
+# ----------------------------------------------- free-slip wall formulas --
+# Each takes the wall coefficients of BoundaryFdm ([pt + 1][c]) and arrays
+# indexed [component, k, j] (matrices [j, k, band]) and updates the wall row
+# in place.
+
+def add_fix_flux_icb_rows(m, ki, r_icb, fdm, g3, coef):
+    """add_fix_flux_icb_poisson_mat (set_sph_scalar_mat_bc.f90:78-101)."""
+    m[:, ki, 1] = m[:, ki, 1] - coef * (fdm[1][2] - g3 * r_icb[2])
+    m[:, ki + 1, 0] = m[:, ki + 1, 0] - coef * fdm[2][2]
+
+
+def add_fix_flux_cmb_rows(m, ko, r_cmb, fdm, g3, coef):
+    """add_fix_flux_cmb_poisson_mat (set_sph_scalar_mat_bc.f90:152-175)."""
+    m[:, ko - 1, 2] = m[:, ko - 1, 2] - coef * fdm[0][2]
+    m[:, ko, 1] = m[:, ko, 1] - coef * (fdm[1][2] - g3 * r_cmb[2])
+
+
+def icb_free_v_and_w(fvp, fvt, u, w, ki):
+    """cal_sph_nod_icb_free_v_and_w (set_sph_exp_free_ICB.f90:60-90)."""
+    d1s = fvp[2][1] * u[0, ki + 1]
+    d2s = fvp[2][2] * u[0, ki + 1]
+    d1t = fvt[1][1] * u[2, ki]
+    u[0, ki] = 0.0
+    u[1, ki] = d1s
+    w[0, ki] = u[2, ki]
+    w[1, ki] = d1t
+    w[2, ki] = -d2s
+
+
+def cmb_free_v_and_w(fvp, fvt, u, w, ko):
+    """cal_sph_nod_cmb_free_v_and_w (set_sph_exp_free_CMB.f90:58-90)."""
+    d1s = fvp[0][1] * u[0, ko - 1]
+    d2s = fvp[0][2] * u[0, ko - 1]
+    d1t = fvt[0][1] * u[2, ko - 1] + fvt[1][1] * u[2, ko]
+    u[0, ko] = 0.0
+    u[1, ko] = d1s
+    w[0, ko] = u[2, ko]
+    w[1, ko] = d1t
+    w[2, ko] = -d2s
+
+
+def icb_free_vpol2(fvp, u, ki):
+    """cal_sph_nod_icb_free_vpol2 (set_sph_exp_free_ICB.f90:94-119)."""
+    d1s = fvp[1][1] * u[0, ki] + fvp[2][1] * u[0, ki + 1]
+    u[0, ki] = 0.0
+    u[1, ki] = d1s
+
+
+def cmb_free_vpol2(fvp, u, ko):
+    """cal_sph_nod_cmb_free_vpol2 (set_sph_exp_free_CMB.f90:94-118)."""
+    d1s = fvp[0][1] * u[0, ko - 1] + fvp[1][1] * u[0, ko]
+    u[0, ko] = 0.0
+    u[1, ko] = d1s
+
+
+def icb_free_rot2(fvp, fvt, r_icb, g3, src, rot, ki):
+    """cal_sph_nod_icb_free_rot2 (set_sph_exp_free_ICB.f90:123-154)."""
+    d2s = fvp[1][2] * src[0, ki] + fvp[2][2] * src[0, ki + 1]
+    d1t = fvt[1][1] * src[2, ki]
+    rot[0, ki] = src[2, ki]
+    rot[1, ki] = d1t
+    rot[2, ki] = -d2s + g3 * r_icb[2] * src[0, ki]
+
+
+def cmb_free_rot2(fvp, fvt, r_cmb, g3, src, rot, ko):
+    """cal_sph_nod_cmb_free_rot2 (set_sph_exp_free_CMB.f90:122-154)."""
+    d2s = fvp[0][2] * src[0, ko - 1] + fvp[1][2] * src[0, ko]
+    d1t = fvt[0][1] * src[2, ko - 1] + fvt[1][1] * src[2, ko]
+    rot[0, ko] = src[2, ko]
+    rot[1, ko] = d1t
+    rot[2, ko] = -d2s + g3 * r_cmb[2] * src[0, ko]
+
+
+def icb_free_w_diffuse2(ffl, fvt, r_icb, g3, coef, w, out, ki):
+    """cal_sph_nod_icb_free_w_diffuse2 (set_sph_exp_free_ICB.f90:194-228):
+    component 0 (the vorticity's poloidal part, the toroidal velocity's
+    image) through the free-slip formula, component 2 through the three-point
+    fixed-value one."""
+    d2s = fvt[1][2] * w[0, ki] + fvt[2][2] * w[0, ki + 1]
+    d2t = ffl[0][2] * w[2, ki] + ffl[1][2] * w[2, ki + 1] + ffl[2][2] * w[2, ki + 2]
+    out[0, ki] = coef * (d2s - g3 * r_icb[2] * w[0, ki])
+    out[2, ki] = coef * (d2t - g3 * r_icb[2] * w[2, ki])
+
+
+def cmb_free_w_diffuse2(ffl, fvt, r_cmb, g3, coef, w, out, ko):
+    """cal_sph_nod_cmb_free_w_diffuse2 (set_sph_exp_free_CMB.f90:196-230)."""
+    d2s = fvt[0][2] * w[0, ko - 1] + fvt[1][2] * w[0, ko]
+    d2t = ffl[2][2] * w[2, ko - 2] + ffl[1][2] * w[2, ko - 1] + ffl[0][2] * w[2, ko]
+    out[0, ko] = coef * (d2s - g3 * r_cmb[2] * w[0, ko])
+    out[2, ko] = coef * (d2t - g3 * r_cmb[2] * w[2, ko])
 
 
 # ------------------------------------------------------------ Gaunt ints --
@@ -323,14 +438,19 @@ class DynamoBenchmarkCase1:
     depth_high_t, depth_low_t = 0.5384615384615384, 1.5384615384615384
     high_temp, low_temp = 1.0, 0.0
     isig = 400
+    # velocity walls (bc_velocity): False non_slip_sph, True free_slip_sph
+    free_icb, free_cmb = False, False
 
 
-def shell_case(L=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6, Ra_mod=1.0e2):
+def shell_case(L=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6, Ra_mod=1.0e2,
+               free_icb=False, free_cmb=False):
     """The Dynamobench case 1 physics (rotating Boussinesq shell, rigid walls,
     insulating magnetic boundaries, fixed temperature) on another grid, time
-    step, Crank-Nicolson weight or modified Rayleigh number."""
+    step, Crank-Nicolson weight or modified Rayleigh number, or with free-slip
+    velocity walls."""
     return type("ShellCase", (DynamoBenchmarkCase1,),
-                dict(L=L, nth=nth, nphi=nphi, nri=nri, dt=dt, coef_imp=coef_imp, Ra_mod=Ra_mod))
+                dict(L=L, nth=nth, nphi=nphi, nri=nri, dt=dt, coef_imp=coef_imp, Ra_mod=Ra_mod,
+                     free_icb=free_icb, free_cmb=free_cmb))
 
 
 class MhdStep:
@@ -352,6 +472,7 @@ class MhdStep:
         self.ar2 = self.ar1 ** 2
         self.d1, self.d2 = fdm_coefs(self.radius)
         self.bc = BoundaryFdm(self.radius, self.kr_in, self.kr_out)
+        self.free_icb, self.free_cmb = bool(c.free_icb), bool(c.free_cmb)
         # rj mode degrees and normalisations (copy_sph_normalization_2_rj: no
         # l = 0 override in rj)
         self.l_of_j = np.array([int(math.isqrt(j)) for j in range(J)])
@@ -431,13 +552,13 @@ class MhdStep:
                                                   - g3 * ar2[k])
                 m[:, k + 1, 0] = m[:, k + 1, 0] - coef * (d2[k, 2] + 2.0 * ar1[k] * d1[k, 2])
 
-        def add_fix_flux_icb(m, coef):
-            m[:, ki, 1] = m[:, ki, 1] - coef * (bc.fix_dr_icb[1][2] - g3 * bc.r_icb[2])
-            m[:, ki + 1, 0] = m[:, ki + 1, 0] - coef * bc.fix_dr_icb[2][2]
+        def add_fix_flux_icb(m, coef, fdm=None):
+            fdm = bc.fix_dr_icb if fdm is None else fdm
+            add_fix_flux_icb_rows(m, ki, bc.r_icb, fdm, g3, coef)
 
-        def add_fix_flux_cmb(m, coef):
-            m[:, ko - 1, 2] = m[:, ko - 1, 2] - coef * bc.fix_dr_cmb[0][2]
-            m[:, ko, 1] = m[:, ko, 1] - coef * (bc.fix_dr_cmb[1][2] - g3 * bc.r_cmb[2])
+        def add_fix_flux_cmb(m, coef, fdm=None):
+            fdm = bc.fix_dr_cmb if fdm is None else fdm
+            add_fix_flux_cmb_rows(m, ko, bc.r_cmb, fdm, g3, coef)
 
         def set_fix_fld_icb(m):
             m[:, ki, 1] = 1.0
@@ -454,12 +575,23 @@ class MhdStep:
         add_vector_poisson(vt, coef_dvt)
         add_vector_poisson(wt, coef_dvt)
         add_vector_poisson(vs, 1.0)
+        # walls (const_radial_mat_vort_2step, const_r_mat_4_vector_sph.f90:
+        # 85-135): free slip puts the free-slip formulas in the toroidal and
+        # poloidal rows, rigid walls fix the toroidal value
         add_fix_flux_icb(wt, coef_dvt)
-        set_fix_fld_icb(vt)
-        add_fix_flux_icb(vs, 1.0)
+        if self.free_icb:
+            add_fix_flux_icb(vt, coef_dvt, bc.free_vt_icb)
+            add_fix_flux_icb(vs, 1.0, bc.free_vp_icb)
+        else:
+            set_fix_fld_icb(vt)
+            add_fix_flux_icb(vs, 1.0)
         add_fix_flux_cmb(wt, coef_dvt)
-        set_fix_fld_cmb(vt)
-        add_fix_flux_cmb(vs, 1.0)
+        if self.free_cmb:
+            add_fix_flux_cmb(vt, coef_dvt, bc.free_vt_cmb)
+            add_fix_flux_cmb(vs, 1.0, bc.free_vp_cmb)
+        else:
+            set_fix_fld_cmb(vt)
+            add_fix_flux_cmb(vs, 1.0)
         vp = self._mat_product_3band(wt, vs)
         # temperature
         coef = self.coef_imp_t * self.coef_d_temp * self.dt
@@ -531,31 +663,38 @@ class MhdStep:
         return d[k, 0] * f[k - 1] + d[k, 1] * f[k] + d[k, 2] * f[k + 1]
 
     def grad_vp_and_vorticity(self):
-        """const_grad_vp_and_vorticity for rigid walls: cal_sph_nod_icb_rigid_velo2,
+        """const_grad_vp_and_vorticity: at rigid walls cal_sph_nod_icb_rigid_velo2,
         cal_sph_nod_icb_rigid_rot2, cal_sph_nod_cmb_rigid_v_and_w
-        (set_sph_exp_rigid_ICB.f90 / _CMB.f90), cal_sph_diff_pol_and_rot2
+        (set_sph_exp_rigid_ICB.f90 / _CMB.f90), at free-slip walls
+        cal_sph_nod_*_free_v_and_w, inside cal_sph_diff_pol_and_rot2
         (cal_sph_exp_rotation.f90)."""
         u, w = self.f["velo"], self.f["vort"]
         bc, g3 = self.bc, self.g3
         ki, ko = self.kr_in - 1, self.kr_out - 1
-        u[0, :ki + 1] = 0.0
-        u[1, :ki + 1] = 0.0
-        u[2, :ki + 1] = 0.0          # vt_ICB_bc = 0
-        fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
-        d2s = fdr[1][2] * u[0, ki] + fdr[2][2] * u[0, ki + 1]
-        d1t = ffl[0][1] * u[2, ki] + ffl[1][1] * u[2, ki + 1] + ffl[2][1] * u[2, ki + 2]
-        w[0, ki] = u[2, ki]
-        w[1, ki] = d1t
-        w[2, ki] = -(d2s - g3 * bc.r_icb[2] * u[0, ki])
-        u[0, ko] = 0.0
-        u[1, ko] = 0.0
-        u[2, ko] = 0.0
-        fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
-        d2s = fdr[0][2] * u[0, ko - 1]
-        d1t = ffl[2][1] * u[2, ko - 2] + ffl[1][1] * u[2, ko - 1] + ffl[0][1] * u[2, ko]
-        w[0, ko] = u[2, ko]
-        w[1, ko] = d1t
-        w[2, ko] = -d2s
+        if self.free_icb:
+            icb_free_v_and_w(bc.free_vp_icb, bc.free_vt_icb, u, w, ki)
+        else:
+            u[0, :ki + 1] = 0.0
+            u[1, :ki + 1] = 0.0
+            u[2, :ki + 1] = 0.0          # vt_ICB_bc = 0
+            fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
+            d2s = fdr[1][2] * u[0, ki] + fdr[2][2] * u[0, ki + 1]
+            d1t = ffl[0][1] * u[2, ki] + ffl[1][1] * u[2, ki + 1] + ffl[2][1] * u[2, ki + 2]
+            w[0, ki] = u[2, ki]
+            w[1, ki] = d1t
+            w[2, ki] = -(d2s - g3 * bc.r_icb[2] * u[0, ki])
+        if self.free_cmb:
+            cmb_free_v_and_w(bc.free_vp_cmb, bc.free_vt_cmb, u, w, ko)
+        else:
+            u[0, ko] = 0.0
+            u[1, ko] = 0.0
+            u[2, ko] = 0.0
+            fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
+            d2s = fdr[0][2] * u[0, ko - 1]
+            d1t = ffl[2][1] * u[2, ko - 2] + ffl[1][1] * u[2, ko - 1] + ffl[0][1] * u[2, ko]
+            w[0, ko] = u[2, ko]
+            w[1, ko] = d1t
+            w[2, ko] = -d2s
         self._diff_pol_and_rot(u, w)
 
     def _diff_pol_and_rot(self, f, rot):
@@ -612,13 +751,19 @@ class MhdStep:
             out[2, k] = coef * (d2t - g3 * self.ar2[k] * w[2, k])
         ki, ko = self.kr_in - 1, self.kr_out - 1
         ffl = bc.fix_fld_icb
-        for c in (0, 2):
-            d2 = ffl[0][2] * w[c, ki] + ffl[1][2] * w[c, ki + 1] + ffl[2][2] * w[c, ki + 2]
-            out[c, ki] = coef * (d2 - g3 * bc.r_icb[2] * w[c, ki])
+        if self.free_icb:
+            icb_free_w_diffuse2(ffl, bc.free_vt_icb, bc.r_icb, g3, coef, w, out, ki)
+        else:
+            for c in (0, 2):
+                d2 = ffl[0][2] * w[c, ki] + ffl[1][2] * w[c, ki + 1] + ffl[2][2] * w[c, ki + 2]
+                out[c, ki] = coef * (d2 - g3 * bc.r_icb[2] * w[c, ki])
         ffl = bc.fix_fld_cmb
-        for c in (0, 2):
-            d2 = ffl[2][2] * w[c, ko - 2] + ffl[1][2] * w[c, ko - 1] + ffl[0][2] * w[c, ko]
-            out[c, ko] = coef * (d2 - g3 * bc.r_cmb[2] * w[c, ko])
+        if self.free_cmb:
+            cmb_free_w_diffuse2(ffl, bc.free_vt_cmb, bc.r_cmb, g3, coef, w, out, ko)
+        else:
+            for c in (0, 2):
+                d2 = ffl[2][2] * w[c, ko - 2] + ffl[1][2] * w[c, ko - 1] + ffl[0][2] * w[c, ko]
+                out[c, ko] = coef * (d2 - g3 * bc.r_cmb[2] * w[c, ko])
 
     def magnetic_diffusion(self):
         """const_sph_mag_diffuse_by_j (const_sph_rotation.f90):
@@ -683,18 +828,27 @@ class MhdStep:
         bc, g1, g3 = self.bc, self.g1, self.g3
         ki, ko = self.kr_in - 1, self.kr_out - 1
         if rigid:
-            fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
-            d2s = fdr[1][2] * src[0, ki] + fdr[2][2] * src[0, ki + 1]
-            d1t = ffl[0][1] * src[2, ki] + ffl[1][1] * src[2, ki + 1] + ffl[2][1] * src[2, ki + 2]
-            rot[0, ki] = src[2, ki]
-            rot[1, ki] = d1t
-            rot[2, ki] = -(d2s - g3 * bc.r_icb[2] * src[0, ki])
-            fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
-            d2s = fdr[0][2] * src[0, ko - 1] + fdr[1][2] * src[0, ko]
-            d1t = ffl[2][1] * src[2, ko - 2] + ffl[1][1] * src[2, ko - 1] + ffl[0][1] * src[2, ko]
-            rot[0, ko] = src[2, ko]
-            rot[1, ko] = d1t
-            rot[2, ko] = -(d2s - g3 * bc.r_cmb[2] * src[0, ko])
+            # sel_bc_sph_vorticity (select_exp_velocity_bc.f90:158-190): the
+            # velocity walls' formulas, cal_sph_nod_icb_free_rot2 /
+            # cal_sph_nod_cmb_free_rot2 at a free-slip wall
+            if self.free_icb:
+                icb_free_rot2(bc.free_vp_icb, bc.free_vt_icb, bc.r_icb, g3, src, rot, ki)
+            else:
+                fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
+                d2s = fdr[1][2] * src[0, ki] + fdr[2][2] * src[0, ki + 1]
+                d1t = ffl[0][1] * src[2, ki] + ffl[1][1] * src[2, ki + 1] + ffl[2][1] * src[2, ki + 2]
+                rot[0, ki] = src[2, ki]
+                rot[1, ki] = d1t
+                rot[2, ki] = -(d2s - g3 * bc.r_icb[2] * src[0, ki])
+            if self.free_cmb:
+                cmb_free_rot2(bc.free_vp_cmb, bc.free_vt_cmb, bc.r_cmb, g3, src, rot, ko)
+            else:
+                fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
+                d2s = fdr[0][2] * src[0, ko - 1] + fdr[1][2] * src[0, ko]
+                d1t = ffl[2][1] * src[2, ko - 2] + ffl[1][1] * src[2, ko - 1] + ffl[0][1] * src[2, ko]
+                rot[0, ko] = src[2, ko]
+                rot[1, ko] = d1t
+                rot[2, ko] = -(d2s - g3 * bc.r_cmb[2] * src[0, ko])
         else:
             rot[0, ki] = src[2, ki]
             rot[1, ki] = (g1 + 1.0) * bc.r_icb[1] * src[2, ki]
@@ -919,12 +1073,19 @@ class MhdStep:
         u[2] = w[0].copy()
         u[0][:, 0] = 0.0
         u[2][:, 0] = 0.0
-        u[0, :ki + 1] = 0.0
-        u[1, :ki + 1] = 0.0
-        u[2, :ki + 1] = 0.0
-        u[0, ko] = 0.0
-        u[1, ko] = 0.0
-        u[2, ko] = 0.0
+        bc = self.bc
+        if self.free_icb:
+            icb_free_vpol2(bc.free_vp_icb, u, ki)
+        else:
+            u[0, :ki + 1] = 0.0
+            u[1, :ki + 1] = 0.0
+            u[2, :ki + 1] = 0.0
+        if self.free_cmb:
+            cmb_free_vpol2(bc.free_vp_cmb, u, ko)
+        else:
+            u[0, ko] = 0.0
+            u[1, ko] = 0.0
+            u[2, ko] = 0.0
         u[0] = self._solve("vp", u[0])
         u[2] = self._solve("vt", u[2])
         self.grad_vp_and_vorticity()

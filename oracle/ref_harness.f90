@@ -14,6 +14,11 @@
 !!   rfftmi, rfftmf, rfftmb                       (fftpack5.1d.f90)
 !!   write_step_data, write_field_data            (m_time_data_IO.f90,
 !!                                                 field_data_IO.f90)
+!!   cal_fdm2_{ICB,CMB}_free_vp / _vt             (m_coef_fdm_free_ICB.f90,
+!!                                                 m_coef_fdm_free_CMB.f90)
+!!   cal_sph_nod_{icb,cmb}_free_*                 (set_sph_exp_free_ICB.f90,
+!!                                                 set_sph_exp_free_CMB.f90)
+!!   add_fix_flux_{icb,cmb}_poisson_mat           (set_sph_scalar_mat_bc.f90)
       module calypso_ref_harness
 !
       use iso_c_binding
@@ -400,5 +405,144 @@
      &    fdata(1:nnod,1:ntot))
       close(91)
       end subroutine ref_write_field_file
+!
+! -----------------------------------------------------------------------
+!> The free-slip wall tables of the four reference routines, computed as
+!! set_bc_sph_mhd.f90:52-56 calls them (on radius(kr_in) and
+!! radius(kr_out-1)): coef(c, pt, t), t = 1 vp ICB, 2 vt ICB, 3 vp CMB,
+!! 4 vt CMB, pt -1..1 the points below / at / above the wall.
+      subroutine ref_free_slip_coefs(nri, kr_in, kr_out, radius, coef)   &
+     &          bind(C, name="ref_free_slip_coefs")
+      use m_coef_fdm_free_ICB
+      use m_coef_fdm_free_CMB
+      integer(c_int), value :: nri, kr_in, kr_out
+      real(c_double), intent(in) :: radius(nri)
+      real(c_double), intent(inout) :: coef(3,-1:1,4)
+      integer :: c, pt
+!
+      call cal_fdm2_ICB_free_vp(radius(kr_in))
+      call cal_fdm2_ICB_free_vt(radius(kr_in))
+      call cal_fdm2_CMB_free_vp(radius(kr_out-1))
+      call cal_fdm2_CMB_free_vt(radius(kr_out-1))
+      do pt = -1, 1
+        do c = 1, 3
+          coef(c,pt,1) = fdm2_free_vp_ICB(pt,c)
+          coef(c,pt,2) = fdm2_free_vt_ICB(pt,c)
+          coef(c,pt,3) = fdm2_free_vp_CMB(pt,c)
+          coef(c,pt,4) = fdm2_free_vt_CMB(pt,c)
+        end do
+      end do
+      end subroutine ref_free_slip_coefs
+!
+! -----------------------------------------------------------------------
+!> One free-slip wall routine on d(nnod, 6), nnod = nri*jmax in rj order
+!! (j fastest), components 1-3 the field and 4-6 its rotation or
+!! diffusion; g3(jmax) = l(l+1); wall 0 the ICB (kr_in), 1 the CMB (kr_out).
+!!   which 0 cal_sph_nod_*_free_v_and_w, 1 *_free_vpol2, 2 *_free_rot2,
+!!         3 *_free_w_diffuse2 (coef_d; fix_fld(0:2,3,2) the three-point
+!!           fixed-value tables of the ICB and the CMB)
+      subroutine ref_free_slip_wall(which, wall, nri, jmax, kr_in,        &
+     &          kr_out, radius, g3, coef_d, fix_fld, d)                   &
+     &          bind(C, name="ref_free_slip_wall")
+      use m_schmidt_poly_on_rtm
+      use m_sph_spectr_data
+      use m_coef_fdm_free_ICB
+      use m_coef_fdm_free_CMB
+      use set_sph_exp_free_ICB
+      use set_sph_exp_free_CMB
+      integer(c_int), value :: which, wall, nri, jmax, kr_in, kr_out
+      real(c_double), value :: coef_d
+      real(c_double), intent(in) :: radius(nri), g3(jmax)
+      real(c_double), intent(in) :: fix_fld(0:2,3,2)
+      real(c_double), intent(inout) :: d(nri*jmax,6)
+      real(kind = 8) :: r_ICB(0:2), r_CMB(0:2)
+!
+      call cal_fdm2_ICB_free_vp(radius(kr_in))
+      call cal_fdm2_ICB_free_vt(radius(kr_in))
+      call cal_fdm2_CMB_free_vp(radius(kr_out-1))
+      call cal_fdm2_CMB_free_vt(radius(kr_out-1))
+      r_ICB(0) = radius(kr_in)
+      r_ICB(1) = 1.0d0 / radius(kr_in)
+      r_ICB(2) = r_ICB(1) * r_ICB(1)
+      r_CMB(0) = radius(kr_out)
+      r_CMB(1) = 1.0d0 / radius(kr_out)
+      r_CMB(2) = r_CMB(1) * r_CMB(1)
+      if(allocated(g_sph_rj)) deallocate(g_sph_rj)
+      allocate(g_sph_rj(jmax,13))
+      g_sph_rj = 0.0d0
+      g_sph_rj(1:jmax,3) = g3(1:jmax)
+      if(allocated(d_rj)) deallocate(d_rj)
+      allocate(d_rj(nri*jmax,6))
+      d_rj = d
+!
+      if(wall .eq. 0) then
+        select case(which)
+        case(0)
+          call cal_sph_nod_icb_free_v_and_w(jmax, kr_in,                &
+     &        fdm2_free_vp_ICB, fdm2_free_vt_ICB, 1, 4)
+        case(1)
+          call cal_sph_nod_icb_free_vpol2(jmax, kr_in,                  &
+     &        fdm2_free_vp_ICB, 1)
+        case(2)
+          call cal_sph_nod_icb_free_rot2(jmax, kr_in, r_ICB,            &
+     &        fdm2_free_vp_ICB, fdm2_free_vt_ICB, 1, 4)
+        case(3)
+          call cal_sph_nod_icb_free_w_diffuse2(jmax, kr_in, r_ICB,      &
+     &        fix_fld(0,1,1), fdm2_free_vt_ICB, coef_d, 1, 4)
+        end select
+      else
+        select case(which)
+        case(0)
+          call cal_sph_nod_cmb_free_v_and_w(jmax, kr_out,               &
+     &        fdm2_free_vp_CMB, fdm2_free_vt_CMB, 1, 4)
+        case(1)
+          call cal_sph_nod_cmb_free_vpol2(jmax, kr_out,                 &
+     &        fdm2_free_vp_CMB, 1)
+        case(2)
+          call cal_sph_nod_cmb_free_rot2(jmax, kr_out, r_CMB,           &
+     &        fdm2_free_vp_CMB, fdm2_free_vt_CMB, 1, 4)
+        case(3)
+          call cal_sph_nod_cmb_free_w_diffuse2(jmax, kr_out, r_CMB,     &
+     &        fix_fld(0,1,2), fdm2_free_vt_CMB, coef_d, 1, 4)
+        end select
+      end if
+      d = d_rj
+      deallocate(d_rj, g_sph_rj)
+      end subroutine ref_free_slip_wall
+!
+! -----------------------------------------------------------------------
+!> add_fix_flux_icb_poisson_mat / add_fix_flux_cmb_poisson_mat
+!! (set_sph_scalar_mat_bc.f90:78-175) on mat(3, nri, jmax): wall 0 the ICB
+!! row kr, 1 the CMB row kr; r3 = (r, 1/r, 1/r^2) of the wall; fdm(3,-1:1)
+!! the wall table as fdm(c, pt).
+      subroutine ref_add_fix_flux_rows(wall, nri, jmax, kr, r3, fdm, g3,  &
+     &          coef, mat) bind(C, name="ref_add_fix_flux_rows")
+      use m_schmidt_poly_on_rtm
+      use set_sph_scalar_mat_bc
+      integer(c_int), value :: wall, nri, jmax, kr
+      real(c_double), value :: coef
+      real(c_double), intent(in) :: r3(0:2), fdm(3,-1:1), g3(jmax)
+      real(c_double), intent(inout) :: mat(3,nri,jmax)
+      real(kind = 8) :: tab(-1:1,3)
+      integer :: c, pt
+!
+      do pt = -1, 1
+        do c = 1, 3
+          tab(pt,c) = fdm(c,pt)
+        end do
+      end do
+      if(allocated(g_sph_rj)) deallocate(g_sph_rj)
+      allocate(g_sph_rj(jmax,13))
+      g_sph_rj = 0.0d0
+      g_sph_rj(1:jmax,3) = g3(1:jmax)
+      if(wall .eq. 0) then
+        call add_fix_flux_icb_poisson_mat(nri, jmax, kr, r3, tab,       &
+     &      coef, mat)
+      else
+        call add_fix_flux_cmb_poisson_mat(nri, jmax, kr, r3, tab,       &
+     &      coef, mat)
+      end if
+      deallocate(g_sph_rj)
+      end subroutine ref_add_fix_flux_rows
 !
       end module calypso_ref_harness
