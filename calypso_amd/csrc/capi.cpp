@@ -519,9 +519,15 @@ int calypso_mhd_create(const calypso_mhd_params *pp, const double *radius, void 
     co.depth_low_t = pp->depth_low_t;
     co.high_temp = pp->high_temp;
     co.low_temp = pp->low_temp;
+    co.bc_velo_icb = pp->bc_velo_icb;
+    co.bc_velo_cmb = pp->bc_velo_cmb;
     if (co.dt <= 0.0) return fail("calypso_mhd_create: dt must be positive");
     if (pp->legendre_mode < CALYPSO_LEG_FP64 || pp->legendre_mode > CALYPSO_LEG_INT8X6)
         return fail("calypso_mhd_create: unknown Legendre mode");
+    for (int bcv : {pp->bc_velo_icb, pp->bc_velo_cmb})
+        if (bcv != CALYPSO_VELO_RIGID && bcv != CALYPSO_VELO_FREE_SLIP)
+            return fail("calypso_mhd_create: velocity boundary kind " + std::to_string(bcv)
+                        + " is not supported (0 rigid, 1 free slip)");
     calypso_mhd *h = new (std::nothrow) calypso_mhd;
     if (!h) return fail("calypso_mhd_create: out of host memory");
     std::string err;

@@ -1,8 +1,8 @@
 // mhd.hip -- the spectral MHD time step on the GPU (gfx950), FP64.
 //
 // One step follows SPH_analyze_MHD (programs/SPH_MHD/SPH_analyzer_MHD.f90:
-// 115-202) for a rotating Boussinesq shell with rigid walls, insulating
-// magnetic boundaries and fixed-temperature boundaries around the
+// 115-202) for a rotating Boussinesq shell with rigid or free-slip walls,
+// insulating magnetic boundaries and fixed-temperature boundaries around the
 // spherical-shell reference temperature:
 //
 //   k_explicit      cal_expricit_sph_euler / _adams
@@ -11,6 +11,7 @@
 //                   solves (cal_sol_sph_fluid_crank.f90:53-391)
 //   lubksb x 5      banded solves, poloidal velocity 5-band, the rest 3-band
 //   k_bc_rows       wall rows of grad / curl (set_sph_exp_rigid_*.f90,
+//                   set_sph_exp_free_*.f90,
 //   k_grad_interior cal_sph_exp_nod_*_ins.f90, cal_sph_exp_rotation.f90)
 //   k_diffusion     diffusion terms and buoyancy (const_sph_diffusion.f90,
 //                   const_sph_rotation.f90, cal_rot_buoyancies_sph_MHD.f90)
@@ -126,7 +127,8 @@ __global__ void k_explicit(MhdDev p, int euler, int first)
 
 // ------------------------------------------- Crank-Nicolson right sides --
 // cal_sol_velo_by_vort_sph_crank: u_pol <- w_tor, u_tor <- w_pol,
-// delete_zero_degree_comp, cal_sph_nod_icb/cmb_rigid_velo2;
+// delete_zero_degree_comp, cal_sph_nod_icb/cmb_rigid_velo2 or
+// cal_sph_nod_icb/cmb_free_vpol2;
 // cal_sol_scalar_sph_crank: set_fixed_scalar_sph at both walls;
 // cal_sol_magne_sph_crank: delete_zero_degree_comp,
 // cal_sph_nod_icb/cmb_ins_mag2.
@@ -136,12 +138,23 @@ __global__ void k_cn_rhs(MhdDev p)
          i += (size_t)gridDim.x * blockDim.x) {
         const NodeKJ kj = node_kj(p, i);
         const int k = kj.k, j = kj.j;
-        double up = FLD(kVort + 2)[i], ut = FLD(kVort)[i];
+        const double *wtor = FLD(kVort + 2);
+        double up = wtor[i], ut = FLD(kVort)[i];
         if (j == 0) {
             up = 0.0;
             ut = 0.0;
         }
-        if (k <= p.ki || k == p.ko) {
+        if (k == p.ki && p.free_i) {
+            // the poloidal wall value after delete_zero_degree_comp is the
+            // neighbour's w_tor (zero for l = 0)
+            const double up1 = j == 0 ? 0.0 : wtor[i + p.J];
+            FLD(kVelo + 1)[i] = p.fvp_i[1][1] * up + p.fvp_i[2][1] * up1;
+            up = 0.0;
+        } else if (k == p.ko && p.free_c) {
+            const double upn = j == 0 ? 0.0 : wtor[i - p.J];
+            FLD(kVelo + 1)[i] = p.fvp_c[0][1] * upn + p.fvp_c[1][1] * up;
+            up = 0.0;
+        } else if (k <= p.ki || k == p.ko) {
             up = 0.0;
             ut = 0.0;
             FLD(kVelo + 1)[i] = 0.0;
@@ -168,8 +181,9 @@ __global__ void k_cn_rhs(MhdDev p)
 
 // ------------------------------------------------ wall rows of grad/curl --
 // cal_sph_nod_icb_rigid_velo2 + cal_sph_nod_icb_rigid_rot2,
-// cal_sph_nod_cmb_rigid_v_and_w, cal_sph_nod_icb_ins_b_and_j,
-// cal_sph_nod_cmb_ins_b_and_j.  One thread per mode.
+// cal_sph_nod_cmb_rigid_v_and_w or cal_sph_nod_icb/cmb_free_v_and_w,
+// cal_sph_nod_icb_ins_b_and_j, cal_sph_nod_cmb_ins_b_and_j.  One thread per
+// mode.
 __global__ void k_bc_rows(MhdDev p)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,7 +192,17 @@ __global__ void k_bc_rows(MhdDev p)
     double *us = FLD(kVelo), *ud = FLD(kVelo + 1), *ut = FLD(kVelo + 2);
     double *ws = FLD(kVort), *wd = FLD(kVort + 1), *wt = FLD(kVort + 2);
     const size_t J = p.J;
-    {   // ICB velocity and vorticity
+    if (p.free_i) {   // ICB free slip
+        const size_t i0 = j + J * p.ki, i1 = i0 + J;
+        const double d1s = p.fvp_i[2][1] * us[i1];
+        const double d2s = p.fvp_i[2][2] * us[i1];
+        const double d1t = p.fvt_i[1][1] * ut[i0];
+        us[i0] = 0.0;
+        ud[i0] = d1s;
+        ws[i0] = ut[i0];
+        wd[i0] = d1t;
+        wt[i0] = -d2s;
+    } else {   // ICB rigid
         for (int k = 0; k <= p.ki; ++k) {
             const size_t i = j + J * k;
             us[i] = 0.0;
@@ -192,7 +216,17 @@ __global__ void k_bc_rows(MhdDev p)
         wd[i0] = d1t;
         wt[i0] = -(d2s - g3 * p.ri[2] * us[i0]);
     }
-    {   // CMB velocity and vorticity
+    if (p.free_c) {   // CMB free slip
+        const size_t i0 = j + J * p.ko, n1 = i0 - J;
+        const double d1s = p.fvp_c[0][1] * us[n1];
+        const double d2s = p.fvp_c[0][2] * us[n1];
+        const double d1t = p.fvt_c[0][1] * ut[n1] + p.fvt_c[1][1] * ut[i0];
+        us[i0] = 0.0;
+        ud[i0] = d1s;
+        ws[i0] = ut[i0];
+        wd[i0] = d1t;
+        wt[i0] = -d2s;
+    } else {   // CMB rigid
         const size_t i0 = j + J * p.ko, n1 = i0 - J, n2 = n1 - J;
         us[i0] = 0.0;
         ud[i0] = 0.0;
@@ -255,7 +289,7 @@ __global__ void k_grad_interior(MhdDev p)
 
 // ------------------------------------------------ diffusion, buoyancy ----
 // const_sph_vorticirty_diffusion (cal_sph_nod_vect_diffuse2 +
-// cal_sph_nod_icb/cmb_rgd_w_diffuse2), const_sph_mag_diffuse_by_j
+// cal_sph_nod_icb/cmb_rgd_w_diffuse2 or cal_sph_nod_icb/cmb_free_w_diffuse2), const_sph_mag_diffuse_by_j
 // (cal_sph_nod_diffuse_by_rot2 + cal_sph_nod_icb/cmb_ins_diffuse2),
 // const_sph_scalar_diffusion (cal_sph_nod_scalar_diffuse2 +
 // cal_sph_fix_scalar_in/out_diffuse2), cal_rot_buoyancy_sph_MHD.
@@ -286,7 +320,10 @@ __global__ void k_diffusion(MhdDev p)
             td[i] = p.coef_d_temp * (d2t + 2.0 * p.ar1[k] * d1s - g3a * t[i]);
         } else if (k == p.ki) {
             const size_t i1 = i + J, i2 = i1 + J;
-            const double dw0 = p.ffl_i[0][2] * w0[i] + p.ffl_i[1][2] * w0[i1] + p.ffl_i[2][2] * w0[i2];
+            // cal_sph_nod_icb_free_w_diffuse2 at a free-slip wall
+            const double dw0 = p.free_i
+                ? p.fvt_i[1][2] * w0[i] + p.fvt_i[2][2] * w0[i1]
+                : p.ffl_i[0][2] * w0[i] + p.ffl_i[1][2] * w0[i1] + p.ffl_i[2][2] * w0[i2];
             const double dw2 = p.ffl_i[0][2] * w2[i] + p.ffl_i[1][2] * w2[i1] + p.ffl_i[2][2] * w2[i2];
             wd0[i] = p.coef_d_velo * (dw0 - g3 * p.ri[2] * w0[i]);
             wd2[i] = p.coef_d_velo * (dw2 - g3 * p.ri[2] * w2[i]);
@@ -301,7 +338,9 @@ __global__ void k_diffusion(MhdDev p)
             td[i] = p.coef_d_temp * (d2tt + 2.0 * p.ri[1] * d1t - g3 * p.ri[2] * fix);
         } else if (k == p.ko) {
             const size_t n1 = i - J, n2 = n1 - J;
-            const double dw0 = p.ffl_c[2][2] * w0[n2] + p.ffl_c[1][2] * w0[n1] + p.ffl_c[0][2] * w0[i];
+            const double dw0 = p.free_c
+                ? p.fvt_c[0][2] * w0[n1] + p.fvt_c[1][2] * w0[i]
+                : p.ffl_c[2][2] * w0[n2] + p.ffl_c[1][2] * w0[n1] + p.ffl_c[0][2] * w0[i];
             const double dw2 = p.ffl_c[2][2] * w2[n2] + p.ffl_c[1][2] * w2[n1] + p.ffl_c[0][2] * w2[i];
             wd0[i] = p.coef_d_velo * (dw0 - g3 * p.rc[2] * w0[i]);
             wd2[i] = p.coef_d_velo * (dw2 - g3 * p.rc[2] * w2[i]);
@@ -490,15 +529,23 @@ __global__ void k_coriolis(MhdDev p)
     }
 }
 
-// Curl of a force with rigid-wall rows (const_sph_force_rot2) -> (pol, tor).
-__device__ __forceinline__ void rot_rigid(const MhdDev &p, int src, int k, int j, size_t i,
-                                          double *pol, double *tor)
+// Curl of a momentum force with the velocity walls' rows
+// (const_sph_force_rot2, sel_bc_sph_vorticity: cal_sph_nod_icb/cmb_rigid_rot2
+// or cal_sph_nod_icb/cmb_free_rot2) -> (pol, tor).
+__device__ __forceinline__ void rot_momentum(const MhdDev &p, int src, int k, int j, size_t i,
+                                             double *pol, double *tor)
 {
     const double *s0 = FLD(src), *s1 = FLD(src + 1), *s2 = FLD(src + 2);
     const double g3 = p.g3[j];
     const size_t J = p.J;
     *pol = s2[i];
-    if (k == p.ki) {
+    if (k == p.ki && p.free_i) {
+        const double d2s = p.fvp_i[1][2] * s0[i] + p.fvp_i[2][2] * s0[i + J];
+        *tor = -d2s + g3 * p.ri[2] * s0[i];
+    } else if (k == p.ko && p.free_c) {
+        const double d2s = p.fvp_c[0][2] * s0[i - J] + p.fvp_c[1][2] * s0[i];
+        *tor = -d2s + g3 * p.rc[2] * s0[i];
+    } else if (k == p.ki) {
         const double d2s = p.fdr_i[1][2] * s0[i] + p.fdr_i[2][2] * s0[i + J];
         *tor = -(d2s - g3 * p.ri[2] * s0[i]);
     } else if (k == p.ko) {
@@ -518,8 +565,8 @@ __global__ void k_rot_forces(MhdDev p, int temp_roundtrip)
         const size_t J = p.J;
         const double g1 = p.g1[j], g3 = p.g3[j];
         double ri_p, ri_t, rl_p, rl_t;
-        rot_rigid(p, kMAdvect, k, j, i, &ri_p, &ri_t);
-        rot_rigid(p, kLorentz, k, j, i, &rl_p, &rl_t);
+        rot_momentum(p, kMAdvect, k, j, i, &ri_p, &ri_t);
+        rot_momentum(p, kLorentz, k, j, i, &rl_p, &rl_t);
         FLD(kRotInertia)[i] = ri_p;
         FLD(kRotInertia + 2)[i] = ri_t;
         FLD(kRotLorentz)[i] = rl_p;
@@ -745,6 +792,36 @@ void radial_fdm(const double *r, int nr, RadialFdm *out)
     }
     f.ri[0] = r[ki]; f.ri[1] = 1.0 / r[ki]; f.ri[2] = f.ri[1] * f.ri[1];
     f.rc[0] = r[ko]; f.rc[1] = 1.0 / r[ko]; f.rc[2] = f.rc[1] * f.rc[1];
+    // free-slip walls (m_coef_fdm_free_ICB.f90:85-173,
+    // m_coef_fdm_free_CMB.f90:85-174): the middle Taylor row is the wall
+    // condition, f - r f' + r^2/2 f'' = 0 (poloidal, with f = 0 at the wall)
+    // or 2 f - r f' = 0 (toroidal); the last row reaches the neighbour.
+    std::memset(f.fvp_i, 0, sizeof f.fvp_i);
+    std::memset(f.fvt_i, 0, sizeof f.fvt_i);
+    std::memset(f.fvp_c, 0, sizeof f.fvp_c);
+    std::memset(f.fvt_c, 0, sizeof f.fvt_c);
+    for (int wall = 0; wall < 2; ++wall) {
+        const double r0 = wall == 0 ? r[ki] : r[ko];
+        const double h = wall == 0 ? dp1 : -dn1;           // neighbour offset
+        const double ap[3][3] = {{1, 0, 0}, {1, -r0, 0.5 * r0 * r0}, {1, h, 0.5 * h * h}};
+        const double at[3][3] = {{1, 0, 0}, {2, -r0, 0}, {1, h, 0.5 * h * h}};
+        double mp[3][3], mt[3][3];
+        inverse33(ap, mp);
+        inverse33(at, mt);
+        double (*vp)[3] = wall == 0 ? f.fvp_i : f.fvp_c;
+        double (*vt)[3] = wall == 0 ? f.fvt_i : f.fvt_c;
+        const int nb = wall == 0 ? 2 : 0;                  // neighbour's row
+        vp[1][0] = 1.0;
+        vp[1][1] = mp[1][0];
+        vp[1][2] = mp[2][0];
+        vp[nb][1] = mp[1][2];
+        vp[nb][2] = mp[2][2];
+        vt[1][0] = 1.0;
+        vt[1][1] = mt[1][0];
+        vt[1][2] = mt[2][0];
+        vt[nb][2] = mt[2][2];
+        if (wall == 1) vt[2][0] = vt[2][1] = vt[2][2] = 1.0;   // unused, as the reference fills it
+    }
 }
 
 int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *radius,
@@ -758,6 +835,12 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
         *err = "mhd_create: nr (L+1)^2 must stay below 2^31 spectral nodes";
         return -1;
     }
+    for (int bcv : {co.bc_velo_icb, co.bc_velo_cmb})
+        if (bcv != kVeloRigid && bcv != kVeloFreeSlip) {
+            *err = "mhd_create: velocity boundary kind " + std::to_string(bcv)
+                 + " is not supported (0 rigid, 1 free slip)";
+            return -1;
+        }
     c->L = L;
     c->nth = nth;
     c->nphi = nphi;
@@ -792,6 +875,12 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     std::memcpy(p.fdr_i, fd.fdr_i, sizeof p.fdr_i);
     std::memcpy(p.ffl_c, fd.ffl_c, sizeof p.ffl_c);
     std::memcpy(p.fdr_c, fd.fdr_c, sizeof p.fdr_c);
+    std::memcpy(p.fvp_i, fd.fvp_i, sizeof p.fvp_i);
+    std::memcpy(p.fvt_i, fd.fvt_i, sizeof p.fvt_i);
+    std::memcpy(p.fvp_c, fd.fvp_c, sizeof p.fvp_c);
+    std::memcpy(p.fvt_c, fd.fvt_c, sizeof p.fvt_c);
+    p.free_i = co.bc_velo_icb == kVeloFreeSlip;
+    p.free_c = co.bc_velo_cmb == kVeloFreeSlip;
     std::memcpy(p.ri, fd.ri, sizeof p.ri);
     std::memcpy(p.rc, fd.rc, sizeof p.rc);
     p.dt = co.dt;
@@ -923,16 +1012,18 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
                                    - coef * (d2[3 * k + 2] + 2.0 * ar1[k] * d1[3 * k + 2]);
             }
     };
-    auto add_fix_flux_icb = [&](std::vector<double> &m, double coef) {
+    // add_fix_flux_icb/cmb_poisson_mat with a wall table (fixed gradient or
+    // free slip)
+    auto add_fix_flux_icb = [&](std::vector<double> &m, double coef, const double (*fdm)[3]) {
         for (int j = 0; j < J; ++j) {
-            M3(m, j, ki, 1) = M3(m, j, ki, 1) - coef * (p.fdr_i[1][2] - g3[j] * p.ri[2]);
-            M3(m, j, ki + 1, 0) = M3(m, j, ki + 1, 0) - coef * p.fdr_i[2][2];
+            M3(m, j, ki, 1) = M3(m, j, ki, 1) - coef * (fdm[1][2] - g3[j] * p.ri[2]);
+            M3(m, j, ki + 1, 0) = M3(m, j, ki + 1, 0) - coef * fdm[2][2];
         }
     };
-    auto add_fix_flux_cmb = [&](std::vector<double> &m, double coef) {
+    auto add_fix_flux_cmb = [&](std::vector<double> &m, double coef, const double (*fdm)[3]) {
         for (int j = 0; j < J; ++j) {
-            M3(m, j, ko - 1, 2) = M3(m, j, ko - 1, 2) - coef * p.fdr_c[0][2];
-            M3(m, j, ko, 1) = M3(m, j, ko, 1) - coef * (p.fdr_c[1][2] - g3[j] * p.rc[2]);
+            M3(m, j, ko - 1, 2) = M3(m, j, ko - 1, 2) - coef * fdm[0][2];
+            M3(m, j, ko, 1) = M3(m, j, ko, 1) - coef * (fdm[1][2] - g3[j] * p.rc[2]);
         }
     };
     auto set_fix_fld_icb = [&](std::vector<double> &m) {
@@ -951,12 +1042,23 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     add_vector_poisson(vt, coef_dvt);
     add_vector_poisson(wt, coef_dvt);
     add_vector_poisson(vs, 1.0);
-    add_fix_flux_icb(wt, coef_dvt);
-    set_fix_fld_icb(vt);
-    add_fix_flux_icb(vs, 1.0);
-    add_fix_flux_cmb(wt, coef_dvt);
-    set_fix_fld_cmb(vt);
-    add_fix_flux_cmb(vs, 1.0);
+    // walls (const_radial_mat_vort_2step, const_r_mat_4_vector_sph.f90:85-135)
+    add_fix_flux_icb(wt, coef_dvt, p.fdr_i);
+    if (p.free_i) {
+        add_fix_flux_icb(vt, coef_dvt, p.fvt_i);
+        add_fix_flux_icb(vs, 1.0, p.fvp_i);
+    } else {
+        set_fix_fld_icb(vt);
+        add_fix_flux_icb(vs, 1.0, p.fdr_i);
+    }
+    add_fix_flux_cmb(wt, coef_dvt, p.fdr_c);
+    if (p.free_c) {
+        add_fix_flux_cmb(vt, coef_dvt, p.fvt_c);
+        add_fix_flux_cmb(vs, 1.0, p.fvp_c);
+    } else {
+        set_fix_fld_cmb(vt);
+        add_fix_flux_cmb(vs, 1.0, p.fdr_c);
+    }
     // poloidal velocity: 5-band product wt * vs (cal_mat_product_3band_mul)
     std::vector<double> vp(5 * nnod, 0.0);
     {

@@ -1,6 +1,7 @@
 // mhd_internal.hpp -- device-resident MHD time step (rotating Boussinesq
-// shell, rigid walls, insulating magnetic boundaries, fixed-temperature
-// boundaries with the spherical-shell reference temperature).
+// shell, rigid or free-slip walls, insulating magnetic boundaries,
+// fixed-temperature boundaries with the spherical-shell reference
+// temperature).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,12 +30,19 @@ enum MhdField : int {
 // Radial finite differences of a grid (nr >= 3): interior three-point
 // weights d1/d2 [nr][3] at offsets -1, 0, +1; wall formulas for a fixed value
 // (ffl_*) and a fixed gradient (fdr_*) at the inner (_i) and outer (_c)
-// walls; powers r, 1/r, 1/r^2 of the wall radii.
+// walls; the free-slip formulas of the poloidal (fvp_*) and toroidal (fvt_*)
+// velocity, [pt + 1][c] with pt -1 / 0 / +1 the point below / at / above the
+// wall and c = 0 value, 1 d/dr, 2 d2/dr2; powers r, 1/r, 1/r^2 of the wall
+// radii.
 struct RadialFdm {
     std::vector<double> d1, d2;
     double ffl_i[3][3], fdr_i[3][3], ffl_c[3][3], fdr_c[3][3];
+    double fvp_i[3][3], fvt_i[3][3], fvp_c[3][3], fvt_c[3][3];
     double ri[3], rc[3];
 };
+
+// velocity wall conditions (bc_velocity of the reference's control file)
+enum MhdVeloBC : int { kVeloRigid = 0, kVeloFreeSlip = 1 };
 void radial_fdm(const double *r, int nr, RadialFdm *out);
 
 struct MhdCoefs {
@@ -44,6 +52,7 @@ struct MhdCoefs {
     double coef_temp = 1, coef_d_temp = 1;
     double coef_magne = 1, coef_d_magne = 1, coef_induct = 1;
     double depth_high_t = 0, depth_low_t = 0, high_temp = 0, low_temp = 0;
+    int bc_velo_icb = kVeloRigid, bc_velo_cmb = kVeloRigid;
 };
 
 // Everything a kernel needs, passed by value.
@@ -62,6 +71,8 @@ struct MhdDev {
     const int *rj_of_rlm = nullptr, *rlm_of_rj = nullptr;
     double *sp_b = nullptr, *sp_f = nullptr;   // rlm buffers, 13 / 19 comps
     double ffl_i[3][3], fdr_i[3][3], ffl_c[3][3], fdr_c[3][3];
+    double fvp_i[3][3], fvt_i[3][3], fvp_c[3][3], fvt_c[3][3];
+    int free_i = 0, free_c = 0;                // free-slip velocity walls
     double ri[3], rc[3];
     double dt, coef_exp_v, coef_exp_t, coef_exp_b;
     double coef_d_velo, coef_d_magne, coef_d_temp, coef_buo, coef_temp, coef_cor;

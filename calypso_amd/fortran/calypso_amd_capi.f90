@@ -22,7 +22,10 @@
         real(c_double) :: coef_magne, coef_d_magne, coef_induct
         real(c_double) :: depth_high_t, depth_low_t
         real(c_double) :: high_temp, low_temp
+        integer(c_int) :: bc_velo_icb, bc_velo_cmb
       end type calypso_mhd_params
+      integer(c_int), parameter :: CALYPSO_VELO_RIGID =     0
+      integer(c_int), parameter :: CALYPSO_VELO_FREE_SLIP = 1
 !
 !>    calypso_scalar_bc of include/calypso_amd.h (same field order)
       integer(c_int), parameter :: CALYPSO_BC_FIXED_VALUE = 0

@@ -63,6 +63,8 @@
       p%depth_low_t = 1.5384615384615384d0
       p%high_temp = 1.0d0
       p%low_temp = 0.0d0
+      p%bc_velo_icb = CALYPSO_VELO_RIGID
+      p%bc_velo_cmb = CALYPSO_VELO_RIGID
 !
       dummy = 0.0d0
       dtemp = 0.0d0

@@ -41,7 +41,12 @@ class MhdParams(ctypes.Structure):
                 ("coef_magne", ctypes.c_double), ("coef_d_magne", ctypes.c_double),
                 ("coef_induct", ctypes.c_double),
                 ("depth_high_t", ctypes.c_double), ("depth_low_t", ctypes.c_double),
-                ("high_temp", ctypes.c_double), ("low_temp", ctypes.c_double)]
+                ("high_temp", ctypes.c_double), ("low_temp", ctypes.c_double),
+                ("bc_velo_icb", ctypes.c_int), ("bc_velo_cmb", ctypes.c_int)]
+
+
+# velocity walls (bc_velocity: non_slip_sph, free_slip_sph)
+VELO_RIGID, VELO_FREE_SLIP = 0, 1
 
 
 def chebyshev_radius(nri: int, r_icb: float, r_cmb: float) -> np.ndarray:
@@ -54,11 +59,12 @@ def chebyshev_radius(nri: int, r_icb: float, r_cmb: float) -> np.ndarray:
 
 
 def dynamobench_case1(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6,
-                      legendre_mode="hip_fp64", Ra=1.0e2):
+                      legendre_mode="hip_fp64", Ra=1.0e2, bc_velo=("non_slip_sph", "non_slip_sph")):
     """Parameters of tests/Dynamobench_case1 (control_MHD, control_sph_shell):
     E = 1e-3, modified Ra = 100, Pr = 1, Pm = 5, r_i/r_o = 0.35 with
     r_o - r_i = 1; returns (MhdParams, radius).  The grid, time step,
-    Crank-Nicolson weight and Rayleigh number can be changed."""
+    Crank-Nicolson weight and Rayleigh number can be changed, and the
+    velocity walls (ICB, CMB) set to "non_slip_sph" or "free_slip_sph"."""
     E, Pr, Pm = 1.0e-3, 1.0, 5.0
     r_cmb = 1.0 / (1.0 - 0.35)
     r_icb = r_cmb * 0.35
@@ -81,6 +87,12 @@ def dynamobench_case1(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_
     p.coef_induct = 1.0 * 1.0
     p.depth_high_t, p.depth_low_t = 0.5384615384615384, 1.5384615384615384
     p.high_temp, p.low_temp = 1.0, 0.0
+    kinds = {"non_slip_sph": VELO_RIGID, "free_slip_sph": VELO_FREE_SLIP}
+    for name in bc_velo:
+        if name not in kinds:
+            raise _lib.CalypsoError(f"velocity boundary {name!r} is not supported "
+                                    "(non_slip_sph, free_slip_sph)")
+    p.bc_velo_icb, p.bc_velo_cmb = kinds[bc_velo[0]], kinds[bc_velo[1]]
     return p, chebyshev_radius(nri, r_icb, r_cmb)
 
 

@@ -297,12 +297,17 @@ int calypso_sph_rj_mode_order(int l_truncation, int *j_global);
  * Replaces the body of module SPH_analyzer_MHD: SPH_initialize_MHD and
  * SPH_analyze_MHD(i_step, iflag_finish) (programs/SPH_MHD/
  * SPH_analyzer_MHD.f90:31-202) for a rotating Boussinesq shell with rigid
- * walls, insulating magnetic boundaries and fixed-temperature walls around
- * the spherical-shell reference temperature (the model of the reference's
- * Dynamobench_case1).  The coefficients are those module
+ * or free-slip walls, insulating magnetic boundaries and fixed-temperature
+ * walls around the spherical-shell reference temperature (the model of the
+ * reference's Dynamobench_case1).  The coefficients are those module
  * m_physical_property holds after set_material_property
  * (material_property.f90); radius[nr] is the radial grid, the inner and
- * outer walls at its ends. */
+ * outer walls at its ends.  bc_velo_icb / bc_velo_cmb choose each velocity
+ * wall as the reference's bc_velocity does (non_slip_sph, free_slip_sph;
+ * set_bc_flag_sph_velo.f90, set_bc_sph_mhd.f90:47-57); any other value is
+ * rejected by calypso_mhd_create. */
+#define CALYPSO_VELO_RIGID     0
+#define CALYPSO_VELO_FREE_SLIP 1
 typedef struct calypso_mhd calypso_mhd_t;
 typedef struct calypso_mhd_params {
     int l_truncation, nth, nphi, nr;
@@ -313,6 +318,7 @@ typedef struct calypso_mhd_params {
     double coef_temp, coef_d_temp;
     double coef_magne, coef_d_magne, coef_induct;
     double depth_high_t, depth_low_t, high_temp, low_temp;
+    int bc_velo_icb, bc_velo_cmb; /* CALYPSO_VELO_RIGID / _FREE_SLIP */
 } calypso_mhd_params;
 
 int calypso_mhd_create(const calypso_mhd_params *params, const double *radius,

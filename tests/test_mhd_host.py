@@ -2,9 +2,13 @@
 (closed forms of the Gaunt and Elsasser integrals) against the oracle's
 restatement of the reference's exact-arithmetic algorithm
 (cal_gaunt_itgs.f90, interact_coriolis_rlm.f90), and the grid helpers."""
-import numpy as np
+import ctypes
 
-from calypso_amd import mhd
+import numpy as np
+import pytest
+
+import calypso_amd
+from calypso_amd import _lib, mhd
 from oracle import mhd_step
 
 
@@ -26,3 +30,22 @@ def test_dynamobench_parameters():
     np.testing.assert_array_equal(r, mhd_step.chebyshev_radius(48, (1 / 0.65) * 0.35, 1 / 0.65))
     assert abs(p.coef_buo - 65000.0) < 1e-9 and p.coef_cor == 2000.0
     assert abs(p.coef_lor - 200.0) < 1e-12 and p.coef_d_magne == 0.2
+
+
+def test_velocity_wall_kinds():
+    """bc_velocity names map to the C ABI's kinds; other names, and other
+    kinds at the C ABI, are rejected before any device work."""
+    p, _ = mhd.dynamobench_case1()
+    assert (p.bc_velo_icb, p.bc_velo_cmb) == (mhd.VELO_RIGID, mhd.VELO_RIGID)
+    p, _ = mhd.dynamobench_case1(bc_velo=("free_slip_sph", "non_slip_sph"))
+    assert (p.bc_velo_icb, p.bc_velo_cmb) == (mhd.VELO_FREE_SLIP, mhd.VELO_RIGID)
+    with pytest.raises(_lib.CalypsoError, match="not supported"):
+        mhd.dynamobench_case1(bc_velo=("rotatable_inner_core", "non_slip_sph"))
+    lib = calypso_amd.load()
+    p, r = mhd.dynamobench_case1()
+    p.bc_velo_cmb = 2
+    h = ctypes.c_void_p()
+    rc = lib.calypso_mhd_create(ctypes.byref(p), r.ctypes.data_as(ctypes.c_void_p), None,
+                                ctypes.byref(h))
+    assert rc != 0 and "velocity boundary kind 2" in lib.calypso_last_error().decode()
+    assert not h.value
