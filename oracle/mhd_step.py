@@ -9,9 +9,24 @@ It is the checker for the GPU step, never the product.  Each routine cites the
 reference routine it restates; numpy arrays are indexed [k, j] with k the
 radial point (0-based) and j the rj mode, j = l(l+1) + m.
 
-Composition, rotating-inner-core walls, the centre point, and the
-diagnostic-only fields (pressure, grad T, viscous diffusion of u) are not part
-of this model and are not restated.
+With DynamoBenchmarkCase2 it restates the model of the reference's
+Dynamobench_case2 test: the radial grid continues through a conducting inner
+core to the centre and beyond the core-mantle boundary
+(count_chebyshev_ext_layers / set_chebyshev_distance_shell), the magnetic
+field is solved from the first point above the centre (sph_to_center:
+set_sph_exp_nod_center.f90, center_sph_matrices.f90,
+m_coef_fdm_to_center.f90) to the insulating CMB and continued above it as a
+potential field (ext_outside_potential_with_j, extend_potential_field.f90), and
+the inner core turns as a solid body about any axis under the viscous,
+Lorentz and Coriolis torques at the ICB (bc_velocity rot_inner_core:
+cal_inner_core_rotation.f90, cal_sph_nod_icb_rotate_velo2,
+inner_core_rot_z_coriolis_rlm).  The rj centre node that the reference adds
+for degree-0 scalars is not kept: with a fixed ICB temperature its l = 0
+solve is the same as the shell's (const_radial_mat_scalar00_sph with
+add_scalar_poisson_mat_no_fld leaves the centre row decoupled).
+
+Composition and the diagnostic-only fields (pressure, grad T, viscous
+diffusion of u) are not part of this model and are not restated.
 """
 from __future__ import annotations
 
@@ -61,12 +76,74 @@ def chebyshev_radius(nri, r_icb, r_cmb):
                      for k in range(nri + 1)])
 
 
-def fdm_coefs(radius):
+def chebyshev_radius_ext(nri, r_icb, r_cmb, r_min, r_max):
+    """count_chebyshev_ext_layers then set_chebyshev_distance_shell
+    (set_radial_grid_sph_shell.f90:60-109, :171-229): Chebyshev points in the
+    shell, mirrored Chebyshev spacing below the ICB down to r_min (then evenly
+    spaced) and above the CMB up to r_max.  Returns (radius, kr_icb, kr_cmb)
+    with 1-based boundary indices."""
+    pi = 4.0 * math.atan(1.0)
+    shell = r_cmb - r_icb
+    half_n = nri // 2
+    dr = 0.5 * shell * (1.0 - math.cos(pi / float(nri)))
+    r, k = r_icb, 0
+    while not (r <= r_min or k >= half_n):
+        k += 1
+        r = r_icb - 0.5 * shell * (1.0 - math.cos(pi * float(k) / float(nri)))
+        dr = 0.5 * shell * (math.cos(pi * float(k - 1) / float(nri))
+                            - math.cos(pi * float(k) / float(nri)))
+    if k == half_n:
+        k = int((r - r_min) / dr) + half_n
+    n_icore = max(k - 1, 0)
+    r, k = r_cmb, 0
+    while not (r >= r_max or k >= half_n):
+        k += 1
+        r = r_cmb + 0.5 * shell * (1.0 - math.cos(pi * float(k) / float(nri)))
+        dr = 0.5 * shell * (-math.cos(pi * float(k) / float(nri))
+                            + math.cos(pi * float(k - 1) / float(nri)))
+    while r < r_max:
+        k += 1
+        r = r + dr
+    n_ext = 0 if k <= 1 else k
+    k_icb = n_icore + 1
+    k_cmb = k_icb + nri
+    n = k_cmb + n_ext
+    rg = np.zeros(n)
+    for kk in range(1, k_icb - half_n):
+        rg[kk - 1] = (r_icb - 0.5 * shell) * float(kk) / float(k_icb - half_n)
+    for kk in range(max(1, k_icb - half_n), k_icb):
+        rg[kk - 1] = r_icb - 0.5 * shell * (1.0 - math.cos(pi * float(kk - k_icb) / float(nri)))
+    for kk in range(k_icb, k_cmb + 1):
+        rg[kk - 1] = r_icb + 0.5 * shell * (1.0 - math.cos(pi * float(kk - k_icb) / float(nri)))
+    ked = min(n, k_cmb + half_n)
+    for kk in range(k_cmb + 1, ked + 1):
+        rg[kk - 1] = r_cmb + 0.5 * shell * (1.0 - math.cos(pi * float(kk - k_cmb) / float(nri)))
+    if ked > k_cmb:
+        dr = rg[ked - 1] - rg[ked - 2]
+    for kk in range(k_cmb + half_n + 1, n + 1):
+        rg[kk - 1] = rg[kk - 2] + dr
+    return rg, k_icb, k_cmb
+
+
+def center_fdm(r1, r2):
+    """cal_2nd_to_center_fixed_fdm (m_coef_fdm_to_center.f90:73-109): the
+    three-point Taylor inverse at the first point with the centre as the
+    point below (value there fixed to zero).  Returns [pt + 1][c] for pt
+    -1 (centre), 0 (first point), +1 (second point), c = 0, 1, 2."""
+    dn, dp = r1, r2 - r1
+    m = inverse33([[1.0, 0.0, 0.0], [1.0, -dn, dn * dn / 2.0], [1.0, dp, dp * dp / 2.0]])
+    return [[m[c][1] for c in range(3)], [m[c][0] for c in range(3)],
+            [m[c][2] for c in range(3)]]
+
+
+def fdm_coefs(radius, below_icb=False):
     """d1nod_mat_fdm_2 / d2nod_mat_fdm_2: 3-point Taylor inverse at every
     point (set_dr_for_nonequi, nod_r_2nd_fdm_coefs_nonequi,
-    SERIAL_src/SPH_SPECTR_src/set_radius_func_noequi.f90; copy order
-    copy_fdm_nod_coefs_from_mat, m_fdm_coefs.f90).  Returns (d1, d2), each
-    (nr, 3) for offsets -1, 0, +1."""
+    SERIAL_src/SPH_SPECTR_src/set_radius_func_noequi.f90:173-197; copy order
+    copy_fdm_nod_coefs_from_mat, m_fdm_coefs.f90).  When the grid has points
+    below the ICB (below_icb), the first point's point below is the centre
+    (distance radius[0]).  Returns (d1, d2), each (nr, 3) for offsets
+    -1, 0, +1."""
     nr = radius.size
     dr = np.zeros(nr)
     for kr in range(1, nr - 1):
@@ -77,7 +154,7 @@ def fdm_coefs(radius):
     d2 = np.zeros((nr, 3))
     for k in range(nr):
         if k == 0:
-            dp, dn = dr[0], dr[0]
+            dp, dn = dr[0], (radius[0] if below_icb else dr[0])
         elif k == nr - 1:
             dp, dn = dr[k], dr[k]
         else:
@@ -442,6 +519,18 @@ class DynamoBenchmarkCase1:
     free_icb, free_cmb = False, False
 
 
+class DynamoBenchmarkCase2(DynamoBenchmarkCase1):
+    """Parameters of the reference's tests/Dynamobench_case2 (control_MHD,
+    control_sph_shell): case 1's shell and numbers, Ra_mod = 110, the radial
+    grid extended to Min_radius 0 and Max_radius 2 (24 inner-core points, 49
+    in the shell, 23 above the CMB: 96 in all), the magnetic field solved to
+    the centre (bc_magnetic_field sph_to_center), a rotatable inner core
+    (bc_velocity rot_inner_core) and the dynamo_benchmark_2 initial field."""
+    Ra_mod = 1.1e2
+    inner_core = True
+    r_min, r_max = 0.0, 2.0
+
+
 def shell_case(L=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6, Ra_mod=1.0e2,
                free_icb=False, free_cmb=False):
     """The Dynamobench case 1 physics (rotating Boussinesq shell, rigid walls,
@@ -463,15 +552,34 @@ class MhdStep:
         self.J = J = (L + 1) ** 2
         r_cmb = c.shell / (1.0 - c.ratio)
         r_icb = r_cmb * c.ratio
-        self.radius = chebyshev_radius(c.nri, r_icb, r_cmb)
-        # set_radius_dat_4_sph_dynamo: r_ICB, r_CMB are the grid's end points
-        self.r_icb, self.r_cmb = float(self.radius[0]), float(self.radius[-1])
+        # inner_core: rotatable inner core with the magnetic field solved to
+        # the centre and a potential field above the CMB (Dynamobench_case2)
+        self.inner_core = bool(getattr(c, "inner_core", False))
+        if self.inner_core:
+            self.radius, self.kr_in, self.kr_out = chebyshev_radius_ext(
+                c.nri, r_icb, r_cmb, c.r_min, c.r_max)
+        else:
+            self.radius = chebyshev_radius(c.nri, r_icb, r_cmb)
+            self.kr_in, self.kr_out = 1, self.radius.size
+        # set_radius_dat_4_sph_dynamo: r_ICB, r_CMB are the grid points of
+        # the ICB and CMB groups
+        self.r_icb = float(self.radius[self.kr_in - 1])
+        self.r_cmb = float(self.radius[self.kr_out - 1])
         nr = self.nr = self.radius.size
-        self.kr_in, self.kr_out = 1, nr
+        # first point of the magnetic field's range (1 = the point above the
+        # centre when the field is solved to the centre)
+        self.kb_in = 1 if self.inner_core else self.kr_in
         self.ar1 = 1.0 / self.radius
         self.ar2 = self.ar1 ** 2
-        self.d1, self.d2 = fdm_coefs(self.radius)
+        self.d1, self.d2 = fdm_coefs(self.radius, self.kr_in > 1)
         self.bc = BoundaryFdm(self.radius, self.kr_in, self.kr_out)
+        if self.inner_core:
+            self.ctr1 = center_fdm(float(self.radius[0]), float(self.radius[1]))
+            self.r_ctr1 = (float(self.radius[0]), 1.0 / float(self.radius[0]),
+                           (1.0 / float(self.radius[0])) ** 2)
+            # cal_fdm1_coef_fix_fld_ICB (cal_fdm_coefs_4_boundaries.f90:135-151)
+            dr1 = float(self.radius[self.kr_in]) - self.r_icb
+            self.fdm1_icb = (-1.0 / dr1, 1.0 / dr1)
         self.free_icb, self.free_cmb = bool(c.free_icb), bool(c.free_cmb)
         # rj mode degrees and normalisations (copy_sph_normalization_2_rj: no
         # l = 0 override in rj)
@@ -499,6 +607,11 @@ class MhdStep:
         dh, dl, hi, lo = c.depth_high_t, c.depth_low_t, c.high_temp, c.low_temp
         self.reft0 = (dh * dl * self.ar1 * (hi - lo) - dh * hi + dl * lo) / (dl - dh)
         self.reft1 = -dh * dl * self.ar2 * (hi - lo) / (dl - dh)
+        # below the ICB the high temperature, above the CMB the low one
+        self.reft0[:self.kr_in - 1] = hi
+        self.reft1[:self.kr_in - 1] = 0.0
+        self.reft0[self.kr_out:] = lo
+        self.reft1[self.kr_out:] = 0.0
         # temperature boundary values after adjust_sph_temp_bc_by_reftemp
         self.T_icb = np.zeros(J)
         self.T_cmb = np.zeros(J)
@@ -539,8 +652,8 @@ class MhdStep:
             m[:, ko + 1:, 1] = 1.0
             return m
 
-        def add_vector_poisson(m, coef):
-            for k in range(ki + 1, ko):
+        def add_vector_poisson(m, coef, k_lo=None):
+            for k in range((ki if k_lo is None else k_lo) + 1, ko):
                 m[:, k - 1, 2] = m[:, k - 1, 2] - coef * d2[k, 0]
                 m[:, k, 1] = m[:, k, 1] - coef * (d2[k, 1] - g3 * ar2[k])
                 m[:, k + 1, 0] = m[:, k + 1, 0] - coef * d2[k, 2]
@@ -592,6 +705,16 @@ class MhdStep:
         else:
             set_fix_fld_cmb(vt)
             add_fix_flux_cmb(vs, 1.0)
+        if self.inner_core:
+            # set_icore_viscous_matrix / set_rotate_icb_vt_sph_mat
+            # (cal_inner_core_rotation.f90:55-71, :185-208): the l = 1
+            # toroidal rows at the ICB carry the viscous torque on the core
+            ar = 1.0 / self.r_icb
+            for j in (1, 2, 3):
+                vt[j, ki, 1] = (1.0 - self.coef_imp_v * self.dt * self.coef_d_velo * 5.0
+                                * (self.fdm1_icb[0] - 2.0 * self.ar1[ki]) * ar)
+                vt[j, ki + 1, 0] = -(self.coef_imp_v * self.dt * self.coef_d_velo * 5.0
+                                     * ar * self.fdm1_icb[1])
         vp = self._mat_product_3band(wt, vs)
         # temperature
         coef = self.coef_imp_t * self.coef_d_temp * self.dt
@@ -602,13 +725,22 @@ class MhdStep:
         # magnetic field
         coef_dbt = self.coef_imp_b * self.coef_d_magne * self.dt
         bs, bt = unit_evo(), unit_evo()
-        add_vector_poisson(bs, coef_dbt)
-        add_vector_poisson(bt, coef_dbt)
+        kb = self.kb_in - 1
+        add_vector_poisson(bs, coef_dbt, kb)
+        add_vector_poisson(bt, coef_dbt, kb)
         g1 = self.g1
-        bs[:, ki, 1] = bs[:, ki, 1] - coef_dbt * (bc.fix_dr_icb[1][2] - g3 * bc.r_icb[2]
-                                                  + (g1 + 1.0) * bc.r_icb[1] * bc.fix_dr_icb[0][2])
-        bs[:, ki + 1, 0] = bs[:, ki + 1, 0] - coef_dbt * bc.fix_dr_icb[2][2]
-        set_fix_fld_icb(bt)
+        if self.inner_core:
+            # add_vector_poisson_mat_center (center_sph_matrices.f90:160-180)
+            ctr, rc = self.ctr1, self.r_ctr1
+            for m in (bs, bt):
+                m[:, 0, 1] = m[:, 0, 1] - coef_dbt * (ctr[1][2] - g3 * rc[2])
+                m[:, 1, 0] = m[:, 1, 0] - coef_dbt * ctr[2][2]
+        else:
+            bs[:, ki, 1] = bs[:, ki, 1] - coef_dbt * (bc.fix_dr_icb[1][2] - g3 * bc.r_icb[2]
+                                                      + (g1 + 1.0) * bc.r_icb[1]
+                                                      * bc.fix_dr_icb[0][2])
+            bs[:, ki + 1, 0] = bs[:, ki + 1, 0] - coef_dbt * bc.fix_dr_icb[2][2]
+            set_fix_fld_icb(bt)
         bs[:, ko - 1, 2] = bs[:, ko - 1, 2] - coef_dbt * bc.fix_dr_cmb[0][2]
         bs[:, ko, 1] = bs[:, ko, 1] - coef_dbt * (bc.fix_dr_cmb[1][2] - g3 * bc.r_cmb[2]
                                                   - g1 * bc.r_cmb[1] * bc.fix_dr_cmb[2][2])
@@ -626,6 +758,7 @@ class MhdStep:
         st, ed = self.kr_in - 1, self.kr_out - 1
         L_, R_ = a_l, a_r
         p = np.zeros((J, nr, 5))
+        p[:, :, 2] = 1.0          # identity outside kr_in..kr_out (m_radial_matrices_sph)
 
         def rows(k, skip5=False, skip1=False):
             if not skip5:
@@ -658,6 +791,53 @@ class MhdStep:
     def _interior(self):
         return range(self.kr_in, self.kr_out - 1)   # 0-based k = kr_in+1..kr_out-1
 
+    def _interior_b(self):
+        return range(self.kb_in, self.kr_out - 1)   # the magnetic field's range
+
+    def _center_rot2(self, src, rot):
+        """cal_sph_nod_center_rot2 (set_sph_exp_nod_center.f90:96-124): the
+        first point with the centre value taken as zero (only the offsets 0,
+        +1 of the three-point formulas)."""
+        d1, d2, g3 = self.d1, self.d2, self.g3
+        d2s = d2[0, 1] * src[0, 0] + d2[0, 2] * src[0, 1]
+        d1t = d1[0, 1] * src[2, 0] + d1[0, 2] * src[2, 1]
+        rot[0, 0] = src[2, 0]
+        rot[1, 0] = d1t
+        rot[2, 0] = -(d2s - g3 * self.ar2[0] * src[0, 0])
+
+    def set_inner_core_rotation(self):
+        """set_inner_core_rotation / set_inner_core_rot_l1
+        (cal_inner_core_rotation.f90:42-52, :212-245): the l = 1 toroidal
+        velocity and vorticity of the solid-body turning core below the ICB."""
+        if not self.inner_core:
+            return
+        u, w = self.f["velo"], self.f["vort"]
+        ki = self.kr_in - 1
+        r = self.radius
+        ar1, ar2 = self.ar1[ki], self.ar2[ki]
+        for j in (1, 2, 3):
+            for k in range(ki):
+                ratio = r[k] * r[k] * ar2
+                u[2, k, j] = ratio * u[2, ki, j]
+                w[0, k, j] = ratio * w[0, ki, j]
+                w[1, k, j] = 2.0 * w[0, ki, j] * r[k] * ar2
+            w[1, ki, j] = 2.0 * w[0, ki, j] * ar1
+
+    def _rotate_velo2(self, u):
+        """cal_sph_nod_icb_rotate_velo2 (set_sph_exp_rigid_ICB.f90:84-146)
+        with no imposed ICB rotation: zero velocity up to the ICB except the
+        l = 1 toroidal modes, which keep their ICB value and turn the core
+        below it as a solid body."""
+        ki = self.kr_in - 1
+        u[0, :ki + 1] = 0.0
+        u[1, :ki + 1] = 0.0
+        keep = u[2, ki, 1:4].copy()
+        u[2, :ki + 1] = 0.0
+        u[2, ki, 1:4] = keep
+        ar2 = self.bc.r_icb[2]
+        for k in range(ki):
+            u[2, k, 1:4] = keep * ar2 * self.radius[k] * self.radius[k]
+
     def _dr(self, f, k, which=1):
         d = self.d1 if which == 1 else self.d2
         return d[k, 0] * f[k - 1] + d[k, 1] * f[k] + d[k, 2] * f[k + 1]
@@ -674,9 +854,12 @@ class MhdStep:
         if self.free_icb:
             icb_free_v_and_w(bc.free_vp_icb, bc.free_vt_icb, u, w, ki)
         else:
-            u[0, :ki + 1] = 0.0
-            u[1, :ki + 1] = 0.0
-            u[2, :ki + 1] = 0.0          # vt_ICB_bc = 0
+            if self.inner_core:
+                self._rotate_velo2(u)
+            else:
+                u[0, :ki + 1] = 0.0
+                u[1, :ki + 1] = 0.0
+                u[2, :ki + 1] = 0.0          # vt_ICB_bc = 0
             fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
             d2s = fdr[1][2] * u[0, ki] + fdr[2][2] * u[0, ki + 1]
             d1t = ffl[0][1] * u[2, ki] + ffl[1][1] * u[2, ki + 1] + ffl[2][1] * u[2, ki + 2]
@@ -697,9 +880,9 @@ class MhdStep:
             w[2, ko] = -d2s
         self._diff_pol_and_rot(u, w)
 
-    def _diff_pol_and_rot(self, f, rot):
+    def _diff_pol_and_rot(self, f, rot, interior=None):
         g3 = self.g3
-        for k in self._interior():
+        for k in (self._interior() if interior is None else interior):
             d1s = self._dr(f[0], k, 1)
             d2s = self._dr(f[0], k, 2)
             d1t = self._dr(f[2], k, 1)
@@ -716,16 +899,22 @@ class MhdStep:
         b, jc = self.f["magne"], self.f["current"]
         bc, g1, g3 = self.bc, self.g1, self.g3
         ki, ko = self.kr_in - 1, self.kr_out - 1
-        fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
-        d1s = (g1 + 1.0) * bc.r_icb[1] * b[0, ki]
-        d2s = (((g1 + 1.0) * bc.r_icb[1] * fdr[0][2] + fdr[1][2]) * b[0, ki]
-               + fdr[2][2] * b[0, ki + 1])
-        d1t = ffl[1][1] * b[2, ki + 1] + ffl[2][1] * b[2, ki + 2]
-        b[1, ki] = d1s
-        b[2, ki] = 0.0
-        jc[0, ki] = 0.0
-        jc[1, ki] = d1t
-        jc[2, ki] = -(d2s - g3 * bc.r_icb[2] * b[0, ki])
+        if self.inner_core:
+            # sph_center_fld_and_curl (set_sph_exp_nod_center.f90:42-71)
+            d1 = self.d1
+            b[1, 0] = d1[0, 1] * b[0, 0] + d1[0, 2] * b[0, 1]
+            self._center_rot2(b, jc)
+        else:
+            fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
+            d1s = (g1 + 1.0) * bc.r_icb[1] * b[0, ki]
+            d2s = (((g1 + 1.0) * bc.r_icb[1] * fdr[0][2] + fdr[1][2]) * b[0, ki]
+                   + fdr[2][2] * b[0, ki + 1])
+            d1t = ffl[1][1] * b[2, ki + 1] + ffl[2][1] * b[2, ki + 2]
+            b[1, ki] = d1s
+            b[2, ki] = 0.0
+            jc[0, ki] = 0.0
+            jc[1, ki] = d1t
+            jc[2, ki] = -(d2s - g3 * bc.r_icb[2] * b[0, ki])
         fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
         d1s = -g1 * bc.r_cmb[1] * b[0, ko]
         d2s = (fdr[0][2] * b[0, ko - 1]
@@ -736,7 +925,15 @@ class MhdStep:
         jc[0, ko] = 0.0
         jc[1, ko] = d1t
         jc[2, ko] = -(d2s - g3 * bc.r_cmb[2] * b[0, ko])
-        self._diff_pol_and_rot(b, jc)
+        self._diff_pol_and_rot(b, jc, self._interior_b())
+        # ext_outside_potential_with_j (extend_potential_field.f90:169-201)
+        for k in range(ko + 1, self.nr):
+            ratio = self.r_cmb * self.ar1[k]
+            pw = np.array([ratio ** int(l) for l in self.l_of_j])
+            b[0, k] = b[0, ko] * pw
+            b[1, k] = -b[0, ko] * pw * g1 * self.ar1[k]
+            b[2, k] = 0.0
+            jc[:, k] = 0.0
 
     def vorticity_diffusion(self):
         """const_sph_vorticirty_diffusion (const_sph_diffusion.f90) with
@@ -757,6 +954,14 @@ class MhdStep:
             for c in (0, 2):
                 d2 = ffl[0][2] * w[c, ki] + ffl[1][2] * w[c, ki + 1] + ffl[2][2] * w[c, ki + 2]
                 out[c, ki] = coef * (d2 - g3 * bc.r_icb[2] * w[c, ki])
+        if self.inner_core:
+            # cal_icore_viscous_drag_l1 (cal_inner_core_rotation.f90:249-276):
+            # the viscous torque on the core replaces the l = 1 ICB row
+            mat_0 = self.fdm1_icb[0] - 2.0 * self.ar1[ki]
+            mat_1 = self.fdm1_icb[1]
+            ar = 1.0 / self.r_icb
+            for j in (1, 2, 3):
+                out[0, ki, j] = 5.0 * coef * ar * (mat_0 * w[0, ki, j] + mat_1 * w[0, ki + 1, j])
         ffl = bc.fix_fld_cmb
         if self.free_cmb:
             cmb_free_w_diffuse2(ffl, bc.free_vt_cmb, bc.r_cmb, g3, coef, w, out, ko)
@@ -771,19 +976,27 @@ class MhdStep:
         cal_sph_nod_icb_ins_diffuse2 / cal_sph_nod_cmb_ins_diffuse2 on B."""
         b, jc, out = self.f["magne"], self.f["current"], self.f["b_diffuse"]
         coef, g1, g3, bc = self.coef_d_magne, self.g1, self.g3, self.bc
-        for k in self._interior():
+        for k in self._interior_b():
             d2s = self._dr(jc[0], k, 2)
             d1t = self._dr(jc[2], k, 1)
             out[0, k] = -coef * jc[2, k]
             out[1, k] = -coef * d1t
             out[2, k] = coef * (d2s - g3 * self.ar2[k] * jc[0, k])
         ki, ko = self.kr_in - 1, self.kr_out - 1
-        fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
-        d2s = (((g1 + 1.0) * bc.r_icb[1] * fdr[0][2] + fdr[1][2]) * b[0, ki]
-               + fdr[2][2] * b[0, ki + 1])
-        d2t = ffl[0][2] * b[2, ki] + ffl[1][2] * b[2, ki + 1] + ffl[2][2] * b[2, ki + 2]
-        out[0, ki] = coef * (d2s - g3 * bc.r_icb[2] * b[0, ki])
-        out[2, ki] = coef * (d2t - g3 * bc.r_icb[2] * b[2, ki])
+        if self.inner_core:
+            # cal_sph_nod_center_diffuse2 (set_sph_exp_nod_center.f90:128-156)
+            d2 = self.d2
+            d2s = d2[0, 1] * b[0, 0] + d2[0, 2] * b[0, 1]
+            d2t = d2[0, 1] * b[2, 0] + d2[0, 2] * b[2, 1]
+            out[0, 0] = coef * (d2s - g3 * self.ar2[0] * b[0, 0])
+            out[2, 0] = coef * (d2t - g3 * self.ar2[0] * b[2, 0])
+        else:
+            fdr, ffl = bc.fix_dr_icb, bc.fix_fld_icb
+            d2s = (((g1 + 1.0) * bc.r_icb[1] * fdr[0][2] + fdr[1][2]) * b[0, ki]
+                   + fdr[2][2] * b[0, ki + 1])
+            d2t = ffl[0][2] * b[2, ki] + ffl[1][2] * b[2, ki + 1] + ffl[2][2] * b[2, ki + 2]
+            out[0, ki] = coef * (d2s - g3 * bc.r_icb[2] * b[0, ki])
+            out[2, ki] = coef * (d2t - g3 * bc.r_icb[2] * b[2, ki])
         fdr, ffl = bc.fix_dr_cmb, bc.fix_fld_cmb
         d2s = (fdr[0][2] * b[0, ko - 1]
                + (fdr[1][2] - g1 * bc.r_cmb[1] * fdr[2][2]) * b[0, ko])
@@ -850,13 +1063,16 @@ class MhdStep:
                 rot[1, ko] = d1t
                 rot[2, ko] = -(d2s - g3 * bc.r_cmb[2] * src[0, ko])
         else:
-            rot[0, ki] = src[2, ki]
-            rot[1, ki] = (g1 + 1.0) * bc.r_icb[1] * src[2, ki]
-            rot[2, ki] = 0.0
+            if self.inner_core:
+                self._center_rot2(src, rot)      # sel_bc_sph_rotation_uxb
+            else:
+                rot[0, ki] = src[2, ki]
+                rot[1, ki] = (g1 + 1.0) * bc.r_icb[1] * src[2, ki]
+                rot[2, ki] = 0.0
             rot[0, ko] = src[2, ko]
             rot[1, ko] = -g1 * bc.r_cmb[1] * src[2, ko]
             rot[2, ko] = 0.0
-        for k in self._interior():
+        for k in (self._interior() if rigid else self._interior_b()):
             d1d = self._dr(src[1], k, 1)
             d1t = self._dr(src[2], k, 1)
             rot[0, k] = src[2, k]
@@ -938,12 +1154,24 @@ class MhdStep:
         f["m_advect"] = take(0)
         rc = take(6)
         rc[0], rc[2] = self.coriolis()
+        ki = self.kr_in - 1
+        if self.inner_core:
+            # inner_core_rot_z_coriolis_rlm (sum_coriolis_terms_rlm.f90:
+            # 316-351): on the core's l = 1 ICB rows only the z rotation's
+            # coupling of the x and y turns remains
+            w0 = f["vort"][0]
+            om0 = 0.5 * self.r_icb ** 2
+            rc[0, ki, 2] = 0.0
+            rc[0, ki, 1] = -(2.0 * self.coef_cor * self.r_icb * om0 * w0[ki, 3])
+            rc[0, ki, 3] = 2.0 * self.coef_cor * self.r_icb * om0 * w0[ki, 1]
         f["rot_Coriolis"] = rc
         f["lorentz"] = take(9)
         f["vp_induct"] = take(12)
         f["h_flux"] = take(15)
         self._force_rot(f["m_advect"], f["rot_inertia"], rigid=True)
         self._force_rot(f["lorentz"], f["rot_Lorentz"], rigid=True)
+        if self.inner_core:
+            self.int_icore_toroidal_lorentz()
         self._force_rot(f["vp_induct"], f["induction"], rigid=False)
         self.scalar_advect()
         ki, ko = self.kr_in - 1, self.kr_out - 1
@@ -954,6 +1182,24 @@ class MhdStep:
         fo[0] = -f["rot_inertia"][0] + f["rot_Coriolis"][0] + f["rot_Lorentz"][0]
         fo[2] = (-f["rot_inertia"][2] + f["rot_Coriolis"][2] + f["rot_Lorentz"][2]
                  + f["rot_buoyancy"])
+
+    def int_icore_toroidal_lorentz(self):
+        """int_icore_tor_lorentz_l1 (cal_inner_core_rotation.f90:280-314):
+        the Lorentz torque on the core, 5/2 r_i^-3 times the trapezoid
+        integral of the l = 1 toroidal Lorentz force times r^2 from the centre
+        (first interval f(r_1) r_1^3) to the ICB, in the ICB rows of the
+        toroidal force and of the poloidal curl."""
+        lor, rl = self.f["lorentz"], self.f["rot_Lorentz"]
+        ki = self.kr_in - 1
+        r = self.radius
+        ar = 1.0 / self.r_icb
+        for j in (1, 2, 3):
+            sk = lor[2, 0, j] * (r[0] * r[0] * r[0])
+            for k in range(ki):
+                sk = sk + ((lor[2, k, j] * (r[k] * r[k]) + lor[2, k + 1, j] * (r[k + 1] * r[k + 1]))
+                           * (r[k + 1] - r[k]))
+            lor[2, ki, j] = 0.5 * 5.0 * sk * (ar * ar * ar)
+            rl[0, ki, j] = lor[2, ki, j]
 
     # ------------------------------------------------------ initialisation --
     def initialize(self, seeded=None):
@@ -994,11 +1240,30 @@ class MhdStep:
         jj = m * (m + 1) + m
         # scalar libm arithmetic per radius (pow, sin, cos as the Fortran has them)
         rc, ri = self.r_cmb, self.r_icb
+        ki, ko = self.kr_in - 1, self.kr_out - 1
         for k, x in enumerate(float(v) for v in r):
+            if k < ki or k > ko:
+                continue
             xr = 2.0 * x - 1.0 * (rc + ri) / shell
             f["temp"][k, jj] = ((1.0 - 3.0 * (xr * xr) + 3.0 * math.pow(xr, 4) - math.pow(xr, 6))
                                 * 0.1 * 3.0 / math.sqrt(2.0 * pi))
         js, jt = 2, 6                       # (l, m) = (1, 0), (2, 0)
+        if self.inner_core:
+            # initial_b_dynamobench_2 (initial_magne_dynamobench.f90:103-147),
+            # from the first point to the CMB
+            for k in range(ko + 1):
+                x = float(r[k])
+                f["magne"][0][k, js] = ((5.0 / 2.0) * (x * x) * (4.0 * rc - 3.0 * x)
+                                        / (rc + 3.0))
+                f["magne"][1][k, js] = (5.0 / 2.0) * x * (8.0 * rc - 9.0 * x) / (rc + 3.0)
+                f["current"][2][k, js] = 5.0 * 6.0 * x / (3.0 + rc)
+                s_ = math.sin(pi * x / rc)
+                c_ = math.cos(pi * x / rc)
+                f["magne"][2][k, jt] = (10.0 / 3.0) * x * s_
+                f["current"][0][k, jt] = f["magne"][2][k, jt]
+                f["current"][1][k, jt] = (10.0 / 3.0) * (s_ + (pi / rc) * x * c_)
+            self._start()
+            return
         for k, x in enumerate(float(v) for v in r):
             f["magne"][0][k, js] = (5.0 / 8.0) * (-3.0 * math.pow(x, 3) + (4.0 * rc) * (x * x)
                                                   - math.pow(ri, 4) / x)
@@ -1016,6 +1281,7 @@ class MhdStep:
         self.sync_temp_by_per_temp()
         # set_sph_field_to_start
         self.grad_vp_and_vorticity()
+        self.set_inner_core_rotation()
         self.vorticity_diffusion()
         self.rot_buoyancy()
         self.scalar_diffusion()
@@ -1076,6 +1342,8 @@ class MhdStep:
         bc = self.bc
         if self.free_icb:
             icb_free_vpol2(bc.free_vp_icb, u, ki)
+        elif self.inner_core:
+            self._rotate_velo2(u)
         else:
             u[0, :ki + 1] = 0.0
             u[1, :ki + 1] = 0.0
@@ -1098,14 +1366,16 @@ class MhdStep:
         b = f["magne"]
         b[0][:, 0] = 0.0
         b[2][:, 0] = 0.0
-        b[1, ki] = (self.g1 + 1.0) * self.bc.r_icb[1] * b[0, ki]
-        b[2, ki] = 0.0
+        if not self.inner_core:
+            b[1, ki] = (self.g1 + 1.0) * self.bc.r_icb[1] * b[0, ki]
+            b[2, ki] = 0.0
         b[1, ko] = -self.g1 * self.bc.r_cmb[1] * b[0, ko]
         b[2, ko] = 0.0
         b[0] = self._solve("bs", b[0])
         b[2] = self._solve("bt", b[2])
         self.grad_bp_and_current()
         # update_after_*
+        self.set_inner_core_rotation()
         self.vorticity_diffusion()
         self.rot_buoyancy()
         self.scalar_diffusion()

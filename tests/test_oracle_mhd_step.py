@@ -1,7 +1,9 @@
 """The oracle's full MHD step against the reference's own Dynamobench_case1
-goldens (sph_pwr_volume_s.dat at steps 50 and 100, compared at the 1e-9
-relative bound of the reference's compare_sph_mean_square.f90:96-122), plus
-closed-form pins of its radial finite differences and Gaunt couplings."""
+and Dynamobench_case2 goldens (sph_pwr_volume_s.dat at steps 50 and 100; the
+reference's compare_sph_mean_square.f90:96-122 allows 1e-9 relative, these
+tests 1e-12), case 2 also against every step of the reference's program built
+from its sources, plus closed-form pins of the radial finite differences
+(at the centre too) and the Gaunt couplings."""
 import json
 import math
 import os
@@ -76,3 +78,67 @@ def test_dynamobench_case1_energies():
             got[str(s.i_step)] = s.energies()
     for step, want in gold["steps"].items():
         np.testing.assert_allclose(got[step], np.array(want), rtol=gold["rtol"], atol=0)
+
+
+# ---------------------------------------------------- Dynamobench case 2 --
+
+GOLD2 = os.path.join(os.path.dirname(__file__), "golden", "dynamobench_case2_energies.json")
+STEPS2 = os.path.join(os.path.dirname(__file__), "golden", "dynamobench_case2_steps.json")
+
+
+def test_case2_radial_grid_matches_reference():
+    """The grid through the inner core and above the CMB equals the radii the
+    reference's gen_sph_grids writes for tests/Dynamobench_case2 (printed to
+    16 digits)."""
+    ref = json.load(open(STEPS2))
+    r, ki, ko = mhd_step.chebyshev_radius_ext(48, 0.35 / 0.65, 1.0 / 0.65, 0.0, 2.0)
+    assert (ki, ko) == (ref["icb_id"], ref["cmb_id"])
+    np.testing.assert_allclose(r, np.array(ref["radius"]), rtol=1e-15, atol=0)
+
+
+def test_center_formulas_exact_for_quadratics_through_the_centre():
+    """fdm2_fix_fld_ctr1 and the first-point stencil of a grid with points
+    below the ICB take the centre as the point below, where a field
+    regular at r = 0 of the form a r + b r^2 vanishes: both are exact for it."""
+    r, _, _ = mhd_step.chebyshev_radius_ext(48, 0.35 / 0.65, 1.0 / 0.65, 0.0, 2.0)
+    f = lambda x: 0.7 * x - 1.3 * x * x
+    ctr = mhd_step.center_fdm(r[0], r[1])
+    d1 = ctr[1][1] * f(r[0]) + ctr[2][1] * f(r[1])
+    d2 = ctr[1][2] * f(r[0]) + ctr[2][2] * f(r[1])
+    assert abs(d1 - (0.7 - 2.6 * r[0])) < 1e-12
+    assert abs(d2 + 2.6) < 1e-9
+    d1c, d2c = mhd_step.fdm_coefs(r, below_icb=True)
+    assert abs(d1c[0, 1] * f(r[0]) + d1c[0, 2] * f(r[1]) - (0.7 - 2.6 * r[0])) < 1e-12
+    assert abs(d2c[0, 1] * f(r[0]) + d2c[0, 2] * f(r[1]) + 2.6) < 1e-9
+    d1s, _ = mhd_step.fdm_coefs(r, below_icb=False)
+    assert abs(d1s[0, 1] * f(r[0]) + d1s[0, 2] * f(r[1]) - (0.7 - 2.6 * r[0])) > 1e-3
+
+
+def test_dynamobench_case2_energies_every_step():
+    """The oracle's Dynamobench_case2 run (rotating conducting inner core,
+    magnetic field to the centre) against the energies the reference's own
+    program writes after each of the 100 steps, and against the published
+    rows for steps 50 and 100."""
+    ref = json.load(open(STEPS2))
+    gold = json.load(open(GOLD2))
+    s = mhd_step.MhdStep(mhd_step.DynamoBenchmarkCase2)
+    s.initialize()
+    for _ in range(100):
+        s.step()
+        np.testing.assert_allclose(s.energies(), np.array(ref["steps"][str(s.i_step)]),
+                                   rtol=ref["rtol"], atol=0, err_msg=f"step {s.i_step}")
+        if str(s.i_step) in gold["steps"]:
+            np.testing.assert_allclose(s.energies(), np.array(gold["steps"][str(s.i_step)]),
+                                       rtol=gold["rtol"], atol=0)
+
+
+def test_case2_torque_is_pinned():
+    """Planted mistake: without the Lorentz torque on the inner core the
+    first step's energies already leave the reference's."""
+    ref = json.load(open(STEPS2))
+    s = mhd_step.MhdStep(mhd_step.DynamoBenchmarkCase2)
+    s.int_icore_toroidal_lorentz = lambda: None
+    s.initialize()
+    s.step()
+    rel = np.abs(s.energies() / np.array(ref["steps"]["1"]) - 1.0)
+    assert rel.max() > 1e-6
