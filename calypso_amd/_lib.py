@@ -62,6 +62,7 @@ SIGNATURES = [
     ("calypso_mhd_snapshot_rtp", _I, [_P, _I, _P, _I, _P, _P]),
     ("calypso_mhd_get_field", _I, [_P, _I, _P]),
     ("calypso_mhd_set_dynamobench_ic", _I, [_P, _I]),
+    ("calypso_mhd_set_dynamobench2_ic", _I, [_P, _I]),
     ("calypso_sph_write_grid_files", _I, [ctypes.c_char_p, _I, _I, _I, _I, _P]),
     ("calypso_sph_rj_mode_order", _I, [_I, _P]),
     ("calypso_mhd_set_rj_order", _I, [_P, _I, _P]),

@@ -521,13 +521,19 @@ int calypso_mhd_create(const calypso_mhd_params *pp, const double *radius, void 
     co.low_temp = pp->low_temp;
     co.bc_velo_icb = pp->bc_velo_icb;
     co.bc_velo_cmb = pp->bc_velo_cmb;
+    co.bc_magne_icb = pp->bc_magne_icb;
+    co.kr_icb = pp->kr_icb > 0 ? pp->kr_icb - 1 : 0;
+    co.kr_cmb = pp->kr_cmb > 0 ? pp->kr_cmb - 1 : -1;
     if (co.dt <= 0.0) return fail("calypso_mhd_create: dt must be positive");
     if (pp->legendre_mode < CALYPSO_LEG_FP64 || pp->legendre_mode > CALYPSO_LEG_INT8X6)
         return fail("calypso_mhd_create: unknown Legendre mode");
-    for (int bcv : {pp->bc_velo_icb, pp->bc_velo_cmb})
-        if (bcv != CALYPSO_VELO_RIGID && bcv != CALYPSO_VELO_FREE_SLIP)
-            return fail("calypso_mhd_create: velocity boundary kind " + std::to_string(bcv)
-                        + " is not supported (0 rigid, 1 free slip)");
+    if (pp->kr_icb < 0 || pp->kr_cmb < 0)
+        return fail("calypso_mhd_create: kr_icb / kr_cmb must be 0 or a 1-based point");
+    {
+        std::string err;
+        if (pp->nr < 5) return fail("calypso_mhd_create: nr must be at least 5");
+        if (calypso::mhd_check_model(pp->nr, radius, co, &err)) return fail(err);
+    }
     calypso_mhd *h = new (std::nothrow) calypso_mhd;
     if (!h) return fail("calypso_mhd_create: out of host memory");
     std::string err;
@@ -581,6 +587,14 @@ int calypso_mhd_set_dynamobench_ic(calypso_mhd_t *mhd, int isig)
     if (!mhd) return fail("calypso_mhd_set_dynamobench_ic: null handle");
     std::string err;
     if (calypso::mhd_set_dynamobench_ic(&mhd->c, isig, &err)) return fail(err);
+    return 0;
+}
+
+int calypso_mhd_set_dynamobench2_ic(calypso_mhd_t *mhd, int isig)
+{
+    if (!mhd) return fail("calypso_mhd_set_dynamobench2_ic: null handle");
+    std::string err;
+    if (calypso::mhd_set_dynamobench2_ic(&mhd->c, isig, &err)) return fail(err);
     return 0;
 }
 

@@ -59,6 +59,17 @@ __device__ __forceinline__ double dr2(const MhdDev &p, const double *f, int k, s
     return p.d2[3 * k + 0] * f[i - p.J] + p.d2[3 * k + 1] * f[i] + p.d2[3 * k + 2] * f[i + p.J];
 }
 
+// x^n for n >= 0 by binary powering (the integer power of Fortran's x**n)
+__device__ inline double powi(double x, int n)
+{
+    double y = (n & 1) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n & 1) y = y * x;
+    }
+    return y;
+}
+
 // ---------------------------------------------------------- explicit ----
 // node i = k J + j -> (k, j) without a 64-bit division sequence: the host
 // keeps nnod below 2^31, so a double quotient and one correction are exact
@@ -158,6 +169,12 @@ __global__ void k_cn_rhs(MhdDev p)
             up = 0.0;
             ut = 0.0;
             FLD(kVelo + 1)[i] = 0.0;
+            if (p.icore && k <= p.ki && j >= 1 && j <= 3) {
+                // cal_sph_nod_icb_rotate_velo2: the l = 1 toroidal velocity
+                // keeps its ICB value and turns the core below as a solid body
+                const double keep = FLD(kVort)[j + (size_t)p.J * p.ki];
+                ut = k == p.ki ? keep : keep * p.ri[2] * p.r[k] * p.r[k];
+            }
         }
         FLD(kVelo)[i] = up;
         FLD(kVelo + 2)[i] = ut;
@@ -168,7 +185,7 @@ __global__ void k_cn_rhs(MhdDev p)
             bp[i] = 0.0;
             bt[i] = 0.0;
         }
-        if (k == p.ki) {
+        if (k == p.ki && !p.icore) {
             bdp[i] = (p.g1[j] + 1.0) * p.ri[1] * bp[i];
             bt[i] = 0.0;
         }
@@ -202,12 +219,18 @@ __global__ void k_bc_rows(MhdDev p)
         ws[i0] = ut[i0];
         wd[i0] = d1t;
         wt[i0] = -d2s;
-    } else {   // ICB rigid
+    } else {   // ICB rigid, or the rotatable core's wall
+        const bool turn = p.icore && j >= 1 && j <= 3;
+        const double keep = turn ? ut[j + J * p.ki] : 0.0;
         for (int k = 0; k <= p.ki; ++k) {
             const size_t i = j + J * k;
             us[i] = 0.0;
             ud[i] = 0.0;
             ut[i] = 0.0;
+        }
+        if (turn) {   // cal_sph_nod_icb_rotate_velo2
+            ut[j + J * p.ki] = keep;
+            for (int k = 0; k < p.ki; ++k) ut[j + J * k] = keep * p.ri[2] * p.r[k] * p.r[k];
         }
         const size_t i0 = j + J * p.ki, i1 = i0 + J, i2 = i1 + J;
         const double d2s = p.fdr_i[1][2] * us[i0] + p.fdr_i[2][2] * us[i1];
@@ -237,9 +260,32 @@ __global__ void k_bc_rows(MhdDev p)
         wd[i0] = d1t;
         wt[i0] = -d2s;
     }
+    if (p.icore && j >= 1 && j <= 3) {
+        // set_inner_core_rotation (cal_inner_core_rotation.f90:212-245):
+        // solid-body rotation below the ICB from the ICB's l = 1 values
+        const size_t i0 = j + J * p.ki;
+        const double a1 = p.ar1[p.ki], a2 = p.ar2[p.ki];
+        for (int k = 0; k < p.ki; ++k) {
+            const size_t i = j + J * k;
+            const double ratio = p.r[k] * p.r[k] * a2;
+            ut[i] = ratio * ut[i0];
+            ws[i] = ratio * ws[i0];
+            wd[i] = 2.0 * ws[i0] * p.r[k] * a2;
+        }
+        wd[i0] = 2.0 * ws[i0] * a1;
+    }
     double *bs = FLD(kMagne), *bd = FLD(kMagne + 1), *bt = FLD(kMagne + 2);
     double *js = FLD(kCurrent), *jd = FLD(kCurrent + 1), *jt = FLD(kCurrent + 2);
-    {   // ICB insulator
+    if (p.icore) {   // sph_center_fld_and_curl: the first point above the centre
+        const size_t i0 = j, i1 = i0 + J;
+        const double d1s = p.d1[1] * bs[i0] + p.d1[2] * bs[i1];
+        const double d2s = p.d2[1] * bs[i0] + p.d2[2] * bs[i1];
+        const double d1t = p.d1[1] * bt[i0] + p.d1[2] * bt[i1];
+        bd[i0] = d1s;
+        js[i0] = bt[i0];
+        jd[i0] = d1t;
+        jt[i0] = -(d2s - g3 * p.ar2[0] * bs[i0]);
+    } else {   // ICB insulator
         const size_t i0 = j + J * p.ki, i1 = i0 + J, i2 = i1 + J;
         const double d1s = (g1 + 1.0) * p.ri[1] * bs[i0];
         const double d2s = ((g1 + 1.0) * p.ri[1] * p.fdr_i[0][2] + p.fdr_i[1][2]) * bs[i0]
@@ -263,19 +309,33 @@ __global__ void k_bc_rows(MhdDev p)
         jd[i0] = d1t;
         jt[i0] = -(d2s - g3 * p.rc[2] * bs[i0]);
     }
+    // ext_outside_potential_with_j (extend_potential_field.f90:169-201): a
+    // potential field above the CMB
+    const int l = (int)g1;
+    for (int k = p.ko + 1; k < p.nr; ++k) {
+        const size_t i = j + J * k, ic = j + J * p.ko;
+        const double pw = powi(p.rc[0] * p.ar1[k], l);
+        bs[i] = bs[ic] * pw;
+        bd[i] = -bs[ic] * pw * g1 * p.ar1[k];
+        bt[i] = 0.0;
+        js[i] = 0.0;
+        jd[i] = 0.0;
+        jt[i] = 0.0;
+    }
 }
 
 // cal_sph_diff_pol_and_rot2 for (u, w) and (B, J) inside the shell.
 __global__ void k_grad_interior(MhdDev p)
 {
-    const size_t ist = (size_t)p.J * (p.ki + 1), ied = (size_t)p.J * p.ko;
+    const size_t ist = (size_t)p.J * (p.kb + 1), ied = (size_t)p.J * p.ko;
     for (size_t i = ist + blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ied;
          i += (size_t)gridDim.x * blockDim.x) {
         const NodeKJ kj = node_kj(p, i);
         const int k = kj.k, j = kj.j;
         const double g3a = p.g3[j] * p.ar2[k];
+        // (u, w) inside the shell, (B, J) from the first point above the centre
 #pragma unroll
-        for (int f = 0; f < 2; ++f) {
+        for (int f = k > p.ki ? 0 : 1; f < 2; ++f) {
             const int src = f == 0 ? kVelo : kMagne, dst = f == 0 ? kVort : kCurrent;
             const double *s = FLD(src), *t = FLD(src + 2);
             const double d1s = dr1(p, s, k, i), d2s = dr2(p, s, k, i), d1t = dr1(p, t, k, i);
@@ -308,14 +368,11 @@ __global__ void k_diffusion(MhdDev p)
         double *wd0 = FLD(kWDiffuse), *wd2 = FLD(kWDiffuse + 2);
         double *bd0 = FLD(kBDiffuse), *bd1 = FLD(kBDiffuse + 1), *bd2 = FLD(kBDiffuse + 2);
         double *td = FLD(kTDiffuse);
+        const double g3a = g3 * p.ar2[k];
+        // vorticity and temperature: the shell
         if (k > p.ki && k < p.ko) {
-            const double g3a = g3 * p.ar2[k];
             wd0[i] = p.coef_d_velo * (dr2(p, w0, k, i) - g3a * w0[i]);
             wd2[i] = p.coef_d_velo * (dr2(p, w2, k, i) - g3a * w2[i]);
-            const double d2s = dr2(p, c0, k, i), d1t = dr1(p, c2, k, i);
-            bd0[i] = -p.coef_d_magne * c2[i];
-            bd1[i] = -p.coef_d_magne * d1t;
-            bd2[i] = p.coef_d_magne * (d2s - g3a * c0[i]);
             const double d1s = dr1(p, t, k, i), d2t = dr2(p, t, k, i);
             td[i] = p.coef_d_temp * (d2t + 2.0 * p.ar1[k] * d1s - g3a * t[i]);
         } else if (k == p.ki) {
@@ -327,11 +384,11 @@ __global__ void k_diffusion(MhdDev p)
             const double dw2 = p.ffl_i[0][2] * w2[i] + p.ffl_i[1][2] * w2[i1] + p.ffl_i[2][2] * w2[i2];
             wd0[i] = p.coef_d_velo * (dw0 - g3 * p.ri[2] * w0[i]);
             wd2[i] = p.coef_d_velo * (dw2 - g3 * p.ri[2] * w2[i]);
-            const double d2s = ((g1 + 1.0) * p.ri[1] * p.fdr_i[0][2] + p.fdr_i[1][2]) * b0[i]
-                             + p.fdr_i[2][2] * b0[i1];
-            const double d2t = p.ffl_i[0][2] * b2[i] + p.ffl_i[1][2] * b2[i1] + p.ffl_i[2][2] * b2[i2];
-            bd0[i] = p.coef_d_magne * (d2s - g3 * p.ri[2] * b0[i]);
-            bd2[i] = p.coef_d_magne * (d2t - g3 * p.ri[2] * b2[i]);
+            if (p.icore && j >= 1 && j <= 3) {
+                // cal_icore_viscous_drag_l1: the viscous torque on the core
+                const double mat0 = p.fdm1_i[0] - 2.0 * p.ar1[k], mat1 = p.fdm1_i[1];
+                wd0[i] = 5.0 * p.coef_d_velo * (1.0 / p.r[k]) * (mat0 * w0[i] + mat1 * w0[i1]);
+            }
             const double fix = p.t_icb[j];
             const double d1t = p.ffl_i[0][1] * fix + p.ffl_i[1][1] * t[i1] + p.ffl_i[2][1] * t[i2];
             const double d2tt = p.ffl_i[0][2] * fix + p.ffl_i[1][2] * t[i1] + p.ffl_i[2][2] * t[i2];
@@ -344,15 +401,37 @@ __global__ void k_diffusion(MhdDev p)
             const double dw2 = p.ffl_c[2][2] * w2[n2] + p.ffl_c[1][2] * w2[n1] + p.ffl_c[0][2] * w2[i];
             wd0[i] = p.coef_d_velo * (dw0 - g3 * p.rc[2] * w0[i]);
             wd2[i] = p.coef_d_velo * (dw2 - g3 * p.rc[2] * w2[i]);
+            const double fix = p.t_cmb[j];
+            const double d1t = p.ffl_c[2][1] * t[n2] + p.ffl_c[1][1] * t[n1] + p.ffl_c[0][1] * fix;
+            const double d2tt = p.ffl_c[2][2] * t[n2] + p.ffl_c[1][2] * t[n1] + p.ffl_c[0][2] * fix;
+            td[i] = p.coef_d_temp * (d2tt + 2.0 * p.rc[1] * d1t - g3 * p.rc[2] * fix);
+        }
+        // magnetic field: from its first point (the centre's neighbour or the ICB)
+        if (k > p.kb && k < p.ko) {
+            const double d2s = dr2(p, c0, k, i), d1t = dr1(p, c2, k, i);
+            bd0[i] = -p.coef_d_magne * c2[i];
+            bd1[i] = -p.coef_d_magne * d1t;
+            bd2[i] = p.coef_d_magne * (d2s - g3a * c0[i]);
+        } else if (k == p.kb && p.icore) {
+            // cal_sph_nod_center_diffuse2
+            const double d2s = p.d2[1] * b0[i] + p.d2[2] * b0[i + J];
+            const double d2t = p.d2[1] * b2[i] + p.d2[2] * b2[i + J];
+            bd0[i] = p.coef_d_magne * (d2s - g3 * p.ar2[0] * b0[i]);
+            bd2[i] = p.coef_d_magne * (d2t - g3 * p.ar2[0] * b2[i]);
+        } else if (k == p.kb) {
+            const size_t i1 = i + J, i2 = i1 + J;
+            const double d2s = ((g1 + 1.0) * p.ri[1] * p.fdr_i[0][2] + p.fdr_i[1][2]) * b0[i]
+                             + p.fdr_i[2][2] * b0[i1];
+            const double d2t = p.ffl_i[0][2] * b2[i] + p.ffl_i[1][2] * b2[i1] + p.ffl_i[2][2] * b2[i2];
+            bd0[i] = p.coef_d_magne * (d2s - g3 * p.ri[2] * b0[i]);
+            bd2[i] = p.coef_d_magne * (d2t - g3 * p.ri[2] * b2[i]);
+        } else if (k == p.ko) {
+            const size_t n1 = i - J, n2 = n1 - J;
             const double d2s = p.fdr_c[0][2] * b0[n1]
                              + (p.fdr_c[1][2] - g1 * p.rc[1] * p.fdr_c[2][2]) * b0[i];
             const double d2t = p.ffl_c[2][2] * b2[n2] + p.ffl_c[1][2] * b2[n1] + p.ffl_c[0][2] * b2[i];
             bd0[i] = p.coef_d_magne * (d2s - g3 * p.rc[2] * b0[i]);
             bd2[i] = p.coef_d_magne * (d2t - g3 * p.rc[2] * b2[i]);
-            const double fix = p.t_cmb[j];
-            const double d1t = p.ffl_c[2][1] * t[n2] + p.ffl_c[1][1] * t[n1] + p.ffl_c[0][1] * fix;
-            const double d2tt = p.ffl_c[2][2] * t[n2] + p.ffl_c[1][2] * t[n1] + p.ffl_c[0][2] * fix;
-            td[i] = p.coef_d_temp * (d2tt + 2.0 * p.rc[1] * d1t - g3 * p.rc[2] * fix);
         }
         if (k >= p.ki && k <= p.ko) FLD(kRotBuoyancy)[i] = p.coef_buo * t[i] * p.r[k];
     }
@@ -524,6 +603,14 @@ __global__ void k_coriolis(MhdDev p)
                    + tw23 * om1 * ws[k2] + tw14 * om0 * sp_dwp_k1 + tw24 * om0 * sp_dwp_k2;
         pol = -p.coef_cor * pol * ar * ar;
         tor = -p.coef_cor * tor * ar * ar;
+        if (p.icore && k == p.ki && j >= 1 && j <= 3) {
+            // inner_core_rot_z_coriolis_rlm (sum_coriolis_terms_rlm.f90:
+            // 316-351): on the core's l = 1 ICB rows only the z rotation's
+            // coupling of the x and y turns remains
+            const double w11 = j == 1 ? ws[base + 3] : ws[base + 1];
+            pol = j == 2 ? 0.0 : 2.0 * p.coef_cor * rr * om0 * w11;
+            if (j == 1) pol = -pol;
+        }
         FLD(kRotCoriolis)[i] = pol;
         FLD(kRotCoriolis + 2)[i] = tor;
     }
@@ -556,14 +643,72 @@ __device__ __forceinline__ void rot_momentum(const MhdDev &p, int src, int k, in
     }
 }
 
+// int_icore_tor_lorentz_l1 (cal_inner_core_rotation.f90:280-314): the
+// Lorentz torque on the core, 5/2 r_i^-3 times the trapezoid integral of the
+// l = 1 toroidal Lorentz force times r^2 from the centre (first interval
+// f(r_1) r_1^3) to the ICB, into the ICB row of the toroidal force (whose
+// wall curl then carries it).  One thread per l = 1 mode, the radial sum in
+// order.
+__global__ void k_icore_torque(MhdDev p)
+{
+    const int j = threadIdx.x + 1;
+    if (j > 3) return;
+    double *lor = FLD(kLorentz + 2);
+    const size_t J = p.J;
+    const double *r = p.r;
+    double sk = lor[j] * (r[0] * r[0] * r[0]);
+    for (int k = 0; k < p.ki; ++k)
+        sk = sk + (lor[j + J * k] * (r[k] * r[k]) + lor[j + J * (k + 1)] * (r[k + 1] * r[k + 1]))
+                * (r[k + 1] - r[k]);
+    const double ar = 1.0 / r[p.ki];
+    lor[j + J * p.ki] = 0.5 * 5.0 * sk * (ar * ar * ar);
+}
+
 __global__ void k_rot_forces(MhdDev p, int temp_roundtrip)
 {
-    for (size_t i = (size_t)p.J * p.ki + blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-         i < (size_t)p.J * (p.ko + 1); i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < p.nnod;
+         i += (size_t)gridDim.x * blockDim.x) {
         const NodeKJ kj = node_kj(p, i);
         const int k = kj.k, j = kj.j;
         const size_t J = p.J;
         const double g1 = p.g1[j], g3 = p.g3[j];
+        // curl of u x B with its rows at the field's ends (const_sph_rotation_uxb:
+        // insulating walls, or cal_sph_nod_center_rot2 above the centre)
+        if (k >= p.kb && k <= p.ko) {
+            const double *v0 = FLD(kVpInduct), *v1 = FLD(kVpInduct + 1), *v2 = FLD(kVpInduct + 2);
+            double in0 = v2[i], in1, in2;
+            if (k == p.kb && p.icore) {
+                in1 = p.d1[1] * v2[i] + p.d1[2] * v2[i + J];
+                const double d2s = p.d2[1] * v0[i] + p.d2[2] * v0[i + J];
+                in2 = -(d2s - g3 * p.ar2[0] * v0[i]);
+            } else if (k == p.kb) {
+                in1 = (g1 + 1.0) * p.ri[1] * v2[i];
+                in2 = 0.0;
+            } else if (k == p.ko) {
+                in1 = -g1 * p.rc[1] * v2[i];
+                in2 = 0.0;
+            } else {
+                in1 = dr1(p, v2, k, i);
+                in2 = -dr1(p, v1, k, i) + g3 * p.ar2[k] * v0[i];
+            }
+            FLD(kInduction)[i] = in0;
+            FLD(kInduction + 1)[i] = in1;
+            FLD(kInduction + 2)[i] = in2;
+        }
+        // trans_per_temp_to_temp_sph then sync_temp_by_per_temp_sph
+        if (temp_roundtrip && j == 0) {
+            double *t = FLD(kTemp);
+            t[i] = t[i] + p.reft0[k];
+            t[i] = t[i] - p.reft0[k];
+        }
+        if (k < p.ki || k > p.ko) {
+            // set_MHD_terms_to_force over every node: outside the shell only
+            // the Coriolis term is nonzero
+            FLD(kForces)[i] = -FLD(kRotInertia)[i] + FLD(kRotCoriolis)[i] + FLD(kRotLorentz)[i];
+            FLD(kForces + 2)[i] = -FLD(kRotInertia + 2)[i] + FLD(kRotCoriolis + 2)[i]
+                                + FLD(kRotLorentz + 2)[i] + FLD(kRotBuoyancy)[i];
+            continue;
+        }
         double ri_p, ri_t, rl_p, rl_t;
         rot_momentum(p, kMAdvect, k, j, i, &ri_p, &ri_t);
         rot_momentum(p, kLorentz, k, j, i, &rl_p, &rl_t);
@@ -571,22 +716,6 @@ __global__ void k_rot_forces(MhdDev p, int temp_roundtrip)
         FLD(kRotInertia + 2)[i] = ri_t;
         FLD(kRotLorentz)[i] = rl_p;
         FLD(kRotLorentz + 2)[i] = rl_t;
-        // curl of u x B with insulating rows (const_sph_rotation_uxb)
-        const double *v0 = FLD(kVpInduct), *v1 = FLD(kVpInduct + 1), *v2 = FLD(kVpInduct + 2);
-        double in0 = v2[i], in1, in2;
-        if (k == p.ki) {
-            in1 = (g1 + 1.0) * p.ri[1] * v2[i];
-            in2 = 0.0;
-        } else if (k == p.ko) {
-            in1 = -g1 * p.rc[1] * v2[i];
-            in2 = 0.0;
-        } else {
-            in1 = dr1(p, v2, k, i);
-            in2 = -dr1(p, v1, k, i) + g3 * p.ar2[k] * v0[i];
-        }
-        FLD(kInduction)[i] = in0;
-        FLD(kInduction + 1)[i] = in1;
-        FLD(kInduction + 2)[i] = in2;
         // divergence of the heat flux (const_sph_scalar_advect)
         const double *h0 = FLD(kHFlux), *h1 = FLD(kHFlux + 1);
         const double gg = fmax(g3, 0.5);
@@ -608,12 +737,6 @@ __global__ void k_rot_forces(MhdDev p, int temp_roundtrip)
         // set_MHD_terms_to_force
         FLD(kForces)[i] = -ri_p + FLD(kRotCoriolis)[i] + rl_p;
         FLD(kForces + 2)[i] = -ri_t + FLD(kRotCoriolis + 2)[i] + rl_t + FLD(kRotBuoyancy)[i];
-        // trans_per_temp_to_temp_sph then sync_temp_by_per_temp_sph
-        if (temp_roundtrip && j == 0) {
-            double *t = FLD(kTemp);
-            t[i] = t[i] + p.reft0[k];
-            t[i] = t[i] - p.reft0[k];
-        }
     }
 }
 
@@ -747,7 +870,14 @@ void coriolis_table(int L, double *cor, int *jk1, int *jk2, int *jl1)
 // Radial finite differences of the grid r[0..nr-1]: the three-point
 // interior weights (set_dr_for_nonequi, nod_r_2nd_fdm_coefs_nonequi) and the
 // one-sided wall formulas (cal_fdm_coefs_4_boundaries.f90).
-void radial_fdm(const double *r, int nr, RadialFdm *out)
+void radial_fdm(const double *r, int nr, RadialFdm *out) { radial_fdm(r, nr, 0, nr - 1, out); }
+
+// set_dr_for_nonequi, nod_r_2nd_fdm_coefs_nonequi
+// (set_radius_func_noequi.f90:173-197: with points below the ICB the first
+// point's lower neighbour is the centre), the wall formulas of
+// cal_fdm_coefs_4_BCs at the points ki and ko, cal_2nd_to_center_fixed_fdm
+// (m_coef_fdm_to_center.f90:73-109), cal_fdm1_coef_fix_fld_ICB.
+void radial_fdm(const double *r, int nr, int ki, int ko, RadialFdm *out)
 {
     RadialFdm &f = *out;
     std::vector<double> dr(nr);
@@ -758,7 +888,7 @@ void radial_fdm(const double *r, int nr, RadialFdm *out)
     dr[0] = r[1] - r[0];
     for (int k = 0; k < nr; ++k) {
         double dp, dn;
-        if (k == 0) { dp = dr[0]; dn = dr[0]; }
+        if (k == 0) { dp = dr[0]; dn = ki > 0 ? r[0] : dr[0]; }
         else if (k == nr - 1) { dp = dr[k]; dn = dr[k]; }
         else { dp = dr[k]; dn = dr[k - 1]; }
         const double a[3][3] = {{1.0, 0.0, 0.0}, {1.0, dp, 0.5 * dp * dp}, {1.0, -dn, 0.5 * dn * dn}};
@@ -767,7 +897,6 @@ void radial_fdm(const double *r, int nr, RadialFdm *out)
         f.d1[3 * k + 0] = m[1][2]; f.d1[3 * k + 1] = m[1][0]; f.d1[3 * k + 2] = m[1][1];
         f.d2[3 * k + 0] = m[2][2]; f.d2[3 * k + 1] = m[2][0]; f.d2[3 * k + 2] = m[2][1];
     }
-    const int ki = 0, ko = nr - 1;
     const double dp1 = r[ki + 1] - r[ki], dp2 = r[ki + 2] - r[ki];
     const double a1[3][3] = {{1, 0, 0}, {1, dp1, dp1 * dp1 / 2.0}, {1, dp2, dp2 * dp2 / 2.0}};
     double m[3][3];
@@ -822,6 +951,64 @@ void radial_fdm(const double *r, int nr, RadialFdm *out)
         vt[nb][2] = mt[2][2];
         if (wall == 1) vt[2][0] = vt[2][1] = vt[2][2] = 1.0;   // unused, as the reference fills it
     }
+    {   // the centre below the first point (value there zero)
+        const double dn = r[0], dp = r[1] - r[0];
+        const double ac[3][3] = {{1, 0, 0}, {1, -dn, dn * dn / 2.0}, {1, dp, dp * dp / 2.0}};
+        double mc[3][3];
+        inverse33(ac, mc);
+        for (int q = 0; q < 3; ++q) {
+            f.ctr1[0][q] = mc[q][1];
+            f.ctr1[1][q] = mc[q][0];
+            f.ctr1[2][q] = mc[q][2];
+        }
+    }
+    f.fdm1_i[0] = -1.0 / dp1;
+    f.fdm1_i[1] = 1.0 / dp1;
+}
+
+// The boundary model a context supports (checked before any device work):
+// rigid or free-slip walls with insulating magnetic walls on a grid that ends
+// at the ICB and CMB, or a rotatable conducting inner core below the ICB with
+// the magnetic field solved to the centre, optionally with points above the
+// CMB.
+int mhd_check_model(int nr, const double *radius, const MhdCoefs &co, std::string *err)
+{
+    if (co.bc_velo_cmb != kVeloRigid && co.bc_velo_cmb != kVeloFreeSlip) {
+        *err = "mhd_create: CMB velocity boundary kind " + std::to_string(co.bc_velo_cmb)
+             + " is not supported (0 rigid, 1 free slip)";
+        return -1;
+    }
+    if (co.bc_velo_icb != kVeloRigid && co.bc_velo_icb != kVeloFreeSlip &&
+        co.bc_velo_icb != kVeloRotInnerCore) {
+        *err = "mhd_create: ICB velocity boundary kind " + std::to_string(co.bc_velo_icb)
+             + " is not supported (0 rigid, 1 free slip, 2 rotatable inner core)";
+        return -1;
+    }
+    if (co.bc_magne_icb != kMagneInsulator && co.bc_magne_icb != kMagneToCenter) {
+        *err = "mhd_create: magnetic boundary kind " + std::to_string(co.bc_magne_icb)
+             + " is not supported (0 insulating ICB, 1 field to the centre)";
+        return -1;
+    }
+    const int ki = co.kr_icb, ko = co.kr_cmb < 0 ? nr - 1 : co.kr_cmb;
+    if (ki < 0 || ko > nr - 1 || ko - ki < 4) {
+        *err = "mhd_create: ICB / CMB points out of the grid or fewer than 5 points between";
+        return -1;
+    }
+    // the rotatable inner core and the field to the centre come together, as
+    // in Dynamobench case 2; points outside the shell only with them
+    const bool icore = co.bc_velo_icb == kVeloRotInnerCore;
+    if (icore != (co.bc_magne_icb == kMagneToCenter) || icore != (ki > 0) ||
+        (!icore && ko != nr - 1)) {
+        *err = "mhd_create: points below the ICB need the rotatable inner core with the "
+               "magnetic field solved to the centre (and these need such points); points "
+               "above the CMB are supported only with them";
+        return -1;
+    }
+    if (icore && !(radius[0] > 0.0)) {
+        *err = "mhd_create: the first point must lie above the centre";
+        return -1;
+    }
+    return 0;
 }
 
 int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *radius,
@@ -835,12 +1022,9 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
         *err = "mhd_create: nr (L+1)^2 must stay below 2^31 spectral nodes";
         return -1;
     }
-    for (int bcv : {co.bc_velo_icb, co.bc_velo_cmb})
-        if (bcv != kVeloRigid && bcv != kVeloFreeSlip) {
-            *err = "mhd_create: velocity boundary kind " + std::to_string(bcv)
-                 + " is not supported (0 rigid, 1 free slip)";
-            return -1;
-        }
+    if (mhd_check_model(nr, radius, co, err)) return -1;
+    const int ki = co.kr_icb, ko = co.kr_cmb < 0 ? nr - 1 : co.kr_cmb;
+    const bool icore = co.bc_velo_icb == kVeloRotInnerCore;
     c->L = L;
     c->nth = nth;
     c->nphi = nphi;
@@ -853,14 +1037,14 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     c->radius.assign(radius, radius + nr);
     const int J = c->J;
     const size_t nnod = (size_t)nr * J;
-    const int ki = 0, ko = nr - 1;
+    const int kb = icore ? 0 : ki;   // the magnetic field's first point
     std::vector<double> r(radius, radius + nr), ar1(nr), ar2(nr);
     for (int k = 0; k < nr; ++k) {
         ar1[k] = 1.0 / r[k];
         ar2[k] = ar1[k] * ar1[k];
     }
     RadialFdm fd;
-    radial_fdm(r.data(), nr, &fd);
+    radial_fdm(r.data(), nr, ki, ko, &fd);
     const std::vector<double> &d1 = fd.d1, &d2 = fd.d2;
     MhdDev &p = c->p;
     p = MhdDev();
@@ -869,6 +1053,10 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     p.L = L;
     p.ki = ki;
     p.ko = ko;
+    p.kb = kb;
+    p.icore = icore ? 1 : 0;
+    p.fdm1_i[0] = fd.fdm1_i[0];
+    p.fdm1_i[1] = fd.fdm1_i[1];
     p.nnod = nnod;
     p.rJ = 1.0 / (double)J;
     std::memcpy(p.ffl_i, fd.ffl_i, sizeof p.ffl_i);
@@ -916,12 +1104,18 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     {
         const double dh = co.depth_high_t, dl = co.depth_low_t, hi = co.high_temp, lo = co.low_temp;
         for (int k = 0; k < nr; ++k) {
+            if (k < ki || k > ko) {   // the high temperature below, the low one above
+                reft0[k] = k < ki ? hi : lo;
+                reft1[k] = 0.0;
+                continue;
+            }
             reft0[k] = (dh * dl * ar1[k] * (hi - lo) - dh * hi + dl * lo) / (dl - dh);
             reft1[k] = -dh * dl * ar2[k] * (hi - lo) / (dl - dh);
         }
         t_icb[0] = hi - reft0[ki];
         t_cmb[0] = lo - reft0[ko];
     }
+    c->reft0 = reft0;
     std::vector<int> jk1(J), jk2(J), jl1(J);
     std::vector<double> cor(12 * (size_t)J, 0.0);
     coriolis_table(L, cor.data(), jk1.data(), jk2.data(), jl1.data());
@@ -993,8 +1187,8 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
             M3(bt, j, k, 1) = 1.0;
             M3(vs, j, k, 1) = (k < ki || k > ko) ? 1.0 : 0.0;
         }
-    auto add_vector_poisson = [&](std::vector<double> &m, double coef) {
-        for (int k = ki + 1; k < ko; ++k)
+    auto add_vector_poisson = [&](std::vector<double> &m, double coef, int k_lo) {
+        for (int k = k_lo + 1; k < ko; ++k)
             for (int j = 0; j < J; ++j) {
                 M3(m, j, k - 1, 2) = M3(m, j, k - 1, 2) - coef * d2[3 * k + 0];
                 M3(m, j, k, 1) = M3(m, j, k, 1) - coef * (d2[3 * k + 1] - g3[j] * ar2[k]);
@@ -1039,9 +1233,9 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
         }
     };
     const double coef_dvt = co.coef_imp_v * co.coef_d_velo * co.dt;
-    add_vector_poisson(vt, coef_dvt);
-    add_vector_poisson(wt, coef_dvt);
-    add_vector_poisson(vs, 1.0);
+    add_vector_poisson(vt, coef_dvt, ki);
+    add_vector_poisson(wt, coef_dvt, ki);
+    add_vector_poisson(vs, 1.0, ki);
     // walls (const_radial_mat_vort_2step, const_r_mat_4_vector_sph.f90:85-135)
     add_fix_flux_icb(wt, coef_dvt, p.fdr_i);
     if (p.free_i) {
@@ -1059,10 +1253,24 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
         set_fix_fld_cmb(vt);
         add_fix_flux_cmb(vs, 1.0, p.fdr_c);
     }
-    // poloidal velocity: 5-band product wt * vs (cal_mat_product_3band_mul)
+    if (icore) {
+        // set_rotate_icb_vt_sph_mat (cal_inner_core_rotation.f90:185-208): the
+        // l = 1 toroidal ICB rows carry the viscous torque on the core
+        const double ar = 1.0 / r[ki];
+        for (int j = 1; j <= 3; ++j) {
+            M3(vt, j, ki, 1) = 1.0 - co.coef_imp_v * co.dt * co.coef_d_velo * 5.0
+                                   * (fd.fdm1_i[0] - 2.0 * ar1[ki]) * ar;
+            M3(vt, j, ki + 1, 0) = -(co.coef_imp_v * co.dt * co.coef_d_velo * 5.0 * ar
+                                     * fd.fdm1_i[1]);
+        }
+    }
+    // poloidal velocity: 5-band product wt * vs (cal_mat_product_3band_mul),
+    // the unit matrix outside the shell
     std::vector<double> vp(5 * nnod, 0.0);
     {
         auto P5 = [&](int j, int k, int b) -> double & { return vp[((size_t)j * nr + k) * 5 + b]; };
+        for (int j = 0; j < J; ++j)
+            for (int k = 0; k < nr; ++k) P5(j, k, 2) = 1.0;
         auto Lm = [&](int j, int k, int b) { return M3(wt, j, k, b); };
         auto Rm = [&](int j, int k, int b) { return M3(vs, j, k, b); };
         auto rows = [&](int j, int k, bool skip5, bool skip1) {
@@ -1091,19 +1299,28 @@ int mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *ra
     set_fix_fld_icb(tm);
     set_fix_fld_cmb(tm);
     const double coef_dbt = co.coef_imp_b * co.coef_d_magne * co.dt;
-    add_vector_poisson(bs, coef_dbt);
-    add_vector_poisson(bt, coef_dbt);
+    add_vector_poisson(bs, coef_dbt, kb);
+    add_vector_poisson(bt, coef_dbt, kb);
     for (int j = 0; j < J; ++j) {
-        M3(bs, j, ki, 1) = M3(bs, j, ki, 1)
-                         - coef_dbt * (p.fdr_i[1][2] - g3[j] * p.ri[2]
-                                       + (g1[j] + 1.0) * p.ri[1] * p.fdr_i[0][2]);
-        M3(bs, j, ki + 1, 0) = M3(bs, j, ki + 1, 0) - coef_dbt * p.fdr_i[2][2];
+        if (icore) {
+            // add_vector_poisson_mat_center (center_sph_matrices.f90:160-180)
+            const double rc2 = ar2[0];
+            for (std::vector<double> *m : {&bs, &bt}) {
+                M3(*m, j, 0, 1) = M3(*m, j, 0, 1) - coef_dbt * (fd.ctr1[1][2] - g3[j] * rc2);
+                M3(*m, j, 1, 0) = M3(*m, j, 1, 0) - coef_dbt * fd.ctr1[2][2];
+            }
+        } else {
+            M3(bs, j, ki, 1) = M3(bs, j, ki, 1)
+                             - coef_dbt * (p.fdr_i[1][2] - g3[j] * p.ri[2]
+                                           + (g1[j] + 1.0) * p.ri[1] * p.fdr_i[0][2]);
+            M3(bs, j, ki + 1, 0) = M3(bs, j, ki + 1, 0) - coef_dbt * p.fdr_i[2][2];
+        }
         M3(bs, j, ko - 1, 2) = M3(bs, j, ko - 1, 2) - coef_dbt * p.fdr_c[0][2];
         M3(bs, j, ko, 1) = M3(bs, j, ko, 1)
                          - coef_dbt * (p.fdr_c[1][2] - g3[j] * p.rc[2]
                                        - g1[j] * p.rc[1] * p.fdr_c[2][2]);
     }
-    set_fix_fld_icb(bt);
+    if (!icore) set_fix_fld_icb(bt);
     set_fix_fld_cmb(bt);
     // LU on the GPU (ludcmp_band / ludcmp_3band)
     const std::vector<double> *mats[5] = {&vp, &vt, &tm, &bs, &bt};
@@ -1328,17 +1545,23 @@ int mhd_snapshot_rtp(MhdContext *c, int nvec, const int *vec, int nscl, const in
 // dynamo_benchmark_1 initial fields (set_initial_velo_sph,
 // set_ini_reference_temp_sph, set_initial_temp_sph,
 // initial_b_dynamobench_1): full temperature, as a restart holds it.
-int mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err)
+namespace {
+
+// The benchmark start (sph_initial_data_control, set_initial_sph_dynamo.f90:
+// set_initial_velo_sph, set_ini_reference_temp_sph, set_initial_temp_sph,
+// initial_b_dynamobench_1 / _2): zero velocity, the full temperature (the
+// reference profile at every point plus the (m, m) perturbation of isig in the
+// shell), as a restart holds it, and benchmark 1's field in the shell or
+// benchmark 2's from the first point to the CMB.
+int set_benchmark_ic(MhdContext *c, int isig, int bench, std::string *err)
 {
-    const int nr = c->nr, J = c->J;
+    const int nr = c->nr, J = c->J, ki = c->p.ki, ko = c->p.ko;
     const size_t n = (size_t)nr * J;
     const double pi = 4.0 * std::atan(1.0);
     const double *r = c->radius.data();
-    const double r_icb = r[0], r_cmb = r[nr - 1], shell = r_cmb - r_icb;
-    const MhdCoefs &co = c->co;
+    const double r_icb = r[ki], r_cmb = r[ko], shell = r_cmb - r_icb;
     std::vector<double> zero(n, 0.0), temp(n, 0.0), bp(n, 0.0), bd(n, 0.0), bt(n, 0.0),
         jp(n, 0.0), jd(n, 0.0), jt(n, 0.0);
-    const double dh = co.depth_high_t, dl = co.depth_low_t, hi = co.high_temp, lo = co.low_temp;
     const int m = (isig % 1000) / 100;
     const int jj = m * (m + 1) + m;
     if (jj >= J) {
@@ -1346,20 +1569,31 @@ int mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err)
         return -1;
     }
     for (int k = 0; k < nr; ++k) {
-        const double ar1 = 1.0 / r[k];
-        temp[(size_t)J * k] = (dh * dl * ar1 * (hi - lo) - dh * hi + dl * lo) / (dl - dh);
-        const double xr = 2.0 * r[k] - 1.0 * (r_cmb + r_icb) / shell;
-        temp[jj + (size_t)J * k] = (1.0 - 3.0 * (xr * xr) + 3.0 * std::pow(xr, 4) - std::pow(xr, 6))
-                                 * 0.1 * 3.0 / std::sqrt(2.0 * pi);
+        temp[(size_t)J * k] = c->reft0[k];
         const double rr = r[k];
         const size_t is = 2 + (size_t)J * k, it = 6 + (size_t)J * k;
-        bp[is] = (5.0 / 8.0) * (-3.0 * std::pow(rr, 3) + (4.0 * r_cmb) * (rr * rr)
-                                - std::pow(r_icb, 4) / rr);
-        bd[is] = (5.0 / 8.0) * (-9.0 * (rr * rr) + 8.0 * r_cmb * rr + std::pow(r_icb, 4) / (rr * rr));
-        jt[is] = (5.0 * 3.0 / 2.0) * rr;
-        bt[it] = (10.0 / 3.0) * rr * std::sin(pi * (rr - r_icb));
-        jp[it] = bt[it];
-        jd[it] = (10.0 / 3.0) * (std::sin(pi * (rr - r_icb)) + pi * rr * std::cos(pi * (rr - r_icb)));
+        if (k >= ki && k <= ko) {
+            const double xr = 2.0 * r[k] - 1.0 * (r_cmb + r_icb) / shell;
+            temp[jj + (size_t)J * k] = (1.0 - 3.0 * (xr * xr) + 3.0 * std::pow(xr, 4) - std::pow(xr, 6))
+                                     * 0.1 * 3.0 / std::sqrt(2.0 * pi);
+        }
+        if (bench == 1 && k >= ki && k <= ko) {
+            bp[is] = (5.0 / 8.0) * (-3.0 * std::pow(rr, 3) + (4.0 * r_cmb) * (rr * rr)
+                                    - std::pow(r_icb, 4) / rr);
+            bd[is] = (5.0 / 8.0) * (-9.0 * (rr * rr) + 8.0 * r_cmb * rr + std::pow(r_icb, 4) / (rr * rr));
+            jt[is] = (5.0 * 3.0 / 2.0) * rr;
+            bt[it] = (10.0 / 3.0) * rr * std::sin(pi * (rr - r_icb));
+            jp[it] = bt[it];
+            jd[it] = (10.0 / 3.0) * (std::sin(pi * (rr - r_icb)) + pi * rr * std::cos(pi * (rr - r_icb)));
+        } else if (bench == 2 && k <= ko) {
+            bp[is] = (5.0 / 2.0) * (rr * rr) * (4.0 * r_cmb - 3.0 * rr) / (r_cmb + 3.0);
+            bd[is] = (5.0 / 2.0) * rr * (8.0 * r_cmb - 9.0 * rr) / (r_cmb + 3.0);
+            jt[is] = 5.0 * 6.0 * rr / (3.0 + r_cmb);
+            const double sn = std::sin(pi * rr / r_cmb), cs = std::cos(pi * rr / r_cmb);
+            bt[it] = (10.0 / 3.0) * rr * sn;
+            jp[it] = bt[it];
+            jd[it] = (10.0 / 3.0) * (sn + (pi / r_cmb) * rr * cs);
+        }
     }
     const std::pair<int, const std::vector<double> *> sets[] = {
         {kVelo, &zero}, {kVelo + 1, &zero}, {kVelo + 2, &zero}, {kTemp, &temp},
@@ -1368,6 +1602,22 @@ int mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err)
     for (const auto &sv : sets)
         if (mhd_field_io(c, sv.first, const_cast<double *>(sv.second->data()), false, err)) return -1;
     return 0;
+}
+
+}  // namespace
+
+int mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err)
+{
+    return set_benchmark_ic(c, isig, 1, err);
+}
+
+int mhd_set_dynamobench2_ic(MhdContext *c, int isig, std::string *err)
+{
+    if (!c->p.icore) {
+        *err = "dynamo_benchmark_2 fields need the inner core (points below the ICB)";
+        return -1;
+    }
+    return set_benchmark_ic(c, isig, 2, err);
 }
 
 // Seeded random initial fields: poloidal and toroidal velocity and magnetic
@@ -1381,6 +1631,10 @@ int mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err)
 int mhd_set_seeded_ic(MhdContext *c, unsigned long long seed, double amp_u, double amp_b,
                       double amp_t, std::string *err)
 {
+    if (c->p.icore) {
+        *err = "seeded fields are defined for a shell without an inner core";
+        return -1;
+    }
     const int nr = c->nr, J = c->J, L = c->L;
     const size_t n = (size_t)nr * J;
     const double *r = c->radius.data();
@@ -1390,13 +1644,7 @@ int mhd_set_seeded_ic(MhdContext *c, unsigned long long seed, double amp_u, doub
     seeded_spectrum(seed, kMagne, L, nr, r, amp_b, 2.0, true, 1, bp.data());
     seeded_spectrum(seed, kMagne + 2, L, nr, r, amp_b, 2.0, true, 1, bt.data());
     seeded_spectrum(seed, kTemp, L, nr, r, amp_t, 2.0, true, 0, temp.data());
-    const MhdCoefs &co = c->co;
-    const double dh = co.depth_high_t, dl = co.depth_low_t, hi = co.high_temp, lo = co.low_temp;
-    for (int k = 0; k < nr; ++k) {
-        const double ar1 = 1.0 / r[k];
-        const double ref = (dh * dl * ar1 * (hi - lo) - dh * hi + dl * lo) / (dl - dh);
-        temp[(size_t)J * k] = ref + temp[(size_t)J * k];
-    }
+    for (int k = 0; k < nr; ++k) temp[(size_t)J * k] = c->reft0[k] + temp[(size_t)J * k];
     const std::pair<int, const std::vector<double> *> sets[] = {
         {kVelo, &up}, {kVelo + 1, &zero}, {kVelo + 2, &ut}, {kTemp, &temp},
         {kMagne, &bp}, {kMagne + 1, &zero}, {kMagne + 2, &bt},
@@ -1436,6 +1684,10 @@ int nonlinear(MhdContext *c, bool temp_roundtrip, std::string *err)
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_coriolis, dim3(nb), dim3(256), 0, s, p);
     HIP_TRY(hipGetLastError());
+    if (p.icore) {
+        hipLaunchKernelGGL(k_icore_torque, dim3(1), dim3(64), 0, s, p);
+        HIP_TRY(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_rot_forces, dim3(nb), dim3(256), 0, s, p, temp_roundtrip ? 1 : 0);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1464,13 +1716,7 @@ int mhd_initialize(MhdContext *c, std::string *err)
     // sync_temp_by_per_temp_sph: the evolved temperature is the perturbation
     std::vector<double> t(p.nnod);
     if (mhd_field_io(c, kTemp, t.data(), true, err)) return -1;
-    const double dh = c->co.depth_high_t, dl = c->co.depth_low_t;
-    const double hi = c->co.high_temp, lo = c->co.low_temp;
-    for (int k = 0; k < p.nr; ++k) {
-        const double ar1 = 1.0 / c->radius[k];
-        const double ref = (dh * dl * ar1 * (hi - lo) - dh * hi + dl * lo) / (dl - dh);
-        t[(size_t)p.J * k] = t[(size_t)p.J * k] - ref;
-    }
+    for (int k = 0; k < p.nr; ++k) t[(size_t)p.J * k] = t[(size_t)p.J * k] - c->reft0[k];
     if (mhd_field_io(c, kTemp, t.data(), false, err)) return -1;
     if (derived_fields(c, err)) return -1;
     if (nonlinear(c, false, err)) return -1;

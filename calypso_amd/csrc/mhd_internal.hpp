@@ -1,7 +1,8 @@
 // mhd_internal.hpp -- device-resident MHD time step (rotating Boussinesq
 // shell, rigid or free-slip walls, insulating magnetic boundaries,
 // fixed-temperature boundaries with the spherical-shell reference
-// temperature).
+// temperature; or a rotatable conducting inner core below the ICB with the
+// magnetic field solved to the centre and points above the CMB).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,16 +35,26 @@ enum MhdField : int {
 // velocity, [pt + 1][c] with pt -1 / 0 / +1 the point below / at / above the
 // wall and c = 0 value, 1 d/dr, 2 d2/dr2; powers r, 1/r, 1/r^2 of the wall
 // radii.
+// The walls are the points ki (ICB) and ko (CMB); with points below the ICB
+// the first point's lower neighbour is the centre (distance r[0]), ctr1 is
+// the centre formula at the first point ([pt + 1][c], pt -1 the centre) and
+// fdm1_i the two-point d/dr at the ICB.
 struct RadialFdm {
     std::vector<double> d1, d2;
     double ffl_i[3][3], fdr_i[3][3], ffl_c[3][3], fdr_c[3][3];
     double fvp_i[3][3], fvt_i[3][3], fvp_c[3][3], fvt_c[3][3];
+    double ctr1[3][3], fdm1_i[2];
     double ri[3], rc[3];
 };
 
-// velocity wall conditions (bc_velocity of the reference's control file)
-enum MhdVeloBC : int { kVeloRigid = 0, kVeloFreeSlip = 1 };
+// velocity wall conditions (bc_velocity of the reference's control file):
+// non_slip_sph, free_slip_sph, rot_inner_core (ICB only)
+enum MhdVeloBC : int { kVeloRigid = 0, kVeloFreeSlip = 1, kVeloRotInnerCore = 2 };
+// magnetic condition below the first shell point (bc_magnetic_field):
+// insulator at the ICB, or sph_to_center (field solved to the centre)
+enum MhdMagneBC : int { kMagneInsulator = 0, kMagneToCenter = 1 };
 void radial_fdm(const double *r, int nr, RadialFdm *out);
+void radial_fdm(const double *r, int nr, int ki, int ko, RadialFdm *out);
 
 struct MhdCoefs {
     double dt = 0;
@@ -53,11 +64,16 @@ struct MhdCoefs {
     double coef_magne = 1, coef_d_magne = 1, coef_induct = 1;
     double depth_high_t = 0, depth_low_t = 0, high_temp = 0, low_temp = 0;
     int bc_velo_icb = kVeloRigid, bc_velo_cmb = kVeloRigid;
+    int kr_icb = 0, kr_cmb = -1;  // 0-based ICB / CMB points (kr_cmb -1: the last)
+    int bc_magne_icb = kMagneInsulator;
 };
 
 // Everything a kernel needs, passed by value.
 struct MhdDev {
     int nr = 0, J = 0, ki = 0, ko = 0, L = 0;
+    int kb = 0;                                // first point of the magnetic field
+    int icore = 0;                             // rotatable inner core, B to the centre
+    double fdm1_i[2];                          // two-point d/dr at the ICB
     double rJ = 0.0;                           // 1 / J
     size_t nnod = 0;
     double *d = nullptr;                       // d_rj, kMhdNumComp * nnod
@@ -89,6 +105,7 @@ struct MhdContext {
     double restart_time = 0.0;
     MhdCoefs co;
     std::vector<double> radius;
+    std::vector<double> reft0;    // reference temperature at every point
     LegContext leg;
     FftContext fft;
     MhdDev p;
@@ -112,10 +129,14 @@ struct MhdContext {
     double *d_ms_out = nullptr;
 };
 
+// the boundary model the step supports, checked before any device work
+int  mhd_check_model(int nr, const double *radius, const MhdCoefs &co, std::string *err);
 int  mhd_create(MhdContext *c, int L, int nth, int nphi, int nr, const double *radius,
                 const MhdCoefs &co, int leg_mode, hipStream_t s, std::string *err);
 void mhd_free(MhdContext *c);
 int  mhd_set_dynamobench_ic(MhdContext *c, int isig, std::string *err);
+// dynamo_benchmark_2: the same temperature, the magnetic field to the centre
+int  mhd_set_dynamobench2_ic(MhdContext *c, int isig, std::string *err);
 int  mhd_set_seeded_ic(MhdContext *c, unsigned long long seed, double amp_u, double amp_b,
                        double amp_t, std::string *err);
 int  mhd_initialize(MhdContext *c, std::string *err);

@@ -23,9 +23,14 @@
         real(c_double) :: depth_high_t, depth_low_t
         real(c_double) :: high_temp, low_temp
         integer(c_int) :: bc_velo_icb, bc_velo_cmb
+        integer(c_int) :: kr_icb, kr_cmb
+        integer(c_int) :: bc_magne_icb
       end type calypso_mhd_params
-      integer(c_int), parameter :: CALYPSO_VELO_RIGID =     0
-      integer(c_int), parameter :: CALYPSO_VELO_FREE_SLIP = 1
+      integer(c_int), parameter :: CALYPSO_VELO_RIGID =          0
+      integer(c_int), parameter :: CALYPSO_VELO_FREE_SLIP =      1
+      integer(c_int), parameter :: CALYPSO_VELO_ROT_INNER_CORE = 2
+      integer(c_int), parameter :: CALYPSO_MAGNE_INSULATOR =     0
+      integer(c_int), parameter :: CALYPSO_MAGNE_TO_CENTER =     1
 !
 !>    calypso_scalar_bc of include/calypso_amd.h (same field order)
       integer(c_int), parameter :: CALYPSO_BC_FIXED_VALUE = 0
@@ -187,6 +192,12 @@
           type(c_ptr), value :: mhd
           integer(c_int), value :: isig
         end function calypso_mhd_set_dynamobench_ic
+        integer(c_int) function calypso_mhd_set_dynamobench2_ic(mhd,     &
+     &      isig) bind(C, name="calypso_mhd_set_dynamobench2_ic")
+          import :: c_int, c_ptr
+          type(c_ptr), value :: mhd
+          integer(c_int), value :: isig
+        end function calypso_mhd_set_dynamobench2_ic
 !
         integer(c_int) function calypso_mhd_initialize(mhd)             &
      &      bind(C, name="calypso_mhd_initialize")

@@ -65,6 +65,9 @@
       p%low_temp = 0.0d0
       p%bc_velo_icb = CALYPSO_VELO_RIGID
       p%bc_velo_cmb = CALYPSO_VELO_RIGID
+      p%kr_icb = 0
+      p%kr_cmb = 0
+      p%bc_magne_icb = CALYPSO_MAGNE_INSULATOR
 !
       dummy = 0.0d0
       dtemp = 0.0d0

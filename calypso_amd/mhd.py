@@ -42,11 +42,15 @@ class MhdParams(ctypes.Structure):
                 ("coef_induct", ctypes.c_double),
                 ("depth_high_t", ctypes.c_double), ("depth_low_t", ctypes.c_double),
                 ("high_temp", ctypes.c_double), ("low_temp", ctypes.c_double),
-                ("bc_velo_icb", ctypes.c_int), ("bc_velo_cmb", ctypes.c_int)]
+                ("bc_velo_icb", ctypes.c_int), ("bc_velo_cmb", ctypes.c_int),
+                ("kr_icb", ctypes.c_int), ("kr_cmb", ctypes.c_int),
+                ("bc_magne_icb", ctypes.c_int)]
 
 
-# velocity walls (bc_velocity: non_slip_sph, free_slip_sph)
-VELO_RIGID, VELO_FREE_SLIP = 0, 1
+# velocity walls (bc_velocity: non_slip_sph, free_slip_sph, rot_inner_core)
+VELO_RIGID, VELO_FREE_SLIP, VELO_ROT_INNER_CORE = 0, 1, 2
+# magnetic condition below the shell (bc_magnetic_field: insulator, sph_to_center)
+MAGNE_INSULATOR, MAGNE_TO_CENTER = 0, 1
 
 
 def chebyshev_radius(nri: int, r_icb: float, r_cmb: float) -> np.ndarray:
@@ -58,6 +62,60 @@ def chebyshev_radius(nri: int, r_icb: float, r_cmb: float) -> np.ndarray:
                      for k in range(nri + 1)])
 
 
+def chebyshev_radius_ext(nri: int, r_icb: float, r_cmb: float, r_min: float, r_max: float):
+    """count_chebyshev_ext_layers then set_chebyshev_distance_shell
+    (set_radial_grid_sph_shell.f90:60-109, :171-229), the grid gen_sph_grids
+    writes for Min_radius / Max_radius: nri Chebyshev intervals in the shell,
+    the mirrored Chebyshev spacing below the ICB down towards r_min (then
+    evenly spaced) and above the CMB up to r_max.  Returns (radius, kr_icb,
+    kr_cmb), the boundary points 1-based as in the reference's groups."""
+    pi = 4.0 * math.atan(1.0)
+    shell = r_cmb - r_icb
+    half = nri // 2
+
+    def below(k):
+        return r_icb - 0.5 * shell * (1.0 - math.cos(pi * float(k) / float(nri)))
+
+    def above(k):
+        return r_cmb + 0.5 * shell * (1.0 - math.cos(pi * float(k) / float(nri)))
+
+    r, k, dr = r_icb, 0, 0.5 * shell * (1.0 - math.cos(pi / float(nri)))
+    while r > r_min and k < half:
+        k += 1
+        r = below(k)
+        dr = 0.5 * shell * (math.cos(pi * float(k - 1) / float(nri)) - math.cos(pi * float(k) / float(nri)))
+    if k == half:
+        k = int((r - r_min) / dr) + half
+    n_icore = max(k - 1, 0)
+    r, k = r_cmb, 0
+    while r < r_max and k < half:
+        k += 1
+        r = above(k)
+        dr = 0.5 * shell * (-math.cos(pi * float(k) / float(nri)) + math.cos(pi * float(k - 1) / float(nri)))
+    while r < r_max:
+        k += 1
+        r = r + dr
+    n_ext = 0 if k <= 1 else k
+    k_icb = n_icore + 1
+    k_cmb = k_icb + nri
+    n = k_cmb + n_ext
+    out = np.zeros(n)
+    for q in range(1, k_icb - half):
+        out[q - 1] = (r_icb - 0.5 * shell) * float(q) / float(k_icb - half)
+    for q in range(max(1, k_icb - half), k_icb):
+        out[q - 1] = below(k_icb - q)
+    for q in range(k_icb, k_cmb + 1):
+        out[q - 1] = r_icb + 0.5 * shell * (1.0 - math.cos(pi * float(q - k_icb) / float(nri)))
+    top = min(n, k_cmb + half)
+    for q in range(k_cmb + 1, top + 1):
+        out[q - 1] = above(q - k_cmb)
+    if top > k_cmb:
+        dr = out[top - 1] - out[top - 2]
+    for q in range(k_cmb + half + 1, n + 1):
+        out[q - 1] = out[q - 2] + dr
+    return out, k_icb, k_cmb
+
+
 def dynamobench_case1(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6,
                       legendre_mode="hip_fp64", Ra=1.0e2, bc_velo=("non_slip_sph", "non_slip_sph")):
     """Parameters of tests/Dynamobench_case1 (control_MHD, control_sph_shell):
@@ -66,8 +124,9 @@ def dynamobench_case1(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_
     Crank-Nicolson weight and Rayleigh number can be changed, and the
     velocity walls (ICB, CMB) set to "non_slip_sph" or "free_slip_sph"."""
     E, Pr, Pm = 1.0e-3, 1.0, 5.0
-    r_cmb = 1.0 / (1.0 - 0.35)
-    r_icb = r_cmb * 0.35
+    # set_ctl_gen_shell_grids.f90:179-185 (fluid_core_size 1, ratio 0.35)
+    r_icb = 1.0 * 0.35 / (1.0 - 0.35)
+    r_cmb = r_icb + 1.0
     p = MhdParams()
     p.l_truncation, p.nth, p.nphi, p.nr = l_truncation, nth, nphi, nri + 1
     p.legendre_mode = set_legendre_trans_mode_ctl(legendre_mode)
@@ -94,6 +153,25 @@ def dynamobench_case1(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_
                                     "(non_slip_sph, free_slip_sph)")
     p.bc_velo_icb, p.bc_velo_cmb = kinds[bc_velo[0]], kinds[bc_velo[1]]
     return p, chebyshev_radius(nri, r_icb, r_cmb)
+
+
+def dynamobench_case2(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_imp=0.6,
+                      legendre_mode="hip_fp64", Ra=1.1e2, r_min=0.0, r_max=2.0):
+    """Parameters of tests/Dynamobench_case2: case 1's shell and numbers with
+    modified Ra = 110, the grid extended to Min_radius 0 and Max_radius 2
+    (24 points in the inner core, 49 in the shell, 23 above the CMB for the
+    default grid), a rotatable inner core (bc_velocity rot_inner_core) and the
+    magnetic field solved to the centre (sph_to_center); returns
+    (MhdParams, radius)."""
+    p, _ = dynamobench_case1(l_truncation, nth, nphi, nri, dt, coef_imp, legendre_mode, Ra)
+    r_icb = 1.0 * 0.35 / (1.0 - 0.35)
+    r_cmb = r_icb + 1.0
+    radius, kr_icb, kr_cmb = chebyshev_radius_ext(nri, r_icb, r_cmb, r_min, r_max)
+    p.nr = radius.size
+    p.kr_icb, p.kr_cmb = kr_icb, kr_cmb
+    p.bc_velo_icb = VELO_ROT_INNER_CORE
+    p.bc_magne_icb = MAGNE_TO_CENTER
+    return p, radius
 
 
 class MhdSimulation:
@@ -150,6 +228,11 @@ class MhdSimulation:
     def set_dynamobench_ic(self, isig=400):
         _lib.check(_lib.load().calypso_mhd_set_dynamobench_ic(self._h, isig),
                    "calypso_mhd_set_dynamobench_ic")
+
+    def set_dynamobench2_ic(self, isig=400):
+        """dynamo_benchmark_2: initial_b_dynamobench_2's field to the centre."""
+        _lib.check(_lib.load().calypso_mhd_set_dynamobench2_ic(self._h, isig),
+                   "calypso_mhd_set_dynamobench2_ic")
 
     def set_rj_order(self, j_global):
         """Take and give d_rj (set_field, get_field, restart files) in the

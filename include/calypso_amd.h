@@ -301,13 +301,24 @@ int calypso_sph_rj_mode_order(int l_truncation, int *j_global);
  * walls around the spherical-shell reference temperature (the model of the
  * reference's Dynamobench_case1).  The coefficients are those module
  * m_physical_property holds after set_material_property
- * (material_property.f90); radius[nr] is the radial grid, the inner and
- * outer walls at its ends.  bc_velo_icb / bc_velo_cmb choose each velocity
- * wall as the reference's bc_velocity does (non_slip_sph, free_slip_sph;
- * set_bc_flag_sph_velo.f90, set_bc_sph_mhd.f90:47-57); any other value is
- * rejected by calypso_mhd_create. */
-#define CALYPSO_VELO_RIGID     0
-#define CALYPSO_VELO_FREE_SLIP 1
+ * (material_property.f90); radius[nr] is the radial grid.  kr_icb and
+ * kr_cmb are the 1-based points of the reference's ICB and CMB radial groups
+ * (0: the grid's first and last points).  bc_velo_icb / bc_velo_cmb choose
+ * each velocity wall as the reference's bc_velocity does (non_slip_sph,
+ * free_slip_sph, and at the ICB rot_inner_core; set_bc_flag_sph_velo.f90,
+ * set_bc_sph_mhd.f90:47-57); bc_magne_icb the magnetic condition below the
+ * shell (insulator, or sph_to_center: set_bc_sph_mhd.f90:147-197).  Points
+ * below the ICB come with the rotatable inner core and the field solved to
+ * the centre (the model of the reference's Dynamobench_case2: the core
+ * turns under the viscous, Lorentz and Coriolis torques,
+ * cal_inner_core_rotation.f90; the field continues above the CMB as a
+ * potential field); any other combination is rejected by
+ * calypso_mhd_create. */
+#define CALYPSO_VELO_RIGID          0
+#define CALYPSO_VELO_FREE_SLIP      1
+#define CALYPSO_VELO_ROT_INNER_CORE 2
+#define CALYPSO_MAGNE_INSULATOR     0
+#define CALYPSO_MAGNE_TO_CENTER     1
 typedef struct calypso_mhd calypso_mhd_t;
 typedef struct calypso_mhd_params {
     int l_truncation, nth, nphi, nr;
@@ -318,7 +329,9 @@ typedef struct calypso_mhd_params {
     double coef_temp, coef_d_temp;
     double coef_magne, coef_d_magne, coef_induct;
     double depth_high_t, depth_low_t, high_temp, low_temp;
-    int bc_velo_icb, bc_velo_cmb; /* CALYPSO_VELO_RIGID / _FREE_SLIP */
+    int bc_velo_icb, bc_velo_cmb; /* CALYPSO_VELO_RIGID / _FREE_SLIP / _ROT_INNER_CORE */
+    int kr_icb, kr_cmb;           /* 1-based ICB / CMB points; 0: first / last point */
+    int bc_magne_icb;             /* CALYPSO_MAGNE_INSULATOR / _TO_CENTER */
 } calypso_mhd_params;
 
 int calypso_mhd_create(const calypso_mhd_params *params, const double *radius,
@@ -353,6 +366,11 @@ int calypso_mhd_snapshot_rtp(calypso_mhd_t *mhd, int nvec, const int *vec_comp,
  * sph_initial_data_control, initial_magne_dynamobench.f90:
  * initial_b_dynamobench_1); isig selects the (m, m) temperature mode. */
 int calypso_mhd_set_dynamobench_ic(calypso_mhd_t *mhd, int isig);
+/* dynamo_benchmark_2 initial fields (initial_b_dynamobench_2,
+ * initial_magne_dynamobench.f90:103-147): the temperature as above, the
+ * magnetic field from the first point above the centre to the CMB; needs
+ * the inner-core model. */
+int calypso_mhd_set_dynamobench2_ic(calypso_mhd_t *mhd, int isig);
 /* Seeded random initial fields (the synthetic inputs of the benchmark
  * configurations; the reference reads such fields from a restart file):
  * poloidal/toroidal velocity and magnetic field for l >= 1 and the

@@ -550,8 +550,10 @@ class MhdStep:
         self.c = c
         L = self.L = c.L
         self.J = J = (L + 1) ** 2
-        r_cmb = c.shell / (1.0 - c.ratio)
-        r_icb = r_cmb * c.ratio
+        # set_ctl_gen_shell_grids.f90:179-185: r_ICB from the shell size and
+        # radius ratio, r_CMB = r_ICB + size
+        r_icb = c.shell * c.ratio / (1.0 - c.ratio)
+        r_cmb = r_icb + c.shell
         # inner_core: rotatable inner core with the magnetic field solved to
         # the centre and a potential field above the CMB (Dynamobench_case2)
         self.inner_core = bool(getattr(c, "inner_core", False))
