@@ -3,7 +3,10 @@
 // One step follows SPH_analyze_MHD (programs/SPH_MHD/SPH_analyzer_MHD.f90:
 // 115-202) for a rotating Boussinesq shell with rigid or free-slip walls,
 // insulating magnetic boundaries and fixed-temperature boundaries around the
-// spherical-shell reference temperature:
+// spherical-shell reference temperature, or the same shell around a
+// rotatable conducting inner core (points ki > 0 below the ICB) with the
+// magnetic field solved from the first point above the centre (kb = 0) and
+// continued above the CMB as a potential field:
 //
 //   k_explicit      cal_expricit_sph_euler / _adams
 //                   (cal_momentum_eq_explicit.f90:54-128)
@@ -20,6 +23,7 @@
 //   k_unpack        rlm -> rj of the forward components; k_coriolis the curl of the
 //                   Coriolis force in spectral space
 //                   (sum_coriolis_terms_rlm.f90:88-165)
+//   k_icore_torque  the Lorentz torque on the inner core (cal_inner_core_rotation.f90)
 //   k_rot_forces    curl of the forces, curl of u x B, divergence of u T with
 //                   the reference-temperature advection, the force sum
 //                   (cal_sph_field_by_rotation.f90, cal_nonlinear.f90)

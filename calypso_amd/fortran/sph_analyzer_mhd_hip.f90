@@ -28,13 +28,14 @@
 ! -----------------------------------------------------------------------
 !
 !>    Create the GPU state; with isig > 0 the dynamo_benchmark_1 initial
-!!    fields are set on the device, otherwise the caller provides velocity
-!!    (nnod_rj,3), magnetic field (nnod_rj,3) and the full temperature.
+!!    fields (dynamo_benchmark_2's with ibench = 2) are set on the device,
+!!    otherwise the caller provides velocity (nnod_rj,3), magnetic field
+!!    (nnod_rj,3) and the full temperature.
 !!    idx_gl_rj_j, when given, is the caller's local rj mode list
 !!    (idx_gl_1d_rj_j(:,1) of m_spheric_parameter): its d_rj arrays, here
 !!    and in sph_mhd_get_field_hip, are then in that mode order.
       subroutine SPH_initialize_MHD_hip(params, radius, isig,           &
-     &          nnod_rj, velo, magne, temp, idx_gl_rj_j)
+     &          nnod_rj, velo, magne, temp, idx_gl_rj_j, ibench)
 !
       type(calypso_mhd_params), intent(in) :: params
       real(c_double), intent(in) :: radius(*)
@@ -42,6 +43,7 @@
       real(c_double), intent(in) :: velo(nnod_rj,3), magne(nnod_rj,3)
       real(c_double), intent(in) :: temp(nnod_rj)
       integer(c_int), intent(in), optional :: idx_gl_rj_j(:)
+      integer(c_int), intent(in), optional :: ibench
       integer(c_int) :: nd
 !
       call check_status(calypso_mhd_create(params, radius, c_null_ptr,  &
@@ -50,7 +52,15 @@
         call check_status(calypso_mhd_set_rj_order(mhd_ctx,             &
      &      int(size(idx_gl_rj_j), c_int), idx_gl_rj_j))
       end if
-      if(isig .gt. 0) then
+      if(isig .gt. 0 .and. present(ibench)) then
+        if(ibench .eq. 2) then
+          call check_status(calypso_mhd_set_dynamobench2_ic(mhd_ctx,    &
+     &        isig))
+        else
+          call check_status(calypso_mhd_set_dynamobench_ic(mhd_ctx,     &
+     &        isig))
+        end if
+      else if(isig .gt. 0) then
         call check_status(calypso_mhd_set_dynamobench_ic(mhd_ctx, isig))
       else
         do nd = 1, 3
