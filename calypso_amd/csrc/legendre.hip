@@ -1793,40 +1793,46 @@ static void plan_candidates3(int ncomp, int nv, int ns, bool fwd, int prec,
         else pds = {2};
         const std::vector<int> tcs = (prec == kBF16x3) ? std::vector<int>{64, 32}
                                      : fwd ? std::vector<int>{64, 32, 16}
-                                           : std::vector<int>{32, 16, 8};
+                                           : std::vector<int>{64, 32, 16, 8};
+        // waves per work-group: 4, or (one row tile per wave, not BF16x3) 8
+        // and, backward, 12: more row tiles (four degree tiles of a parity,
+        // or eight / twelve colatitude tiles) share one staged input, and the
+        // staging slots of the extra waves allow longer chunks (fewer
+        // barriers per MFMA).  A 4-wave plan keeps its LDS to 80 KB (two
+        // work-groups per CU); the larger ones run one work-group per CU.
+        const std::vector<int> nws_all = fwd ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 12};
         for (int rpw : rpws)
-            for (int pd : pds) {
-                int taken = 0;
-                for (int tc : tcs) {
-                    if (taken == 2) break;
-                    if (tc % (pd * kstep)) continue;
-                    const long long nst = fwd ? (long long)tc * kt * 2 * ncomp
-                                              : (long long)2 * tc * ncomp * kt * 2;
-                    if (nst > (long long)(fwd ? kSlotF : kSlotB) * kThreads) continue;
-                    LegPlan p;
-                    p.ncomp = ncomp;
-                    p.nvec = nv;
-                    p.nscl = ns;
-                    p.fwd = fwd;
-                    p.KT = kt;
-                    p.mct1 = kInst3[i][0];
-                    p.mct2 = kInst3[i][1];
-                    p.rpw = rpw;
-                    p.pd = pd;
-                    p.tc = tc;
-                    p.ver = 3;
-                    if (plan_lds(p, fwd, prec) > 80 * 1024) continue;
-                    out->push_back(p);
-                    ++taken;
-                    // the same plan with 8 waves: twice the row tiles (four
-                    // degree tiles of a parity, or eight colatitude tiles)
-                    // per row group share one staged input
-                    if (rpw == 1 && prec != kBF16x3) {
-                        p.nw = 8;
+            for (int pd : pds)
+                for (int nw : nws_all) {
+                    if (nw != 4 && (rpw != 1 || prec == kBF16x3)) continue;
+                    // twelve waves (three per SIMD, 168 registers each) only
+                    // with one column tile per GEMM: more accumulators spill
+                    if (nw == 12 && (kInst3[i][0] != 1 || kInst3[i][1] != 1)) continue;
+                    int taken = 0;
+                    for (int tc : tcs) {
+                        if (taken == 2) break;
+                        if (tc % (pd * kstep)) continue;
+                        const long long nst = fwd ? (long long)tc * kt * 2 * ncomp
+                                                  : (long long)2 * tc * ncomp * kt * 2;
+                        if (nst > (long long)(fwd ? kSlotF : kSlotB) * 64 * nw) continue;
+                        LegPlan p;
+                        p.ncomp = ncomp;
+                        p.nvec = nv;
+                        p.nscl = ns;
+                        p.fwd = fwd;
+                        p.KT = kt;
+                        p.mct1 = kInst3[i][0];
+                        p.mct2 = kInst3[i][1];
+                        p.rpw = rpw;
+                        p.pd = pd;
+                        p.tc = tc;
+                        p.ver = 3;
+                        p.nw = nw;
+                        if (plan_lds(p, fwd, prec) > (nw == 4 ? 80 : 150) * 1024) continue;
                         out->push_back(p);
+                        ++taken;
                     }
                 }
-            }
     }
 }
 
@@ -2331,6 +2337,9 @@ static int launch3_n(LegContext *c, const LegPlan &p, bool fwd, const double *in
             if (p.rpw == 2 && p.pd == 4) return launch3_f<T, 2, NC1, NC2, 4>(c, p, in, out, s, err);
         } else if (p.nw == 8) {
             if (p.rpw == 1 && p.pd == 2) return launch3_b<T, 1, NC1, NC2, 2, 8>(c, p, in, out, s, err);
+        } else if (p.nw == 12) {
+            if constexpr (NC1 == 1 && NC2 == 1)
+                if (p.rpw == 1 && p.pd == 2) return launch3_b<T, 1, NC1, NC2, 2, 12>(c, p, in, out, s, err);
         } else {
             if (p.rpw == 1 && p.pd == 2) return launch3_b<T, 1, NC1, NC2, 2>(c, p, in, out, s, err);
             if constexpr (NC2 == 1)
