@@ -522,6 +522,7 @@ int calypso_mhd_create(const calypso_mhd_params *pp, const double *radius, void 
     co.bc_velo_icb = pp->bc_velo_icb;
     co.bc_velo_cmb = pp->bc_velo_cmb;
     co.bc_magne_icb = pp->bc_magne_icb;
+    co.coef_press = pp->coef_press;
     co.kr_icb = pp->kr_icb > 0 ? pp->kr_icb - 1 : 0;
     co.kr_cmb = pp->kr_cmb > 0 ? pp->kr_cmb - 1 : -1;
     if (co.dt <= 0.0) return fail("calypso_mhd_create: dt must be positive");
@@ -571,6 +572,14 @@ int calypso_mhd_snapshot_rtp(calypso_mhd_t *mhd, int nvec, const int *vec_comp, 
     std::string err;
     if (calypso::mhd_snapshot_rtp(&mhd->c, nvec, vec_comp, nscl, scl_comp, host_rtp, &err))
         return fail(err);
+    return 0;
+}
+
+int calypso_mhd_pressure(calypso_mhd_t *mhd, double *host)
+{
+    if (!mhd || !host) return fail("calypso_mhd_pressure: null argument");
+    std::string err;
+    if (calypso::mhd_pressure(&mhd->c, host, &err)) return fail(err);
     return 0;
 }
 

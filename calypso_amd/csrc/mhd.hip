@@ -1359,10 +1359,12 @@ void mhd_free(MhdContext *c)
     leg_context_free(&c->leg);
     fft_context_free(&c->fft);
     double *bufs[] = {c->d_rj, c->d_sp_b, c->d_sp_f, c->d_vr_b, c->d_vr_f, c->d_consts,
-                      c->d_radius, c->d_scratch, c->d_ms_out, c->d_grid_b, c->d_grid_f};
+                      c->d_radius, c->d_scratch, c->d_ms_out, c->d_grid_b, c->d_grid_f,
+                      c->d_press, c->d_lu_p};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     if (c->d_iconsts) (void)hipFree(c->d_iconsts);
+    if (c->d_piv_p) (void)hipFree(c->d_piv_p);
     for (int q = 0; q < 5; ++q) {
         if (c->d_lu[q]) (void)hipFree(c->d_lu[q]);
         if (c->d_piv[q]) (void)hipFree(c->d_piv[q]);
@@ -1848,6 +1850,157 @@ int mhd_energies(MhdContext *c, double *out7, std::string *err)
             out7[jb.o + 2] = h[2];
         }
     }
+    return 0;
+}
+
+// ------------------------------------------------------------- pressure --
+// Right-hand side of the pressure Poisson equation at an output step
+// (pressure_4_sph_mhd, lead_fields_4_sph_mhd.f90:70-95), one thread per node:
+// -div(advection) + div(Lorentz force) + div(buoyancy) with the Coriolis
+// term zero (the Coriolis vector is formed neither on the grid nor in rlm,
+// cal_nonlinear_sph_MHD.f90:78-80, const_coriolis_sph_rlm.f90:92-95).
+// Inside the shell the divergence cal_sph_nod_vect_div2
+// (cal_sph_exp_rotation.f90:210-239) and the buoyancy's
+// cal_div_buoyancy_sph_MHD (cal_div_buoyancies_sph_MHD.f90:124-145) over the
+// perturbation temperature and its radial gradient (cal_sph_nod_gradient_2,
+// cal_sph_exp_1st_diff.f90:40-75); on the walls the Lorentz force's
+// divergence by the one-sided formulas cal_sph_nod_nobc_in/out_div2
+// (cal_sph_exp_nod_none_bc.f90:133-160, 269-296), and the advection and
+// buoyancy terms replaced by radial forces (s_const_radial_forces_on_bc,
+// const_radial_forces_on_bc.f90:26-60: the advection's radial part is
+// overwritten by the Lorentz force's; cal_r_buoyancy_on_sph,
+// cal_r_buoyancies_on_sph.f90:97-113).  Zero outside the shell.
+__global__ void k_press_rhs(MhdDev p, double coef_buo, double *rhs)
+{
+    const size_t J = p.J;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < p.nnod;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int j = (int)(e % J), k = (int)(e / J);
+        if (k < p.ki || k > p.ko) {
+            rhs[e] = 0.0;
+            continue;
+        }
+        const double *t = p.d + (size_t)kTemp * p.nnod + j;
+        const double *lp = p.d + (size_t)kLorentz * p.nnod + j;
+        const double *ld = p.d + (size_t)(kLorentz + 1) * p.nnod + j;
+        const double *ap = p.d + (size_t)kMAdvect * p.nnod + j;
+        const double *ad = p.d + (size_t)(kMAdvect + 1) * p.nnod + j;
+        const double g3 = p.g3[j];
+        const double gh = fmax(g3, 0.5);
+        double div_in, div_lor, div_buo;
+        if (k == p.ki) {
+            const double d1 = p.ffl_i[0][1] * lp[k * J] + p.ffl_i[1][1] * lp[(k + 1) * J] +
+                              p.ffl_i[2][1] * lp[(k + 2) * J];
+            div_lor = (d1 - ld[k * J]) * gh * p.ri[2];
+        } else if (k == p.ko) {
+            const double d1 = p.ffl_c[2][1] * lp[(k - 2) * J] + p.ffl_c[1][1] * lp[(k - 1) * J] +
+                              p.ffl_c[0][1] * lp[k * J];
+            div_lor = (d1 - ld[k * J]) * gh * p.rc[2];
+        } else {
+            const double *w = p.d1 + 3 * k;
+            const double d1l = w[0] * lp[(k - 1) * J] + w[1] * lp[k * J] + w[2] * lp[(k + 1) * J];
+            const double d1a = w[0] * ap[(k - 1) * J] + w[1] * ap[k * J] + w[2] * ap[(k + 1) * J];
+            const double d1t = w[0] * t[(k - 1) * J] + w[1] * t[k * J] + w[2] * t[(k + 1) * J];
+            div_lor = (d1l - ld[k * J]) * gh * p.ar2[k];
+            div_in = (d1a - ad[k * J]) * gh * p.ar2[k];
+            // poloidal part of grad T: dT/dr / (l(l+1)) r^2 (1 for l = 0)
+            const double g13 = (j == 0) ? 1.0 : 1.0 / g3;
+            const double gt = d1t * g13 * (p.r[k] * p.r[k]);
+            div_buo = coef_buo * (3.0 * t[k * J] + gt * p.r[k] * g3 * p.ar1[k]);
+        }
+        if (k == p.ki || k == p.ko) {
+            div_in = lp[k * J] * gh * p.ar2[k];
+            div_buo = coef_buo * t[k * J] * p.r[k];
+        }
+        rhs[e] = -div_in + div_lor + div_buo;
+    }
+}
+
+// adjust_by_ave_pressure_on_CMB (set_reference_sph_mhd.f90:53-73): the
+// degree-0 pressure relative to its CMB value, on one thread.
+__global__ void k_press_adjust(MhdDev p, double *press)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double ref = press[(size_t)p.ko * p.J];
+    for (int k = p.ki; k <= p.ko; ++k) press[(size_t)k * p.J] = press[(size_t)k * p.J] - ref;
+}
+
+// The pressure matrix of const_radial_mat_4_press_sph
+// (const_r_mat_4_scalar_sph.f90:73-129): set_unit_mat_4_poisson and
+// add_scalar_poisson_mat_sph (set_radial_mat_sph.f90:141-204) with
+// coef_p = -coef_press, the walls' add_icb/cmb_scalar_poisson_mat
+// (set_sph_scalar_mat_bc.f90:105-128, 179-202) by the fixed-gradient
+// formulas, factored by ludcmp_3band on the device.
+static int pressure_matrix(MhdContext *c, std::string *err)
+{
+    const int nr = c->nr, J = c->J, ki = c->p.ki, ko = c->p.ko;
+    RadialFdm fd;
+    radial_fdm(c->radius.data(), nr, ki, ko, &fd);
+    const std::vector<double> &d1 = fd.d1, &d2 = fd.d2;
+    std::vector<double> ar1(nr), ar2(nr), g3(J);
+    for (int k = 0; k < nr; ++k) {
+        ar1[k] = 1.0 / c->radius[k];
+        ar2[k] = ar1[k] * ar1[k];
+    }
+    for (int j = 0; j < J; ++j) {
+        const int l = (int)std::sqrt((double)j + 0.5);
+        g3[j] = (double)(l * (l + 1));
+    }
+    std::vector<double> m((size_t)3 * nr * J, 0.0);
+    auto M3 = [&](int j, int k, int b) -> double & { return m[((size_t)j * nr + k) * 3 + b]; };
+    const double coef_p = -c->co.coef_press;
+    for (int j = 0; j < J; ++j) {
+        for (int k = 0; k < nr; ++k) M3(j, k, 1) = (k < ki || k > ko) ? 1.0 : 0.0;
+        for (int k = ki + 1; k < ko; ++k) {
+            M3(j, k - 1, 2) = M3(j, k - 1, 2) - coef_p * (d2[3 * k + 0] + 2.0 * ar1[k] * d1[3 * k + 0]);
+            M3(j, k, 1) = M3(j, k, 1) -
+                          coef_p * (d2[3 * k + 1] + 2.0 * ar1[k] * d1[3 * k + 1] - g3[j] * ar2[k]);
+            M3(j, k + 1, 0) = M3(j, k + 1, 0) - coef_p * (d2[3 * k + 2] + 2.0 * ar1[k] * d1[3 * k + 2]);
+        }
+        M3(j, ki, 1) = M3(j, ki, 1) -
+                       coef_p * (fd.fdr_i[1][2] + 2.0 * fd.ri[1] * fd.fdr_i[1][1] - g3[j] * fd.ri[2]);
+        M3(j, ki + 1, 0) = M3(j, ki + 1, 0) - coef_p * (fd.fdr_i[2][2] + 2.0 * fd.ri[1] * fd.fdr_i[2][1]);
+        M3(j, ko - 1, 2) = M3(j, ko - 1, 2) - coef_p * (fd.fdr_c[0][2] + 2.0 * fd.rc[1] * fd.fdr_c[0][1]);
+        M3(j, ko, 1) = M3(j, ko, 1) -
+                       coef_p * (fd.fdr_c[1][2] + 2.0 * fd.rc[1] * fd.fdr_c[1][1] - g3[j] * fd.rc[2]);
+    }
+    if (factor_band_for_solve(3, nr, J, m.data(), &c->d_lu_p, &c->d_piv_p, c->stream, err)) {
+        *err = "pressure matrix: " + *err;
+        return -1;
+    }
+    return 0;
+}
+
+int mhd_pressure(MhdContext *c, double *host, std::string *err)
+{
+    if (!(c->co.coef_press != 0.0)) {
+        *err = "pressure: coef_press is not set (coef_4_press, E^-1 for the benchmarks)";
+        return -1;
+    }
+    if (!c->initialized) {
+        *err = "pressure: the state is not initialised (calypso_mhd_initialize)";
+        return -1;
+    }
+    MhdDev &p = c->p;
+    hipStream_t s = c->stream;
+    if (!c->d_press) {
+        HIP_TRY(hipMalloc(&c->d_press, sizeof(double) * p.nnod));
+        if (pressure_matrix(c, err)) return -1;
+    }
+    hipLaunchKernelGGL(k_press_rhs, dim3(blocks_for(p.nnod)), dim3(256), 0, s, p, c->co.coef_buo,
+                       c->d_press);
+    HIP_TRY(hipGetLastError());
+    if (lubksb_t_dev(3, p.nr, p.J, c->d_lu_p, c->d_piv_p, c->d_press, s, err)) return -1;
+    hipLaunchKernelGGL(k_press_adjust, dim3(1), dim3(64), 0, s, p, c->d_press);
+    HIP_TRY(hipGetLastError());
+    const int J = c->J, nr = c->nr;
+    std::vector<double> own((size_t)J * nr);
+    HIP_TRY(hipMemcpyAsync(own.data(), c->d_press, sizeof(double) * p.nnod, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < nr; ++k)
+        for (int j = 0; j < J; ++j)
+            host[(size_t)j + (size_t)J * k] =
+                own[(size_t)(c->rj_order.empty() ? j : c->rj_order[j]) + (size_t)J * k];
     return 0;
 }
 

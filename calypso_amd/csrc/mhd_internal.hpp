@@ -66,6 +66,7 @@ struct MhdCoefs {
     int bc_velo_icb = kVeloRigid, bc_velo_cmb = kVeloRigid;
     int kr_icb = 0, kr_cmb = -1;  // 0-based ICB / CMB points (kr_cmb -1: the last)
     int bc_magne_icb = kMagneInsulator;
+    double coef_press = 0;        // coef_4_press; 0: the pressure is not asked for
 };
 
 // Everything a kernel needs, passed by value.
@@ -125,6 +126,9 @@ struct MhdContext {
     double *d_lu[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // vp vt temp bs bt
     int *d_piv[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::vector<int> rj_order;    // caller's local rj mode j -> l(l+1)+m; empty: the same
+    double *d_press = nullptr;    // pressure (nr*J), its 3-band LU and pivots, built
+    double *d_lu_p = nullptr;     // at the first pressure request
+    int *d_piv_p = nullptr;
     double *d_scratch = nullptr;  // energies: full temperature + per-mode sums
     double *d_ms_out = nullptr;
 };
@@ -142,6 +146,7 @@ int  mhd_set_seeded_ic(MhdContext *c, unsigned long long seed, double amp_u, dou
 int  mhd_initialize(MhdContext *c, std::string *err);
 int  mhd_steps(MhdContext *c, int nsteps, std::string *err);
 int  mhd_energies(MhdContext *c, double *out7, std::string *err);
+int  mhd_pressure(MhdContext *c, double *host, std::string *err);
 int  mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err);
 // the same with the host array in the caller's rj mode order (mhd_set_rj_order)
 int  mhd_field_io_caller(MhdContext *c, int comp, double *host, bool to_host, std::string *err);

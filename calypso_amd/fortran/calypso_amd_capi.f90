@@ -25,6 +25,7 @@
         integer(c_int) :: bc_velo_icb, bc_velo_cmb
         integer(c_int) :: kr_icb, kr_cmb
         integer(c_int) :: bc_magne_icb
+        real(c_double) :: coef_press
       end type calypso_mhd_params
       integer(c_int), parameter :: CALYPSO_VELO_RIGID =          0
       integer(c_int), parameter :: CALYPSO_VELO_FREE_SLIP =      1
@@ -218,6 +219,13 @@
           type(c_ptr), value :: mhd
           real(c_double), intent(inout) :: ene(7)
         end function calypso_mhd_energies
+!
+        integer(c_int) function calypso_mhd_pressure(mhd, press)        &
+     &      bind(C, name="calypso_mhd_pressure")
+          import :: c_int, c_double, c_ptr
+          type(c_ptr), value :: mhd
+          real(c_double), intent(inout) :: press(*)
+        end function calypso_mhd_pressure
 !
         integer(c_int) function calypso_mhd_time(mhd, time)            &
      &      bind(C, name="calypso_mhd_time")

@@ -68,6 +68,7 @@
       p%kr_icb = 0
       p%kr_cmb = 0
       p%bc_magne_icb = CALYPSO_MAGNE_INSULATOR
+      p%coef_press = 1.0d0 * 1.0d-3**(-1)
 !
       dummy = 0.0d0
       dtemp = 0.0d0

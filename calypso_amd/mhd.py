@@ -44,7 +44,8 @@ class MhdParams(ctypes.Structure):
                 ("high_temp", ctypes.c_double), ("low_temp", ctypes.c_double),
                 ("bc_velo_icb", ctypes.c_int), ("bc_velo_cmb", ctypes.c_int),
                 ("kr_icb", ctypes.c_int), ("kr_cmb", ctypes.c_int),
-                ("bc_magne_icb", ctypes.c_int)]
+                ("bc_magne_icb", ctypes.c_int),
+                ("coef_press", ctypes.c_double)]
 
 
 # velocity walls (bc_velocity: non_slip_sph, free_slip_sph, rot_inner_core)
@@ -144,6 +145,7 @@ def dynamobench_case1(l_truncation=31, nth=48, nphi=96, nri=48, dt=5.0e-5, coef_
     p.coef_magne = 1.0 * 1.0
     p.coef_d_magne = 1.0 * Pm ** -1.0
     p.coef_induct = 1.0 * 1.0
+    p.coef_press = 1.0 * E ** -1.0
     p.depth_high_t, p.depth_low_t = 0.5384615384615384, 1.5384615384615384
     p.high_temp, p.low_temp = 1.0, 0.0
     kinds = {"non_slip_sph": VELO_RIGID, "free_slip_sph": VELO_FREE_SLIP}
@@ -259,6 +261,15 @@ class MhdSimulation:
         the captured Adams-Bashforth step as a HIP graph; on by default)."""
         _lib.check(_lib.load().calypso_mhd_set_option(self._h, name.encode(), int(value)),
                    "calypso_mhd_set_option")
+
+    def pressure(self) -> np.ndarray:
+        """The pressure of the current state as the reference computes it at
+        an output step (pressure_4_sph_mhd, lead_fields_4_sph_mhd.f90:70-95),
+        (nr, J) in the caller's rj order, zero outside the shell."""
+        out = np.empty((self.nr, self.J))
+        _lib.check(_lib.load().calypso_mhd_pressure(self._h, out.ctypes.data_as(_P)),
+                   "calypso_mhd_pressure")
+        return out
 
     def energies(self) -> np.ndarray:
         out = np.zeros(7)

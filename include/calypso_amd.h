@@ -332,6 +332,8 @@ typedef struct calypso_mhd_params {
     int bc_velo_icb, bc_velo_cmb; /* CALYPSO_VELO_RIGID / _FREE_SLIP / _ROT_INNER_CORE */
     int kr_icb, kr_cmb;           /* 1-based ICB / CMB points; 0: first / last point */
     int bc_magne_icb;             /* CALYPSO_MAGNE_INSULATOR / _TO_CENTER */
+    double coef_press;            /* coef_4_press (E^-1 for the benchmarks); only
+                                   * calypso_mhd_pressure reads it, 0: no pressure */
 } calypso_mhd_params;
 
 int calypso_mhd_create(const calypso_mhd_params *params, const double *radius,
@@ -385,6 +387,16 @@ int calypso_mhd_set_seeded_ic(calypso_mhd_t *mhd, unsigned long long seed, doubl
 int calypso_mhd_initialize(calypso_mhd_t *mhd);
 /* nsteps calls of SPH_analyze_MHD (Euler on step 1, then Adams-Bashforth). */
 int calypso_mhd_step(calypso_mhd_t *mhd, int nsteps);
+/* The pressure of the current state as an output step computes it
+ * (pressure_4_sph_mhd, lead_fields_4_sph_mhd.f90:70-95, replacing
+ * cal_sol_pressure_by_div_v, cal_sol_sph_fluid_crank.f90:136-175): the
+ * divergence of the advection, Lorentz force and buoyancy with the walls'
+ * radial forces, the Poisson solve of const_radial_mat_4_press_sph
+ * (const_r_mat_4_scalar_sph.f90:73-129) and adjust_by_ave_pressure_on_CMB.
+ * host: nr*J doubles in the caller's rj order, zero outside the shell.  The
+ * matrix is factored on the first call (5 nr J doubles of device memory,
+ * kept until calypso_mhd_destroy); params.coef_press must be set. */
+int calypso_mhd_pressure(calypso_mhd_t *mhd, double *host);
 /* out[7]: K_ene_pol, K_ene_tor, K_ene, temperature, M_ene_pol, M_ene_tor,
  * M_ene over the shell (sph_pwr_volume_s.dat, output_rms_sph_mhd_control). */
 int calypso_mhd_energies(calypso_mhd_t *mhd, double *out);

@@ -600,6 +600,7 @@ class MhdStep:
         self.coef_magne = 1.0
         self.coef_d_magne = 1.0 * c.Pm ** -1.0
         self.coef_induct = 1.0
+        self.coef_press = 1.0 * c.E ** -1.0
         self.coef_imp_v = self.coef_imp_t = self.coef_imp_b = c.coef_imp
         self.coef_exp_v = 1.0 - c.coef_imp
         self.coef_exp_t = 1.0 - c.coef_imp
@@ -1393,6 +1394,98 @@ class MhdStep:
         self.trans_per_temp_to_temp()
         self._last_energy = None
         self.sync_temp_by_per_temp()
+
+    def pressure(self):
+        """pressure_4_sph_mhd (lead_fields_4_sph_mhd.f90:70-95) at an output
+        step, from the fields after the step and the forces of its nonlinear
+        stage: the divergences of the advection and Lorentz force
+        (const_sph_div_force, const_sph_divergence.f90:56-78, with
+        cal_sph_nod_vect_div2 cal_sph_exp_rotation.f90:210-239 inside and
+        cal_sph_nod_nobc_in/out_div2 cal_sph_exp_nod_none_bc.f90:133-160,
+        269-296 on the walls), of the buoyancy (cal_div_buoyancy_sph_MHD
+        cal_div_buoyancies_sph_MHD.f90:124-145 over the perturbation
+        temperature and its gradient: cal_sph_nod_gradient_2
+        cal_sph_exp_1st_diff.f90:40-75, dsdr_sph_fix_scalar_in/out_2
+        cal_sph_exp_fixed_scalar.f90:76-106, 208-238,
+        normalize_sph_average_grad :79-97); the wall rows replaced by radial
+        forces (s_const_radial_forces_on_bc const_radial_forces_on_bc.f90:
+        26-60: the advection's then the Lorentz force's radial part both
+        into the inertia term, so the Lorentz force's stays; the buoyancy's
+        coef T r, cal_r_buoyancy_on_sph cal_r_buoyancies_on_sph.f90:97-113);
+        the sum set_MHD_terms_to_div_force (cal_div_of_forces.f90:81-95: the
+        Coriolis vector is not formed on the grid or in rlm, so its term is
+        zero); the Poisson solve with the matrix of
+        const_radial_mat_4_press_sph (const_r_mat_4_scalar_sph.f90:73-129:
+        set_unit_mat_4_poisson, add_scalar_poisson_mat_sph
+        set_radial_mat_sph.f90:141-204, add_icb/cmb_scalar_poisson_mat
+        set_sph_scalar_mat_bc.f90:105-128, 179-202, ludcmp_3band), and
+        adjust_by_ave_pressure_on_CMB (set_reference_sph_mhd.f90:53-73).
+        Returns d_rj of the pressure, (nr, J)."""
+        f, nr, J = self.f, self.nr, self.J
+        ki, ko = self.kr_in - 1, self.kr_out - 1
+        bc, g3, r = self.bc, self.g3, self.radius
+        gh = np.maximum(g3, 0.5)
+        g13 = np.where(self.l_of_j == 0, 1.0, 1.0 / np.where(g3 == 0.0, 1.0, g3))
+        t = f["temp"]
+
+        def div(v):
+            out = np.zeros((nr, J))
+            for k in range(ki + 1, ko):
+                out[k] = (self._dr(v[0], k, 1) - v[1, k]) * gh * self.ar2[k]
+            ffl = bc.fix_fld_icb
+            d1 = ffl[0][1] * v[0, ki] + ffl[1][1] * v[0, ki + 1] + ffl[2][1] * v[0, ki + 2]
+            out[ki] = (d1 - v[1, ki]) * gh * bc.r_icb[2]
+            ffl = bc.fix_fld_cmb
+            d1 = ffl[2][1] * v[0, ko - 2] + ffl[1][1] * v[0, ko - 1] + ffl[0][1] * v[0, ko]
+            out[ko] = (d1 - v[1, ko]) * gh * bc.r_cmb[2]
+            return out
+
+        # gradient of the perturbation temperature (poloidal part)
+        gt = np.zeros((nr, J))
+        for k in range(ki + 1, ko):
+            gt[k] = self._dr(t, k, 1) * g13 * r[k] ** 2
+        ffl = bc.fix_fld_icb
+        d1 = ffl[0][1] * self.T_icb + ffl[1][1] * t[ki + 1] + ffl[2][1] * t[ki + 2]
+        gt[ki] = d1 * g13 * bc.r_icb[0] ** 2
+        ffl = bc.fix_fld_cmb
+        d1 = ffl[2][1] * t[ko - 2] + ffl[1][1] * t[ko - 1] + ffl[0][1] * self.T_cmb
+        gt[ko] = d1 * g13 * bc.r_cmb[0] ** 2
+        gt[:, 0] = 2.0 * gt[:, 0]
+
+        div_in = div(f["m_advect"])
+        div_lor = div(f["lorentz"])
+        div_buo = np.zeros((nr, J))
+        for k in range(ki, ko + 1):
+            div_buo[k] = self.coef_buo * (3.0 * t[k] + gt[k] * r[k] * g3 * self.ar1[k])
+        for k in (ki, ko):
+            div_buo[k] = self.coef_buo * t[k] * r[k]
+            div_in[k] = f["lorentz"][0, k] * gh * self.ar2[k]
+        rhs = -div_in + div_lor + div_buo
+        self.press_terms = {"div_inertia": div_in, "div_Lorentz_force": div_lor,
+                            "div_buoyancy": div_buo, "grad_temp_pol": gt}
+
+        coef_p = -self.coef_press
+        d1c, d2c, ar1 = self.d1, self.d2, self.ar1
+        m = np.zeros((J, nr, 3))
+        m[:, :ki, 1] = 1.0
+        m[:, ko + 1:, 1] = 1.0
+        for k in range(ki + 1, ko):
+            m[:, k - 1, 2] = m[:, k - 1, 2] - coef_p * (d2c[k, 0] + 2.0 * ar1[k] * d1c[k, 0])
+            m[:, k, 1] = m[:, k, 1] - coef_p * (d2c[k, 1] + 2.0 * ar1[k] * d1c[k, 1]
+                                                - g3 * self.ar2[k])
+            m[:, k + 1, 0] = m[:, k + 1, 0] - coef_p * (d2c[k, 2] + 2.0 * ar1[k] * d1c[k, 2])
+        fdr = bc.fix_dr_icb
+        m[:, ki, 1] = m[:, ki, 1] - coef_p * (fdr[1][2] + 2.0 * bc.r_icb[1] * fdr[1][1]
+                                              - g3 * bc.r_icb[2])
+        m[:, ki + 1, 0] = m[:, ki + 1, 0] - coef_p * (fdr[2][2] + 2.0 * bc.r_icb[1] * fdr[2][1])
+        fdr = bc.fix_dr_cmb
+        m[:, ko - 1, 2] = m[:, ko - 1, 2] - coef_p * (fdr[0][2] + 2.0 * bc.r_cmb[1] * fdr[0][1])
+        m[:, ko, 1] = m[:, ko, 1] - coef_p * (fdr[1][2] + 2.0 * bc.r_cmb[1] * fdr[1][1]
+                                              - g3 * bc.r_cmb[2])
+        lu, piv = ludcmp(3, m)[:2]
+        p = lubksb(3, lu, piv, rhs)
+        p[ki:ko + 1, 0] = p[ki:ko + 1, 0] - p[ko, 0]
+        return p
 
     def energies(self):
         """The sph_pwr_volume_s.dat columns K_ene_pol, K_ene_tor, K_ene,

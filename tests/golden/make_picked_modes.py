@@ -1,12 +1,14 @@
 """Writes tests/golden/picked_modes_case{1,2}.json: spectral coefficients of
-velocity, temperature and magnetic field at chosen (l, m) modes on every
-radial point after steps 10 and 20 of the reference's tests/Dynamobench_case1
+velocity, temperature, magnetic field and pressure at chosen (l, m) modes on
+every radial point after steps 10 and 20 of the reference's tests/Dynamobench_case1
 and tests/Dynamobench_case2, as the reference's own MHD program writes them
 through its pickup monitor (picked_sph_prefix, pickup_sph_spectr.f90).  The
 programs are the reference's grid generator and MHD program built from its
 sources (`make -C oracle grids mhd`).  The reference's control files are used
 as they are except for one subdomain and one thread, the grid and restart
-paths, an output row every 10 steps, 20 steps, and the pickup block.  (1, 1)
+paths, an output row every 10 steps, 20 steps, the pressure monitored
+(Monitor_On: it is solved at every output step either way) and the pickup
+block.  (1, 1)
 stays at round-off in these runs and pins the inner core's x/y turns of case 2
 only through their absence.
 
@@ -56,6 +58,8 @@ def run_case(case):
     mhd = re.sub(r"'rst_4/rst'", "'rst/rst'", mhd)
     mhd = re.sub(r"i_step_check_ctl\s+\d+", "i_step_check_ctl    10", mhd)
     mhd = re.sub(r"i_step_finish_ctl\s+\d+", "i_step_finish_ctl    20", mhd)
+    mhd = re.sub(r"nod_value_ctl\s+pressure\s+\w+\s+Monitor_Off",
+                 "nod_value_ctl  pressure                Viz_Off  Monitor_On", mhd)
     mhd = re.sub(r"(?s)\s*begin sph_monitor_ctl.*?end sph_monitor_ctl\s*", "\n", mhd)
     mhd = mhd.replace("end MHD_control", monitor_block() + "end MHD_control")
     with tempfile.TemporaryDirectory() as tmp:

@@ -4,15 +4,19 @@ modes after steps 10 and 20 of Dynamobench case 1 and case 2, as the
 reference's MHD program (built from its sources) writes them through its
 pickup monitor (tests/golden/picked_modes_case*.json, written by
 tests/golden/make_picked_modes.py), within 1e-12 of each column's largest
-value."""
+value; the output-step pressure (calypso_mhd_pressure) against the same
+pickup's pressure column within the oracle's bounds, and against the oracle
+after three steps."""
 import json
 import os
 
 import numpy as np
 import pytest
 
-from calypso_amd import mhd
+from calypso_amd import _lib, mhd
 from oracle import mhd_step
+
+from test_oracle_mhd_step import PRESS_TOL_L, PRESS_TOL_L0, pressure_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -43,12 +47,68 @@ def test_fields_match_reference_pickup(torch_cuda, case):
         cols = [sim.get_field(0), sim.get_field(2), sim.get_field(1), t,
                 sim.get_field(6), sim.get_field(8), sim.get_field(7)]
         got = np.stack([np.stack([c[:, j] for c in cols], 1) for j in js])
-        want = np.array([ref["steps"][str(step)][lm] for lm in ref["modes"]])
+        want = np.array([ref["steps"][str(step)][lm] for lm in ref["modes"]])[:, :, :7]
         errs = []
-        for c, name in enumerate(ref["columns"]):
+        for c, name in enumerate(ref["columns"][:7]):
             scale = np.abs(want[:, :, c]).max()
             errs.append(np.abs(got[:, :, c] - want[:, :, c]).max() / scale)
         print(f"[reference fields] case {case} step {step}: "
               + " ".join(f"{n}={e:.1e}" for n, e in zip(ref["columns"], errs)), flush=True)
         assert max(errs) < 1e-12, dict(zip(ref["columns"], errs))
+        e0, e1 = pressure_errors(ref, step, sim.pressure())
+        print(f"[reference fields] case {case} step {step}: pressure l=0 {e0:.1e} "
+              f"l>=1 {e1:.1e}", flush=True)
+        assert e0 < PRESS_TOL_L0 and e1 < PRESS_TOL_L, (e0, e1)
+    sim.close()
+
+
+@pytest.mark.parametrize("case", [1, 2])
+def test_pressure_matches_oracle(torch_cuda, case):
+    """calypso_mhd_pressure after three steps against the oracle's pressure
+    of the same run, relative to each mode group's largest value: 1e-12 for
+    l >= 1 (reached: 6e-16 / 2e-14 for case 1 / 2), 1e-10 for the degree-0
+    system that is singular up to rounding (reached: 8e-13 / 2e-12); zero
+    outside the shell; twice in a row (the matrix is factored once)."""
+    print(f"[pressure] Dynamobench case {case} vs oracle", flush=True)
+    bench = mhd_step.DynamoBenchmarkCase1 if case == 1 else mhd_step.DynamoBenchmarkCase2
+    o = mhd_step.MhdStep(bench)
+    o.initialize()
+    for _ in range(3):
+        o.step()
+    want = o.pressure()
+    p, r = mhd.dynamobench_case1() if case == 1 else mhd.dynamobench_case2()
+    sim = mhd.MhdSimulation(p, r)
+    if case == 1:
+        sim.set_dynamobench_ic(400)
+    else:
+        sim.set_dynamobench2_ic(400)
+    sim.initialize()
+    sim.step(3)
+    got = sim.pressure()
+    again = sim.pressure()
+    assert np.array_equal(got, again)
+    ki, ko = o.kr_in - 1, o.kr_out - 1
+    assert not got[:ki].any() and not got[ko + 1:].any()
+    e0 = np.abs(got[:, 0] - want[:, 0]).max() / np.abs(want[:, 0]).max()
+    e1 = np.abs(got[:, 1:] - want[:, 1:]).max() / np.abs(want[:, 1:]).max()
+    print(f"[pressure] case {case}: l=0 {e0:.1e} l>=1 {e1:.1e}", flush=True)
+    assert e0 < 1e-10 and e1 < 1e-12, (e0, e1)
+    sim.close()
+
+
+def test_pressure_needs_coefficient_and_state(torch_cuda):
+    """calypso_mhd_pressure refuses a model without coef_press and a state
+    that is not initialised, with a message and no device work."""
+    p, r = mhd.dynamobench_case1()
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_dynamobench_ic(400)
+    with pytest.raises(_lib.CalypsoError, match="not initialised"):
+        sim.pressure()
+    sim.close()
+    p.coef_press = 0.0
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_dynamobench_ic(400)
+    sim.initialize()
+    with pytest.raises(_lib.CalypsoError, match="coef_press"):
+        sim.pressure()
     sim.close()
