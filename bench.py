@@ -288,11 +288,12 @@ def large_legendre_bench(torch, steps=3):
             "scalars": ns5, "modes": out}
 
 
-def ensemble_bench(torch, nshell=64, steps=20, nstream=8, balanced=True):
+def ensemble_bench(torch, nshell=64, steps=20, nstream=4, balanced=True):
     """Ensemble of independent shells, Lmax in {31,47,63,95}, Nr in
     {32,48,64,96}, nth = 1.5(L+1), nphi = 2 nth, seeded Ra in [50, 500]: each
     shell's step is a captured graph; the shells are dealt over `nstream`
-    streams (default priority) longest first, each to the stream with the
+    streams (default priority; four, the hardware queues a process gets)
+    longest first, each to the stream with the
     least work so far (work of a shell ~ Nr nth (L+1)^2, the Legendre
     transforms), and stepped together.  Reported: shell-steps per second of
     the ensemble and of the same shells stepped one after another."""

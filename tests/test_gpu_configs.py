@@ -174,9 +174,9 @@ def test_c4_ensemble_matches_shells_alone(torch_cuda):
         sim.step(nsteps)
         alone.append((sim.energies(), sim.get_field(0), sim.get_field(12)))
         sim.close()
-    streams = [torch.cuda.Stream() for _ in range(8)]
-    print("[C4] 64 shells together on 8 streams", flush=True)
-    sims = [make(q, streams[q % 8].cuda_stream) for q in range(len(specs))]
+    streams = [torch.cuda.Stream() for _ in range(4)]
+    print("[C4] 64 shells together on 4 streams", flush=True)
+    sims = [make(q, streams[q % 4].cuda_stream) for q in range(len(specs))]
     for _ in range(nsteps):
         for sim in sims:
             sim.step(1)
