@@ -4,7 +4,11 @@ free_slip_sph) against the oracle's restatement, for each choice of walls
 start), and the invariant that pins the walls physically: between two
 free-slip walls a solid-body rotation (toroidal l = 1, m = 0 velocity
 proportional to r^2) keeps its amplitude to rounding, while a rigid wall
-brakes it."""
+brakes it; and Dynamobench case 1 with free-slip walls against the energies
+the reference's own MHD program writes after every one of 20 steps."""
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -68,3 +72,26 @@ def test_free_slip_keeps_solid_body_rotation(torch_cuda):
     change, _ = _solid_body((False, False))
     print(f"   rigid/rigid: change {change:.1e}", flush=True)
     assert change > 1e-2
+
+
+@pytest.mark.parametrize("run,walls", [("free_free", (True, True)), ("free_rigid", (True, False))])
+def test_free_slip_energies_match_reference_program(torch_cuda, run, walls):
+    """Dynamobench case 1 with its velocity walls changed to free slip, on the
+    GPU, against the energies of the reference's own MHD program (built from
+    its sources) after every one of 20 steps
+    (tests/golden/dynamobench_free_slip_steps.json): within 1e-12."""
+    print(f"[free slip] Dynamobench case 1, {run}: 20 steps vs the reference program", flush=True)
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "dynamobench_free_slip_steps.json")))["runs"][run]
+    p, r = mhd.dynamobench_case1(bc_velo=(NAMES[walls[0]], NAMES[walls[1]]))
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_dynamobench_ic(400)
+    sim.initialize()
+    worst = 0.0
+    for n in range(1, 21):
+        sim.step(1)
+        want = np.array(ref["steps"][str(n)])
+        worst = max(worst, float(np.max(np.abs(sim.energies() / want - 1.0))))
+    print(f"[free slip] {run}: worst relative energy difference {worst:.1e}", flush=True)
+    assert worst < 1e-12, worst
+    sim.close()

@@ -5,6 +5,8 @@ reference's own routines compiled from its sources (oracle/_ref), and the
 whole step keeps a solid-body rotation exactly, as free-slip walls must and
 rigid walls must not."""
 import ctypes
+import json
+import os
 
 import numpy as np
 import pytest
@@ -159,3 +161,22 @@ def test_planted_toroidal_wall_mistake_breaks_solid_rotation():
         mvt = ms.inverse33([[1.0, 0.0, 0.0], [1.0, -r0, 0.0], [1.0, h, 0.5 * h * h]])
         bc.free_vt_icb = [[0.0, 0.0, 0.0], [1.0, mvt[1][0], mvt[2][0]], [0.0, 0.0, mvt[2][2]]]
     assert _solid_body_change(True, True, bc_hook=wrong) > 1e-6
+
+
+FREE_STEPS = os.path.join(os.path.dirname(__file__), "golden", "dynamobench_free_slip_steps.json")
+
+
+@pytest.mark.parametrize("run,walls", [("free_free", (True, True)), ("free_rigid", (True, False))])
+def test_free_slip_energies_match_reference_program(run, walls):
+    """The oracle's Dynamobench case 1 with free-slip velocity walls against
+    the energies the reference's own MHD program (built from its sources)
+    writes after every one of 20 steps of the same run
+    (tests/golden/dynamobench_free_slip_steps.json): within 1e-12 (reached:
+    2e-13)."""
+    ref = json.load(open(FREE_STEPS))["runs"][run]
+    s = ms.MhdStep(ms.shell_case(free_icb=walls[0], free_cmb=walls[1]))
+    s.initialize()
+    for _ in range(20):
+        s.step()
+        np.testing.assert_allclose(s.energies(), np.array(ref["steps"][str(s.i_step)]),
+                                   rtol=1e-12, atol=0, err_msg=f"{run} step {s.i_step}")
