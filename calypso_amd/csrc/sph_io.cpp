@@ -7,7 +7,8 @@
 //  * restart files <prefix>.<step>.<domain>.fst in the ASCII field format:
 //    write_step_data (m_time_data_IO.f90:107-119) and write_field_data
 //    (field_data_IO.f90:267-298), the fields chosen by
-//    set_sph_restart_comp_to_IO (set_sph_restart_IO.f90:98-140), the name
+//    set_sph_restart_comp_to_IO (set_sph_restart_IO.f90:98-140), vectors as
+//    (pol, tor, dpol/dr) like copy_each_sph_vector_to_IO, the name
 //    by set_SPH_fld_file_name (set_field_file_names.f90:125-157); and the
 //    reader (read_step_data, read_field_data: names matched, any order).
 // The restart temperature is the full T (the step evolves T - T0; the
@@ -54,6 +55,12 @@ struct RstField {
 const RstField kRst[] = {{"velocity", kVelo, 3},           {"temperature", kTemp, 1},
                          {"magnetic_field", kMagne, 3},    {"previous_momentum", kPreMom, 3},
                          {"previous_induction", kPreUxB, 3}, {"previous_heat", kPreHeat, 1}};
+
+// The file holds a vector as (pol, tor, dpol/dr) where d_rj holds (pol,
+// dpol/dr, tor): copy_each_sph_vector_to_IO / _from_IO
+// (copy_rj_phys_data_4_IO.f90:219-236, 284-300).  d_rj component of the
+// file's column q.
+int file_comp(int ncomp, int q) { return ncomp == 3 ? (q == 0 ? 0 : 3 - q) : q; }
 
 }  // namespace
 
@@ -103,7 +110,8 @@ int mhd_write_restart(MhdContext *c, const std::string &path, std::string *err)
     for (const RstField &r : kRst)
         for (int q = 0; q < r.ncomp; ++q, ++n) {
             comp[n].resize(nnod);
-            if (mhd_field_io_caller(c, r.comp0 + q, comp[n].data(), true, err)) return -1;
+            if (mhd_field_io_caller(c, r.comp0 + file_comp(r.ncomp, q), comp[n].data(), true, err))
+                return -1;
         }
     // T - T0 -> T at l = 0 (the caller's position of that mode)
     size_t j00 = 0;
@@ -247,7 +255,8 @@ int mhd_read_restart(MhdContext *c, const std::string &path, std::string *err)
         }
         if (!r) continue;   // a field this step does not evolve
         for (int q = 0; q < nc; ++q)
-            if (mhd_field_io_caller(c, r->comp0 + q, buf.data() + (size_t)q * nnod, false, err))
+            if (mhd_field_io_caller(c, r->comp0 + file_comp(nc, q), buf.data() + (size_t)q * nnod,
+                                    false, err))
                 return -1;
         ++seen;
     }

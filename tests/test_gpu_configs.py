@@ -3,9 +3,12 @@ FP64 GPU path:
 
 * C1  L=31, 48 x 96 grid, 49 radial points, E=1e-3, Ra=100, Pr=1, Pm=5,
       seeded random start, dt=1e-4, Crank-Nicolson weight 0.5: every stage
-      field against the oracle after the initialisation and three steps, and
-      the energies after 50 and 100 steps against the oracle's
-      (tests/golden/seeded_c1_energies.json);
+      field against the oracle after the initialisation and three steps, the
+      energies after 50 and 100 steps against the oracle's
+      (tests/golden/seeded_c1_energies.json), and after every one of 100
+      steps against the reference's own program started from the same
+      fields (tests/golden/seeded_c1_reference_steps.json), also through a
+      restart file;
 * C3  L=127, 192 x 384 grid, 65 radial points, dt=5e-5: the FP64 step
       against the oracle after two steps, then 200 steps with every Legendre
       precision against the FP64 step (energy drift and fields);
@@ -91,6 +94,61 @@ def test_c1_seeded_100_steps_energies(torch_cuda):
         rel = np.max(np.abs(got[step] - want) / np.abs(want))
         print(f"   step {step}: max rel diff {rel:.2e}", flush=True)
         assert rel < 1e-10, (step, got[step], want)
+
+
+REF_C1 = os.path.join(os.path.dirname(__file__), "golden", "seeded_c1_reference_steps.json")
+
+
+def test_c1_seeded_matches_reference_program(torch_cuda):
+    """C1 on the GPU against the reference's own MHD program started from the
+    same seeded fields (its restart file): energies after every one of 100
+    steps within 1e-12."""
+    ref = json.load(open(REF_C1))
+    assert tuple(ref["seeded"].values()) == SEEDED
+    print("[C1] seeded start, 100 steps vs the reference program", flush=True)
+    p, r = mhd.dynamobench_case1(dt=ref["dt"], coef_imp=ref["coef_imp"])
+    sim = _seeded_sim(p, r)
+    worst = 0.0
+    for n in range(1, 101):
+        sim.step(1)
+        want = np.array(ref["steps"][str(n)])
+        worst = max(worst, float(np.max(np.abs(sim.energies() / want - 1.0))))
+    sim.close()
+    print(f"   worst relative energy difference {worst:.1e}", flush=True)
+    assert worst < 1e-12, worst
+
+
+def test_c1_restart_file_layout_and_resume(torch_cuda, tmp_path):
+    """The restart file holds each vector as (pol, tor, dpol/dr), the layout
+    of copy_each_sph_vector_to_IO (copy_rj_phys_data_4_IO.f90:219-236) that
+    the reference program read in the C1 reference run; a state read back
+    from it resumes that run: energies of steps 1-10 within 1e-12 of the
+    reference program's."""
+    ref = json.load(open(REF_C1))
+    p, r = mhd.dynamobench_case1(dt=ref["dt"], coef_imp=ref["coef_imp"])
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_seeded_ic(*SEEDED)
+    u = [sim.get_field(c) for c in (0, 1, 2)]
+    path = sim.write_restart(str(tmp_path / "rst"), 0)
+    sim.close()
+    lines = open(path).read().splitlines()
+    i = lines.index("velocity")
+    nnod = u[0].size
+    block = np.array([[float(x) for x in lines[i + 1 + k].split()] for k in range(nnod)])
+    # 16 significant digits in the file ('1pE25.15')
+    np.testing.assert_allclose(block[:, 0], u[0].reshape(-1), rtol=2e-15, atol=0)
+    np.testing.assert_allclose(block[:, 1], u[2].reshape(-1), rtol=2e-15, atol=0)  # toroidal
+    np.testing.assert_allclose(block[:, 2], u[1].reshape(-1), rtol=2e-15, atol=0)
+    print("[C1] resume from the restart file, 10 steps vs the reference program", flush=True)
+    sim = mhd.MhdSimulation(p, r)
+    sim.read_restart(path)
+    sim.initialize()
+    for n in range(1, 11):
+        sim.step(1)
+        want = np.array(ref["steps"][str(n)])
+        np.testing.assert_allclose(sim.energies(), want, rtol=1e-12, atol=0,
+                                   err_msg=f"step {n}")
+    sim.close()
 
 
 # --------------------------------------------------------------------- C3 --

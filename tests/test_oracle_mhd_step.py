@@ -244,3 +244,21 @@ def test_pressure_wall_forces_are_pinned():
     bad = s.pressure()
     assert max(pressure_errors(ref, 10, good)) < PRESS_TOL_L0
     assert pressure_errors(ref, 10, bad)[1] > 1e-4
+
+
+def test_seeded_c1_matches_reference_program():
+    """The oracle from the seeded C1 start (seed 2024, dt = 1e-4,
+    Crank-Nicolson weight 0.5) against the energies the reference's own MHD
+    program writes after every one of 100 steps when it starts from the same
+    fields as its restart file (tests/golden/seeded_c1_reference_steps.json,
+    written by tests/golden/make_seeded_reference.py): within 1e-12
+    (reached: 3.9e-13)."""
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "seeded_c1_reference_steps.json")))
+    seeded = tuple(ref["seeded"].values())
+    s = mhd_step.MhdStep(mhd_step.shell_case(dt=ref["dt"], coef_imp=ref["coef_imp"]))
+    s.initialize(seeded=seeded)
+    for _ in range(100):
+        s.step()
+        np.testing.assert_allclose(s.energies(), np.array(ref["steps"][str(s.i_step)]),
+                                   rtol=1e-12, atol=0, err_msg=f"step {s.i_step}")
