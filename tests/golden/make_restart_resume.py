@@ -1,9 +1,11 @@
 """Writes tests/golden/restart_resume_l7/: a restart file written by the
 reference's own MHD program (built from its sources by `make -C oracle grids
 mhd`) after step 10 of tests/Dynamobench_case1's physics on a small grid
-(L = 7, 12 x 24, 12 Chebyshev intervals), gzipped, and the energies
+(L = 7, 12 x 24, 12 Chebyshev intervals) and the one after step 20, gzipped,
+and the energies
 (sph_pwr_volume_s.dat columns) after each of the run's 20 steps.  A run of
-the library read from that file must continue the reference's run.
+the library read from the first file must continue the reference's run and
+write the second.
 
 The control files are the reference's, except: one subdomain and one thread,
 the grid (truncation, points, intervals) and restart paths, an energy row
@@ -57,9 +59,11 @@ def main():
         rows = open(os.path.join(tmp, "sph_pwr_volume_s.dat")).read().splitlines()
         print(sorted(os.listdir(os.path.join(tmp, "rst"))))
         rst = open(os.path.join(tmp, "rst", "rst.1.0.fst"), "rb").read()
+        rst2 = open(os.path.join(tmp, "rst", "rst.2.0.fst"), "rb").read()
     os.makedirs(OUT, exist_ok=True)
-    with gzip.GzipFile(os.path.join(OUT, "rst.1.0.fst.gz"), "wb", mtime=0) as g:
-        g.write(rst)
+    for name, data in (("rst.1.0.fst.gz", rst), ("rst.2.0.fst.gz", rst2)):
+        with gzip.GzipFile(os.path.join(OUT, name), "wb", mtime=0) as g:
+            g.write(data)
     steps = {}
     for line in rows[8:]:
         v = line.split()
@@ -68,7 +72,8 @@ def main():
                          "tests/Dynamobench_case1's physics at L=7, 12 x 24, 12 intervals; "
                          "restart after step 10, sph_pwr_volume_s.dat after every step",
                "grid": {"l_truncation": L, "nth": NTH, "nphi": NPHI, "nri": NRI},
-               "restart": "rst.1.0.fst.gz", "restart_step": 10, "steps": steps},
+               "restart": "rst.1.0.fst.gz", "restart_step": 10,
+               "restart_end": "rst.2.0.fst.gz", "steps": steps},
               open(os.path.join(OUT, "energies.json"), "w"), indent=0)
     print("wrote", OUT, len(steps), "rows", len(rst), "bytes of restart")
 
