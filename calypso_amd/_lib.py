@@ -72,6 +72,7 @@ SIGNATURES = [
     ("calypso_mhd_step", _I, [_P, _I]),
     ("calypso_mhd_energies", _I, [_P, _P]),
     ("calypso_mhd_pressure", _I, [_P, _P]),
+    ("calypso_mhd_energy_fluxes", _I, [_P, _I, _P, _P]),
     ("calypso_mhd_info", _I, [_P, _P]),
     ("calypso_mhd_set_option", _I, [_P, ctypes.c_char_p, ctypes.c_longlong]),
     ("calypso_mhd_time", _I, [_P, _P]),

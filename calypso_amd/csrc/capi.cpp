@@ -583,6 +583,14 @@ int calypso_mhd_pressure(calypso_mhd_t *mhd, double *host)
     return 0;
 }
 
+int calypso_mhd_energy_fluxes(calypso_mhd_t *mhd, int nflux, const int *flux_ids, double *host)
+{
+    if (!mhd || !flux_ids || !host) return fail("calypso_mhd_energy_fluxes: null argument");
+    std::string err;
+    if (calypso::mhd_energy_fluxes(&mhd->c, nflux, flux_ids, host, &err)) return fail(err);
+    return 0;
+}
+
 int calypso_mhd_get_field(calypso_mhd_t *mhd, int comp, double *host)
 {
     if (!mhd || !host) return fail("calypso_mhd_get_field: null argument");

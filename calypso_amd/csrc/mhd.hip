@@ -1548,6 +1548,149 @@ int mhd_snapshot_rtp(MhdContext *c, int nvec, const int *vec, int nscl, const in
     return rc;
 }
 
+// Output-step energy fluxes on the grid (s_cal_energy_flux_rtp,
+// cal_energy_flux_rtp.f90:101-150 and cal_buoyancy_flux_rtp_smp :222-252).
+// g holds 13 grid components, node fastest: velocity, magnetic field, current
+// density and magnetic induction as (r, theta, phi), then the temperature
+// perturbation.  The Lorentz force is the step's coef_lor (J x B)
+// (cal_cross_prod_w_coef); flux ids: 0 Lorentz_work u.F, 1
+// work_against_Lorentz -u.F, 2 magnetic_ene_generation B.induction, 3
+// buoyancy_flux coef_buo T u_r r.
+struct FluxIds {
+    int n;
+    int id[kMaxEnergyFluxes];
+};
+
+__global__ void k_energy_flux(size_t nnod, int nr, const double *__restrict__ g,
+                              const double *__restrict__ radius, double coef_lor,
+                              double coef_buo, FluxIds ids, double *__restrict__ out)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnod;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double u1 = g[i], u2 = g[nnod + i], u3 = g[2 * nnod + i];
+        const double b1 = g[3 * nnod + i], b2 = g[4 * nnod + i], b3 = g[5 * nnod + i];
+        const double j1 = g[6 * nnod + i], j2 = g[7 * nnod + i], j3 = g[8 * nnod + i];
+        const double n1 = g[9 * nnod + i], n2 = g[10 * nnod + i], n3 = g[11 * nnod + i];
+        const double t = g[12 * nnod + i];
+        const double f1 = (j2 * b3 - j3 * b2) * coef_lor;
+        const double f2 = (j3 * b1 - j1 * b3) * coef_lor;
+        const double f3 = (j1 * b2 - j2 * b1) * coef_lor;
+        for (int q = 0; q < ids.n; ++q) {
+            double v;
+            switch (ids.id[q]) {
+            case 0: v = f1 * u1 + f2 * u2 + f3 * u3; break;
+            case 1: v = -1.0 * (f1 * u1 + f2 * u2 + f3 * u3); break;
+            case 2: v = n1 * b1 + n2 * b2 + n3 * b3; break;
+            default: v = coef_buo * t * u1 * radius[i % nr]; break;
+            }
+            out[(size_t)q * nnod + i] = v;
+        }
+    }
+}
+
+// rlm spectrum (component fastest) back to d_rj order: out(comp, k, j)
+__global__ void k_scatter_rlm(MhdDev p, int ncomp, const double *__restrict__ sp,
+                              double *__restrict__ out)
+{
+    const size_t n = (size_t)ncomp * p.J * p.nr;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int comp = (int)(e % ncomp);
+        const size_t q = e / ncomp;
+        const int jr = (int)(q % p.J), k = (int)(q / p.J);
+        out[(size_t)comp * p.nnod + (size_t)k * p.J + p.rj_of_rlm[jr]] = sp[e];
+    }
+}
+
+// enegy_fluxes_4_sph_mhd (lead_fields_4_sph_mhd.f90:98-121): the snapshot's
+// backward transform of u, B, J, the induction and T, the fluxes on the grid,
+// and the forward transform of the flux scalars (sph_forward_trans_snapshot_MHD)
+// to spectra in the caller's rj order, host(nflux, nr, J).
+int mhd_energy_fluxes(MhdContext *c, int nflux, const int *ids, double *host, std::string *err)
+{
+    if (nflux < 1 || nflux > kMaxEnergyFluxes || !ids || !host) {
+        *err = "energy fluxes: need 1 <= nflux <= 8, the flux ids and an output array";
+        return -1;
+    }
+    FluxIds fid{};
+    fid.n = nflux;
+    for (int q = 0; q < nflux; ++q) {
+        if (ids[q] < 0 || ids[q] > 3) {
+            *err = "energy fluxes: flux id must be 0 (Lorentz_work), 1 (work_against_Lorentz), "
+                   "2 (magnetic_ene_generation) or 3 (buoyancy_flux)";
+            return -1;
+        }
+        fid.id[q] = ids[q];
+    }
+    if (!c->initialized) {
+        *err = "energy fluxes: the state is not initialised (calypso_mhd_initialize)";
+        return -1;
+    }
+    static const int slot[13] = {0, 1, 2, 6, 7, 8, 9, 10, 11, kInduction, kInduction + 1,
+                                 kInduction + 2, 12};
+    const int ncomp = 13;
+    const size_t nnod_rtp = (size_t)c->nth * c->nr * c->nphi;
+    const size_t nrlm = (size_t)ncomp * c->J * c->nr;
+    const size_t nrtm = (size_t)ncomp * c->nth * c->nr * c->M;
+    const size_t nj = (size_t)c->J * c->nr;
+    hipStream_t s = c->stream;
+    double *d_sp = nullptr, *d_vr = nullptr, *d_grid = nullptr, *d_flux = nullptr;
+    int *d_slot = nullptr;
+    auto release = [&]() {
+        if (d_sp) (void)hipFree(d_sp);
+        if (d_vr) (void)hipFree(d_vr);
+        if (d_grid) (void)hipFree(d_grid);
+        if (d_flux) (void)hipFree(d_flux);
+        if (d_slot) (void)hipFree(d_slot);
+    };
+    int rc = -1;
+    do {
+        hipError_t e = hipMalloc(&d_sp, nrlm * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&d_vr, nrtm * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&d_grid, (size_t)ncomp * nnod_rtp * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&d_flux, (size_t)nflux * nnod_rtp * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&d_slot, ncomp * sizeof(int));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_slot, slot, ncomp * sizeof(int), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) {
+            *err = std::string("energy fluxes: ") + hipGetErrorString(e);
+            break;
+        }
+        hipLaunchKernelGGL(k_gather_rlm, dim3(blocks_for(nrlm)), dim3(256), 0, s, c->p, ncomp,
+                           d_slot, d_sp);
+        if (leg_backward_dev(&c->leg, ncomp, 4, 1, d_sp, d_vr, s, err)) break;
+        if (fft_backward_dev(&c->fft, ncomp, d_vr, d_grid, s, err)) break;
+        hipLaunchKernelGGL(k_energy_flux, dim3(blocks_for(nnod_rtp)), dim3(256), 0, s, nnod_rtp,
+                           c->nr, d_grid, c->d_radius, c->co.coef_lor, c->co.coef_buo, fid,
+                           d_flux);
+        // the forward transform reuses the backward's buffers (nflux <= 13 components)
+        if (fft_forward_dev(&c->fft, nflux, d_flux, d_vr, s, err)) break;
+        if (leg_forward_dev(&c->leg, nflux, 0, nflux, d_vr, d_sp, s, err)) break;
+        hipLaunchKernelGGL(k_scatter_rlm, dim3(blocks_for(nflux * nj)), dim3(256), 0, s, c->p,
+                           nflux, d_sp, d_grid);
+        e = hipGetLastError();
+        std::vector<double> own((size_t)nflux * nj);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(own.data(), d_grid, own.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            *err = std::string("energy fluxes: ") + hipGetErrorString(e);
+            break;
+        }
+        const int J = c->J;
+        for (int q = 0; q < nflux; ++q)
+            for (int k = 0; k < c->nr; ++k)
+                for (int j = 0; j < J; ++j)
+                    host[(size_t)q * nj + (size_t)j + (size_t)J * k] =
+                        own[(size_t)q * nj + (size_t)(c->rj_order.empty() ? j : c->rj_order[j]) +
+                            (size_t)J * k];
+        rc = 0;
+    } while (false);
+    release();
+    return rc;
+}
+
 // dynamo_benchmark_1 initial fields (set_initial_velo_sph,
 // set_ini_reference_temp_sph, set_initial_temp_sph,
 // initial_b_dynamobench_1): full temperature, as a restart holds it.

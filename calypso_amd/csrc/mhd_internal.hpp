@@ -147,6 +147,10 @@ int  mhd_initialize(MhdContext *c, std::string *err);
 int  mhd_steps(MhdContext *c, int nsteps, std::string *err);
 int  mhd_energies(MhdContext *c, double *out7, std::string *err);
 int  mhd_pressure(MhdContext *c, double *host, std::string *err);
+// output-step energy fluxes (up to kMaxEnergyFluxes flux ids) as spectra
+constexpr int kMaxEnergyFluxes = 8;
+int  mhd_energy_fluxes(MhdContext *c, int nflux, const int *ids, double *host,
+                       std::string *err);
 int  mhd_field_io(MhdContext *c, int comp, double *host, bool to_host, std::string *err);
 // the same with the host array in the caller's rj mode order (mhd_set_rj_order)
 int  mhd_field_io_caller(MhdContext *c, int comp, double *host, bool to_host, std::string *err);

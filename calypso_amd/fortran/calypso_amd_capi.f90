@@ -226,6 +226,14 @@
           type(c_ptr), value :: mhd
           real(c_double), intent(inout) :: press(*)
         end function calypso_mhd_pressure
+        integer(c_int) function calypso_mhd_energy_fluxes(mhd, nflux,    &
+     &      flux_ids, fluxes) bind(C, name="calypso_mhd_energy_fluxes")
+          import :: c_int, c_ptr, c_double
+          type(c_ptr), value :: mhd
+          integer(c_int), value :: nflux
+          integer(c_int), intent(in) :: flux_ids(*)
+          real(c_double), intent(inout) :: fluxes(*)
+        end function calypso_mhd_energy_fluxes
 !
         integer(c_int) function calypso_mhd_time(mhd, time)            &
      &      bind(C, name="calypso_mhd_time")

@@ -271,6 +271,20 @@ class MhdSimulation:
                    "calypso_mhd_pressure")
         return out
 
+    ENERGY_FLUXES = ("Lorentz_work", "work_against_Lorentz", "magnetic_ene_generation",
+                     "buoyancy_flux")
+
+    def energy_fluxes(self, names=ENERGY_FLUXES) -> np.ndarray:
+        """Spectra of the output-step energy fluxes named (enegy_fluxes_4_sph_mhd,
+        lead_fields_4_sph_mhd.f90:98-121; the reference's field labels,
+        m_phys_labels.f90:425-441), (len(names), nr, J) in the caller's rj order."""
+        ids = np.array([self.ENERGY_FLUXES.index(n) for n in names], dtype=np.int32)
+        out = np.empty((len(ids), self.nr, self.J))
+        _lib.check(_lib.load().calypso_mhd_energy_fluxes(
+            self._h, len(ids), ids.ctypes.data_as(_P), out.ctypes.data_as(_P)),
+            "calypso_mhd_energy_fluxes")
+        return out
+
     def energies(self) -> np.ndarray:
         out = np.zeros(7)
         _lib.check(_lib.load().calypso_mhd_energies(self._h, out.ctypes.data_as(_P)),

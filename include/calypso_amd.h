@@ -397,6 +397,20 @@ int calypso_mhd_step(calypso_mhd_t *mhd, int nsteps);
  * matrix is factored on the first call (5 nr J doubles of device memory,
  * kept until calypso_mhd_destroy); params.coef_press must be set. */
 int calypso_mhd_pressure(calypso_mhd_t *mhd, double *host);
+/* The energy fluxes of the current state as an output step computes them
+ * (enegy_fluxes_4_sph_mhd, lead_fields_4_sph_mhd.f90:98-121, replacing
+ * s_cal_energy_flux_rtp, cal_energy_flux_rtp.f90:86-150, between
+ * sph_back_trans_snapshot_MHD and sph_forward_trans_snapshot_MHD,
+ * sph_transforms_4_MHD.f90:200-278): u, B, J, the magnetic induction and the
+ * temperature perturbation to the grid, there the scalar fluxes, and their
+ * forward transform.  flux_ids[nflux] (1 <= nflux <= 8): 0 Lorentz_work
+ * u.(coef_lor J x B), 1 work_against_Lorentz (its negative), 2
+ * magnetic_ene_generation B.(curl(u x B)), 3 buoyancy_flux coef_buo T u_r r
+ * (m_phys_labels.f90:425-441).  host(nflux, nr, J): spectra in the caller's
+ * rj order.  Temporary device buffers of about 13 grids are allocated and
+ * freed per call. */
+int calypso_mhd_energy_fluxes(calypso_mhd_t *mhd, int nflux, const int *flux_ids,
+                              double *host);
 /* out[7]: K_ene_pol, K_ene_tor, K_ene, temperature, M_ene_pol, M_ene_tor,
  * M_ene over the shell (sph_pwr_volume_s.dat, output_rms_sph_mhd_control). */
 int calypso_mhd_energies(calypso_mhd_t *mhd, double *out);

@@ -1487,6 +1487,43 @@ class MhdStep:
         p[ki:ko + 1, 0] = p[ki:ko + 1, 0] - p[ko, 0]
         return p
 
+    def energy_fluxes(self):
+        """enegy_fluxes_4_sph_mhd (lead_fields_4_sph_mhd.f90:98-121) at an
+        output step: the snapshot's backward transform of u, B, J, the
+        magnetic induction (the curl of u x B of the step's nonlinear stage)
+        and the temperature perturbation (sph_back_trans_snapshot_MHD), the
+        scalar fluxes on the grid (s_cal_energy_flux_rtp,
+        cal_energy_flux_rtp.f90:101-150: the Lorentz force coef_lor J x B as
+        cal_cross_prod_w_coef makes it, dotted with u; its negative; B dotted
+        with the induction; cal_buoyancy_flux_rtp_smp :222-252 coef_buo T u_r r
+        over the perturbation temperature, the spherical-shell reference's
+        branch), and their forward transform (sph_forward_trans_snapshot_MHD).
+        Returns (4, nr, J): Lorentz_work, work_against_Lorentz,
+        magnetic_ene_generation, buoyancy_flux."""
+        f, J, nr = self.f, self.J, self.nr
+        perm = self.rj_of_rlm
+        sp = np.zeros((nr, J, 13))
+        for c0, name in ((0, "velo"), (3, "magne"), (6, "current"), (9, "induction")):
+            for c in range(3):
+                sp[:, :, c0 + c] = f[name][c][:, perm]
+        sp[:, :, 12] = f["temp"][:, perm]
+        nth, nphi = self.c.nth, self.c.nphi
+        vr = leg_bwd(self.grid, 13, 4, 1, sp.ravel())
+        g = fft_bwd(self.L, nth, nr, nphi, 13, vr).reshape(13, -1)
+        u, b, cj, ind, t = g[0:3], g[3:6], g[6:9], g[9:12], g[12]
+        fl = [(cj[1] * b[2] - cj[2] * b[1]) * self.coef_lor,
+              (cj[2] * b[0] - cj[0] * b[2]) * self.coef_lor,
+              (cj[0] * b[1] - cj[1] * b[0]) * self.coef_lor]
+        ujb = fl[0] * u[0] + fl[1] * u[1] + fl[2] * u[2]
+        r = np.tile(self.radius, t.size // nr)
+        flux = np.stack([ujb, -1.0 * ujb, ind[0] * b[0] + ind[1] * b[1] + ind[2] * b[2],
+                         self.coef_buo * t * u[0] * r])
+        vr2 = fft_fwd(self.L, nth, nr, nphi, 4, flux)
+        sp2 = leg_fwd(self.grid, 4, 0, 4, vr2).reshape(nr, J, 4)
+        inv = np.empty(J, dtype=np.int64)
+        inv[perm] = np.arange(J)
+        return np.stack([sp2[:, inv, q] for q in range(4)])
+
     def energies(self):
         """The sph_pwr_volume_s.dat columns K_ene_pol, K_ene_tor, K_ene,
         temperature, M_ene_pol, M_ene_tor, M_ene over the whole shell, with the

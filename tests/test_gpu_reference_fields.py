@@ -7,6 +7,7 @@ tests/golden/make_picked_modes.py), within 1e-12 of each column's largest
 value; the output-step pressure (calypso_mhd_pressure) against the same
 pickup's pressure column within the oracle's bounds, and against the oracle
 after three steps."""
+import ctypes
 import json
 import os
 
@@ -16,7 +17,8 @@ import pytest
 from calypso_amd import _lib, mhd
 from oracle import mhd_step
 
-from test_oracle_mhd_step import PRESS_TOL_L, PRESS_TOL_L0, pressure_errors
+from test_oracle_mhd_step import (FLUX_TOL, PRESS_TOL_L, PRESS_TOL_L0, energy_flux_errors,
+                                 pressure_errors)
 
 pytestmark = pytest.mark.gpu
 
@@ -111,4 +113,61 @@ def test_pressure_needs_coefficient_and_state(torch_cuda):
     sim.initialize()
     with pytest.raises(_lib.CalypsoError, match="coef_press"):
         sim.pressure()
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_energy_fluxes_match_oracle_and_reference(torch_cuda):
+    """calypso_mhd_energy_fluxes (enegy_fluxes_4_sph_mhd) on Dynamobench
+    case 1: after 3 steps against the oracle's fluxes of the same run within
+    1e-12 of each flux's largest value; after steps 10 and 20 against what
+    the reference's own MHD program writes through its pickup monitor
+    (tests/golden/energy_fluxes_case1.json) within the oracle's bound; a
+    subset in another order gives the same rows to rounding (the component
+    count selects the transform's tile plan)."""
+    print("[energy fluxes] Dynamobench case 1 vs oracle and reference", flush=True)
+    o = mhd_step.MhdStep(mhd_step.DynamoBenchmarkCase1)
+    o.initialize()
+    for _ in range(3):
+        o.step()
+    want = o.energy_fluxes()
+    p, r = mhd.dynamobench_case1()
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_dynamobench_ic(400)
+    sim.initialize()
+    sim.step(3)
+    got = sim.energy_fluxes()
+    for q, name in enumerate(mhd.MhdSimulation.ENERGY_FLUXES):
+        e = np.abs(got[q] - want[q]).max() / np.abs(want[q]).max()
+        print(f"[energy fluxes] step 3 {name}: {e:.1e}", flush=True)
+        assert e < 1e-12, (name, e)
+    sub = sim.energy_fluxes(("buoyancy_flux", "Lorentz_work"))
+    for a, b in ((sub[0], got[3]), (sub[1], got[0])):
+        assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "energy_fluxes_case1.json")))
+    for step in (10, 20):
+        sim.step(step - sim.i_step)
+        errs = energy_flux_errors(ref, step, sim.energy_fluxes())
+        print(f"[energy fluxes] step {step} vs reference: {errs}", flush=True)
+        assert max(errs.values()) < FLUX_TOL, (step, errs)
+    sim.close()
+
+
+@pytest.mark.gpu
+def test_energy_fluxes_reject_bad_requests(torch_cuda):
+    """An unknown flux id, too many fluxes and a state that is not
+    initialised are refused with a message."""
+    p, r = mhd.dynamobench_case1()
+    sim = mhd.MhdSimulation(p, r)
+    sim.set_dynamobench_ic(400)
+    with pytest.raises(_lib.CalypsoError, match="not initialised"):
+        sim.energy_fluxes()
+    sim.initialize()
+    h = _lib.load()
+    out = np.empty(9 * sim.nr * sim.J)
+    for ids in (np.array([4], np.int32), np.zeros(9, np.int32)):
+        rc = h.calypso_mhd_energy_fluxes(sim._h, len(ids), ids.ctypes.data_as(ctypes.c_void_p),
+                                         out.ctypes.data_as(ctypes.c_void_p))
+        assert rc != 0
     sim.close()

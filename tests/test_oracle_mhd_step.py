@@ -262,3 +262,60 @@ def test_seeded_c1_matches_reference_program():
         s.step()
         np.testing.assert_allclose(s.energies(), np.array(ref["steps"][str(s.i_step)]),
                                    rtol=1e-12, atol=0, err_msg=f"step {s.i_step}")
+
+
+def energy_flux_errors(ref, step, fluxes):
+    """Per-flux error of spectra (4, nr, J) against the fixture's columns at
+    its modes, relative to each column's largest value."""
+    errs = {}
+    for q, name in enumerate(ref["columns"]):
+        want = np.array([ref["steps"][str(step)][lm] for lm in ref["modes"]])[:, :, q]
+        scale = np.abs(want).max()
+        e = 0.0
+        for i, lm in enumerate(ref["modes"]):
+            l, m = (int(x) for x in lm.split(","))
+            e = max(e, np.abs(fluxes[q][:, l * (l + 1) + m] - want[i]).max() / scale)
+        errs[name] = e
+    return errs
+
+
+# bound of the energy fluxes against the reference's pickup (15 printed
+# digits), relative to each column's largest value
+FLUX_TOL = 1e-11
+
+
+def test_energy_fluxes_match_reference_pickup():
+    """The oracle's output-step energy fluxes (enegy_fluxes_4_sph_mhd):
+    Lorentz_work, work_against_Lorentz, magnetic_ene_generation and
+    buoyancy_flux against what the reference's own MHD program writes through
+    its pickup monitor after steps 10 and 20 of Dynamobench case 1
+    (tests/golden/energy_fluxes_case1.json, by make_energy_fluxes.py)."""
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "energy_fluxes_case1.json")))
+    s = mhd_step.MhdStep(mhd_step.DynamoBenchmarkCase1)
+    s.initialize()
+    for step in (10, 20):
+        while s.i_step < step:
+            s.step()
+        errs = energy_flux_errors(ref, step, s.energy_fluxes())
+        print(step, errs)
+        assert max(errs.values()) < FLUX_TOL, (step, errs)
+
+
+def test_energy_flux_terms_are_pinned():
+    """A dropped radius factor in the buoyancy flux or a wrong sign of the
+    Lorentz work is far outside the bound."""
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "energy_fluxes_case1.json")))
+    s = mhd_step.MhdStep(mhd_step.DynamoBenchmarkCase1)
+    s.initialize()
+    while s.i_step < 10:
+        s.step()
+    good = s.energy_fluxes()
+    bad = good.copy()
+    bad[[0, 1]] = bad[[1, 0]]
+    errs = energy_flux_errors(ref, 10, bad)
+    assert errs["Lorentz_work"] > 1.0 and errs["work_against_Lorentz"] > 1.0
+    s.radius = np.ones_like(s.radius)
+    errs = energy_flux_errors(ref, 10, s.energy_fluxes())
+    assert errs["buoyancy_flux"] > 1e-2 and errs["Lorentz_work"] < FLUX_TOL
